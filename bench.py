@@ -1,0 +1,197 @@
+#!/usr/bin/env python
+"""Benchmark: chiral Lebwohl-Lasher checkerboard sweeps on one MI355X.
+
+A "step" is one full sweep (director half-sweeps on both colours, then
+chirality half-sweeps on both colours, then the on-device step adaptation and,
+every nout sweeps, the observable reduction) of an L^3 FP64 lattice already
+resident in HBM.  The metric is site-updates/s = L^3 * sweeps / s.
+
+Default workload: L = 256 (16,777,216 sites), seeded isotropic directors and
++-1 spins, eps = 1, K = 1.0, g = 0.5, J = 0, kT = 0.7, d0 = 0.3, nout = 50, open
+boundaries and reference-compatible arithmetic (the drop-in path's defaults),
+site-keyed Philox random numbers.
+
+Prints ONE JSON line.  The roofline object is for the dominant kernel, the
+director half-sweep, timed with HIP events on the stream it runs on; its
+algorithmic traffic is 37 B per site per launch (read all directors 24 B and
+all spins 1 B, write the active half's directors 12 B).  The CPU baseline is
+the reference's own run() compiled from source (serial), timed on a bounded
+sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "site-updates/s at L=256 FP64 + 64-replica batched scan; % of HBM roofline"
+HBM_PEAK_GBS = 8000.0
+BYTES_DIR = 37.0    # director half-sweep, per lattice site per launch
+BYTES_SPIN = 25.5   # chirality half-sweep
+BYTES_SWEEP = 125.0  # per site-update, four half-sweeps
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(L, st0, args):
+    """Time the reference's run() (oracle/_ref, built from the reference
+    source) on a bounded sample: args.cpu_sweeps sweeps of the same L^3
+    workload, serial, host random_number included.  Falls back to the C
+    oracle when the reference build is absent."""
+    import numpy as np
+
+    from oracle.oracle import REF_SO, RNG_PHILOX, Oracle, Reference, copy_state
+
+    st = copy_state(st0)
+    n = args.cpu_sweeps
+    cpu_model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    cpu_model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    if os.path.exists(REF_SO):
+        ref = Reference(stream=False)
+        t0 = time.perf_counter()
+        # nstart = 2 skips the header line; nout beyond the range prints no rows
+        ref.run(st, 2, 1 + n, 10 ** 9, args.g, args.K, args.d0, args.kT)
+        dt = time.perf_counter() - t0
+        kind, what = "reference", "reference chll_mod.f90 run() built from source with flang -O2 (serial)"
+    else:
+        t0 = time.perf_counter()
+        Oracle().run(st, 2, 1 + n, 10 ** 9, args.g, args.K, args.d0, args.kT, rng=RNG_PHILOX, seed=1)
+        dt = time.perf_counter() - t0
+        kind, what = "port", "C restatement of the reference (oracle), serial"
+    N = L ** 3
+    return {"value": N * n / dt, "unit": "site-updates/s", "cores": 1, "kind": kind,
+            "sample": f"{n} sweeps of the L={L} workload ({what}); {dt:.2f} s wall on {cpu_model}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--L", type=int, default=256)
+    ap.add_argument("--nout", type=int, default=50)
+    ap.add_argument("--arith", default="compat", choices=["compat", "fp64"])
+    ap.add_argument("--bc", default="open", choices=["open", "periodic"])
+    ap.add_argument("--rng", default="philox", choices=["philox", "stream"])
+    ap.add_argument("--g", type=float, default=0.5)
+    ap.add_argument("--K", type=float, default=1.0)
+    ap.add_argument("--kT", type=float, default=0.7)
+    ap.add_argument("--d0", type=float, default=0.3)
+    ap.add_argument("--cpu-sweeps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="no per-kernel events")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; runs on more than one GPU are out of scope "
+              "(the lattice would shard into k-slabs with one halo plane per phase)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from chllmodel_amd.lattice import Lattice
+    from chllmodel_amd.states import isotropic_state
+
+    L = args.L
+    N = L ** 3
+    log(f"bench: L={L} N={N} bc={args.bc} arith={args.arith} rng={args.rng} steps={args.steps} warmup={args.warmup}")
+    st0 = isotropic_state(L, L, L, 20261203)
+    torch.cuda.init()
+    stream = torch.cuda.Stream()
+    lat = Lattice(L, L, L, bc=args.bc)
+    lat.set_stream(stream.cuda_stream)
+    lat.upload_state(st0)
+    lat.set_step(args.d0)
+    p = lat.params(g=args.g, KK=args.K, kbt=args.kT, rng=args.rng, arith=args.arith, seed=20261203, adapt=True)
+    W, K = args.warmup, args.steps
+    lat.sweeps(1, W, args.nout, params=p, want_rows=False)
+    lat.sync()
+    torch.cuda.synchronize()
+    log("bench: warmup done")
+    if not args.no_timing:
+        lat.set_timing(True)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    lat.sweeps(W + 1, W + K, args.nout, params=p, want_rows=False)
+    ev1.record(stream)
+    lat.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = ev0.elapsed_time(ev1)
+    kt = lat.timing() if not args.no_timing else {}
+    row = lat.measure(args.g, args.K)
+    d_now = lat.get_step()
+    lat.close()
+    log(f"bench: timed {K} sweeps in {wall:.4f} s (events {ev_ms:.3f} ms); E_true={row['E_true']:.6f} "
+        f"eta={row['eta_exact']:.6f} d={d_now:.5f}")
+
+    sups = N * K / wall
+    kernels = {}
+    for name, (ms, cnt) in kt.items():
+        if cnt:
+            kernels[name] = {"launches": cnt, "avg_us": 1e3 * ms / cnt, "total_ms": ms}
+    roof = None
+    if "director" in kernels:
+        avg_s = kernels["director"]["avg_us"] * 1e-6
+        ach = BYTES_DIR * N / avg_s / 1e9
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", f"director_L{L}_{args.bc}_{args.arith}_traffic.json")
+        if os.path.exists(tf):
+            with open(tf) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": traffic, "kernel": "k_dir (director half-sweep)",
+                "algorithmic_bytes_per_launch": BYTES_DIR * N}
+        if "chirality" in kernels:
+            kernels["chirality"]["GBps"] = BYTES_SPIN * N / (kernels["chirality"]["avg_us"] * 1e-6) / 1e9
+        kernels["director"]["GBps"] = ach
+    out = {
+        "metric": METRIC,
+        "value": sups,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * wall / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"ChLL Metropolis sweep, L={L} single lattice, {args.bc} boundaries, "
+                               f"{args.arith} arithmetic, {args.rng} RNG, K={args.K} g={args.g} kT={args.kT} "
+                               f"d0={args.d0} nout={args.nout}",
+                   "L": L, "sites": N, "bc": args.bc, "arith": args.arith, "rng": args.rng, "nout": args.nout},
+        "sweeps_per_s": K / wall,
+        "sweep_hbm_frac": sups * BYTES_SWEEP / (HBM_PEAK_GBS * 1e9),
+        "roofline": roof,
+        "kernels": kernels,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
+        log(f"bench: CPU baseline, {args.cpu_sweeps} sweeps at L={L}")
+        out["cpu_baseline"] = cpu_baseline(L, st0, args)
+        out["speedup_vs_cpu"] = sups / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,224 @@
+"""``ChLLSim``: the host-side simulation class of mdeutsc2/ChLLModel
+(chll/chll.py:38-304) on top of the MI355X library.
+
+Same constructor, methods, array layouts (Fortran-ordered float64 directors,
+int32 spins), output folders and file formats as the reference; the two native
+calls go through the same ctypes signatures (``ftn_init``, ``ftn_run``,
+chll/chll.py:9-36) into libchll_mod.so built from this package's HIP sources.
+
+Keyword-only extensions (defaults reproduce the reference): ``bc`` ('open' |
+'periodic'), ``arith`` ('compat' | 'fp64'), ``rng`` ('philox' | 'stream'),
+``seed`` (Philox key / stream seed) and ``verbose``.
+
+Deliberate differences from the reference, each a crash or a silent loss
+there: ``save_params`` is skipped when ``name`` is None (the reference raises
+TypeError, chll/chll.py:141); the initialisers are vectorised with numpy in
+the reference's draw order instead of Python triple loops.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import ARITH, BC, RNG, ROW_NAMES
+
+libchll_mod = _lib.lib()
+ftn_init = libchll_mod.init
+ftn_run = libchll_mod.run
+
+
+class ChLLSim:
+    def __init__(self, ni, nj, nk, kbt, d, KK, g, rho, name=None, overwrite=False, *,
+                 bc="open", arith="compat", rng="philox", seed=None, verbose=True):
+        # parameters (chll/chll.py:39-49)
+        self.name = name
+        self.ni = ni
+        self.nj = nj
+        self.nk = nk
+        self.kbt = kbt
+        self.d = d
+        self.KK = KK
+        self.g = g
+        self.rho = rho
+        # internal parameters (:51-54)
+        self.overwrite = overwrite
+        self.simpath = None
+        self.nsub = int((self.ni * self.nj * self.nk) / 2)
+        # options of this implementation
+        if bc not in BC or arith not in ARITH or rng not in ("philox", "stream"):
+            raise ValueError(f"bad option bc={bc!r} arith={arith!r} rng={rng!r}")
+        self.bc, self.arith, self.rng, self.seed, self.verbose = bc, arith, rng, seed, verbose
+        self.d_out = None  # adapted step returned by the last run (the reference drops it, chll.py:170)
+        self.ext = None    # full measurement rows of the last run (see ROW_NAMES)
+        # arrays, Fortran-ordered (:56-82)
+        self.nx = np.zeros((self.ni, self.nj, self.nk), dtype=np.float64, order="F")
+        self.ny = np.zeros((self.ni, self.nj, self.nk), dtype=np.float64, order="F")
+        self.nz = np.zeros((self.ni, self.nj, self.nk), dtype=np.float64, order="F")
+        self.s = np.zeros((self.ni, self.nj, self.nk), dtype=np.int32, order="F")
+        self.dope = np.zeros((self.ni, self.nj, self.nk), dtype=np.int32, order="F")
+        self.sl1 = np.zeros(self.nsub, dtype=np.int32, order="F")
+        self.sl2 = np.zeros(self.nsub, dtype=np.int32, order="F")
+        self.m = None
+        self.msteps = None
+        self.setup_folders()
+
+    # -- initial states (chll/chll.py:84-115) ---------------------------------
+    def _default_directors(self):
+        if np.mean(np.sqrt(self.nx * self.nx + self.ny * self.ny + self.nz * self.nz)) == 0.0:
+            print("> Nx,Ny,Nz not initialized, setting nx[:,:,:] = 1.0")
+            self.nx[:, :, :] = 1.0
+            self.ny[:, :, :] = 0.0
+            self.nz[:, :, :] = 0.0
+
+    def _ftn_init(self):
+        ftn_init(self.nx, self.ny, self.nz, self.s, self.dope, self.sl1, self.sl2,
+                 ct.c_int(self.ni), ct.c_int(self.nj), ct.c_int(self.nk), ct.c_int(self.nsub))
+        err = _lib.last_error()
+        if err:
+            raise _lib.ChllError(err)
+        print("> Init Done!")
+
+    def init_rand_s(self):
+        """Spins +1, then -1 where np.random.rand() < 0.5, one global-stream draw
+        per site in i-major order (chll/chll.py:85-90)."""
+        u = np.random.rand(self.ni, self.nj, self.nk)
+        self.s[:, :, :] = np.where(u < 0.5, -1, 1).astype(np.int32)
+        self._default_directors()
+        self._ftn_init()
+
+    def init_with_s(self, init_s):
+        self.s[:, :, :] = init_s
+        self._default_directors()
+        self._ftn_init()
+
+    # -- output folders and files (chll/chll.py:117-262) ----------------------
+    def setup_folders(self):
+        data_folder = "./data"
+        if self.name is None:
+            print("> Not saving sim")
+        else:
+            if os.path.exists(data_folder):
+                simname_folder_path = os.path.join(data_folder, self.name)
+                if os.path.exists(simname_folder_path):
+                    print(f"> ERROR: '{self.name}' in '{data_folder}'!")
+                    if self.overwrite:
+                        print(f"> Overwriting '{self.name}' in '{data_folder}' ... ")
+                        os.makedirs(simname_folder_path, exist_ok=True)
+                    else:
+                        exit()
+                else:
+                    os.makedirs(simname_folder_path, exist_ok=True)
+                    print(f"> {simname_folder_path}")
+                self.simpath = simname_folder_path
+            else:
+                print(f"ERROR: '{data_folder}' does not exist.")
+                exit()
+
+    def save_params(self, nsteps, nout):
+        if self.name is None:
+            return
+        params_name = self.name + self.name + "_params.txt"
+        with open(os.path.join(self.simpath, params_name), "w") as f:
+            f.write(self.name + "\n")
+            f.write("nsteps = " + str(nsteps) + " nout = " + str(nout) + "\n")
+            f.write("ni,nj,nk = " + str(self.ni) + " " + str(self.nj) + " " + str(self.nk) + "\n")
+            f.write("kbt = " + str(self.kbt) + "\n")
+            f.write("d = " + str(self.d) + "\n")
+            f.write("KK = " + str(self.KK) + "\n")
+            f.write("g = " + str(self.g) + "\n")
+            f.write("rho = " + str(self.rho) + "\n")
+
+    def _configure(self):
+        L = _lib.lib()
+        rng = RNG[self.rng]
+        _lib.check(L.chll_configure(rng, ARITH[self.arith], BC[self.bc], 1 if self.verbose else 0))
+        if self.seed is not None:
+            if rng == RNG["philox"]:
+                _lib.check(L.chll_set_seed(int(self.seed), 0))
+                self.seed = None  # later runs continue the run counter
+            else:
+                _lib.check(L.chll_set_stream(int(self.seed), 0))
+                self.seed = None
+
+    def run(self, nsteps, nout, save=False):
+        """chll/chll.py:154-178."""
+        self.save_params(nsteps, nout)
+        self.msteps = np.zeros((int(nsteps / nout)), dtype=np.int32, order="F")
+        self.m = np.zeros((int(nsteps / nout), 5), dtype=np.float64, order="F")
+        self._configure()
+        print("> RUNNING")
+        st = time.perf_counter()
+        d = ct.c_double(self.d)
+        ftn_run(ct.c_int(1), ct.c_int(nsteps), ct.c_int(nout),
+                self.nx, self.ny, self.nz, self.s, self.dope, self.sl1, self.sl2,
+                ct.c_double(self.g), ct.c_double(self.KK), d, ct.c_double(self.kbt),
+                ct.c_int(self.ni), ct.c_int(self.nj), ct.c_int(self.nk), ct.c_int(self.nsub),
+                self.msteps, self.m)
+        et = time.perf_counter()
+        err = _lib.last_error()
+        if err:
+            raise _lib.ChllError(err)
+        self.d_out = d.value
+        rows = max(int(nsteps / nout), 0)
+        ext = np.zeros((max(rows, 1), _lib.ROW_COLS))
+        got = _lib.lib().chll_last_rows(ext.ctypes.data_as(ct.POINTER(ct.c_double)), rows)
+        self.ext = {k: ext[:got, i].copy() for i, k in enumerate(ROW_NAMES)}
+        if save:
+            self.save_macros()
+            self.save_config()
+        print("> DONE! ", et - st)
+
+    def save_config(self):
+        if self.name is not None:
+            if os.path.exists(self.simpath):
+                datafile_name = self.name + "_data.npz"
+                datafile_path = os.path.join(self.simpath, datafile_name)
+                if os.path.isfile(datafile_path) and not self.overwrite:
+                    print(f"> ERROR: '{datafile_path}' exists and overwrite=False")
+                    exit()
+                np.savez_compressed(datafile_path, nx=self.nx, ny=self.ny, nz=self.nz, s=self.s, dope=self.dope)
+                print(f"> Wrote '{datafile_path}'")
+            else:
+                print(f"> ERROR: '{self.simpath}' does not exist")
+                exit()
+
+    def save_macros(self):
+        if self.name is not None:
+            if os.path.exists(self.simpath):
+                csv_name = self.name + ".csv"
+                csv_path = os.path.join(self.simpath, csv_name)
+                if os.path.isfile(csv_path) and not self.overwrite:
+                    print(f"> ERROR: '{csv_path}' exists and overwrite=False")
+                    exit()
+                with open(csv_path, "w") as fcsv:
+                    fcsv.write("Step,TotalEnergy,Eexcess,Paccept,Faccept,D\n")
+                    for i in range(len(self.msteps)):
+                        fcsv.write(str(int(self.msteps[i])) + "," + str(self.m[i, 0]) + "," + str(self.m[i, 1]) + ","
+                                   + str(self.m[i, 2]) + "," + str(self.m[i, 3]) + "," + str(self.m[i, 4]) + "\n")
+                print(f"> Wrote '{csv_path}'")
+            else:
+                print(f"> ERROR: '{self.simpath}' does not exist")
+                exit()
+
+    def plot_config(self):
+        """Quiver plot of the y = nj/2 slice (chll/chll.py:264-282)."""
+        import matplotlib.pyplot as plt
+
+        X, Z = np.meshgrid(np.arange(0, self.ni), np.arange(0, self.nk))
+        mp = int(np.floor(self.nj / 2))
+        u = self.nx[:, mp, :]
+        w = self.nz[:, mp, :]
+        s_color = self.s[:, mp, :]
+        fig = plt.figure(figsize=(10, 10))
+        ax = fig.add_subplot(111)
+        q = ax.quiver(X, Z, u, w, s_color, pivot="mid", headlength=0, headwidth=0, headaxislength=0,
+                      scale_units="xy", scale=0.75, cmap="coolwarm")
+        ax.set_xlabel("X")
+        ax.set_ylabel("Z")
+        ax.set_title(self.name)
+        fig.colorbar(q)
+        plt.savefig(os.path.join(self.simpath, self.name + ".png"))

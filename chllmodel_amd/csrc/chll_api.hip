@@ -1,0 +1,584 @@
+// chll_api.hip -- C ABI of libchll_mod.so: the drop-in init/run entry points
+// of chll/chll_mod.f90 and the device-resident lattice extensions.
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/chll_mod.h"
+#include "chll_kernels.h"
+
+using namespace chll;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return -1;
+}
+
+#define HIPCHECK(x)                                                                   \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess) return fail("%s: %s", #x, hipGetErrorString(e_));       \
+    } while (0)
+
+template <class T>
+int dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) return fail("hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    return 0;
+}
+
+template <class T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+// Options of the drop-in run() (set by chll_configure / chll_set_*).
+struct RunConfig {
+    int rng = RNG_PHILOX, arith = ARITH_COMPAT, bc = BC_OPEN, verbose = 1;
+    uint64_t seed = 0x43484C4C2D4D4335ull;  // "CHLL-MC5"
+    uint32_t run_index = 0;
+    uint64_t stream_seed = 0, stream_pos = 0;
+    const double *buffer = nullptr;
+    int64_t buffer_len = 0;
+    std::vector<double> last_rows;
+};
+
+std::mutex g_mu;
+RunConfig g_cfg;
+bool g_cfg_env_read = false;
+
+}  // namespace
+
+enum { KIND_DIR = 0, KIND_SPIN = 1, KIND_MEASURE = 2, KIND_FINAL = 3, KIND_FILL = 4, KIND_COUNT = 5 };
+
+struct chll_lattice {
+    Geom G{};
+    int bc = BC_OPEN;
+    hipStream_t own = nullptr, stream = nullptr;
+    Fields F{};
+    double *stage = nullptr;
+    int32_t *stage_s = nullptr;
+    int *bad = nullptr;
+    uint32_t *counts = nullptr;
+    uint32_t nblk = 0;
+    double *part = nullptr;
+    long long *spart = nullptr;
+    double *rows_dev = nullptr;
+    int32_t rows_cap = 0;
+    double *d_dev = nullptr;
+    double *ustream = nullptr;
+    long long *counts_dev = nullptr;
+    size_t counts_cap = 0;
+    double *pinned = nullptr;  // host: 8 doubles
+    bool timing = false;
+    struct Ev { int kind; hipEvent_t a, b; };
+    std::vector<Ev> ev;
+    size_t ev_used = 0;
+};
+
+namespace {
+
+int check_geometry(int32_t ni, int32_t nj, int32_t nk, int32_t bc)
+{
+    if (ni < 2 || nj < 1 || nk < 1) return fail("lattice dims must be ni>=2, nj>=1, nk>=1 (got %d,%d,%d)", ni, nj, nk);
+    if (ni % 2) return fail("ni must be even for the colour-split layout (got %d)", ni);
+    const uint64_t n3 = (uint64_t)ni * nj * nk;
+    if (n3 >= (1ull << 32)) return fail("lattice too large: N = %llu sites (limit 2^32)", (unsigned long long)n3);
+    if (bc != BC_OPEN && bc != BC_PERIODIC) return fail("unknown boundary condition %d", bc);
+    if (bc == BC_PERIODIC && ((nj % 2) || (nk % 2)))
+        return fail("periodic boundaries need even nj and nk for a valid checkerboard (got %d,%d)", nj, nk);
+    return 0;
+}
+
+int tbegin(chll_lattice *L, int kind, size_t *idx)
+{
+    if (!L->timing) return 0;
+    if (L->ev_used == L->ev.size()) {
+        chll_lattice::Ev e{kind, nullptr, nullptr};
+        HIPCHECK(hipEventCreate(&e.a));
+        HIPCHECK(hipEventCreate(&e.b));
+        L->ev.push_back(e);
+    }
+    *idx = L->ev_used++;
+    L->ev[*idx].kind = kind;
+    HIPCHECK(hipEventRecord(L->ev[*idx].a, L->stream));
+    return 0;
+}
+
+int tend(chll_lattice *L, size_t idx)
+{
+    if (!L->timing) return 0;
+    HIPCHECK(hipEventRecord(L->ev[idx].b, L->stream));
+    return 0;
+}
+
+int launch_check()
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail("kernel launch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *chll_last_error(void) { return g_err.c_str(); }
+
+int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lattice **out)
+{
+    if (!out) return fail("chll_lattice_create: null output pointer");
+    *out = nullptr;
+    if (check_geometry(ni, nj, nk, bc)) return -1;
+    chll_lattice *L = new chll_lattice();
+    Geom &G = L->G;
+    G.ni = (uint32_t)ni; G.nj = (uint32_t)nj; G.nk = (uint32_t)nk;
+    G.half = G.ni / 2;
+    G.plane = G.half * G.nj;
+    G.nsub = (uint32_t)(((uint64_t)ni * nj * nk) / 2);
+    G.fd_half = make_fastdiv(G.half);
+    G.fd_nj = make_fastdiv(G.nj);
+    L->bc = bc;
+    const size_t ns = G.nsub, n3 = 2 * ns;
+    L->nblk = phase_blocks(G);
+    int rc = 0;
+    rc |= dalloc(&L->F.nx, n3);
+    rc |= dalloc(&L->F.ny, n3);
+    rc |= dalloc(&L->F.nz, n3);
+    rc |= dalloc(&L->F.s, n3);
+    rc |= dalloc(&L->bad, 1);
+    rc |= dalloc(&L->counts, 4 * (size_t)L->nblk);
+    rc |= dalloc(&L->part, 8 * (size_t)L->nblk);
+    rc |= dalloc(&L->spart, (size_t)L->nblk);
+    rc |= dalloc(&L->d_dev, 1);
+    if (!rc && hipHostMalloc((void **)&L->pinned, 8 * sizeof(double)) != hipSuccess) rc = fail("hipHostMalloc failed");
+    if (!rc && hipStreamCreate(&L->own) != hipSuccess) rc = fail("hipStreamCreate failed");
+    if (rc) {
+        std::string keep = g_err;
+        chll_lattice_destroy(L);
+        g_err = keep;
+        return -1;
+    }
+    L->stream = L->own;
+    *out = L;
+    return 0;
+}
+
+int chll_lattice_destroy(chll_lattice *L)
+{
+    if (!L) return 0;
+    if (L->stream) (void)hipStreamSynchronize(L->stream);
+    dfree(L->F.nx); dfree(L->F.ny); dfree(L->F.nz); dfree(L->F.s);
+    dfree(L->stage); dfree(L->stage_s); dfree(L->bad); dfree(L->counts);
+    dfree(L->part); dfree(L->spart); dfree(L->rows_dev); dfree(L->d_dev);
+    dfree(L->ustream); dfree(L->counts_dev);
+    if (L->pinned) (void)hipHostFree(L->pinned);
+    for (auto &e : L->ev) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    if (L->own) (void)hipStreamDestroy(L->own);
+    delete L;
+    return 0;
+}
+
+int chll_lattice_set_stream(chll_lattice *L, void *s)
+{
+    if (!L) return fail("null lattice");
+    L->stream = s ? (hipStream_t)s : L->own;
+    return 0;
+}
+
+static int ensure_stage(chll_lattice *L)
+{
+    const size_t n3 = 2 * (size_t)L->G.nsub;
+    if (!L->stage && dalloc(&L->stage, n3)) return -1;
+    if (!L->stage_s && dalloc(&L->stage_s, n3)) return -1;
+    return 0;
+}
+
+int chll_lattice_upload(chll_lattice *L, const double *nx, const double *ny, const double *nz, const int32_t *s)
+{
+    if (!L || !nx || !ny || !nz || !s) return fail("chll_lattice_upload: null argument");
+    if (ensure_stage(L)) return -1;
+    const Geom &G = L->G;
+    const size_t n3 = 2 * (size_t)G.nsub;
+    const double *src[3] = {nx, ny, nz};
+    double *dst[3] = {L->F.nx, L->F.ny, L->F.nz};
+    for (int c = 0; c < 3; ++c) {
+        HIPCHECK(hipMemcpyAsync(L->stage, src[c], n3 * sizeof(double), hipMemcpyHostToDevice, L->stream));
+        launch_scatter_f64(G, L->stage, dst[c], L->stream);
+        if (launch_check()) return -1;
+    }
+    HIPCHECK(hipMemsetAsync(L->bad, 0, sizeof(int), L->stream));
+    HIPCHECK(hipMemcpyAsync(L->stage_s, s, n3 * sizeof(int32_t), hipMemcpyHostToDevice, L->stream));
+    launch_scatter_spin(G, L->stage_s, L->F.s, L->bad, L->stream);
+    if (launch_check()) return -1;
+    int bad = 0;
+    HIPCHECK(hipMemcpyAsync(&L->pinned[0], L->bad, sizeof(int), hipMemcpyDeviceToHost, L->stream));
+    HIPCHECK(hipStreamSynchronize(L->stream));
+    memcpy(&bad, &L->pinned[0], sizeof(int));
+    if (bad) return fail("spin values must lie in [-127, 127]");
+    return 0;
+}
+
+int chll_lattice_download(chll_lattice *L, double *nx, double *ny, double *nz, int32_t *s)
+{
+    if (!L || !nx || !ny || !nz || !s) return fail("chll_lattice_download: null argument");
+    if (ensure_stage(L)) return -1;
+    const Geom &G = L->G;
+    const size_t n3 = 2 * (size_t)G.nsub;
+    double *dst[3] = {nx, ny, nz};
+    const double *src[3] = {L->F.nx, L->F.ny, L->F.nz};
+    for (int c = 0; c < 3; ++c) {
+        launch_gather_f64(G, src[c], L->stage, L->stream);
+        if (launch_check()) return -1;
+        HIPCHECK(hipMemcpyAsync(dst[c], L->stage, n3 * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    }
+    launch_gather_spin(G, L->F.s, L->stage_s, L->stream);
+    if (launch_check()) return -1;
+    HIPCHECK(hipMemcpyAsync(s, L->stage_s, n3 * sizeof(int32_t), hipMemcpyDeviceToHost, L->stream));
+    HIPCHECK(hipStreamSynchronize(L->stream));
+    return 0;
+}
+
+int chll_lattice_set_step(chll_lattice *L, double d)
+{
+    if (!L) return fail("null lattice");
+    HIPCHECK(hipStreamSynchronize(L->stream));
+    L->pinned[1] = d;
+    HIPCHECK(hipMemcpyAsync(L->d_dev, &L->pinned[1], sizeof(double), hipMemcpyHostToDevice, L->stream));
+    HIPCHECK(hipStreamSynchronize(L->stream));
+    return 0;
+}
+
+int chll_lattice_get_step(chll_lattice *L, double *d)
+{
+    if (!L || !d) return fail("null argument");
+    HIPCHECK(hipMemcpyAsync(&L->pinned[2], L->d_dev, sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    HIPCHECK(hipStreamSynchronize(L->stream));
+    *d = L->pinned[2];
+    return 0;
+}
+
+int chll_lattice_sync(chll_lattice *L)
+{
+    if (!L) return fail("null lattice");
+    HIPCHECK(hipStreamSynchronize(L->stream));
+    return 0;
+}
+
+int chll_lattice_set_timing(chll_lattice *L, int32_t on)
+{
+    if (!L) return fail("null lattice");
+    HIPCHECK(hipStreamSynchronize(L->stream));
+    L->timing = on != 0;
+    L->ev_used = 0;
+    return 0;
+}
+
+int chll_lattice_timing(chll_lattice *L, double *ms, int64_t *launches, int32_t nkinds)
+{
+    if (!L) return fail("null lattice");
+    HIPCHECK(hipStreamSynchronize(L->stream));
+    for (int k = 0; k < nkinds; ++k) { if (ms) ms[k] = 0.0; if (launches) launches[k] = 0; }
+    for (size_t i = 0; i < L->ev_used; ++i) {
+        float t = 0.f;
+        HIPCHECK(hipEventElapsedTime(&t, L->ev[i].a, L->ev[i].b));
+        const int k = L->ev[i].kind;
+        if (k < nkinds) {
+            if (ms) ms[k] += t;
+            if (launches) launches[k] += 1;
+        }
+    }
+    L->ev_used = 0;
+    return 0;
+}
+
+static int ensure_rows(chll_lattice *L, int32_t rows)
+{
+    if (rows <= L->rows_cap) return 0;
+    dfree(L->rows_dev);
+    L->rows_cap = 0;
+    if (dalloc(&L->rows_dev, (size_t)rows * kRowCols)) return -1;
+    L->rows_cap = rows;
+    return 0;
+}
+
+int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t nstop, int32_t nout,
+                        double *rows, int32_t max_rows, int64_t *counts)
+{
+    if (!L || !p) return fail("chll_lattice_sweeps: null argument");
+    if (p->rng != RNG_PHILOX && p->rng != RNG_STREAM && p->rng != RNG_BUFFER) return fail("unknown rng mode %d", p->rng);
+    if (p->arith != ARITH_COMPAT && p->arith != ARITH_FP64) return fail("unknown arithmetic mode %d", p->arith);
+    if (max_rows < 0) max_rows = 0;
+    const int64_t nsw = (int64_t)nstop - nstart + 1;
+    if (nsw <= 0) return 0;
+    const Geom &G = L->G;
+    const uint64_t ns = G.nsub;
+    if (p->rng == RNG_BUFFER) {
+        if (!p->buffer) return fail("rng=buffer without a uniform buffer");
+        if ((int64_t)(p->stream_pos + (uint64_t)nsw * 10 * ns) > p->buffer_len)
+            return fail("uniform buffer too short: %lld sweeps need %llu draws from position %llu, buffer holds %lld",
+                        (long long)nsw, (unsigned long long)(nsw * 10 * ns), (unsigned long long)p->stream_pos,
+                        (long long)p->buffer_len);
+    }
+    if (p->rng != RNG_PHILOX && !L->ustream && dalloc(&L->ustream, 6 * ns)) return -1;
+    if (ensure_rows(L, max_rows)) return -1;
+    if (counts && (size_t)nsw * 2 > L->counts_cap) {
+        dfree(L->counts_dev);
+        L->counts_cap = 0;
+        if (dalloc(&L->counts_dev, (size_t)nsw * 2)) return -1;
+        L->counts_cap = (size_t)nsw * 2;
+    }
+    hipStream_t st = L->stream;
+    PhaseArgs A{};
+    A.g = p->g; A.KK = p->KK; A.kbt = p->kbt; A.inv_kbt = 1.0 / p->kbt;
+    A.d = L->d_dev; A.seed = p->seed; A.run_index = p->run_index; A.replica = p->replica;
+    FinArgs FA{};
+    FA.counts = L->counts; FA.nblk_phase = L->nblk; FA.part = L->part; FA.spart = L->spart;
+    FA.nblk_meas = L->nblk; FA.n3 = 2 * ns; FA.d = L->d_dev; FA.rows = L->rows_dev;
+    FA.counts_out = counts ? L->counts_dev : nullptr; FA.adapt = p->adapt;
+    int32_t row = 0;
+    size_t ti = 0;
+    for (int64_t t = 0; t < nsw; ++t) {
+        const int32_t istep = nstart + (int32_t)t;
+        if (p->rng == RNG_STREAM) {
+            if (tbegin(L, KIND_FILL, &ti)) return -1;
+            launch_stream_fill(L->ustream, p->seed, p->stream_pos, G.nsub, st);
+            if (launch_check() || tend(L, ti)) return -1;
+            p->stream_pos += 10 * ns;
+        } else if (p->rng == RNG_BUFFER) {
+            static const uint64_t off[6] = {0, 1, 3, 4, 7, 9};
+            for (int q = 0; q < 6; ++q)
+                HIPCHECK(hipMemcpyAsync(L->ustream + q * ns, p->buffer + p->stream_pos + off[q] * ns,
+                                        ns * sizeof(double), hipMemcpyHostToDevice, st));
+            p->stream_pos += 10 * ns;
+        }
+        A.istep = (uint32_t)istep;
+        for (int ph = 0; ph < 4; ++ph) {
+            const int kind = ph >> 1, colour = ph & 1;
+            if (p->rng != RNG_PHILOX) {
+                if (kind == 0) { A.u_phi = L->ustream + (2 * colour) * ns; A.u_acc = L->ustream + (2 * colour + 1) * ns; }
+                else { A.u_phi = nullptr; A.u_acc = L->ustream + (4 + colour) * ns; }
+            }
+            if (tbegin(L, kind == 0 ? KIND_DIR : KIND_SPIN, &ti)) return -1;
+            launch_phase(kind, p->arith, p->rng == RNG_PHILOX ? RNG_PHILOX : RNG_STREAM, L->bc, G, L->F, colour, A,
+                         L->counts + (size_t)ph * L->nblk, st);
+            if (launch_check() || tend(L, ti)) return -1;
+        }
+        const bool meas = nout > 0 && (istep % nout) == 0 && row < max_rows;
+        if (meas) {
+            if (tbegin(L, KIND_MEASURE, &ti)) return -1;
+            launch_measure(L->bc, G, L->F, p->g, p->KK, L->part, L->spart, st);
+            if (launch_check() || tend(L, ti)) return -1;
+        }
+        FA.istep = istep; FA.measure = meas ? 1 : 0; FA.row = row; FA.sweep_index = t;
+        if (tbegin(L, KIND_FINAL, &ti)) return -1;
+        launch_finalize(FA, st);
+        if (launch_check() || tend(L, ti)) return -1;
+        if (meas) ++row;
+    }
+    if (rows && row > 0)
+        HIPCHECK(hipMemcpyAsync(rows, L->rows_dev, (size_t)row * kRowCols * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (counts)
+        HIPCHECK(hipMemcpyAsync(counts, L->counts_dev, (size_t)nsw * 2 * sizeof(long long), hipMemcpyDeviceToHost, st));
+    if (rows || counts) HIPCHECK(hipStreamSynchronize(st));
+    return row;
+}
+
+int chll_lattice_measure(chll_lattice *L, double g, double KK, double *row)
+{
+    if (!L || !row) return fail("chll_lattice_measure: null argument");
+    if (ensure_rows(L, 1)) return -1;
+    hipStream_t st = L->stream;
+    // counters of the last sweep are reused as they stand; only the row matters here
+    HIPCHECK(hipMemsetAsync(L->counts, 0, 4 * (size_t)L->nblk * sizeof(uint32_t), st));
+    launch_measure(L->bc, L->G, L->F, g, KK, L->part, L->spart, st);
+    if (launch_check()) return -1;
+    FinArgs FA{};
+    FA.counts = L->counts; FA.nblk_phase = L->nblk; FA.part = L->part; FA.spart = L->spart;
+    FA.nblk_meas = L->nblk; FA.n3 = 2 * (uint64_t)L->G.nsub; FA.d = L->d_dev; FA.rows = L->rows_dev;
+    FA.istep = 0; FA.measure = 1; FA.row = 0; FA.adapt = 0;
+    launch_finalize(FA, st);
+    if (launch_check()) return -1;
+    HIPCHECK(hipMemcpyAsync(row, L->rows_dev, kRowCols * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Drop-in entry points.
+
+static void read_env_once()
+{
+    if (g_cfg_env_read) return;
+    g_cfg_env_read = true;
+    const char *q = getenv("CHLL_QUIET");
+    if (q && *q && strcmp(q, "0") != 0) g_cfg.verbose = 0;
+}
+
+int chll_configure(int32_t rng, int32_t arith, int32_t bc, int32_t verbose)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    read_env_once();
+    if (rng != RNG_PHILOX && rng != RNG_STREAM && rng != RNG_BUFFER) return fail("unknown rng mode %d", rng);
+    if (arith != ARITH_COMPAT && arith != ARITH_FP64) return fail("unknown arithmetic mode %d", arith);
+    if (bc != BC_OPEN && bc != BC_PERIODIC) return fail("unknown boundary condition %d", bc);
+    g_cfg.rng = rng; g_cfg.arith = arith; g_cfg.bc = bc; g_cfg.verbose = verbose;
+    return 0;
+}
+
+int chll_set_seed(uint64_t seed, uint32_t run_index)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_cfg.seed = seed;
+    g_cfg.run_index = run_index;
+    return 0;
+}
+
+int chll_set_stream(uint64_t seed, uint64_t pos)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_cfg.stream_seed = seed;
+    g_cfg.stream_pos = pos;
+    return 0;
+}
+
+uint64_t chll_get_stream_pos(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_cfg.stream_pos;
+}
+
+int chll_set_uniform_buffer(const double *u, int64_t n, uint64_t pos)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_cfg.buffer = u;
+    g_cfg.buffer_len = u ? n : 0;
+    g_cfg.stream_pos = pos;
+    return 0;
+}
+
+int32_t chll_last_rows(double *out, int32_t rows)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!out) return fail("null output");
+    const int32_t have = (int32_t)(g_cfg.last_rows.size() / kRowCols);
+    const int32_t n = rows < have ? rows : have;
+    if (n > 0) memcpy(out, g_cfg.last_rows.data(), (size_t)n * kRowCols * sizeof(double));
+    return n;
+}
+
+void init(double *nx, double *ny, double *nz, int32_t *s, int32_t *dope, int32_t *sl1, int32_t *sl2,
+          const int32_t *ni, const int32_t *nj, const int32_t *nk, const int32_t *nsub)
+{
+    (void)nx; (void)ny; (void)nz; (void)s; (void)dope;
+    g_err.clear();
+    if (!ni || !nj || !nk || !nsub || !sl1 || !sl2) { fail("init: null argument"); fprintf(stderr, "chll init: %s\n", g_err.c_str()); return; }
+    const int64_t n3 = (int64_t)*ni * *nj * *nk;
+    if (*ni < 1 || *nj < 1 || *nk < 1 || (n3 & 1) || *nsub != n3 / 2) {
+        fail("init: need positive dims with N even and nsub = N/2 (ni,nj,nk,nsub = %d,%d,%d,%d)", *ni, *nj, *nk, *nsub);
+        fprintf(stderr, "chll init: %s\n", g_err.c_str());
+        return;
+    }
+    // chll/chll_mod.f90:23-36: Fortran order; 1-based i+j+k odd -> sl1, even -> sl2.
+    int64_t a = 0, b = 0;
+    for (int32_t k = 1; k <= *nk; ++k)
+        for (int32_t j = 1; j <= *nj; ++j) {
+            const int32_t base = (j - 1) * *ni + (k - 1) * (*ni * *nj);
+            for (int32_t i = 1; i <= *ni; ++i) {
+                if ((i + j + k) & 1) sl1[a++] = base + i;
+                else sl2[b++] = base + i;
+            }
+        }
+}
+
+static chll_lattice *g_run_lat = nullptr;
+
+static void print_fixed(double v)
+{
+    char buf[64];
+    snprintf(buf, sizeof buf, "%12.6f", v);
+    if (strlen(buf) > 12) printf(" ************");  // Fortran F12.6 overflow
+    else printf(" %s", buf);
+}
+
+void run(const int32_t *nstart, const int32_t *nstop, const int32_t *nout,
+         double *nx, double *ny, double *nz, int32_t *s, int32_t *dope, int32_t *sl1, int32_t *sl2,
+         double *g, double *KK, double *d, double *kbt,
+         const int32_t *ni, const int32_t *nj, const int32_t *nk, int32_t *nsub,
+         int32_t *msteps, double *m)
+{
+    (void)dope; (void)sl1; (void)sl2;
+    std::lock_guard<std::mutex> lk(g_mu);
+    read_env_once();
+    g_err.clear();
+    auto bail = [](void) { fprintf(stderr, "chll run: %s\n", g_err.c_str()); };
+    if (!nstart || !nstop || !nout || !nx || !ny || !nz || !s || !g || !KK || !d || !kbt || !ni || !nj || !nk ||
+        !nsub) {
+        fail("run: null argument");
+        return bail();
+    }
+    if (*nout <= 0) { fail("run: nout must be positive (got %d)", *nout); return bail(); }
+    if (check_geometry(*ni, *nj, *nk, g_cfg.bc)) return bail();
+    const int32_t rows = (1 + *nstop - *nstart) / *nout;
+    if (rows > 0 && (!msteps || !m)) { fail("run: null msteps/m"); return bail(); }
+    if (g_cfg.rng == RNG_BUFFER && !g_cfg.buffer) { fail("run: rng=buffer without chll_set_uniform_buffer"); return bail(); }
+    if (g_run_lat && (g_run_lat->G.ni != (uint32_t)*ni || g_run_lat->G.nj != (uint32_t)*nj ||
+                      g_run_lat->G.nk != (uint32_t)*nk || g_run_lat->bc != g_cfg.bc)) {
+        chll_lattice_destroy(g_run_lat);
+        g_run_lat = nullptr;
+    }
+    if (!g_run_lat && chll_lattice_create(*ni, *nj, *nk, g_cfg.bc, &g_run_lat)) return bail();
+    chll_lattice *L = g_run_lat;
+    *nsub = (int32_t)L->G.nsub;
+    if (chll_lattice_upload(L, nx, ny, nz, s)) return bail();
+    if (chll_lattice_set_step(L, *d)) return bail();
+    chll_params p{};
+    p.g = *g; p.KK = *KK; p.kbt = *kbt; p.rng = g_cfg.rng; p.arith = g_cfg.arith; p.adapt = 1;
+    if (g_cfg.rng == RNG_PHILOX) { p.seed = g_cfg.seed; p.run_index = g_cfg.run_index++; }
+    else { p.seed = g_cfg.stream_seed; p.stream_pos = g_cfg.stream_pos; }
+    p.buffer = g_cfg.buffer; p.buffer_len = g_cfg.buffer_len;
+    std::vector<double> rbuf((size_t)(rows > 0 ? rows : 1) * kRowCols, 0.0);
+    const int got = chll_lattice_sweeps(L, &p, *nstart, *nstop, *nout, rbuf.data(), rows > 0 ? rows : 0, nullptr);
+    if (got < 0) return bail();
+    if (g_cfg.rng != RNG_PHILOX) g_cfg.stream_pos = p.stream_pos;
+    if (chll_lattice_download(L, nx, ny, nz, s)) return bail();
+    if (chll_lattice_get_step(L, d)) return bail();
+    for (int r = 0; r < got; ++r) {
+        const double *row = &rbuf[(size_t)r * kRowCols];
+        msteps[r] = (int32_t)row[0];
+        for (int c = 0; c < 5; ++c) m[r + (size_t)c * rows] = row[1 + c];
+    }
+    g_cfg.last_rows.assign(rbuf.begin(), rbuf.begin() + (size_t)got * kRowCols);
+    if (g_cfg.verbose) {
+        // chll/chll_mod.f90:60 header and :98 rows, format (I8, 5(1X, F12.6))
+        if (*nstart == 1) printf("Steps       Energy      E. excess      Paccept      Faccept      D\n");
+        for (int r = 0; r < got; ++r) {
+            const double *row = &rbuf[(size_t)r * kRowCols];
+            printf("%8d", (int32_t)row[0]);
+            for (int c = 1; c <= 5; ++c) print_fixed(row[c]);
+            printf("\n");
+        }
+        fflush(stdout);
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,322 @@
+// chll_device.h -- device-side building blocks of the chiral Lebwohl-Lasher
+// sweep for gfx950: lattice geometry of the colour-split layout, counter-based
+// Philox4x32-10, the site-energy bond terms and the cone trial move.
+//
+// Arithmetic: this library is compiled with -ffp-contract=off.  The "compat"
+// functions evaluate every binary64 operation in the order chll/chll_mod.f90
+// writes it and round the running site energy to binary32 after every bond
+// (the reference's `real function energy`, chll/chll_mod.f90:363-412), so a
+// deterministic run reproduces the reference's accept/reject decisions.  The
+// "fp64" functions keep energies in binary64, use explicit fused multiply-adds
+// and evaluate energy differences directly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace chll {
+
+enum : int { ARITH_COMPAT = 0, ARITH_FP64 = 1 };
+enum : int { RNG_PHILOX = 0, RNG_STREAM = 1, RNG_BUFFER = 2 };
+enum : int { BC_OPEN = 0, BC_PERIODIC = 1 };
+
+// 2*pi as the reference forms it, 2 * (4 * atan(1d0)) (chll/chll_mod.f90:127-128).
+constexpr double kTwoPi = 6.283185307179586;
+
+// Unsigned 32-bit division by a run-time constant d >= 1 (Granlund &
+// Montgomery, "Division by invariant integers using multiplication", PLDI'94,
+// Fig. 4.1 with N = 32): l = ceil(log2 d), mul = floor(2^32 (2^l - d) / d) + 1,
+// q = (mulhi(mul, n) + n) >> l, the sum taken in 64 bits.  Exact for every
+// n < 2^32.
+struct FastDiv {
+    uint32_t d, mul, shift;
+};
+
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    FastDiv f;
+    f.d = d;
+    uint32_t l = 0;
+    while ((uint64_t(1) << l) < d) ++l;
+    f.shift = l;
+    f.mul = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1);
+    return f;
+}
+
+__host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    uint32_t t = __umulhi(n, f.mul);
+#else
+    uint32_t t = (uint32_t)(((uint64_t)n * f.mul) >> 32);
+#endif
+    return (uint32_t)(((uint64_t)t + n) >> f.shift);
+}
+
+// Colour-split structure-of-arrays layout.  Site (i,j,k) (0-based, Fortran
+// order l = i + ni*(j + nj*k)) has colour c = (i+j+k) & 1 -- colour 0 is the
+// reference's sl1 (1-based i+j+k odd), colour 1 is sl2 (chll/chll_mod.f90:23-36)
+// -- and packed index m = l >> 1 within its colour.  With ni even, m is also
+// the site's rank in its sublattice list, i.e. the index of its uniforms in
+// rand1 / rand2.  Each field stores colour 0 at [0, nsub) and colour 1 at
+// [nsub, 2 nsub).
+struct Geom {
+    uint32_t ni, nj, nk;
+    uint32_t half;   // ni / 2: packed sites per row and colour
+    uint32_t plane;  // ni * nj / 2: packed sites per plane and colour
+    uint32_t nsub;   // N / 2
+    FastDiv fd_half, fd_nj;
+};
+
+struct Site {
+    uint32_t i, j, k;
+};
+
+__device__ inline Site decode(const Geom &G, uint32_t m, int colour)
+{
+    uint32_t row = fdiv(m, G.fd_half);
+    uint32_t h = m - row * G.half;
+    uint32_t k = fdiv(row, G.fd_nj);
+    uint32_t j = row - k * G.nj;
+    uint32_t p = (j + k + (uint32_t)colour) & 1u;
+    return Site{2u * h + p, j, k};
+}
+
+// Packed indices (in the other colour) of the six neighbours in the order
+// +x, -x, +y, -y, +z, -z, and a presence mask (bit b set = bond b exists).
+// Open boundaries drop absent bonds (chll/chll_mod.f90:136-141, 371-410);
+// periodic boundaries wrap them.
+template <int BC>
+__device__ inline uint32_t neighbours(const Geom &G, uint32_t m, const Site &s, uint32_t nb[6])
+{
+    const uint32_t p = s.i & 1u;
+    uint32_t mask = 0;
+    // +x
+    if (s.i + 1 < G.ni) { nb[0] = m + p; mask |= 1u; }
+    else if (BC == BC_PERIODIC) { nb[0] = m + 1 - G.half; mask |= 1u; }
+    // -x
+    if (s.i >= 1) { nb[1] = m + p - 1; mask |= 2u; }
+    else if (BC == BC_PERIODIC) { nb[1] = m + G.half - 1; mask |= 2u; }
+    // +y
+    if (s.j + 1 < G.nj) { nb[2] = m + G.half; mask |= 4u; }
+    else if (BC == BC_PERIODIC) { nb[2] = m - (G.nj - 1) * G.half; mask |= 4u; }
+    // -y
+    if (s.j >= 1) { nb[3] = m - G.half; mask |= 8u; }
+    else if (BC == BC_PERIODIC) { nb[3] = m + (G.nj - 1) * G.half; mask |= 8u; }
+    // +z
+    if (s.k + 1 < G.nk) { nb[4] = m + G.plane; mask |= 16u; }
+    else if (BC == BC_PERIODIC) { nb[4] = m - (G.nk - 1) * G.plane; mask |= 16u; }
+    // -z
+    if (s.k >= 1) { nb[5] = m - G.plane; mask |= 32u; }
+    else if (BC == BC_PERIODIC) { nb[5] = m + (G.nk - 1) * G.plane; mask |= 32u; }
+    if (!(mask & 1u)) nb[0] = m;
+    if (!(mask & 2u)) nb[1] = m;
+    if (!(mask & 4u)) nb[2] = m;
+    if (!(mask & 8u)) nb[3] = m;
+    if (!(mask & 16u)) nb[4] = m;
+    if (!(mask & 32u)) nb[5] = m;
+    return mask;
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    }
+    return c;
+}
+
+// 53-bit uniform in [0,1) from two words (hi first).
+__device__ inline double u53(uint32_t hi, uint32_t lo)
+{
+    uint64_t x = ((uint64_t)hi << 32) | lo;
+    return (double)(x >> 11) * 0x1.0p-53;
+}
+
+// Deterministic-mode stream u_n: block n/2 with counter (q_lo, q_hi, 0,
+// 0x5354524D) and key (seed_lo, seed_hi); even n -> words (y, x), odd n ->
+// words (w, z).
+__device__ inline double stream_uniform(uint64_t seed, uint64_t n)
+{
+    uint64_t q = n >> 1;
+    U4 w = philox4x32_10(U4{(uint32_t)q, (uint32_t)(q >> 32), 0u, 0x5354524Du}, (uint32_t)seed,
+                         (uint32_t)(seed >> 32));
+    return (n & 1) ? u53(w.w, w.z) : u53(w.y, w.x);
+}
+
+// Production site-keyed draws.  Counter (m, istep, (run_index << 4) |
+// (colour << 1) | kind, replica), key (seed_lo, seed_hi).  kind 0 (director
+// trial): u_phi = words (y, x), u_acc = words (w, z).  kind 1 (chirality
+// trial): u_acc = words (y, x).
+__device__ inline U4 site_block(uint64_t seed, uint32_t run_index, uint32_t replica, uint32_t m,
+                                uint32_t istep, int colour, int kind)
+{
+    return philox4x32_10(U4{m, istep, (run_index << 4) | ((uint32_t)colour << 1) | (uint32_t)kind, replica},
+                         (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// ---------------------------------------------------------------------------
+__device__ inline double fl32(double x) { return (double)(float)x; }
+
+// Neighbour data gathered once per trial.
+struct Nbr {
+    double x[6], y[6], z[6];
+    int s[6];
+    uint32_t mask;
+};
+
+// Reference-compatible site energy E(n, snew) (chll/chll_mod.f90:363-412):
+// bonds in the order +x, -x, +y, -y, +z, -z; each bond
+//   e = fl32( (e + (1 - c*c)) -/+ ((K*c)*q)*f ),  c = (nx*mx + ny*my) + nz*mz,
+//   f = 0.5 * (snew + s_b);
+// then e = fl32(e - g*snew).  Absent bonds are skipped.
+__device__ inline double energy_compat(const Nbr &N, double nx, double ny, double nz, int snew, double g,
+                                       double KK)
+{
+    double e = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const double mx = N.x[b], my = N.y[b], mz = N.z[b];
+        const double dott = nx * mx + ny * my + nz * mz;
+        double cr;
+        if (b < 2) cr = ny * mz - nz * my;
+        else if (b < 4) cr = nz * mx - nx * mz;
+        else cr = nx * my - ny * mx;
+        const double sfac = 0.5 * (double)(snew + N.s[b]);
+        double r;
+        if ((b & 1) == 0) r = e + (1.0 - dott * dott) - KK * dott * cr * sfac;
+        else r = e + (1.0 - dott * dott) + KK * dott * cr * sfac;
+        r = fl32(r);
+        e = (N.mask >> b) & 1u ? r : e;
+    }
+    return fl32(e - g * (double)snew);
+}
+
+// Cone trial move (chll/chll_mod.f90:149-183), operation for operation.
+__device__ inline void trial_compat(double nnx, double nny, double nnz, double d, double u_phi, double &ox,
+                                    double &oy, double &oz)
+{
+    const double phi = u_phi * kTwoPi;
+    double sphi, cphi;
+    sincos(phi, &sphi, &cphi);
+    if (__builtin_expect(fabs(nnz) > 0.999, 0)) {
+        const double xx = nnx + d * cphi;
+        const double yy = nny + d * sphi;
+        const double zz = nnz;
+        const double rsq = sqrt(xx * xx + yy * yy + zz * zz);
+        ox = xx / rsq; oy = yy / rsq; oz = zz / rsq;
+    } else {
+        double ux = -nny, uy = nnx, uz = 0.0;
+        double rsq = sqrt(ux * ux + uy * uy);
+        ux = ux / rsq; uy = uy / rsq; uz = uz / rsq;
+        double vx = -nnz * nnx, vy = -nnz * nny, vz = nnx * nnx + nny * nny;
+        rsq = sqrt(vx * vx + vy * vy + vz * vz);
+        vx = vx / rsq; vy = vy / rsq; vz = vz / rsq;
+        const double dc = d * cphi, ds = d * sphi;
+        const double tx = nnx + dc * ux + ds * vx;
+        const double ty = nny + dc * uy + ds * vy;
+        const double tz = nnz + dc * uz + ds * vz;
+        rsq = sqrt(tx * tx + ty * ty + tz * tz);
+        ox = tx / rsq; oy = ty / rsq; oz = tz / rsq;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pure binary64 variants.  Per bond b with neighbour director m_b:
+//   c_b = n.m_b, q_b = (n x m_b)_axis(b), sgn_b = +1 for +x/+y/+z, -1 otherwise.
+// Director move n -> n' at fixed spin:
+//   dE = sum_b [ (c_b^2 - c'_b^2) - sgn_b K f_b (c'_b q'_b - c_b q_b) ].
+// Chirality flip sigma -> -sigma at fixed director:
+//   dE = sigma * (K * sum_b sgn_b c_b q_b + 2 g).
+__device__ inline double cq_sum(const Nbr &N, double nx, double ny, double nz, double &csq)
+{
+    double acc = 0.0, c2 = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const double mx = N.x[b], my = N.y[b], mz = N.z[b];
+        const double c = fma(nz, mz, fma(ny, my, nx * mx));
+        double q;
+        if (b < 2) q = fma(ny, mz, -nz * my);
+        else if (b < 4) q = fma(nz, mx, -nx * mz);
+        else q = fma(nx, my, -ny * mx);
+        const bool on = (N.mask >> b) & 1u;
+        const double cqv = on ? c * q : 0.0;
+        acc = (b & 1) == 0 ? acc + cqv : acc - cqv;
+        c2 = on ? fma(c, c, c2) : c2;
+    }
+    csq = c2;
+    return acc;
+}
+
+// Weighted sum for the director move: sum_b sgn_b f_b c_b q_b and sum_b c_b^2.
+__device__ inline double cqf_sum(const Nbr &N, int sigma, double nx, double ny, double nz, double &csq)
+{
+    double acc = 0.0, c2 = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const double mx = N.x[b], my = N.y[b], mz = N.z[b];
+        const double c = fma(nz, mz, fma(ny, my, nx * mx));
+        double q;
+        if (b < 2) q = fma(ny, mz, -nz * my);
+        else if (b < 4) q = fma(nz, mx, -nx * mz);
+        else q = fma(nx, my, -ny * mx);
+        const bool on = (N.mask >> b) & 1u;
+        const double f = 0.5 * (double)(sigma + N.s[b]);
+        const double v = on ? c * q * f : 0.0;
+        acc = (b & 1) == 0 ? acc + v : acc - v;
+        c2 = on ? fma(c, c, c2) : c2;
+    }
+    csq = c2;
+    return acc;
+}
+
+// Cone trial in binary64 with reciprocal square roots: the same geometry
+// (tangent basis u = z x n / |.|, v = n x u, displacement d on a cone of
+// half-angle atan d at azimuth 2 pi u_phi), without the compat operation order.
+__device__ inline void trial_fp64(double nnx, double nny, double nnz, double d, double u_phi, double &ox,
+                                  double &oy, double &oz)
+{
+    const double phi = u_phi * kTwoPi;
+    double sphi, cphi;
+    sincos(phi, &sphi, &cphi);
+    double tx, ty, tz;
+    if (__builtin_expect(fabs(nnz) > 0.999, 0)) {
+        tx = fma(d, cphi, nnx);
+        ty = fma(d, sphi, nny);
+        tz = nnz;
+    } else {
+        const double rho2 = fma(nnx, nnx, nny * nny);
+        const double ir = rsqrt(rho2);
+        const double ux = -nny * ir, uy = nnx * ir;
+        const double ivn = rsqrt(fma(nnz * nnz, rho2, rho2 * rho2));
+        const double vx = -nnz * nnx * ivn, vy = -nnz * nny * ivn, vz = rho2 * ivn;
+        const double dc = d * cphi, ds = d * sphi;
+        tx = fma(ds, vx, fma(dc, ux, nnx));
+        ty = fma(ds, vy, fma(dc, uy, nny));
+        tz = fma(ds, vz, nnz);
+    }
+    const double inv = rsqrt(fma(tz, tz, fma(ty, ty, tx * tx)));
+    ox = tx * inv; oy = ty * inv; oz = tz * inv;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8);
+// remap so each XCD walks one contiguous range of logical blocks and its L2
+// sees the neighbouring rows and planes of the tiles it updates.
+__device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb)
+{
+    const uint32_t per = nb >> 3, rem = nb & 7u;
+    const uint32_t x = b & 7u, q = b >> 3;
+    return x * per + (x < rem ? x : rem) + q;
+}
+
+}  // namespace chll

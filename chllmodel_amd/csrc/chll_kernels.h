@@ -1,0 +1,54 @@
+// chll_kernels.h -- argument blocks and launchers of the sweep kernels.
+#pragma once
+
+#include "chll_device.h"
+
+namespace chll {
+
+// Device fields, colour-split: colour 0 at [0, nsub), colour 1 at [nsub, 2 nsub).
+struct Fields {
+    double *nx, *ny, *nz;
+    int8_t *s;
+};
+
+// Per-launch arguments of a half-sweep.
+struct PhaseArgs {
+    double g, KK, kbt, inv_kbt;
+    const double *d;      // device cone step (updated by the finalize kernel)
+    const double *u_phi;  // deterministic mode: uniforms indexed by packed site
+    const double *u_acc;
+    uint64_t seed;
+    uint32_t run_index, replica, istep;
+};
+
+constexpr int kRowCols = 16;
+
+struct FinArgs {
+    const uint32_t *counts;  // 4 phases x nblk_phase per-block counts
+    uint32_t nblk_phase;
+    const double *part;      // nblk_meas x 8 measurement partials
+    const long long *spart;  // nblk_meas spin partials
+    uint32_t nblk_meas;
+    uint64_t n3;
+    double *d;               // device cone step, in/out
+    double *rows;            // rows x kRowCols
+    long long *counts_out;   // optional per-sweep (naccept, nflip)
+    int32_t istep, measure, row, adapt;
+    int64_t sweep_index;
+};
+
+uint32_t phase_blocks(const Geom &G);
+
+void launch_scatter_f64(const Geom &G, const double *src, double *dst, hipStream_t s);
+void launch_gather_f64(const Geom &G, const double *src, double *dst, hipStream_t s);
+void launch_scatter_spin(const Geom &G, const int32_t *src, int8_t *dst, int *bad, hipStream_t s);
+void launch_gather_spin(const Geom &G, const int8_t *src, int32_t *dst, hipStream_t s);
+// kind 0 = director, 1 = chirality
+void launch_phase(int kind, int arith, int rng, int bc, const Geom &G, const Fields &F, int colour,
+                  const PhaseArgs &A, uint32_t *counts, hipStream_t s);
+void launch_measure(int bc, const Geom &G, const Fields &F, double g, double KK, double *part, long long *spart,
+                    hipStream_t s);
+void launch_finalize(const FinArgs &A, hipStream_t s);
+void launch_stream_fill(double *out, uint64_t seed, uint64_t base, uint32_t nsub, hipStream_t s);
+
+}  // namespace chll

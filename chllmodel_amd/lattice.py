@@ -1,0 +1,142 @@
+"""Device-resident lattice: the state stays in HBM across sweep calls.
+
+This is the Python face of the ``chll_lattice_*`` extension of
+libchll_mod.so.  The reference keeps no device state between ``run`` calls
+(chll/chll_mod.f90:61, 111); ``Lattice`` is what a caller uses to sweep, measure
+and time without a host round trip per call.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from ._lib import ARITH, BC, RNG, ROW_COLS, ROW_NAMES, TIMING_KINDS, ChllParams, check, lib
+
+_f64p = ct.POINTER(ct.c_double)
+_i32p = ct.POINTER(ct.c_int32)
+_i64p = ct.POINTER(ct.c_int64)
+
+
+def _fortran(a, dtype):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+class Lattice:
+    """An (ni, nj, nk) chiral Lebwohl-Lasher lattice in GPU memory."""
+
+    def __init__(self, ni: int, nj: int, nk: int, bc: str = "open"):
+        self.shape = (int(ni), int(nj), int(nk))
+        self.bc = bc
+        self.N = ni * nj * nk
+        h = ct.c_void_p()
+        check(lib().chll_lattice_create(ni, nj, nk, BC[bc], ct.byref(h)))
+        self._h = h
+        self._buf_keep = None
+
+    # -- state ---------------------------------------------------------------
+    def upload(self, nx, ny, nz, s) -> None:
+        nx, ny, nz = (_fortran(a, np.float64) for a in (nx, ny, nz))
+        s = _fortran(s, np.int32)
+        for a in (nx, ny, nz, s):
+            if a.shape != self.shape:
+                raise ValueError(f"field shape {a.shape} != lattice {self.shape}")
+        check(lib().chll_lattice_upload(self._h, nx.ctypes.data_as(_f64p), ny.ctypes.data_as(_f64p),
+                                        nz.ctypes.data_as(_f64p), s.ctypes.data_as(_i32p)))
+
+    def upload_state(self, st: dict) -> None:
+        self.upload(st["nx"], st["ny"], st["nz"], st["s"])
+
+    def download(self) -> dict:
+        out = {k: np.zeros(self.shape, dtype=np.float64, order="F") for k in ("nx", "ny", "nz")}
+        out["s"] = np.zeros(self.shape, dtype=np.int32, order="F")
+        check(lib().chll_lattice_download(self._h, out["nx"].ctypes.data_as(_f64p), out["ny"].ctypes.data_as(_f64p),
+                                          out["nz"].ctypes.data_as(_f64p), out["s"].ctypes.data_as(_i32p)))
+        return out
+
+    def set_step(self, d: float) -> None:
+        check(lib().chll_lattice_set_step(self._h, float(d)))
+
+    def get_step(self) -> float:
+        v = ct.c_double()
+        check(lib().chll_lattice_get_step(self._h, ct.byref(v)))
+        return v.value
+
+    def set_stream(self, hip_stream_ptr: int | None) -> None:
+        """Launch on a caller-owned HIP stream (e.g. torch.cuda.Stream().cuda_stream)."""
+        check(lib().chll_lattice_set_stream(self._h, ct.c_void_p(hip_stream_ptr or 0)))
+
+    def sync(self) -> None:
+        check(lib().chll_lattice_sync(self._h))
+
+    # -- sweeps --------------------------------------------------------------
+    def params(self, *, g, KK, kbt, rng="philox", arith="compat", seed=0, stream_pos=0, buffer=None,
+               run_index=0, replica=0, adapt=True) -> ChllParams:
+        p = ChllParams()
+        p.g, p.KK, p.kbt = float(g), float(KK), float(kbt)
+        p.rng, p.arith = RNG[rng], ARITH[arith]
+        p.seed, p.stream_pos = int(seed), int(stream_pos)
+        if buffer is not None:
+            self._buf_keep = np.ascontiguousarray(buffer, dtype=np.float64)
+            p.buffer = self._buf_keep.ctypes.data_as(_f64p)
+            p.buffer_len = self._buf_keep.size
+        p.run_index, p.replica = int(run_index), int(replica)
+        p.adapt = 1 if adapt else 0
+        return p
+
+    def sweeps(self, nstart: int, nstop: int, nout: int = 0, *, params: ChllParams | None = None,
+               want_rows: bool = True, want_counts: bool = False, **kw) -> dict:
+        """Run sweeps istep = nstart..nstop.  Returns {'rows': (R,16) array,
+        'counts': (nsweeps,2) array or None, 'stream_pos': int}."""
+        p = params if params is not None else self.params(**kw)
+        nsw = max(nstop - nstart + 1, 0)
+        max_rows = 0
+        if nout > 0 and nsw > 0:
+            max_rows = nstop // nout - (nstart - 1) // nout
+        rows = np.zeros((max(max_rows, 1), ROW_COLS), dtype=np.float64) if want_rows else None
+        counts = np.zeros((max(nsw, 1), 2), dtype=np.int64) if want_counts else None
+        got = check(lib().chll_lattice_sweeps(
+            self._h, ct.byref(p), int(nstart), int(nstop), int(nout),
+            rows.ctypes.data_as(_f64p) if rows is not None else None, max_rows,
+            counts.ctypes.data_as(_i64p) if counts is not None else None))
+        return {"rows": rows[:got] if rows is not None else None,
+                "counts": counts[:nsw] if counts is not None else None,
+                "stream_pos": p.stream_pos}
+
+    def measure(self, g: float, KK: float) -> dict:
+        row = np.zeros(ROW_COLS, dtype=np.float64)
+        check(lib().chll_lattice_measure(self._h, float(g), float(KK), row.ctypes.data_as(_f64p)))
+        return dict(zip(ROW_NAMES, row.tolist()))
+
+    # -- timing --------------------------------------------------------------
+    def set_timing(self, on: bool) -> None:
+        check(lib().chll_lattice_set_timing(self._h, 1 if on else 0))
+
+    def timing(self) -> dict:
+        ms = np.zeros(len(TIMING_KINDS), dtype=np.float64)
+        n = np.zeros(len(TIMING_KINDS), dtype=np.int64)
+        check(lib().chll_lattice_timing(self._h, ms.ctypes.data_as(_f64p), n.ctypes.data_as(_i64p), len(TIMING_KINDS)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(TIMING_KINDS)}
+
+    # -- lifetime ------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().chll_lattice_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
+
+def rows_to_dict(rows: np.ndarray) -> dict:
+    return {name: rows[:, i].copy() for i, name in enumerate(ROW_NAMES)}

@@ -1,0 +1,140 @@
+/*
+ * chll_mod.h -- C ABI of libchll_mod.so, the MI355X (gfx950) implementation
+ * of the chiral Lebwohl-Lasher Metropolis sweep of mdeutsc2/ChLLModel.
+ *
+ * The two drop-in entry points keep the reference's symbol names, argument
+ * order, by-reference scalars and Fortran-ordered array layouts, so the
+ * reference's Python binding (chll/chll.py:6-36) loads this library unchanged.
+ * Everything prefixed chll_ is an extension with no counterpart in the
+ * reference.  Extension functions return 0 on success and -1 on error, with
+ * the message available from chll_last_error().
+ */
+#ifndef CHLL_MOD_H
+#define CHLL_MOD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in entry points                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces `subroutine init(nx,ny,nz,s,dope,sl1,sl2,ni,nj,nk,nsub)
+ * bind(c, name="init")`, chll/chll_mod.f90:9-38 (Python argtypes
+ * chll/chll.py:9-18).  Fills sl1[0..nsub) / sl2[0..nsub) with the 1-based
+ * Fortran linear indices of the sites whose 1-based i+j+k is odd / even, in
+ * Fortran order.  The other arrays are not touched.  Requires N = ni*nj*nk
+ * even and nsub = N/2 (the reference overflows sl1 otherwise); on a bad size
+ * nothing is written and chll_last_error() says why. */
+void init(double *nx, double *ny, double *nz, int32_t *s, int32_t *dope, int32_t *sl1, int32_t *sl2,
+          const int32_t *ni, const int32_t *nj, const int32_t *nk, const int32_t *nsub);
+
+/* Replaces `subroutine run(nstart,nstop,nout,nx,ny,nz,s,dope,sl1,sl2,g,KK,d,
+ * kbt,ni,nj,nk,nsub,msteps,m) bind(c, name="run")`, chll/chll_mod.f90:40-115
+ * (Python argtypes chll/chll.py:20-36).  Runs sweeps istep = nstart..nstop on
+ * the GPU: director(sl1), director(sl2), chirality(sl1), chirality(sl2) per
+ * sweep; adapts d while istep < 10000; every nout sweeps records
+ * msteps[r] = istep and the column-major row m[r + c*rows] =
+ * (E/site, eta, paccept, faccept, d), rows = (1+nstop-nstart)/nout, and prints
+ * the reference's table to stdout (silenced by chll_configure(..., verbose=0)
+ * or CHLL_QUIET=1).  nx, ny, nz, s are updated in place, *d receives the
+ * adapted step and *nsub = N/2.  Arrays are Fortran-ordered (ni,nj,nk);
+ * ni must be even.  No device state survives the call.  Boundary condition,
+ * arithmetic and random source are those set by chll_configure (default:
+ * open boundaries, reference-compatible arithmetic, Philox). */
+void run(const int32_t *nstart, const int32_t *nstop, const int32_t *nout,
+         double *nx, double *ny, double *nz, int32_t *s, int32_t *dope, int32_t *sl1, int32_t *sl2,
+         double *g, double *KK, double *d, double *kbt,
+         const int32_t *ni, const int32_t *nj, const int32_t *nk, int32_t *nsub,
+         int32_t *msteps, double *m);
+
+/* ------------------------------------------------------------------------ */
+/* Extensions                                                               */
+/* ------------------------------------------------------------------------ */
+
+#define CHLL_RNG_PHILOX 0 /* site-keyed Philox4x32-10 counters on the device */
+#define CHLL_RNG_STREAM 1 /* documented uniform stream u_n in the reference's draw schedule */
+#define CHLL_RNG_BUFFER 2 /* caller's host array of uniforms in the reference's draw schedule */
+
+#define CHLL_ARITH_COMPAT 0 /* reference arithmetic: binary32-rounded site energies */
+#define CHLL_ARITH_FP64 1   /* binary64 energies, fused multiply-adds */
+
+#define CHLL_BC_OPEN 0     /* free boundaries, as the reference's sweep */
+#define CHLL_BC_PERIODIC 1 /* periodic in i, j, k (ni, nj, nk even) */
+
+/* Measurement row columns (chll_lattice_sweeps, chll_last_rows). */
+#define CHLL_ROW_COLS 16
+/* 0 istep | 1 E: reference etot | 2 eta | 3 paccept | 4 faccept | 5 d |
+ * 6 E_true/N: Hamiltonian of the sweep under its boundary condition |
+ * 7 exact mean spin | 8..13 <nx nx>, <ny ny>, <nz nz>, <nx ny>, <nx nz>, <ny nz> |
+ * 14 naccept | 15 nflip */
+
+/* Message of the last failed call (empty string when none). */
+const char *chll_last_error(void);
+
+/* Options of subsequent run() calls.  verbose: 1 prints the reference's table. */
+int chll_configure(int32_t rng, int32_t arith, int32_t bc, int32_t verbose);
+
+/* Philox key and run counter for run(); run_index advances by one per run()
+ * call so consecutive calls draw fresh numbers. */
+int chll_set_seed(uint64_t seed, uint32_t run_index);
+
+/* CHLL_RNG_STREAM: stream seed and next draw index (advances 10*nsub per sweep). */
+int chll_set_stream(uint64_t seed, uint64_t pos);
+uint64_t chll_get_stream_pos(void);
+
+/* CHLL_RNG_BUFFER: host array of n uniforms, next draw index pos.  The array
+ * must stay valid while run() consumes it. */
+int chll_set_uniform_buffer(const double *u, int64_t n, uint64_t pos);
+
+/* Copy up to `rows` full measurement rows (row-major, CHLL_ROW_COLS each) of
+ * the last run() call; returns the number copied or -1. */
+int32_t chll_last_rows(double *out, int32_t rows);
+
+/* Device-resident lattice -------------------------------------------------- */
+
+typedef struct chll_params {
+    double g, KK, kbt;
+    int32_t rng, arith;
+    uint64_t seed;       /* Philox key, or the stream seed */
+    uint64_t stream_pos; /* STREAM / BUFFER: next draw index, advanced on return */
+    const double *buffer;/* BUFFER: host uniforms */
+    int64_t buffer_len;
+    uint32_t run_index, replica;
+    int32_t adapt;       /* 1: reference step adaptation while istep < 10000 */
+    int32_t pad;
+} chll_params;
+
+typedef struct chll_lattice chll_lattice;
+
+int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lattice **out);
+int chll_lattice_destroy(chll_lattice *lat);
+/* Launch on the caller's HIP stream (NULL restores the lattice's own stream). */
+int chll_lattice_set_stream(chll_lattice *lat, void *hip_stream);
+/* Host Fortran-ordered fields <-> device colour-split layout. */
+int chll_lattice_upload(chll_lattice *lat, const double *nx, const double *ny, const double *nz, const int32_t *s);
+int chll_lattice_download(chll_lattice *lat, double *nx, double *ny, double *nz, int32_t *s);
+int chll_lattice_set_step(chll_lattice *lat, double d);
+int chll_lattice_get_step(chll_lattice *lat, double *d);
+/* Sweeps istep = nstart..nstop.  A row is measured when nout > 0 and
+ * istep % nout == 0, up to max_rows rows.  When rows or counts (2 per sweep:
+ * naccept, nflip) is non-NULL the call waits and copies them; otherwise it
+ * only enqueues work. */
+int chll_lattice_sweeps(chll_lattice *lat, chll_params *p, int32_t nstart, int32_t nstop, int32_t nout,
+                        double *rows, int32_t max_rows, int64_t *counts);
+/* Measure one row of the current state without sweeping (row-major, CHLL_ROW_COLS). */
+int chll_lattice_measure(chll_lattice *lat, double g, double KK, double *row);
+int chll_lattice_sync(chll_lattice *lat);
+/* Per-kernel HIP-event timing: kinds 0 director, 1 chirality, 2 measure,
+ * 3 finalize, 4 uniform fill. */
+int chll_lattice_set_timing(chll_lattice *lat, int32_t on);
+int chll_lattice_timing(chll_lattice *lat, double *ms, int64_t *launches, int32_t nkinds);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHLL_MOD_H */

@@ -1,0 +1,225 @@
+"""Parity of the HIP sweep (libchll_mod.so on the GPU) with the reference.
+
+Bar (deterministic modes): spins and accept/flip counts bit-exact, directors
+within 1e-12, msteps / eta / paccept / faccept / d bit-exact, the reported
+energy within one binary32 ulp (the reference sums it serially in binary64,
+the GPU in a tree, before both round to binary32).
+
+* golden vectors written by the reference's compiled source (tests/golden);
+* the oracle on seeded inputs, for every mode: the reference's draw schedule
+  fed by the documented stream or an explicit host buffer, the production
+  site-keyed Philox draws, reference-compatible and FP64 arithmetic, open and
+  periodic boundaries;
+* the drop-in ``run`` entry point called with the reference's ctypes signature.
+"""
+import ctypes as ct
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from chllmodel_amd import _lib
+from chllmodel_amd.lattice import Lattice
+from oracle.oracle import (ARITH_COMPAT, ARITH_FP64, BC_OPEN, BC_PERIODIC, RNG_BUFFER, RNG_PHILOX, RNG_STREAM,
+                           Oracle, copy_state, isotropic_state)
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "ref_*.npz")))
+DIR_TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+def f32_ulp(x):
+    return float(np.spacing(np.float32(abs(x))))
+
+
+def assert_rows_match(rows_gpu, msteps, m):
+    """rows_gpu: (R,16) GPU rows; msteps, m: reference layout (R,), (R,5)."""
+    assert rows_gpu.shape[0] == len(msteps)
+    assert np.array_equal(rows_gpu[:, 0].astype(np.int32), msteps)
+    for r in range(len(msteps)):
+        assert abs(rows_gpu[r, 1] - m[r, 0]) <= f32_ulp(m[r, 0]), (r, rows_gpu[r, 1], m[r, 0])
+    assert np.array_equal(rows_gpu[:, 2:6], m[:, 1:5])
+
+
+def assert_state_match(a, b, tol=DIR_TOL):
+    assert np.array_equal(a["s"], b["s"])
+    for f in ("nx", "ny", "nz"):
+        err = np.abs(a[f] - b[f]).max()
+        assert err <= tol, (f, err)
+
+
+def gpu_run(st0, bc, nstart, nstop, nout, d, want_counts=True, **kw):
+    with Lattice(*st0["nx"].shape, bc=bc) as lat:
+        lat.upload_state(st0)
+        lat.set_step(d)
+        out = lat.sweeps(nstart, nstop, nout, want_counts=want_counts, **kw)
+        out["state"] = lat.download()
+        out["d"] = lat.get_step()
+    return out
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_golden_vectors(path):
+    z = np.load(path, allow_pickle=False)
+    st0 = {k: np.asfortranarray(z[k + "0"]) for k in ("nx", "ny", "nz", "s")}
+    p = z["params"]
+    nstart, nstop, nout = int(p[0]), int(p[1]), int(p[2])
+    kw = dict(g=p[3], KK=p[4], kbt=p[6], arith="compat", seed=int(p[7]), stream_pos=0)
+    if str(z["mode"]) == "buffer":
+        kw.update(rng="buffer", buffer=z["buffer"])
+    else:
+        kw.update(rng="stream")
+    print(f"golden {os.path.basename(path)} dims={st0['nx'].shape}", flush=True)
+    out = gpu_run(st0, "open", nstart, nstop, nout, float(p[5]), want_counts=False, **kw)
+    assert_rows_match(out["rows"], z["msteps"], z["m"])
+    assert out["d"] == float(z["d_out"])
+    assert out["stream_pos"] == int(z["stream_pos_out"])
+    ref = {k: z[k] for k in ("nx", "ny", "nz", "s")}
+    assert_state_match(out["state"], ref)
+
+
+CASES = [
+    # dims, bc, rng, arith, nsweeps, nout
+    ((4, 4, 4), "open", "stream", "compat", 12, 3),
+    ((6, 4, 5), "open", "stream", "compat", 8, 2),
+    ((32, 16, 8), "open", "stream", "compat", 6, 3),
+    ((16, 16, 16), "periodic", "stream", "compat", 6, 2),
+    ((8, 6, 4), "periodic", "philox", "compat", 10, 5),
+    ((32, 32, 32), "open", "philox", "compat", 6, 3),
+    ((24, 20, 18), "periodic", "philox", "compat", 5, 5),
+    ((16, 16, 16), "open", "philox", "fp64", 6, 3),
+    ((12, 8, 10), "periodic", "philox", "fp64", 6, 2),
+    ((16, 8, 8), "open", "stream", "fp64", 5, 5),
+    ((64, 64, 64), "open", "philox", "compat", 3, 3),
+]
+
+
+@pytest.mark.parametrize("dims,bc,rng,arith,nsw,nout", CASES,
+                         ids=[f"{'x'.join(map(str, c[0]))}-{c[1]}-{c[2]}-{c[3]}" for c in CASES])
+def test_matches_oracle(orc, dims, bc, rng, arith, nsw, nout):
+    st0 = isotropic_state(*dims, 1000 + sum(dims))
+    g, K, d, kbt = 0.45, 1.3, 0.3, 0.6
+    seed, run_index, replica = 0x1234ABCD5678, 3, 0
+    print(f"oracle parity dims={dims} bc={bc} rng={rng} arith={arith}", flush=True)
+    out = gpu_run(st0, bc, 1, nsw, nout, d, g=g, KK=K, kbt=kbt, rng=rng, arith=arith, seed=seed,
+                  run_index=run_index)
+    st = copy_state(st0)
+    o_rng = {"stream": RNG_STREAM, "philox": RNG_PHILOX}[rng]
+    ms, m, d_out, pos, counts = orc.run(st, 1, nsw, nout, g, K, d, kbt, rng=o_rng, seed=seed, run_index=run_index,
+                                        replica=replica, bc={"open": BC_OPEN, "periodic": BC_PERIODIC}[bc],
+                                        arith={"compat": ARITH_COMPAT, "fp64": ARITH_FP64}[arith],
+                                        want_counts=True)
+    assert np.array_equal(out["counts"], counts)
+    assert_state_match(out["state"], st)
+    assert_rows_match(out["rows"], ms, m)
+    assert out["d"] == d_out
+    if rng == "stream":
+        assert out["stream_pos"] == pos
+
+
+def test_measure_row(orc):
+    dims = (12, 10, 8)
+    st = isotropic_state(*dims, 77)
+    g, K = 0.35, 1.7
+    for bc, obc in (("open", BC_OPEN), ("periodic", BC_PERIODIC)):
+        with Lattice(*dims, bc=bc) as lat:
+            lat.upload_state(st)
+            row = lat.measure(g, K)
+        N = np.prod(dims)
+        assert abs(row["E"] - orc.etot(st, g, K)) <= f32_ulp(row["E"])
+        assert abs(row["E_true"] - orc.energy_true(st, g, K, bc=obc) / N) < 1e-13
+        assert row["eta_exact"] == st["s"].sum() / N
+        Q = {"Qxx": st["nx"] ** 2, "Qyy": st["ny"] ** 2, "Qzz": st["nz"] ** 2, "Qxy": st["nx"] * st["ny"],
+             "Qxz": st["nx"] * st["nz"], "Qyz": st["ny"] * st["nz"]}
+        for k, v in Q.items():
+            assert abs(row[k] - v.mean()) < 1e-14
+
+
+def test_upload_download_roundtrip():
+    dims = (10, 6, 7)
+    st = isotropic_state(*dims, 5)
+    st["s"][0, 0, 0] = 5
+    st["s"][9, 5, 6] = -100
+    with Lattice(*dims) as lat:
+        lat.upload_state(st)
+        back = lat.download()
+    for f in ("nx", "ny", "nz", "s"):
+        assert np.array_equal(back[f], st[f])
+
+
+def test_upload_rejects_wide_spins():
+    dims = (4, 4, 4)
+    st = isotropic_state(*dims, 5)
+    st["s"][1, 2, 3] = 300
+    with Lattice(*dims) as lat:
+        with pytest.raises(_lib.ChllError, match="spin values"):
+            lat.upload_state(st)
+
+
+def _dropin_run(st, nstart, nstop, nout, g, KK, d, kbt):
+    ni, nj, nk = st["nx"].shape
+    nsub = ni * nj * nk // 2
+    rows = (1 + nstop - nstart) // nout
+    msteps = np.zeros(rows, dtype=np.int32)
+    m = np.zeros((rows, 5), dtype=np.float64, order="F")
+    sl1 = np.zeros(nsub, np.int32)
+    sl2 = np.zeros(nsub, np.int32)
+    dd = ct.c_double(d)
+    ns = ct.c_int(0)
+    _lib.lib().run(ct.c_int(nstart), ct.c_int(nstop), ct.c_int(nout), st["nx"], st["ny"], st["nz"], st["s"],
+                   st["dope"], sl1, sl2, ct.c_double(g), ct.c_double(KK), dd, ct.c_double(kbt),
+                   ct.c_int(ni), ct.c_int(nj), ct.c_int(nk), ns, msteps, m)
+    assert _lib.last_error() == ""
+    return msteps, m, dd.value, ns.value
+
+
+def test_dropin_run_stream_mode(orc):
+    """The reference's own calling convention (chll/chll.py:160-173) in the
+    deterministic stream mode, twice in a row: the second call continues the
+    stream exactly as the reference build continues its random_number draws."""
+    L = _lib.lib()
+    dims = (8, 6, 10)
+    st0 = isotropic_state(*dims, 404)
+    a, b = copy_state(st0), copy_state(st0)
+    _lib.check(L.chll_configure(1, 0, 0, 0))
+    _lib.check(L.chll_set_stream(55, 0))
+    try:
+        for call in range(2):
+            ms, m, d_out, ns = _dropin_run(a, 1, 12, 4, 0.5, 1.0, 0.3, 0.7)
+            pos0 = 0 if call == 0 else pos
+            oms, om, od, pos = orc.run(b, 1, 12, 4, 0.5, 1.0, 0.3, 0.7, rng=RNG_STREAM, seed=55, stream_pos=pos0)
+            assert ns == np.prod(dims) // 2
+            assert np.array_equal(ms, oms) and d_out == od
+            for r in range(len(ms)):
+                assert abs(m[r, 0] - om[r, 0]) <= f32_ulp(om[r, 0])
+            assert np.array_equal(m[:, 1:], om[:, 1:])
+            assert_state_match(a, b)
+            assert L.chll_get_stream_pos() == pos
+    finally:
+        _lib.check(L.chll_configure(0, 0, 0, 1))
+
+
+def test_dropin_run_philox_run_counter(orc):
+    """Philox mode: consecutive run() calls use run_index 0, 1, ... after
+    chll_set_seed, so call n equals the oracle with run_index n."""
+    L = _lib.lib()
+    dims = (6, 6, 6)
+    st0 = isotropic_state(*dims, 405)
+    a, b = copy_state(st0), copy_state(st0)
+    _lib.check(L.chll_configure(0, 0, 0, 0))
+    _lib.check(L.chll_set_seed(999, 0))
+    try:
+        for call in range(3):
+            ms, m, d_out, _ = _dropin_run(a, 1, 6, 3, 0.2, 0.8, 0.25, 0.9)
+            oms, om, od, _ = orc.run(b, 1, 6, 3, 0.2, 0.8, 0.25, 0.9, rng=RNG_PHILOX, seed=999, run_index=call)
+            assert d_out == od and np.array_equal(m[:, 1:], om[:, 1:])
+            assert_state_match(a, b)
+    finally:
+        _lib.check(L.chll_configure(0, 0, 0, 1))
