@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="run one plain elementwise copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -123,6 +125,15 @@ def main():
     lat.sync()
     torch.cuda.synchronize()
     log("bench: warmup done")
+    if args.copy_calib_mib > 0:
+        n = args.copy_calib_mib * (1 << 20) // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 1.0, out=dst)  # reads and writes copy_calib_mib MiB once
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"bench: calibration copy of {args.copy_calib_mib} MiB done")
     if not args.no_timing:
         lat.set_timing(True)
     ev0 = torch.cuda.Event(enable_timing=True)

@@ -79,7 +79,7 @@ struct chll_lattice {
     int32_t *stage_s = nullptr;
     int *bad = nullptr;
     uint32_t *counts = nullptr;
-    uint32_t nblk = 0;
+    uint32_t nblk = 0, nblk_meas = 0;
     double *part = nullptr;
     long long *spart = nullptr;
     double *rows_dev = nullptr;
@@ -166,9 +166,11 @@ int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lat
     rc |= dalloc(&L->F.nz, n3);
     rc |= dalloc(&L->F.s, n3);
     rc |= dalloc(&L->bad, 1);
-    rc |= dalloc(&L->counts, 4 * (size_t)L->nblk);
-    rc |= dalloc(&L->part, 8 * (size_t)L->nblk);
-    rc |= dalloc(&L->spart, (size_t)L->nblk);
+    L->nblk_meas = measure_blocks(G);
+    rc |= dalloc(&L->counts, (size_t)kCountWords);
+    rc |= dalloc(&L->part, 8 * (size_t)L->nblk_meas);
+    rc |= dalloc(&L->spart, (size_t)L->nblk_meas);
+    if (!rc && hipMemset(L->counts, 0, kCountWords * sizeof(uint32_t)) != hipSuccess) rc = fail("hipMemset failed");
     rc |= dalloc(&L->d_dev, 1);
     if (!rc && hipHostMalloc((void **)&L->pinned, 8 * sizeof(double)) != hipSuccess) rc = fail("hipHostMalloc failed");
     if (!rc && hipStreamCreate(&L->own) != hipSuccess) rc = fail("hipStreamCreate failed");
@@ -352,8 +354,8 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
     A.g = p->g; A.KK = p->KK; A.kbt = p->kbt; A.inv_kbt = 1.0 / p->kbt;
     A.d = L->d_dev; A.seed = p->seed; A.run_index = p->run_index; A.replica = p->replica;
     FinArgs FA{};
-    FA.counts = L->counts; FA.nblk_phase = L->nblk; FA.part = L->part; FA.spart = L->spart;
-    FA.nblk_meas = L->nblk; FA.n3 = 2 * ns; FA.d = L->d_dev; FA.rows = L->rows_dev;
+    FA.counts = L->counts; FA.part = L->part; FA.spart = L->spart;
+    FA.nblk_meas = L->nblk_meas; FA.n3 = 2 * ns; FA.d = L->d_dev; FA.rows = L->rows_dev;
     FA.counts_out = counts ? L->counts_dev : nullptr; FA.adapt = p->adapt;
     int32_t row = 0;
     size_t ti = 0;
@@ -380,7 +382,7 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
             }
             if (tbegin(L, kind == 0 ? KIND_DIR : KIND_SPIN, &ti)) return -1;
             launch_phase(kind, p->arith, p->rng == RNG_PHILOX ? RNG_PHILOX : RNG_STREAM, L->bc, G, L->F, colour, A,
-                         L->counts + (size_t)ph * L->nblk, st);
+                         L->counts + (size_t)ph * kSlots * kSlotStride, st);
             if (launch_check() || tend(L, ti)) return -1;
         }
         const bool meas = nout > 0 && (istep % nout) == 0 && row < max_rows;
@@ -408,13 +410,12 @@ int chll_lattice_measure(chll_lattice *L, double g, double KK, double *row)
     if (!L || !row) return fail("chll_lattice_measure: null argument");
     if (ensure_rows(L, 1)) return -1;
     hipStream_t st = L->stream;
-    // counters of the last sweep are reused as they stand; only the row matters here
-    HIPCHECK(hipMemsetAsync(L->counts, 0, 4 * (size_t)L->nblk * sizeof(uint32_t), st));
+    // the accept/flip slots are zero between sweeps, so the row's counts read 0
     launch_measure(L->bc, L->G, L->F, g, KK, L->part, L->spart, st);
     if (launch_check()) return -1;
     FinArgs FA{};
-    FA.counts = L->counts; FA.nblk_phase = L->nblk; FA.part = L->part; FA.spart = L->spart;
-    FA.nblk_meas = L->nblk; FA.n3 = 2 * (uint64_t)L->G.nsub; FA.d = L->d_dev; FA.rows = L->rows_dev;
+    FA.counts = L->counts; FA.part = L->part; FA.spart = L->spart;
+    FA.nblk_meas = L->nblk_meas; FA.n3 = 2 * (uint64_t)L->G.nsub; FA.d = L->d_dev; FA.rows = L->rows_dev;
     FA.istep = 0; FA.measure = 1; FA.row = 0; FA.adapt = 0;
     launch_finalize(FA, st);
     if (launch_check()) return -1;

@@ -32,7 +32,7 @@ struct FastDiv {
     uint32_t d, mul, shift;
 };
 
-inline FastDiv make_fastdiv(uint32_t d)
+__host__ __device__ inline FastDiv make_fastdiv(uint32_t d)
 {
     FastDiv f;
     f.d = d;
@@ -124,20 +124,23 @@ struct U4 {
     uint32_t x, y, z, w;
 };
 
-__device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
+// Each round's two 32x32->64 products are written as 64-bit multiplies so
+// they lower to one v_mad_u64_u32 each instead of a v_mul_hi_u32 /
+// v_mul_lo_u32 pair.
+__host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
-        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        c = U4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0};
     }
     return c;
 }
 
 // 53-bit uniform in [0,1) from two words (hi first).
-__device__ inline double u53(uint32_t hi, uint32_t lo)
+__host__ __device__ inline double u53(uint32_t hi, uint32_t lo)
 {
     uint64_t x = ((uint64_t)hi << 32) | lo;
     return (double)(x >> 11) * 0x1.0p-53;
@@ -146,7 +149,7 @@ __device__ inline double u53(uint32_t hi, uint32_t lo)
 // Deterministic-mode stream u_n: block n/2 with counter (q_lo, q_hi, 0,
 // 0x5354524D) and key (seed_lo, seed_hi); even n -> words (y, x), odd n ->
 // words (w, z).
-__device__ inline double stream_uniform(uint64_t seed, uint64_t n)
+__host__ __device__ inline double stream_uniform(uint64_t seed, uint64_t n)
 {
     uint64_t q = n >> 1;
     U4 w = philox4x32_10(U4{(uint32_t)q, (uint32_t)(q >> 32), 0u, 0x5354524Du}, (uint32_t)seed,
@@ -158,7 +161,7 @@ __device__ inline double stream_uniform(uint64_t seed, uint64_t n)
 // (colour << 1) | kind, replica), key (seed_lo, seed_hi).  kind 0 (director
 // trial): u_phi = words (y, x), u_acc = words (w, z).  kind 1 (chirality
 // trial): u_acc = words (y, x).
-__device__ inline U4 site_block(uint64_t seed, uint32_t run_index, uint32_t replica, uint32_t m,
+__host__ __device__ inline U4 site_block(uint64_t seed, uint32_t run_index, uint32_t replica, uint32_t m,
                                 uint32_t istep, int colour, int kind)
 {
     return philox4x32_10(U4{m, istep, (run_index << 4) | ((uint32_t)colour << 1) | (uint32_t)kind, replica},
@@ -167,6 +170,17 @@ __device__ inline U4 site_block(uint64_t seed, uint32_t run_index, uint32_t repl
 
 // ---------------------------------------------------------------------------
 __device__ inline double fl32(double x) { return (double)(float)x; }
+
+// x / b from y = RN(1/b): q = RN(x y), r = x - b q (exact with an FMA),
+// q' = RN(q + r y) = RN(x / b) (Markstein, IBM J. Res. Dev. 34(1), 1990, for
+// binary64 without underflow or overflow).  Three FMA-unit operations replace
+// a full IEEE division when several quotients share a divisor.
+__host__ __device__ inline double div_by_recip(double x, double b, double y)
+{
+    const double q = x * y;
+    const double r = fma(-q, b, x);
+    return fma(r, y, q);
+}
 
 // Neighbour data gathered once per trial.
 struct Nbr {
@@ -202,7 +216,8 @@ __device__ inline double energy_compat(const Nbr &N, double nx, double ny, doubl
     return fl32(e - g * (double)snew);
 }
 
-// Cone trial move (chll/chll_mod.f90:149-183), operation for operation.
+// Cone trial move (chll/chll_mod.f90:149-183), operation for operation; the
+// componentwise divisions by a shared norm are exact quotients via div_by_recip.
 __device__ inline void trial_compat(double nnx, double nny, double nnz, double d, double u_phi, double &ox,
                                     double &oy, double &oz)
 {
@@ -214,20 +229,24 @@ __device__ inline void trial_compat(double nnx, double nny, double nnz, double d
         const double yy = nny + d * sphi;
         const double zz = nnz;
         const double rsq = sqrt(xx * xx + yy * yy + zz * zz);
-        ox = xx / rsq; oy = yy / rsq; oz = zz / rsq;
+        const double ir = 1.0 / rsq;
+        ox = div_by_recip(xx, rsq, ir); oy = div_by_recip(yy, rsq, ir); oz = div_by_recip(zz, rsq, ir);
     } else {
         double ux = -nny, uy = nnx, uz = 0.0;
         double rsq = sqrt(ux * ux + uy * uy);
-        ux = ux / rsq; uy = uy / rsq; uz = uz / rsq;
+        double ir = 1.0 / rsq;
+        ux = div_by_recip(ux, rsq, ir); uy = div_by_recip(uy, rsq, ir); uz = div_by_recip(uz, rsq, ir);
         double vx = -nnz * nnx, vy = -nnz * nny, vz = nnx * nnx + nny * nny;
         rsq = sqrt(vx * vx + vy * vy + vz * vz);
-        vx = vx / rsq; vy = vy / rsq; vz = vz / rsq;
+        ir = 1.0 / rsq;
+        vx = div_by_recip(vx, rsq, ir); vy = div_by_recip(vy, rsq, ir); vz = div_by_recip(vz, rsq, ir);
         const double dc = d * cphi, ds = d * sphi;
         const double tx = nnx + dc * ux + ds * vx;
         const double ty = nny + dc * uy + ds * vy;
         const double tz = nnz + dc * uz + ds * vz;
         rsq = sqrt(tx * tx + ty * ty + tz * tz);
-        ox = tx / rsq; oy = ty / rsq; oz = tz / rsq;
+        ir = 1.0 / rsq;
+        ox = div_by_recip(tx, rsq, ir); oy = div_by_recip(ty, rsq, ir); oz = div_by_recip(tz, rsq, ir);
     }
 }
 

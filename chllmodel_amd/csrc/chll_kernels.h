@@ -13,7 +13,7 @@ struct Fields {
 
 // Per-launch arguments of a half-sweep.
 struct PhaseArgs {
-    double g, KK, kbt, inv_kbt;
+    double g, KK, kbt, inv_kbt;  // inv_kbt = RN(1/kbt)
     const double *d;      // device cone step (updated by the finalize kernel)
     const double *u_phi;  // deterministic mode: uniforms indexed by packed site
     const double *u_acc;
@@ -23,9 +23,18 @@ struct PhaseArgs {
 
 constexpr int kRowCols = 16;
 
+// Accept/flip counters: each block adds its count atomically into one of
+// kSlots slots per phase (slot = block % kSlots, slots kSlotStride words apart
+// so they sit on separate cache lines); the finalize kernel sums and re-zeroes
+// them.
+constexpr int kSlots = 64;
+constexpr int kSlotStride = 32;
+constexpr int kCountWords = 4 * kSlots * kSlotStride;
+// Cap on measurement blocks (grid-stride), bounding the finalize kernel's sum.
+constexpr uint32_t kMeasBlocksMax = 2048;
+
 struct FinArgs {
-    const uint32_t *counts;  // 4 phases x nblk_phase per-block counts
-    uint32_t nblk_phase;
+    uint32_t *counts;        // 4 phases x kSlots slots, zeroed after reading
     const double *part;      // nblk_meas x 8 measurement partials
     const long long *spart;  // nblk_meas spin partials
     uint32_t nblk_meas;
@@ -38,6 +47,7 @@ struct FinArgs {
 };
 
 uint32_t phase_blocks(const Geom &G);
+uint32_t measure_blocks(const Geom &G);
 
 void launch_scatter_f64(const Geom &G, const double *src, double *dst, hipStream_t s);
 void launch_gather_f64(const Geom &G, const double *src, double *dst, hipStream_t s);

@@ -73,8 +73,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_spin(Geom G, const int8_t *__
 }
 
 // ---------------------------------------------------------------------------
-// Block-wide sum of one 0/1 flag per thread, written to out[blk].
-__device__ inline void block_count(bool flag, uint32_t *out, uint32_t blk)
+// Block-wide sum of one 0/1 flag per thread, added to slot blk % kSlots.
+__device__ inline void block_count(bool flag, uint32_t *slots, uint32_t blk)
 {
     __shared__ uint32_t wsum[kBlock / 64];
     const uint64_t bal = __ballot(flag);
@@ -85,7 +85,7 @@ __device__ inline void block_count(bool flag, uint32_t *out, uint32_t blk)
         uint32_t t = 0;
 #pragma unroll
         for (int w = 0; w < kBlock / 64; ++w) t += wsum[w];
-        out[blk] = t;
+        if (t) atomicAdd(&slots[(blk & (kSlots - 1)) * kSlotStride], t);
     }
 }
 
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_dir(Geom G, Fields F, int colour, Ph
             const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
             trial_compat(nx, ny, nz, d, u_phi, tx, ty, tz);
             const double enew = energy_compat(N, tx, ty, tz, sg, A.g, A.KK);
-            acc = (enew < eold) || (u_acc <= exp(-(enew - eold) / A.kbt));
+            acc = (enew < eold) || (u_acc <= exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt)));
         } else {
             double c2o, c2n;
             const double ao = cqf_sum(N, sg, nx, ny, nz, c2o);
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(kBlock) void k_spin(Geom G, Fields F, int colour, P
         if (ARITH == ARITH_COMPAT) {
             const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
             const double enew = energy_compat(N, nx, ny, nz, -sg, A.g, A.KK);
-            acc = (enew < eold) || (u_acc <= exp(-(enew - eold) / A.kbt));
+            acc = (enew < eold) || (u_acc <= exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt)));
         } else {
             double c2;
             const double a = cq_sum(N, nx, ny, nz, c2);
@@ -223,12 +223,11 @@ template <int BC>
 __global__ __launch_bounds__(kBlock) void k_measure(Geom G, Fields F, double g, double KK,
                                                     double *__restrict__ part, long long *__restrict__ spart)
 {
-    const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
     double v[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) v[q] = 0.0;
     long long ssum = 0;
-    if (m < G.nsub) {
+    for (uint32_t m = blockIdx.x * kBlock + threadIdx.x; m < G.nsub; m += gridDim.x * kBlock) {
         const uint32_t row = fdiv(m, G.fd_half);
         const uint32_t h = m - row * G.half;
         const uint32_t k = fdiv(row, G.fd_nj);
@@ -324,9 +323,12 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs A)
     __shared__ double red[kFinBlock / 64][10];
     const uint32_t t = threadIdx.x;
     unsigned long long ca = 0, cf = 0;
-    for (uint32_t b = t; b < A.nblk_phase; b += kFinBlock) {
-        ca += (unsigned long long)A.counts[b] + A.counts[A.nblk_phase + b];
-        cf += (unsigned long long)A.counts[2 * A.nblk_phase + b] + A.counts[3 * A.nblk_phase + b];
+    for (uint32_t q = t; q < 4 * kSlots; q += kFinBlock) {
+        uint32_t *slot = A.counts + (q / kSlots) * (kSlots * kSlotStride) + (q % kSlots) * kSlotStride;
+        const uint32_t c = *slot;
+        *slot = 0u;
+        if (q < 2 * kSlots) ca += c;
+        else cf += c;
     }
     double v[8];
 #pragma unroll
@@ -431,6 +433,12 @@ static inline dim3 grid1(uint32_t n) { return dim3((n + kBlock - 1) / kBlock); }
 
 uint32_t phase_blocks(const Geom &G) { return (G.nsub + kBlock - 1) / kBlock; }
 
+uint32_t measure_blocks(const Geom &G)
+{
+    const uint32_t b = phase_blocks(G);
+    return b < kMeasBlocksMax ? b : kMeasBlocksMax;
+}
+
 void launch_scatter_f64(const Geom &G, const double *src, double *dst, hipStream_t s)
 {
     hipLaunchKernelGGL(k_scatter_f64, grid1(G.nsub), dim3(kBlock), 0, s, G, src, dst);
@@ -485,9 +493,9 @@ void launch_measure(int bc, const Geom &G, const Fields &F, double g, double KK,
                     hipStream_t s)
 {
     if (bc == BC_OPEN)
-        hipLaunchKernelGGL(k_measure<BC_OPEN>, grid1(G.nsub), dim3(kBlock), 0, s, G, F, g, KK, part, spart);
+        hipLaunchKernelGGL(k_measure<BC_OPEN>, dim3(measure_blocks(G)), dim3(kBlock), 0, s, G, F, g, KK, part, spart);
     else
-        hipLaunchKernelGGL(k_measure<BC_PERIODIC>, grid1(G.nsub), dim3(kBlock), 0, s, G, F, g, KK, part, spart);
+        hipLaunchKernelGGL(k_measure<BC_PERIODIC>, dim3(measure_blocks(G)), dim3(kBlock), 0, s, G, F, g, KK, part, spart);
 }
 
 void launch_finalize(const FinArgs &A, hipStream_t s)
