@@ -8,8 +8,10 @@ resident in HBM.  The metric is site-updates/s = L^3 * sweeps / s.
 
 Default workload: L = 256 (16,777,216 sites), seeded isotropic directors and
 +-1 spins, eps = 1, K = 1.0, g = 0.5, J = 0, kT = 0.7, d0 = 0.3, nout = 50, open
-boundaries and reference-compatible arithmetic (the drop-in path's defaults),
-site-keyed Philox random numbers.
+boundaries (the reference's), binary64 energies, site-keyed Philox random
+numbers; the reference-compatible arithmetic (binary32-rounded site energies)
+is timed beside it under "variants".  A second section times the 64-replica
+batched phase-diagram scan against the same replicas run one after another.
 
 Prints ONE JSON line.  The roofline object is for the dominant kernel, the
 director half-sweep, timed with HIP events on the stream it runs on; its
@@ -84,7 +86,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--L", type=int, default=256)
     ap.add_argument("--nout", type=int, default=50)
-    ap.add_argument("--arith", default="compat", choices=["compat", "fp64"])
+    ap.add_argument("--arith", default="fp64", choices=["compat", "fp64"])
     ap.add_argument("--bc", default="open", choices=["open", "periodic"])
     ap.add_argument("--rng", default="philox", choices=["philox", "stream"])
     ap.add_argument("--g", type=float, default=0.5)
@@ -94,6 +96,9 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events")
+    ap.add_argument("--no-variants", action="store_true", help="skip the other arithmetic mode")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-replica scan")
+    ap.add_argument("--batched-blocks", type=int, default=20, help="timed blocks of 10 sweeps")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run one plain elementwise copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -115,16 +120,6 @@ def main():
     st0 = isotropic_state(L, L, L, 20261203)
     torch.cuda.init()
     stream = torch.cuda.Stream()
-    lat = Lattice(L, L, L, bc=args.bc)
-    lat.set_stream(stream.cuda_stream)
-    lat.upload_state(st0)
-    lat.set_step(args.d0)
-    p = lat.params(g=args.g, KK=args.K, kbt=args.kT, rng=args.rng, arith=args.arith, seed=20261203, adapt=True)
-    W, K = args.warmup, args.steps
-    lat.sweeps(1, W, args.nout, params=p, want_rows=False)
-    lat.sync()
-    torch.cuda.synchronize()
-    log("bench: warmup done")
     if args.copy_calib_mib > 0:
         n = args.copy_calib_mib * (1 << 20) // 8
         src = torch.rand(n, dtype=torch.float64, device="cuda")
@@ -134,31 +129,43 @@ def main():
         torch.cuda.synchronize()
         del src, dst
         log(f"bench: calibration copy of {args.copy_calib_mib} MiB done")
-    if not args.no_timing:
-        lat.set_timing(True)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    lat.sweeps(W + 1, W + K, args.nout, params=p, want_rows=False)
-    ev1.record(stream)
-    lat.sync()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ev_ms = ev0.elapsed_time(ev1)
-    kt = lat.timing() if not args.no_timing else {}
-    row = lat.measure(args.g, args.K)
-    d_now = lat.get_step()
-    lat.close()
-    log(f"bench: timed {K} sweeps in {wall:.4f} s (events {ev_ms:.3f} ms); E_true={row['E_true']:.6f} "
-        f"eta={row['eta_exact']:.6f} d={d_now:.5f}")
 
-    sups = N * K / wall
-    kernels = {}
-    for name, (ms, cnt) in kt.items():
-        if cnt:
-            kernels[name] = {"launches": cnt, "avg_us": 1e3 * ms / cnt, "total_ms": ms}
+    def time_single(arith, timing):
+        """W warm-up sweeps, then K timed sweeps bracketed by synchronisation."""
+        lat = Lattice(L, L, L, bc=args.bc)
+        lat.set_stream(stream.cuda_stream)
+        lat.upload_state(st0)
+        lat.set_step(args.d0)
+        p = lat.params(g=args.g, KK=args.K, kbt=args.kT, rng=args.rng, arith=arith, seed=20261203, adapt=True)
+        W, K = args.warmup, args.steps
+        lat.sweeps(1, W, args.nout, params=p, want_rows=False)
+        lat.sync()
+        torch.cuda.synchronize()
+        if timing:
+            lat.set_timing(True)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        lat.sweeps(W + 1, W + K, args.nout, params=p, want_rows=False)
+        ev1.record(stream)
+        lat.sync()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kt = lat.timing() if timing else {}
+        row = lat.measure(args.g, args.K)
+        lat.close()
+        log(f"bench: {arith}: {K} sweeps in {wall:.4f} s (events {ev0.elapsed_time(ev1):.3f} ms); "
+            f"E_true={row['E_true']:.6f} eta={row['eta_exact']:.6f}")
+        kernels = {}
+        for name, (ms, cnt) in kt.items():
+            if cnt:
+                kernels[name] = {"launches": cnt, "avg_us": 1e3 * ms / cnt, "total_ms": ms}
+        return N * K / wall, wall, kernels
+
+    W, K = args.warmup, args.steps
+    sups, wall, kernels = time_single(args.arith, not args.no_timing)
     roof = None
     if "director" in kernels:
         avg_s = kernels["director"]["avg_us"] * 1e-6
@@ -174,6 +181,12 @@ def main():
         if "chirality" in kernels:
             kernels["chirality"]["GBps"] = BYTES_SPIN * N / (kernels["chirality"]["avg_us"] * 1e-6) / 1e9
         kernels["director"]["GBps"] = ach
+    variants = {}
+    if not args.no_variants:
+        other = "compat" if args.arith == "fp64" else "fp64"
+        v_sups, v_wall, v_k = time_single(other, not args.no_timing)
+        variants[other] = {"site_updates_per_s": v_sups, "ms_per_step": 1e3 * v_wall / K,
+                           "kernels": {k: round(v["avg_us"], 2) for k, v in v_k.items()}}
     out = {
         "metric": METRIC,
         "value": sups,
@@ -195,13 +208,75 @@ def main():
         "sweep_hbm_frac": sups * BYTES_SWEEP / (HBM_PEAK_GBS * 1e9),
         "roofline": roof,
         "kernels": kernels,
+        "variants": variants,
         "cpu_baseline": None,
     }
+    if not args.no_batched:
+        out["batched"] = batched_scan(args, stream)
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         log(f"bench: CPU baseline, {args.cpu_sweeps} sweeps at L={L}")
         out["cpu_baseline"] = cpu_baseline(L, st0, args)
         out["speedup_vs_cpu"] = sups / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
+
+
+def batched_scan(args, stream):
+    """64 replicas in 5 parallel-tempering ladders (g = 0, 0.5, 1, 2, 3 with
+    L = 32, 48, 64, 96, 128; kT geometric in [0.2, 2]; K = 1): one hipGraph
+    replay per block of 10 sweeps, against the same replicas swept one after
+    another with the single-lattice path (measurement every 10 sweeps)."""
+    import torch
+
+    from chllmodel_amd.ensemble import Ensemble, autotune, phase_scan_replicas
+    from chllmodel_amd.lattice import Lattice
+    from chllmodel_amd.states import isotropic_state
+
+    reps = phase_scan_replicas()
+    sites = sum(r["ni"] * r["nj"] * r["nk"] for r in reps)
+    Kb, nb = 10, args.batched_blocks
+    log(f"bench: batched scan, {len(reps)} replicas, {sites} sites, autotuning")
+    tune = autotune(reps, bc="open", arith=args.arith, K=Kb, use_cache=False)
+    ens = Ensemble(reps, bc="open", arith=args.arith)
+    ens.set_stream(stream.cuda_stream)
+    ens.set_block(tune["best"])
+    ens.init_isotropic(20261203)
+    ens.run(2, Kb, seed=20261203, want_rows=False, use_graph=True)
+    ens.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.run(nb, Kb, seed=20261203, want_rows=False, use_graph=True)
+    ens.sync()
+    torch.cuda.synchronize()
+    t_b = time.perf_counter() - t0
+    stt = ens.state()
+    ens.close()
+    nsw = nb * Kb
+    log(f"bench: batched {nsw} sweeps in {t_b:.4f} s")
+    # the same replicas one after another, single-lattice path
+    t_seq = 0.0
+    for i, r in enumerate(reps):
+        lat = Lattice(r["ni"], r["nj"], r["nk"])
+        lat.set_stream(stream.cuda_stream)
+        lat.upload_state(isotropic_state(r["ni"], r["nj"], r["nk"], 20261203 + i))
+        lat.set_step(r["d0"])
+        p = lat.params(g=r["g"], KK=r["KK"], kbt=r["kbt"], arith=args.arith, seed=20261203, replica=i)
+        lat.sweeps(1, 2, Kb, params=p, want_rows=False)
+        lat.sync()
+        t0 = time.perf_counter()
+        lat.sweeps(3, 2 + nsw, Kb, params=p, want_rows=False)
+        lat.sync()
+        t_seq += time.perf_counter() - t0
+        lat.close()
+    log(f"bench: sequential {nsw} sweeps x {len(reps)} replicas in {t_seq:.4f} s")
+    att, acc = stt["swap_att"], stt["swap_acc"]
+    return {"workload": "64 replicas, 5 PT ladders g={0,0.5,1,2,3} x L={32,48,64,96,128}, kT in [0.2,2] "
+                        "geometric, K=1, swaps every 10 sweeps, one hipGraph replay per 10 sweeps",
+            "replicas": len(reps), "sites": sites, "sweeps": nsw, "arith": args.arith,
+            "site_updates_per_s": sites * nsw / t_b, "ms_per_sweep": 1e3 * t_b / nsw,
+            "sequential_site_updates_per_s": sites * nsw / t_seq, "sequential_ms_per_sweep": 1e3 * t_seq / nsw,
+            "speedup_vs_sequential": t_seq / t_b, "block_size": tune["best"],
+            "autotune_ms_per_sweep": tune["ms_per_sweep"],
+            "swap_acceptance": float(acc.sum() / max(att.sum(), 1))}
 
 
 if __name__ == "__main__":

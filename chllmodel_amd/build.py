@@ -16,8 +16,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libchll_mod.so")
-SOURCES = ["chll_kernels.hip", "chll_api.hip"]
-HEADERS = ["chll_device.h", "chll_kernels.h"]
+SOURCES = ["chll_kernels.hip", "chll_api.hip", "chll_ensemble.hip"]
+HEADERS = ["chll_device.h", "chll_kernels.h", "chll_site.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CHLL_OFFLOAD_ARCH", "gfx950")

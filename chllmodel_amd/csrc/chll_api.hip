@@ -149,22 +149,31 @@ int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lat
     if (!out) return fail("chll_lattice_create: null output pointer");
     *out = nullptr;
     if (check_geometry(ni, nj, nk, bc)) return -1;
+    if ((uint64_t)ni * nj * nk + kFieldPad >= (1ull << 32)) return fail("lattice too large for 32-bit site indices");
     chll_lattice *L = new chll_lattice();
     Geom &G = L->G;
     G.ni = (uint32_t)ni; G.nj = (uint32_t)nj; G.nk = (uint32_t)nk;
     G.half = G.ni / 2;
     G.plane = G.half * G.nj;
     G.nsub = (uint32_t)(((uint64_t)ni * nj * nk) / 2);
+    G.zero = 2 * G.nsub;
     G.fd_half = make_fastdiv(G.half);
     G.fd_nj = make_fastdiv(G.nj);
     L->bc = bc;
     const size_t ns = G.nsub, n3 = 2 * ns;
     L->nblk = phase_blocks(G);
     int rc = 0;
-    rc |= dalloc(&L->F.nx, n3);
-    rc |= dalloc(&L->F.ny, n3);
-    rc |= dalloc(&L->F.nz, n3);
-    rc |= dalloc(&L->F.s, n3);
+    // fields carry kFieldPad zeroed elements after the 2 nsub sites (the zero site)
+    rc |= dalloc(&L->F.nx, n3 + kFieldPad);
+    rc |= dalloc(&L->F.ny, n3 + kFieldPad);
+    rc |= dalloc(&L->F.nz, n3 + kFieldPad);
+    rc |= dalloc(&L->F.s, n3 + kFieldPad);
+    if (!rc) {
+        HIPCHECK(hipMemset(L->F.nx + n3, 0, kFieldPad * sizeof(double)));
+        HIPCHECK(hipMemset(L->F.ny + n3, 0, kFieldPad * sizeof(double)));
+        HIPCHECK(hipMemset(L->F.nz + n3, 0, kFieldPad * sizeof(double)));
+        HIPCHECK(hipMemset(L->F.s + n3, 0, kFieldPad * sizeof(int8_t)));
+    }
     rc |= dalloc(&L->bad, 1);
     L->nblk_meas = measure_blocks(G);
     rc |= dalloc(&L->counts, (size_t)kCountWords);

@@ -60,11 +60,18 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f)
 // the site's rank in its sublattice list, i.e. the index of its uniforms in
 // rand1 / rand2.  Each field stores colour 0 at [0, nsub) and colour 1 at
 // [nsub, 2 nsub).
+//
+// Index 2 nsub of every field is a padding "zero site": director (0,0,0),
+// spin 0.  Absent neighbours (open boundaries) point there, so a bond to it
+// has c = q = 0 and adds nothing to a binary64 energy difference.
+constexpr uint32_t kFieldPad = 64;  // padding elements after the 2 nsub sites
+
 struct Geom {
     uint32_t ni, nj, nk;
     uint32_t half;   // ni / 2: packed sites per row and colour
     uint32_t plane;  // ni * nj / 2: packed sites per plane and colour
     uint32_t nsub;   // N / 2
+    uint32_t zero;   // 2 nsub: index of the zero site
     FastDiv fd_half, fd_nj;
 };
 
@@ -82,39 +89,41 @@ __device__ inline Site decode(const Geom &G, uint32_t m, int colour)
     return Site{2u * h + p, j, k};
 }
 
-// Packed indices (in the other colour) of the six neighbours in the order
-// +x, -x, +y, -y, +z, -z, and a presence mask (bit b set = bond b exists).
-// Open boundaries drop absent bonds (chll/chll_mod.f90:136-141, 371-410);
+// Field indices of the six neighbours (all of the other colour, whose block
+// starts at oth) in the order +x, -x, +y, -y, +z, -z, and a presence mask
+// (bit b set = bond b exists).  Open boundaries drop absent bonds
+// (chll/chll_mod.f90:136-141, 371-410) and point them at the zero site;
 // periodic boundaries wrap them.
 template <int BC>
-__device__ inline uint32_t neighbours(const Geom &G, uint32_t m, const Site &s, uint32_t nb[6])
+__device__ inline uint32_t neighbours(const Geom &G, uint32_t m, const Site &s, uint32_t oth, uint32_t nb[6])
 {
     const uint32_t p = s.i & 1u;
+    const uint32_t o = oth + m;
     uint32_t mask = 0;
     // +x
-    if (s.i + 1 < G.ni) { nb[0] = m + p; mask |= 1u; }
-    else if (BC == BC_PERIODIC) { nb[0] = m + 1 - G.half; mask |= 1u; }
+    if (s.i + 1 < G.ni) { nb[0] = o + p; mask |= 1u; }
+    else if (BC == BC_PERIODIC) { nb[0] = o + 1 - G.half; mask |= 1u; }
+    else nb[0] = G.zero;
     // -x
-    if (s.i >= 1) { nb[1] = m + p - 1; mask |= 2u; }
-    else if (BC == BC_PERIODIC) { nb[1] = m + G.half - 1; mask |= 2u; }
+    if (s.i >= 1) { nb[1] = o + p - 1; mask |= 2u; }
+    else if (BC == BC_PERIODIC) { nb[1] = o + G.half - 1; mask |= 2u; }
+    else nb[1] = G.zero;
     // +y
-    if (s.j + 1 < G.nj) { nb[2] = m + G.half; mask |= 4u; }
-    else if (BC == BC_PERIODIC) { nb[2] = m - (G.nj - 1) * G.half; mask |= 4u; }
+    if (s.j + 1 < G.nj) { nb[2] = o + G.half; mask |= 4u; }
+    else if (BC == BC_PERIODIC) { nb[2] = o - (G.nj - 1) * G.half; mask |= 4u; }
+    else nb[2] = G.zero;
     // -y
-    if (s.j >= 1) { nb[3] = m - G.half; mask |= 8u; }
-    else if (BC == BC_PERIODIC) { nb[3] = m + (G.nj - 1) * G.half; mask |= 8u; }
+    if (s.j >= 1) { nb[3] = o - G.half; mask |= 8u; }
+    else if (BC == BC_PERIODIC) { nb[3] = o + (G.nj - 1) * G.half; mask |= 8u; }
+    else nb[3] = G.zero;
     // +z
-    if (s.k + 1 < G.nk) { nb[4] = m + G.plane; mask |= 16u; }
-    else if (BC == BC_PERIODIC) { nb[4] = m - (G.nk - 1) * G.plane; mask |= 16u; }
+    if (s.k + 1 < G.nk) { nb[4] = o + G.plane; mask |= 16u; }
+    else if (BC == BC_PERIODIC) { nb[4] = o - (G.nk - 1) * G.plane; mask |= 16u; }
+    else nb[4] = G.zero;
     // -z
-    if (s.k >= 1) { nb[5] = m - G.plane; mask |= 32u; }
-    else if (BC == BC_PERIODIC) { nb[5] = m + (G.nk - 1) * G.plane; mask |= 32u; }
-    if (!(mask & 1u)) nb[0] = m;
-    if (!(mask & 2u)) nb[1] = m;
-    if (!(mask & 4u)) nb[2] = m;
-    if (!(mask & 8u)) nb[3] = m;
-    if (!(mask & 16u)) nb[4] = m;
-    if (!(mask & 32u)) nb[5] = m;
+    if (s.k >= 1) { nb[5] = o - G.plane; mask |= 32u; }
+    else if (BC == BC_PERIODIC) { nb[5] = o + (G.nk - 1) * G.plane; mask |= 32u; }
+    else nb[5] = G.zero;
     return mask;
 }
 
@@ -257,7 +266,25 @@ __device__ inline void trial_compat(double nnx, double nny, double nnz, double d
 //   dE = sum_b [ (c_b^2 - c'_b^2) - sgn_b K f_b (c'_b q'_b - c_b q_b) ].
 // Chirality flip sigma -> -sigma at fixed director:
 //   dE = sigma * (K * sum_b sgn_b c_b q_b + 2 g).
-__device__ inline double cq_sum(const Nbr &N, double nx, double ny, double nz, double &csq)
+__device__ inline double cq_sum(const Nbr &N, double nx, double ny, double nz)
+{
+    double acc = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const double mx = N.x[b], my = N.y[b], mz = N.z[b];
+        const double c = fma(nz, mz, fma(ny, my, nx * mx));
+        double q;
+        if (b < 2) q = fma(ny, mz, -nz * my);
+        else if (b < 4) q = fma(nz, mx, -nx * mz);
+        else q = fma(nx, my, -ny * mx);
+        acc = (b & 1) == 0 ? fma(c, q, acc) : fma(-c, q, acc);
+    }
+    return acc;
+}
+
+// Weighted sum for the director move: sum_b sgn_b f_b c_b q_b and sum_b c_b^2.
+// Absent bonds read the zero site (c = q = 0) and add exactly nothing.
+__device__ inline double cqf_sum(const Nbr &N, double nx, double ny, double nz, const double f[6], double &csq)
 {
     double acc = 0.0, c2 = 0.0;
 #pragma unroll
@@ -268,46 +295,59 @@ __device__ inline double cq_sum(const Nbr &N, double nx, double ny, double nz, d
         if (b < 2) q = fma(ny, mz, -nz * my);
         else if (b < 4) q = fma(nz, mx, -nx * mz);
         else q = fma(nx, my, -ny * mx);
-        const bool on = (N.mask >> b) & 1u;
-        const double cqv = on ? c * q : 0.0;
-        acc = (b & 1) == 0 ? acc + cqv : acc - cqv;
-        c2 = on ? fma(c, c, c2) : c2;
+        acc = fma(c * q, f[b], acc);
+        c2 = fma(c, c, c2);
     }
     csq = c2;
     return acc;
 }
 
-// Weighted sum for the director move: sum_b sgn_b f_b c_b q_b and sum_b c_b^2.
-__device__ inline double cqf_sum(const Nbr &N, int sigma, double nx, double ny, double nz, double &csq)
+// sin and cos of 2 pi u for u in [0, 1): w = 4u and f = w - rint(w) are exact,
+// so 2 pi u = n pi/2 + x with x = f pi/2 in [-pi/4, pi/4]; sin x and cos x
+// from their Taylor series through x^17 and x^18 (truncation below 1e-19 on
+// that interval), then rotated by the quadrant n.
+__device__ inline void sincos_2pi(double u, double &s, double &c)
 {
-    double acc = 0.0, c2 = 0.0;
-#pragma unroll
-    for (int b = 0; b < 6; ++b) {
-        const double mx = N.x[b], my = N.y[b], mz = N.z[b];
-        const double c = fma(nz, mz, fma(ny, my, nx * mx));
-        double q;
-        if (b < 2) q = fma(ny, mz, -nz * my);
-        else if (b < 4) q = fma(nz, mx, -nx * mz);
-        else q = fma(nx, my, -ny * mx);
-        const bool on = (N.mask >> b) & 1u;
-        const double f = 0.5 * (double)(sigma + N.s[b]);
-        const double v = on ? c * q * f : 0.0;
-        acc = (b & 1) == 0 ? acc + v : acc - v;
-        c2 = on ? fma(c, c, c2) : c2;
-    }
-    csq = c2;
-    return acc;
+    const double w = 4.0 * u;
+    const double n = rint(w);
+    const double x = (w - n) * 1.5707963267948966;
+    const double z = x * x;
+    double ps = -1.0 / 355687428096000.0;
+    ps = fma(ps, z, 1.0 / 1307674368000.0);
+    ps = fma(ps, z, -1.0 / 6227020800.0);
+    ps = fma(ps, z, 1.0 / 39916800.0);
+    ps = fma(ps, z, -1.0 / 362880.0);
+    ps = fma(ps, z, 1.0 / 5040.0);
+    ps = fma(ps, z, -1.0 / 120.0);
+    ps = fma(ps, z, 1.0 / 6.0);
+    const double sx = fma(-x * z, ps, x);
+    double pc = 1.0 / 6402373705728000.0;
+    pc = fma(pc, z, -1.0 / 20922789888000.0);
+    pc = fma(pc, z, 1.0 / 87178291200.0);
+    pc = fma(pc, z, -1.0 / 479001600.0);
+    pc = fma(pc, z, 1.0 / 3628800.0);
+    pc = fma(pc, z, -1.0 / 40320.0);
+    pc = fma(pc, z, 1.0 / 720.0);
+    pc = fma(pc, z, -1.0 / 24.0);
+    pc = fma(pc, z, 0.5);
+    const double cx = fma(-z, pc, 1.0);
+    const int q = (int)n;
+    const bool swap = q & 1;
+    double ss = swap ? cx : sx;
+    double cc = swap ? sx : cx;
+    s = (q & 2) ? -ss : ss;
+    c = ((q + 1) & 2) ? -cc : cc;
 }
 
 // Cone trial in binary64 with reciprocal square roots: the same geometry
 // (tangent basis u = z x n / |.|, v = n x u, displacement d on a cone of
 // half-angle atan d at azimuth 2 pi u_phi), without the compat operation order.
+// Same branch as the reference for |nz| > 0.999 (chll/chll_mod.f90:149-157).
 __device__ inline void trial_fp64(double nnx, double nny, double nnz, double d, double u_phi, double &ox,
                                   double &oy, double &oz)
 {
-    const double phi = u_phi * kTwoPi;
     double sphi, cphi;
-    sincos(phi, &sphi, &cphi);
+    sincos_2pi(u_phi, sphi, cphi);
     double tx, ty, tz;
     if (__builtin_expect(fabs(nnz) > 0.999, 0)) {
         tx = fma(d, cphi, nnx);
@@ -326,16 +366,6 @@ __device__ inline void trial_fp64(double nnx, double nny, double nnz, double d, 
     }
     const double inv = rsqrt(fma(tz, tz, fma(ty, ty, tx * tx)));
     ox = tx * inv; oy = ty * inv; oz = tz * inv;
-}
-
-// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8);
-// remap so each XCD walks one contiguous range of logical blocks and its L2
-// sees the neighbouring rows and planes of the tiles it updates.
-__device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb)
-{
-    const uint32_t per = nb >> 3, rem = nb & 7u;
-    const uint32_t x = b & 7u, q = b >> 3;
-    return x * per + (x < rem ? x : rem) + q;
 }
 
 }  // namespace chll

@@ -1,0 +1,677 @@
+// chll_ensemble.hip -- many replicas of uneven size and (kT, g) swept by one
+// launch per half-sweep, with parallel-tempering swaps on the device, captured
+// in a hipGraph.
+//
+// The reference scans temperature / field grids by running one ChLLSim after
+// another (e.g. scripts/phase_diagram_large1.py loops over K x T); here every
+// replica of a scan lives in HBM at once.  A launch's block b belongs to one
+// replica (table blk_rep, replica-major, largest replicas first) and runs the
+// same per-site trial as the single-lattice kernels (chll_site.h).
+//
+// Replicas are grouped into ladders: replicas of one ladder share (ni, nj, nk,
+// g, K) and sit at distinct temperature slots, listed in ascending kT.  Each
+// slot owns its kT and its adaptive cone step d; a parallel-tempering swap
+// exchanges the slots of two replicas at neighbouring temperatures with
+// probability min(1, exp((1/kT_t - 1/kT_{t+1}) (E_a - E_b))), E the replica's
+// binary64 lattice Hamiltonian.
+//
+// One "block" of K sweeps is: K x (four half-sweeps + a per-replica epilogue),
+// then the observable reduction, the measuring epilogue and the swap kernel,
+// which also advances the device sweep counter.  Nothing reads host memory, so
+// the block is captured once and replayed.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#include "../../include/chll_mod.h"
+#include "chll_site.h"
+
+namespace chll {
+
+constexpr int kEnsCols = CHLL_ENS_COLS;
+
+// Replica descriptor in device memory (read by whole blocks: uniform loads).
+struct RepDesc {
+    Geom G;
+    Fields F;
+    uint32_t blk_start, nblk;  // this replica's blocks in a launch
+    uint32_t replica;          // Philox counter word
+    uint32_t pad;
+    double g, KK;
+    uint32_t *counts;          // 4 phases x kSlots x kSlotStride
+};
+
+// Ensemble-wide device state.
+struct EnsDev {
+    const RepDesc *R;
+    const uint32_t *blk_rep;  // per launch block: replica << 24 | local block
+    double *kbt, *inv_kbt;    // per temperature slot
+    double *d;                // per temperature slot (adaptive cone step)
+    int32_t *rep_slot;        // replica -> slot
+    int32_t *slot_rep;        // slot -> replica
+    const int32_t *lad_start; // nlad + 1 slot offsets
+    double *E;                // per replica: Hamiltonian at the last measurement
+    uint32_t *istep;          // next sweep index
+    int32_t *row;             // next measurement row
+    unsigned long long *swap_att, *swap_acc;  // per slot pair (t, t+1)
+    double *part;             // per block: 8 measurement partials
+    long long *spart;         // per block: spin sum
+    double *rows;             // max_rows x nrep x kEnsCols
+    int32_t max_rows, nrep, nlad, adapt;
+    uint64_t seed;
+    uint32_t run_index, K;
+};
+
+template <int ARITH, int BC, int BS>
+__global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int kind, int colour, uint32_t istep_off)
+{
+    const uint32_t code = E.blk_rep[blockIdx.x];
+    const uint32_t r = code >> 24, lb = code & 0xFFFFFFu;
+    const RepDesc &D = E.R[r];
+    const int s = E.rep_slot[r];
+    PhaseArgs A;
+    A.g = D.g; A.KK = D.KK; A.kbt = E.kbt[s]; A.inv_kbt = E.inv_kbt[s]; A.d = E.d + s;
+    A.u_phi = nullptr; A.u_acc = nullptr;
+    A.seed = E.seed; A.run_index = E.run_index; A.replica = D.replica; A.istep = *E.istep + istep_off;
+    const uint32_t m = lb * BS + threadIdx.x;
+    bool acc = false;
+    if (m < D.G.nsub) {
+        if (kind == 0) acc = dir_site<ARITH, RNG_PHILOX, BC>(D.G, D.F, colour, A, m);
+        else acc = spin_site<ARITH, RNG_PHILOX, BC>(D.G, D.F, colour, A, m);
+    }
+    block_count<BS>(acc, D.counts + (size_t)(2 * kind + colour) * kSlots * kSlotStride, lb);
+}
+
+template <int BC, int BS>
+__global__ __launch_bounds__(BS) void k_measure_ens(EnsDev E)
+{
+    const uint32_t code = E.blk_rep[blockIdx.x];
+    const uint32_t r = code >> 24, lb = code & 0xFFFFFFu;
+    const RepDesc &D = E.R[r];
+    MeasureAcc acc;
+    const uint32_t m = lb * BS + threadIdx.x;
+    if (m < D.G.nsub) measure_site<BC>(D.G, D.F, D.g, D.KK, m, acc);
+    block_reduce_measure<BS>(acc, E.part + (size_t)blockIdx.x * 8, E.spart + blockIdx.x);
+}
+
+// Per-replica epilogue (one block per replica): accept/flip totals, step
+// adaptation of the replica's temperature slot (chll/chll_mod.f90:83-90), and
+// on measuring sweeps the observable row and the replica's energy for the swap.
+constexpr int kEnsFin = 256;
+
+__global__ __launch_bounds__(kEnsFin) void k_finalize_ens(EnsDev E, uint32_t istep_off, int measure)
+{
+    const int r = blockIdx.x;
+    const RepDesc &D = E.R[r];
+    const uint32_t t = threadIdx.x;
+    unsigned long long ca = 0, cf = 0;
+    for (uint32_t q = t; q < 4 * kSlots; q += kEnsFin) {
+        uint32_t *slot = D.counts + (q / kSlots) * (kSlots * kSlotStride) + (q % kSlots) * kSlotStride;
+        const uint32_t c = *slot;
+        *slot = 0u;
+        if (q < 2 * kSlots) ca += c;
+        else cf += c;
+    }
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = 0.0;
+    long long ss = 0;
+    if (measure) {
+        for (uint32_t b = t; b < D.nblk; b += kEnsFin) {
+            const size_t gb = (size_t)D.blk_start + b;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] += E.part[gb * 8 + q];
+            ss += E.spart[gb];
+        }
+    }
+    double cad = (double)ca, cfd = (double)cf, ssd = (double)ss;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] += __shfl_xor(v[q], off, 64);
+        cad += __shfl_xor(cad, off, 64);
+        cfd += __shfl_xor(cfd, off, 64);
+        ssd += __shfl_xor(ssd, off, 64);
+    }
+    __shared__ double red[kEnsFin / 64][11];
+    const uint32_t lane = t & 63u, wid = t >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) red[wid][q] = v[q];
+        red[wid][8] = cad;
+        red[wid][9] = cfd;
+        red[wid][10] = ssd;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    double tot[11];
+#pragma unroll
+    for (int q = 0; q < 11; ++q) {
+        double a = 0.0;
+#pragma unroll
+        for (int w = 0; w < kEnsFin / 64; ++w) a += red[w][q];
+        tot[q] = a;
+    }
+    const uint32_t istep = *E.istep + istep_off;
+    const int s = E.rep_slot[r];
+    const double n3 = 2.0 * (double)D.G.nsub;
+    const double n3f = (double)(float)n3;
+    const double paccept = (double)(float)((double)(float)(int32_t)(long long)tot[8] / n3f);
+    double d = E.d[s];
+    if (E.adapt && istep < 10000) {
+        if (paccept < 0.4) d = d * 0.99500000476837158203125;
+        else if (paccept > 0.6) d = d / 0.99500000476837158203125;
+    }
+    E.d[s] = d;
+    if (measure) {
+        E.E[r] = tot[1];
+        const int row = *E.row;
+        if (row < E.max_rows) {
+            double *o = E.rows + ((size_t)row * E.nrep + r) * kEnsCols;
+            o[0] = (double)istep;
+            o[1] = E.kbt[s];
+            o[2] = tot[1] / n3;
+            o[3] = tot[10] / n3;
+            o[4] = tot[2] / n3;
+            o[5] = tot[3] / n3;
+            o[6] = tot[4] / n3;
+            o[7] = tot[5] / n3;
+            o[8] = tot[6] / n3;
+            o[9] = tot[7] / n3;
+            o[10] = tot[8] / n3;
+            o[11] = tot[9] / n3;
+            o[12] = d;
+            o[13] = (double)s;
+            o[14] = (double)(float)(tot[0] / n3f);
+            o[15] = 0.0;
+        }
+    }
+}
+
+// Parallel-tempering swaps, one thread per ladder, then advance the sweep and
+// row counters.  Round q = istep / K alternates even and odd slot pairs.
+__global__ void k_swap_ens(EnsDev E, int do_swap)
+{
+    const int l = threadIdx.x;
+    const uint32_t istep_last = *E.istep + E.K - 1;
+    __syncthreads();
+    if (do_swap && l < E.nlad) {
+        const uint32_t round = istep_last / E.K;
+        const int t0 = E.lad_start[l], t1 = E.lad_start[l + 1];
+        for (int t = t0 + (int)(round & 1u); t + 1 < t1; t += 2) {
+            const int a = E.slot_rep[t], b = E.slot_rep[t + 1];
+            const double delta = (E.inv_kbt[t] - E.inv_kbt[t + 1]) * (E.E[a] - E.E[b]);
+            const U4 w = philox4x32_10(U4{(uint32_t)t, round, E.run_index, 0x50540000u}, (uint32_t)E.seed,
+                                       (uint32_t)(E.seed >> 32));
+            const double u = u53(w.y, w.x);
+            E.swap_att[t] += 1ull;
+            if (delta >= 0.0 || u < exp(delta)) {
+                E.slot_rep[t] = b;
+                E.slot_rep[t + 1] = a;
+                E.rep_slot[a] = t + 1;
+                E.rep_slot[b] = t;
+                E.swap_acc[t] += 1ull;
+            }
+        }
+    }
+    __syncthreads();
+    if (l == 0) {
+        *E.istep = istep_last + 1;
+        *E.row = *E.row + 1;
+    }
+}
+
+}  // namespace chll
+
+using namespace chll;
+
+namespace {
+
+thread_local std::string e_err;
+
+int efail(const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    e_err = buf;
+    return -1;
+}
+
+#define ECHECK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) return efail("%s: %s", #x, hipGetErrorString(e_));       \
+    } while (0)
+
+}  // namespace
+
+struct chll_ensemble {
+    int nrep = 0, nlad = 0, bc = BC_OPEN, arith = ARITH_COMPAT, bs = 256;
+    std::vector<chll_replica_desc> rd;
+    std::vector<RepDesc> rep;       // host copy of descriptors
+    std::vector<int32_t> lad_start;
+    std::vector<uint32_t> order;    // replica order of blocks (largest first)
+    hipStream_t own = nullptr, stream = nullptr;
+    RepDesc *R = nullptr;
+    uint32_t *blk_rep = nullptr;
+    uint32_t nblocks = 0;
+    EnsDev E{};
+    double *stage = nullptr;
+    int32_t *stage_s = nullptr;
+    int *bad = nullptr;
+    size_t stage_n = 0;
+    int32_t rows_cap = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_K = 0, graph_bs = 0, graph_adapt = -1;
+    uint64_t graph_seed = 0;
+    uint32_t graph_run = 0;
+    double *pinned = nullptr;
+};
+
+namespace {
+
+template <class T>
+int ealloc(T **p, size_t n)
+{
+    *p = nullptr;
+    hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+    if (e != hipSuccess) return efail("hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+    return 0;
+}
+
+void drop_graph(chll_ensemble *X)
+{
+    if (X->gexec) (void)hipGraphExecDestroy(X->gexec);
+    if (X->graph) (void)hipGraphDestroy(X->graph);
+    X->gexec = nullptr;
+    X->graph = nullptr;
+    X->graph_K = 0;
+}
+
+// (Re)build the block table and descriptors for block size X->bs.
+int build_tables(chll_ensemble *X)
+{
+    std::vector<uint32_t> table;
+    uint32_t start = 0;
+    for (uint32_t r : X->order) {
+        RepDesc &D = X->rep[r];
+        D.nblk = (D.G.nsub + X->bs - 1) / X->bs;
+        D.blk_start = start;
+        if (D.nblk >= (1u << 24)) return efail("replica %u too large for the block table", r);
+        for (uint32_t b = 0; b < D.nblk; ++b) table.push_back((r << 24) | b);
+        start += D.nblk;
+    }
+    X->nblocks = start;
+    if (X->blk_rep) (void)hipFree(X->blk_rep);
+    X->blk_rep = nullptr;
+    if (X->E.part) (void)hipFree(X->E.part);
+    if (X->E.spart) (void)hipFree(X->E.spart);
+    X->E.part = nullptr;
+    X->E.spart = nullptr;
+    if (ealloc(&X->blk_rep, table.size()) || ealloc(&X->E.part, (size_t)X->nblocks * 8) ||
+        ealloc(&X->E.spart, X->nblocks))
+        return -1;
+    ECHECK(hipMemcpy(X->blk_rep, table.data(), table.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    ECHECK(hipMemcpy(X->R, X->rep.data(), X->rep.size() * sizeof(RepDesc), hipMemcpyHostToDevice));
+    X->E.R = X->R;
+    X->E.blk_rep = X->blk_rep;
+    drop_graph(X);
+    return 0;
+}
+
+template <int ARITH, int BC, int BS>
+void launch_block_t(chll_ensemble *X, int K, hipStream_t st)
+{
+    const EnsDev &E = X->E;
+    for (int t = 0; t < K; ++t) {
+        for (int ph = 0; ph < 4; ++ph)
+            hipLaunchKernelGGL((k_phase_ens<ARITH, BC, BS>), dim3(X->nblocks), dim3(BS), 0, st, E, ph >> 1, ph & 1,
+                               (uint32_t)t);
+        if (t + 1 == K) hipLaunchKernelGGL((k_measure_ens<BC, BS>), dim3(X->nblocks), dim3(BS), 0, st, E);
+        hipLaunchKernelGGL(k_finalize_ens, dim3(X->nrep), dim3(kEnsFin), 0, st, E, (uint32_t)t, t + 1 == K ? 1 : 0);
+    }
+    hipLaunchKernelGGL(k_swap_ens, dim3(1), dim3(256), 0, st, E, 1);
+}
+
+template <int ARITH, int BC>
+int launch_block_bs(chll_ensemble *X, int K, hipStream_t st)
+{
+    switch (X->bs) {
+    case 64: launch_block_t<ARITH, BC, 64>(X, K, st); break;
+    case 128: launch_block_t<ARITH, BC, 128>(X, K, st); break;
+    case 256: launch_block_t<ARITH, BC, 256>(X, K, st); break;
+    case 512: launch_block_t<ARITH, BC, 512>(X, K, st); break;
+    default: return efail("unsupported block size %d", X->bs);
+    }
+    return 0;
+}
+
+// Enqueue one block of K sweeps (+ measurement + swap).
+int launch_block(chll_ensemble *X, int K, hipStream_t st)
+{
+    int rc;
+    if (X->arith == ARITH_COMPAT) rc = X->bc == BC_OPEN ? launch_block_bs<ARITH_COMPAT, BC_OPEN>(X, K, st)
+                                                        : launch_block_bs<ARITH_COMPAT, BC_PERIODIC>(X, K, st);
+    else rc = X->bc == BC_OPEN ? launch_block_bs<ARITH_FP64, BC_OPEN>(X, K, st)
+                               : launch_block_bs<ARITH_FP64, BC_PERIODIC>(X, K, st);
+    if (rc) return rc;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return efail("ensemble launch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *chll_ensemble_last_error(void) { return e_err.c_str(); }
+
+int chll_ensemble_create(const chll_replica_desc *reps, int32_t nrep, int32_t bc, int32_t arith, chll_ensemble **out)
+{
+    e_err.clear();
+    if (!reps || !out || nrep < 1 || nrep > 255) return efail("need 1..255 replicas");
+    if (bc != BC_OPEN && bc != BC_PERIODIC) return efail("unknown boundary condition %d", bc);
+    if (arith != ARITH_COMPAT && arith != ARITH_FP64) return efail("unknown arithmetic %d", arith);
+    *out = nullptr;
+    // validate ladders: contiguous, same lattice and Hamiltonian, ascending kT
+    std::vector<int32_t> lad_start;
+    for (int r = 0; r < nrep; ++r) {
+        const chll_replica_desc &a = reps[r];
+        if (a.ni < 2 || a.ni % 2 || a.nj < 1 || a.nk < 1)
+            return efail("replica %d: dims must have ni even >= 2 (got %d,%d,%d)", r, a.ni, a.nj, a.nk);
+        if (bc == BC_PERIODIC && ((a.nj % 2) || (a.nk % 2)))
+            return efail("replica %d: periodic boundaries need even nj, nk", r);
+        if ((uint64_t)a.ni * a.nj * a.nk + kFieldPad >= (1ull << 32)) return efail("replica %d too large", r);
+        if (!(a.kbt > 0.0)) return efail("replica %d: kbt must be positive", r);
+        if (r == 0 || a.ladder != reps[r - 1].ladder) {
+            for (int q = 0; q < r; ++q)
+                if (reps[q].ladder == a.ladder) return efail("replica %d: ladder %d is not contiguous", r, a.ladder);
+            lad_start.push_back(r);
+        } else {
+            const chll_replica_desc &p = reps[r - 1];
+            if (p.ni != a.ni || p.nj != a.nj || p.nk != a.nk || p.g != a.g || p.KK != a.KK)
+                return efail("replica %d: a ladder needs one lattice size, g and K", r);
+            if (!(a.kbt > p.kbt)) return efail("replica %d: kbt must ascend within a ladder", r);
+        }
+    }
+    lad_start.push_back(nrep);
+    chll_ensemble *X = new chll_ensemble();
+    X->nrep = nrep;
+    X->nlad = (int)lad_start.size() - 1;
+    X->bc = bc;
+    X->arith = arith;
+    X->rd.assign(reps, reps + nrep);
+    X->lad_start = lad_start;
+    X->rep.resize(nrep);
+    int rc = 0;
+    for (int r = 0; r < nrep && !rc; ++r) {
+        RepDesc &D = X->rep[r];
+        memset(&D, 0, sizeof D);
+        Geom &G = D.G;
+        G.ni = reps[r].ni; G.nj = reps[r].nj; G.nk = reps[r].nk;
+        G.half = G.ni / 2;
+        G.plane = G.half * G.nj;
+        G.nsub = (uint32_t)(((uint64_t)G.ni * G.nj * G.nk) / 2);
+        G.zero = 2 * G.nsub;
+        G.fd_half = make_fastdiv(G.half);
+        G.fd_nj = make_fastdiv(G.nj);
+        const size_t n = 2 * (size_t)G.nsub + kFieldPad;
+        rc |= ealloc(&D.F.nx, n) | ealloc(&D.F.ny, n) | ealloc(&D.F.nz, n) | ealloc(&D.F.s, n);
+        rc |= ealloc(&D.counts, (size_t)kCountWords);
+        if (!rc) {
+            rc |= hipMemset(D.F.nx, 0, n * sizeof(double)) != hipSuccess;
+            rc |= hipMemset(D.F.ny, 0, n * sizeof(double)) != hipSuccess;
+            rc |= hipMemset(D.F.nz, 0, n * sizeof(double)) != hipSuccess;
+            rc |= hipMemset(D.F.s, 0, n) != hipSuccess;
+            rc |= hipMemset(D.counts, 0, kCountWords * sizeof(uint32_t)) != hipSuccess;
+            if (rc) efail("hipMemset failed");
+        }
+        D.replica = (uint32_t)r;
+        D.g = reps[r].g;
+        D.KK = reps[r].KK;
+    }
+    // block order: largest replicas first
+    X->order.resize(nrep);
+    for (int r = 0; r < nrep; ++r) X->order[r] = r;
+    std::stable_sort(X->order.begin(), X->order.end(),
+                     [&](uint32_t a, uint32_t b) { return X->rep[a].G.nsub > X->rep[b].G.nsub; });
+    EnsDev &E = X->E;
+    E.nrep = nrep;
+    E.nlad = X->nlad;
+    rc = rc || ealloc(&X->R, nrep) || ealloc(&E.kbt, nrep) || ealloc(&E.inv_kbt, nrep) || ealloc(&E.d, nrep) ||
+         ealloc(&E.rep_slot, nrep) || ealloc(&E.slot_rep, nrep) || ealloc((int32_t **)&E.lad_start, X->nlad + 1) ||
+         ealloc(&E.E, nrep) || ealloc(&E.istep, 1) || ealloc(&E.row, 1) || ealloc(&E.swap_att, nrep) ||
+         ealloc(&E.swap_acc, nrep) || ealloc(&X->bad, 1);
+    if (!rc && hipHostMalloc((void **)&X->pinned, 16 * sizeof(double)) != hipSuccess) rc = efail("hipHostMalloc");
+    if (!rc && hipStreamCreate(&X->own) != hipSuccess) rc = efail("hipStreamCreate");
+    if (!rc) {
+        std::vector<double> kb(nrep), ikb(nrep), d(nrep);
+        std::vector<int32_t> ident(nrep);
+        for (int r = 0; r < nrep; ++r) {
+            kb[r] = reps[r].kbt;
+            ikb[r] = 1.0 / reps[r].kbt;
+            d[r] = reps[r].d0;
+            ident[r] = r;
+        }
+        uint32_t one = 1;
+        int32_t zero = 0;
+        std::vector<unsigned long long> z(nrep, 0ull);
+        rc |= hipMemcpy(E.kbt, kb.data(), nrep * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
+        rc |= hipMemcpy(E.inv_kbt, ikb.data(), nrep * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
+        rc |= hipMemcpy(E.d, d.data(), nrep * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
+        rc |= hipMemcpy(E.rep_slot, ident.data(), nrep * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess;
+        rc |= hipMemcpy(E.slot_rep, ident.data(), nrep * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess;
+        rc |= hipMemcpy((void *)E.lad_start, lad_start.data(), lad_start.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice) != hipSuccess;
+        rc |= hipMemcpy(E.istep, &one, sizeof one, hipMemcpyHostToDevice) != hipSuccess;
+        rc |= hipMemcpy(E.row, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess;
+        rc |= hipMemcpy(E.swap_att, z.data(), nrep * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess;
+        rc |= hipMemcpy(E.swap_acc, z.data(), nrep * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess;
+        if (rc) efail("initial copies failed");
+    }
+    if (!rc) rc = build_tables(X);
+    if (rc) {
+        std::string keep = e_err;
+        chll_ensemble_destroy(X);
+        e_err = keep;
+        return -1;
+    }
+    X->stream = X->own;
+    *out = X;
+    return 0;
+}
+
+int chll_ensemble_destroy(chll_ensemble *X)
+{
+    if (!X) return 0;
+    if (X->stream) (void)hipStreamSynchronize(X->stream);
+    drop_graph(X);
+    for (RepDesc &D : X->rep) {
+        (void)hipFree(D.F.nx); (void)hipFree(D.F.ny); (void)hipFree(D.F.nz); (void)hipFree(D.F.s);
+        (void)hipFree(D.counts);
+    }
+    EnsDev &E = X->E;
+    void *ptrs[] = {X->R, X->blk_rep, E.kbt, E.inv_kbt, E.d, E.rep_slot, E.slot_rep, (void *)E.lad_start, E.E,
+                    E.istep, E.row, E.swap_att, E.swap_acc, E.part, E.spart, E.rows, X->stage, X->stage_s, X->bad};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (X->pinned) (void)hipHostFree(X->pinned);
+    if (X->own) (void)hipStreamDestroy(X->own);
+    delete X;
+    return 0;
+}
+
+int chll_ensemble_set_stream(chll_ensemble *X, void *s)
+{
+    if (!X) return efail("null ensemble");
+    X->stream = s ? (hipStream_t)s : X->own;
+    drop_graph(X);
+    return 0;
+}
+
+int chll_ensemble_set_block(chll_ensemble *X, int32_t bs)
+{
+    if (!X) return efail("null ensemble");
+    if (bs != 64 && bs != 128 && bs != 256 && bs != 512) return efail("block size must be 64, 128, 256 or 512");
+    ECHECK(hipStreamSynchronize(X->stream));
+    if (bs == X->bs) return 0;
+    X->bs = bs;
+    return build_tables(X);
+}
+
+static int ens_stage(chll_ensemble *X, size_t n)
+{
+    if (n <= X->stage_n) return 0;
+    if (X->stage) (void)hipFree(X->stage);
+    if (X->stage_s) (void)hipFree(X->stage_s);
+    X->stage = nullptr;
+    X->stage_s = nullptr;
+    X->stage_n = 0;
+    if (ealloc(&X->stage, n) || ealloc(&X->stage_s, n)) return -1;
+    X->stage_n = n;
+    return 0;
+}
+
+int chll_ensemble_upload(chll_ensemble *X, int32_t r, const double *nx, const double *ny, const double *nz,
+                         const int32_t *s)
+{
+    if (!X || r < 0 || r >= X->nrep || !nx || !ny || !nz || !s) return efail("chll_ensemble_upload: bad argument");
+    const RepDesc &D = X->rep[r];
+    const size_t n3 = 2 * (size_t)D.G.nsub;
+    if (ens_stage(X, n3)) return -1;
+    const double *src[3] = {nx, ny, nz};
+    double *dst[3] = {D.F.nx, D.F.ny, D.F.nz};
+    for (int c = 0; c < 3; ++c) {
+        ECHECK(hipMemcpyAsync(X->stage, src[c], n3 * sizeof(double), hipMemcpyHostToDevice, X->stream));
+        launch_scatter_f64(D.G, X->stage, dst[c], X->stream);
+    }
+    ECHECK(hipMemsetAsync(X->bad, 0, sizeof(int), X->stream));
+    ECHECK(hipMemcpyAsync(X->stage_s, s, n3 * sizeof(int32_t), hipMemcpyHostToDevice, X->stream));
+    launch_scatter_spin(D.G, X->stage_s, D.F.s, X->bad, X->stream);
+    ECHECK(hipGetLastError());
+    ECHECK(hipMemcpyAsync(X->pinned, X->bad, sizeof(int), hipMemcpyDeviceToHost, X->stream));
+    ECHECK(hipStreamSynchronize(X->stream));
+    int bad;
+    memcpy(&bad, X->pinned, sizeof bad);
+    if (bad) return efail("spin values must lie in [-127, 127]");
+    return 0;
+}
+
+int chll_ensemble_download(chll_ensemble *X, int32_t r, double *nx, double *ny, double *nz, int32_t *s)
+{
+    if (!X || r < 0 || r >= X->nrep || !nx || !ny || !nz || !s) return efail("chll_ensemble_download: bad argument");
+    const RepDesc &D = X->rep[r];
+    const size_t n3 = 2 * (size_t)D.G.nsub;
+    if (ens_stage(X, n3)) return -1;
+    double *dst[3] = {nx, ny, nz};
+    const double *src[3] = {D.F.nx, D.F.ny, D.F.nz};
+    for (int c = 0; c < 3; ++c) {
+        launch_gather_f64(D.G, src[c], X->stage, X->stream);
+        ECHECK(hipMemcpyAsync(dst[c], X->stage, n3 * sizeof(double), hipMemcpyDeviceToHost, X->stream));
+    }
+    launch_gather_spin(D.G, D.F.s, X->stage_s, X->stream);
+    ECHECK(hipMemcpyAsync(s, X->stage_s, n3 * sizeof(int32_t), hipMemcpyDeviceToHost, X->stream));
+    ECHECK(hipGetLastError());
+    ECHECK(hipStreamSynchronize(X->stream));
+    return 0;
+}
+
+int chll_ensemble_run(chll_ensemble *X, uint64_t seed, uint32_t run_index, int32_t nblocks, int32_t K, int32_t adapt,
+                      int32_t use_graph, double *rows, int32_t max_rows)
+{
+    if (!X) return efail("null ensemble");
+    if (nblocks < 0 || K < 1) return efail("need nblocks >= 0 and sweeps per block K >= 1");
+    if (max_rows < 0) max_rows = 0;
+    hipStream_t st = X->stream;
+    EnsDev &E = X->E;
+    if (max_rows > X->rows_cap) {
+        ECHECK(hipStreamSynchronize(st));
+        if (E.rows) (void)hipFree(E.rows);
+        E.rows = nullptr;
+        X->rows_cap = 0;
+        if (ealloc(&E.rows, (size_t)max_rows * X->nrep * kEnsCols)) return -1;
+        X->rows_cap = max_rows;
+        drop_graph(X);
+    }
+    // a captured graph bakes in the kernel arguments: rebuild it when they change
+    if (X->graph && (X->graph_K != K || X->graph_bs != X->bs || X->graph_adapt != adapt || X->graph_seed != seed ||
+                     X->graph_run != run_index || E.max_rows != max_rows))
+        drop_graph(X);
+    E.seed = seed;
+    E.run_index = run_index;
+    E.K = (uint32_t)K;
+    E.adapt = adapt ? 1 : 0;
+    E.max_rows = max_rows;
+    const int32_t zero = 0;
+    memcpy(X->pinned, &zero, sizeof zero);
+    ECHECK(hipMemcpyAsync(E.row, X->pinned, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    if (use_graph && nblocks > 0 && !X->gexec) {
+        ECHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = launch_block(X, K, st);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return -1;
+        }
+        if (ce != hipSuccess) return efail("hipStreamEndCapture: %s", hipGetErrorString(ce));
+        X->graph = g;
+        ECHECK(hipGraphInstantiate(&X->gexec, g, nullptr, nullptr, 0));
+        X->graph_K = K;
+        X->graph_bs = X->bs;
+        X->graph_adapt = adapt;
+        X->graph_seed = seed;
+        X->graph_run = run_index;
+    }
+    for (int b = 0; b < nblocks; ++b) {
+        if (use_graph) ECHECK(hipGraphLaunch(X->gexec, st));
+        else if (launch_block(X, K, st)) return -1;
+    }
+    if (rows && max_rows > 0) {
+        const int32_t n = nblocks < max_rows ? nblocks : max_rows;
+        if (n > 0)
+            ECHECK(hipMemcpyAsync(rows, E.rows, (size_t)n * X->nrep * kEnsCols * sizeof(double),
+                                  hipMemcpyDeviceToHost, st));
+        ECHECK(hipStreamSynchronize(st));
+    }
+    return 0;
+}
+
+int chll_ensemble_sync(chll_ensemble *X)
+{
+    if (!X) return efail("null ensemble");
+    ECHECK(hipStreamSynchronize(X->stream));
+    return 0;
+}
+
+int chll_ensemble_state(chll_ensemble *X, int32_t *rep_slot, double *slot_kbt, double *slot_d, int64_t *swap_att,
+                        int64_t *swap_acc, uint32_t *istep)
+{
+    if (!X) return efail("null ensemble");
+    hipStream_t st = X->stream;
+    const EnsDev &E = X->E;
+    const size_t n = X->nrep;
+    ECHECK(hipStreamSynchronize(st));
+    if (rep_slot) ECHECK(hipMemcpy(rep_slot, E.rep_slot, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (slot_kbt) ECHECK(hipMemcpy(slot_kbt, E.kbt, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (slot_d) ECHECK(hipMemcpy(slot_d, E.d, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (swap_att) ECHECK(hipMemcpy(swap_att, E.swap_att, n * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (swap_acc) ECHECK(hipMemcpy(swap_acc, E.swap_acc, n * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (istep) ECHECK(hipMemcpy(istep, E.istep, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int32_t chll_ensemble_blocks(chll_ensemble *X)
+{
+    if (!X) return efail("null ensemble");
+    return (int32_t)X->nblocks;
+}
+
+}  // extern "C"

@@ -9,6 +9,7 @@
 // observable reductions into one row (chll/chll_mod.f90:91-105).  Nothing in a
 // run of sweeps waits for the host.
 #include "chll_kernels.h"
+#include "chll_site.h"
 
 namespace chll {
 
@@ -73,83 +74,14 @@ __global__ __launch_bounds__(kBlock) void k_gather_spin(Geom G, const int8_t *__
 }
 
 // ---------------------------------------------------------------------------
-// Block-wide sum of one 0/1 flag per thread, added to slot blk % kSlots.
-__device__ inline void block_count(bool flag, uint32_t *slots, uint32_t blk)
-{
-    __shared__ uint32_t wsum[kBlock / 64];
-    const uint64_t bal = __ballot(flag);
-    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-    if (lane == 0) wsum[wid] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = 0;
-#pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) t += wsum[w];
-        if (t) atomicAdd(&slots[(blk & (kSlots - 1)) * kSlotStride], t);
-    }
-}
-
-__device__ inline void load_nbrs(const Fields &F, uint32_t oth, const uint32_t nb[6], Nbr &N)
-{
-#pragma unroll
-    for (int b = 0; b < 6; ++b) {
-        const uint32_t q = oth + nb[b];
-        N.x[b] = F.nx[q];
-        N.y[b] = F.ny[q];
-        N.z[b] = F.nz[q];
-        N.s[b] = F.s[q];
-    }
-}
-
 // Director half-sweep over one colour (chll/chll_mod.f90:117-202).
 template <int ARITH, int RNG, int BC>
 __global__ __launch_bounds__(kBlock) void k_dir(Geom G, Fields F, int colour, PhaseArgs A,
                                                 uint32_t *__restrict__ counts)
 {
-    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t m = blk * kBlock + threadIdx.x;
-    bool acc = false;
-    if (m < G.nsub) {
-        const Site st = decode(G, m, colour);
-        uint32_t nb[6];
-        Nbr N;
-        N.mask = neighbours<BC>(G, m, st, nb);
-        const uint32_t own = colour ? G.nsub + m : m;
-        const uint32_t oth = colour ? 0u : G.nsub;
-        load_nbrs(F, oth, nb, N);
-        const double nx = F.nx[own], ny = F.ny[own], nz = F.nz[own];
-        const int sg = F.s[own];
-        const double d = *A.d;
-        double u_phi, u_acc;
-        if (RNG == RNG_PHILOX) {
-            const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour, 0);
-            u_phi = u53(w.y, w.x);
-            u_acc = u53(w.w, w.z);
-        } else {
-            u_phi = A.u_phi[m];
-            u_acc = A.u_acc[m];
-        }
-        double tx, ty, tz;
-        if (ARITH == ARITH_COMPAT) {
-            const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
-            trial_compat(nx, ny, nz, d, u_phi, tx, ty, tz);
-            const double enew = energy_compat(N, tx, ty, tz, sg, A.g, A.KK);
-            acc = (enew < eold) || (u_acc <= exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt)));
-        } else {
-            double c2o, c2n;
-            const double ao = cqf_sum(N, sg, nx, ny, nz, c2o);
-            trial_fp64(nx, ny, nz, d, u_phi, tx, ty, tz);
-            const double an = cqf_sum(N, sg, tx, ty, tz, c2n);
-            const double dE = (c2o - c2n) - A.KK * (an - ao);
-            acc = (dE < 0.0) || (u_acc <= exp(-dE * A.inv_kbt));
-        }
-        if (acc) {
-            F.nx[own] = tx;
-            F.ny[own] = ty;
-            F.nz[own] = tz;
-        }
-    }
-    block_count(acc, counts, blk);
+    const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
+    const bool acc = m < G.nsub && dir_site<ARITH, RNG, BC>(G, F, colour, A, m);
+    block_count<kBlock>(acc, counts, blockIdx.x);
 }
 
 // Chirality half-sweep over one colour (chll/chll_mod.f90:204-245).
@@ -157,158 +89,21 @@ template <int ARITH, int RNG, int BC>
 __global__ __launch_bounds__(kBlock) void k_spin(Geom G, Fields F, int colour, PhaseArgs A,
                                                  uint32_t *__restrict__ counts)
 {
-    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t m = blk * kBlock + threadIdx.x;
-    bool acc = false;
-    if (m < G.nsub) {
-        const Site st = decode(G, m, colour);
-        uint32_t nb[6];
-        Nbr N;
-        N.mask = neighbours<BC>(G, m, st, nb);
-        const uint32_t own = colour ? G.nsub + m : m;
-        const uint32_t oth = colour ? 0u : G.nsub;
-        load_nbrs(F, oth, nb, N);
-        const double nx = F.nx[own], ny = F.ny[own], nz = F.nz[own];
-        const int sg = F.s[own];
-        double u_acc;
-        if (RNG == RNG_PHILOX) {
-            const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour, 1);
-            u_acc = u53(w.y, w.x);
-        } else {
-            u_acc = A.u_acc[m];
-        }
-        if (ARITH == ARITH_COMPAT) {
-            const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
-            const double enew = energy_compat(N, nx, ny, nz, -sg, A.g, A.KK);
-            acc = (enew < eold) || (u_acc <= exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt)));
-        } else {
-            double c2;
-            const double a = cq_sum(N, nx, ny, nz, c2);
-            const double dE = (double)sg * fma(A.KK, a, 2.0 * A.g);
-            acc = (dE < 0.0) || (u_acc <= exp(-dE * A.inv_kbt));
-        }
-        if (acc) F.s[own] = (int8_t)(-sg);
-    }
-    block_count(acc, counts, blk);
+    const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
+    const bool acc = m < G.nsub && spin_site<ARITH, RNG, BC>(G, F, colour, A, m);
+    block_count<kBlock>(acc, counts, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
-// Observables.  Per site: the reference's reported energy term (periodic i
-// and j, open k, +y bond x-component from the diagonal site (i+1, j+1, k),
-// chll/chll_mod.f90:427-452), the Hamiltonian's own +x/+y/+z bonds under the
-// sweep's boundary condition, the six Q-tensor sums n_a n_b and the spin.
-__device__ inline uint32_t pidx(const Geom &G, uint32_t i, uint32_t j, uint32_t k)
-{
-    return (i >> 1) + G.half * (j + G.nj * k);
-}
-
-__device__ inline uint32_t colour_base(const Geom &G, uint32_t i, uint32_t j, uint32_t k)
-{
-    return ((i + j + k) & 1u) ? G.nsub : 0u;
-}
-
-__device__ inline double bond_term(double ax, double ay, double az, double bx, double by, double bz, int axis,
-                                   int sa, int sb, double KK)
-{
-    const double dd = ax * bx + ay * by + az * bz;
-    double cr;
-    if (axis == 0) cr = ay * bz - az * by;
-    else if (axis == 1) cr = az * bx - ax * bz;
-    else cr = ax * by - ay * bx;
-    const double sf = 0.5 * (double)(sa + sb);
-    return (1.0 - dd * dd) - KK * dd * cr * sf;
-}
-
+// Observables over the whole lattice (grid-stride), one partial per block.
 template <int BC>
 __global__ __launch_bounds__(kBlock) void k_measure(Geom G, Fields F, double g, double KK,
                                                     double *__restrict__ part, long long *__restrict__ spart)
 {
-    double v[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = 0.0;
-    long long ssum = 0;
-    for (uint32_t m = blockIdx.x * kBlock + threadIdx.x; m < G.nsub; m += gridDim.x * kBlock) {
-        const uint32_t row = fdiv(m, G.fd_half);
-        const uint32_t h = m - row * G.half;
-        const uint32_t k = fdiv(row, G.fd_nj);
-        const uint32_t j = row - k * G.nj;
-        const uint32_t jp = (j + 1 == G.nj) ? 0u : j + 1;
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const uint32_t i = 2u * h + (uint32_t)b;
-            const uint32_t ip = (i + 1 == G.ni) ? 0u : i + 1;
-            const uint32_t c = colour_base(G, i, j, k) + m;
-            const double ax = F.nx[c], ay = F.ny[c], az = F.nz[c];
-            const int sa = F.s[c];
-            // +x neighbour (periodic wrap in i)
-            const uint32_t qx = colour_base(G, ip, j, k) + pidx(G, ip, j, k);
-            const double bx0 = F.nx[qx], by0 = F.ny[qx], bz0 = F.nz[qx];
-            const int sx = F.s[qx];
-            // +y neighbour (periodic wrap in j) and the diagonal site's x component
-            const uint32_t qy = colour_base(G, i, jp, k) + pidx(G, i, jp, k);
-            const double bx1 = F.nx[qy], by1 = F.ny[qy], bz1 = F.nz[qy];
-            const int sy = F.s[qy];
-            const uint32_t qd = colour_base(G, ip, jp, k) + pidx(G, ip, jp, k);
-            const double dgx = F.nx[qd];
-            const double tx = bond_term(ax, ay, az, bx0, by0, bz0, 0, sa, sx, KK);
-            // reference's +y bond: dot uses nx of the diagonal site
-            double ety;
-            {
-                const double dd = ax * dgx + ay * by1 + az * bz1;
-                const double cr = az * bx1 - ax * bz1;
-                const double sf = 0.5 * (double)(sa + sy);
-                ety = (1.0 - dd * dd) - KK * dd * cr * sf;
-            }
-            const double ty = bond_term(ax, ay, az, bx1, by1, bz1, 1, sa, sy, KK);
-            double tz = 0.0, tzp = 0.0;
-            if (k + 1 < G.nk) {
-                const uint32_t qz = colour_base(G, i, j, k + 1) + pidx(G, i, j, k + 1);
-                tz = bond_term(ax, ay, az, F.nx[qz], F.ny[qz], F.nz[qz], 2, sa, F.s[qz], KK);
-                tzp = tz;
-            } else if (BC == BC_PERIODIC) {
-                const uint32_t qz = colour_base(G, i, j, 0) + pidx(G, i, j, 0);
-                tzp = bond_term(ax, ay, az, F.nx[qz], F.ny[qz], F.nz[qz], 2, sa, F.s[qz], KK);
-            }
-            const double gs = g * (double)sa;
-            v[0] += ((tx + ety) + tz) - gs;
-            const bool hx = (BC == BC_PERIODIC) || (i + 1 < G.ni);
-            const bool hy = (BC == BC_PERIODIC) || (j + 1 < G.nj);
-            v[1] += ((hx ? tx : 0.0) + (hy ? ty : 0.0) + tzp) - gs;
-            v[2] += ax * ax;
-            v[3] += ay * ay;
-            v[4] += az * az;
-            v[5] += ax * ay;
-            v[6] += ax * az;
-            v[7] += ay * az;
-            ssum += sa;
-        }
-    }
-    // deterministic block reduction: wave butterflies, then a fixed-order LDS pass
-    __shared__ double wv[kBlock / 64][8];
-    __shared__ long long ws[kBlock / 64];
-    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] += __shfl_xor(v[q], off, 64);
-        ssum += __shfl_xor(ssum, off, 64);
-    }
-    if (lane == 0) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) wv[wid][q] = v[q];
-        ws[wid] = ssum;
-    }
-    __syncthreads();
-    if (threadIdx.x < 8) {
-        double t = 0.0;
-        for (int w = 0; w < kBlock / 64; ++w) t += wv[w][threadIdx.x];
-        part[(size_t)blockIdx.x * 8 + threadIdx.x] = t;
-    }
-    if (threadIdx.x == 8) {
-        long long t = 0;
-        for (int w = 0; w < kBlock / 64; ++w) t += ws[w];
-        spart[blockIdx.x] = t;
-    }
+    MeasureAcc acc;
+    for (uint32_t m = blockIdx.x * kBlock + threadIdx.x; m < G.nsub; m += gridDim.x * kBlock)
+        measure_site<BC>(G, F, g, KK, m, acc);
+    block_reduce_measure<kBlock>(acc, part + (size_t)blockIdx.x * 8, spart + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,246 @@
+// chll_site.h -- per-site work of the sweep, shared by the single-lattice
+// kernels (chll_kernels.hip) and the batched replica kernels
+// (chll_ensemble.hip): one director trial, one chirality trial, one site pair's
+// observables, and the block reductions.
+#pragma once
+
+#include "chll_kernels.h"
+
+namespace chll {
+
+// Block-wide sum of one 0/1 flag per thread, added to slot blk % kSlots.
+template <int BS>
+__device__ inline void block_count(bool flag, uint32_t *slots, uint32_t blk)
+{
+    __shared__ uint32_t wsum[BS / 64];
+    const uint64_t bal = __ballot(flag);
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    if (lane == 0) wsum[wid] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < BS / 64; ++w) t += wsum[w];
+        if (t) atomicAdd(&slots[(blk & (kSlots - 1)) * kSlotStride], t);
+    }
+}
+
+__device__ inline void load_nbrs(const Fields &F, const uint32_t nb[6], Nbr &N)
+{
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const uint32_t q = nb[b];
+        N.x[b] = F.nx[q];
+        N.y[b] = F.ny[q];
+        N.z[b] = F.nz[q];
+        N.s[b] = F.s[q];
+    }
+}
+
+// One director trial at packed site m of `colour` (chll/chll_mod.f90:130-200);
+// returns whether it was accepted (and then stores the new director).
+template <int ARITH, int RNG, int BC>
+__device__ inline bool dir_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m)
+{
+    const Site st = decode(G, m, colour);
+    uint32_t nb[6];
+    Nbr N;
+    const uint32_t own = colour ? G.nsub + m : m;
+    const uint32_t oth = colour ? 0u : G.nsub;
+    N.mask = neighbours<BC>(G, m, st, oth, nb);
+    load_nbrs(F, nb, N);
+    const double nx = F.nx[own], ny = F.ny[own], nz = F.nz[own];
+    const int sg = F.s[own];
+    const double d = *A.d;
+    double u_phi, u_acc;
+    if (RNG == RNG_PHILOX) {
+        const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour, 0);
+        u_phi = u53(w.y, w.x);
+        u_acc = u53(w.w, w.z);
+    } else {
+        u_phi = A.u_phi[m];
+        u_acc = A.u_acc[m];
+    }
+    double tx, ty, tz;
+    bool acc;
+    if (ARITH == ARITH_COMPAT) {
+        const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
+        trial_compat(nx, ny, nz, d, u_phi, tx, ty, tz);
+        const double enew = energy_compat(N, tx, ty, tz, sg, A.g, A.KK);
+        acc = (enew < eold) || (u_acc <= exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt)));
+    } else {
+        double f[6];
+#pragma unroll
+        for (int b = 0; b < 6; ++b) f[b] = (b & 1) ? -0.5 * (double)(sg + N.s[b]) : 0.5 * (double)(sg + N.s[b]);
+        double c2o, c2n;
+        const double ao = cqf_sum(N, nx, ny, nz, f, c2o);
+        trial_fp64(nx, ny, nz, d, u_phi, tx, ty, tz);
+        const double an = cqf_sum(N, tx, ty, tz, f, c2n);
+        const double dE = (c2o - c2n) - A.KK * (an - ao);
+        acc = (dE < 0.0) || (u_acc <= exp(-dE * A.inv_kbt));
+    }
+    if (acc) {
+        F.nx[own] = tx;
+        F.ny[own] = ty;
+        F.nz[own] = tz;
+    }
+    return acc;
+}
+
+// One chirality trial at packed site m of `colour` (chll/chll_mod.f90:214-243).
+template <int ARITH, int RNG, int BC>
+__device__ inline bool spin_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m)
+{
+    const Site st = decode(G, m, colour);
+    uint32_t nb[6];
+    Nbr N;
+    const uint32_t own = colour ? G.nsub + m : m;
+    const uint32_t oth = colour ? 0u : G.nsub;
+    N.mask = neighbours<BC>(G, m, st, oth, nb);
+    load_nbrs(F, nb, N);
+    const double nx = F.nx[own], ny = F.ny[own], nz = F.nz[own];
+    const int sg = F.s[own];
+    double u_acc;
+    if (RNG == RNG_PHILOX) {
+        const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour, 1);
+        u_acc = u53(w.y, w.x);
+    } else {
+        u_acc = A.u_acc[m];
+    }
+    bool acc;
+    if (ARITH == ARITH_COMPAT) {
+        const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
+        const double enew = energy_compat(N, nx, ny, nz, -sg, A.g, A.KK);
+        acc = (enew < eold) || (u_acc <= exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt)));
+    } else {
+        const double a = cq_sum(N, nx, ny, nz);
+        const double dE = (double)sg * fma(A.KK, a, 2.0 * A.g);
+        acc = (dE < 0.0) || (u_acc <= exp(-dE * A.inv_kbt));
+    }
+    if (acc) F.s[own] = (int8_t)(-sg);
+    return acc;
+}
+
+// ---------------------------------------------------------------------------
+// Observables.  Per site: the reference's reported energy term (periodic i
+// and j, open k, +y bond x-component from the diagonal site (i+1, j+1, k),
+// chll/chll_mod.f90:427-452), the Hamiltonian's own +x/+y/+z bonds under the
+// sweep's boundary condition, the six Q-tensor sums n_a n_b and the spin.
+struct MeasureAcc {
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long s = 0;
+};
+
+__device__ inline uint32_t pidx(const Geom &G, uint32_t i, uint32_t j, uint32_t k)
+{
+    return (i >> 1) + G.half * (j + G.nj * k);
+}
+
+__device__ inline uint32_t colour_base(const Geom &G, uint32_t i, uint32_t j, uint32_t k)
+{
+    return ((i + j + k) & 1u) ? G.nsub : 0u;
+}
+
+__device__ inline double bond_term(double ax, double ay, double az, double bx, double by, double bz, int axis,
+                                   int sa, int sb, double KK)
+{
+    const double dd = ax * bx + ay * by + az * bz;
+    double cr;
+    if (axis == 0) cr = ay * bz - az * by;
+    else if (axis == 1) cr = az * bx - ax * bz;
+    else cr = ax * by - ay * bx;
+    const double sf = 0.5 * (double)(sa + sb);
+    return (1.0 - dd * dd) - KK * dd * cr * sf;
+}
+
+// Both sites (i = 2h and 2h+1) of packed index m.
+template <int BC>
+__device__ inline void measure_site(const Geom &G, const Fields &F, double g, double KK, uint32_t m, MeasureAcc &A)
+{
+    const uint32_t row = fdiv(m, G.fd_half);
+    const uint32_t h = m - row * G.half;
+    const uint32_t k = fdiv(row, G.fd_nj);
+    const uint32_t j = row - k * G.nj;
+    const uint32_t jp = (j + 1 == G.nj) ? 0u : j + 1;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        const uint32_t i = 2u * h + (uint32_t)b;
+        const uint32_t ip = (i + 1 == G.ni) ? 0u : i + 1;
+        const uint32_t c = colour_base(G, i, j, k) + m;
+        const double ax = F.nx[c], ay = F.ny[c], az = F.nz[c];
+        const int sa = F.s[c];
+        const uint32_t qx = colour_base(G, ip, j, k) + pidx(G, ip, j, k);
+        const double bx0 = F.nx[qx], by0 = F.ny[qx], bz0 = F.nz[qx];
+        const int sx = F.s[qx];
+        const uint32_t qy = colour_base(G, i, jp, k) + pidx(G, i, jp, k);
+        const double bx1 = F.nx[qy], by1 = F.ny[qy], bz1 = F.nz[qy];
+        const int sy = F.s[qy];
+        const uint32_t qd = colour_base(G, ip, jp, k) + pidx(G, ip, jp, k);
+        const double dgx = F.nx[qd];
+        const double tx = bond_term(ax, ay, az, bx0, by0, bz0, 0, sa, sx, KK);
+        // the reference's +y bond: its dot product takes nx of the diagonal site
+        double ety;
+        {
+            const double dd = ax * dgx + ay * by1 + az * bz1;
+            const double cr = az * bx1 - ax * bz1;
+            const double sf = 0.5 * (double)(sa + sy);
+            ety = (1.0 - dd * dd) - KK * dd * cr * sf;
+        }
+        const double ty = bond_term(ax, ay, az, bx1, by1, bz1, 1, sa, sy, KK);
+        double tz = 0.0, tzp = 0.0;
+        if (k + 1 < G.nk) {
+            const uint32_t qz = colour_base(G, i, j, k + 1) + pidx(G, i, j, k + 1);
+            tz = bond_term(ax, ay, az, F.nx[qz], F.ny[qz], F.nz[qz], 2, sa, F.s[qz], KK);
+            tzp = tz;
+        } else if (BC == BC_PERIODIC) {
+            const uint32_t qz = colour_base(G, i, j, 0) + pidx(G, i, j, 0);
+            tzp = bond_term(ax, ay, az, F.nx[qz], F.ny[qz], F.nz[qz], 2, sa, F.s[qz], KK);
+        }
+        const double gs = g * (double)sa;
+        A.v[0] += ((tx + ety) + tz) - gs;
+        const bool hx = (BC == BC_PERIODIC) || (i + 1 < G.ni);
+        const bool hy = (BC == BC_PERIODIC) || (j + 1 < G.nj);
+        A.v[1] += ((hx ? tx : 0.0) + (hy ? ty : 0.0) + tzp) - gs;
+        A.v[2] += ax * ax;
+        A.v[3] += ay * ay;
+        A.v[4] += az * az;
+        A.v[5] += ax * ay;
+        A.v[6] += ax * az;
+        A.v[7] += ay * az;
+        A.s += sa;
+    }
+}
+
+// Deterministic block reduction of MeasureAcc: wave butterflies, then a
+// fixed-order pass over the waves; writes 8 doubles to part and one int to spart.
+template <int BS>
+__device__ inline void block_reduce_measure(MeasureAcc &A, double *part, long long *spart)
+{
+    __shared__ double wv[BS / 64][8];
+    __shared__ long long ws[BS / 64];
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) A.v[q] += __shfl_xor(A.v[q], off, 64);
+        A.s += __shfl_xor(A.s, off, 64);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) wv[wid][q] = A.v[q];
+        ws[wid] = A.s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        double t = 0.0;
+        for (int w = 0; w < BS / 64; ++w) t += wv[w][threadIdx.x];
+        part[threadIdx.x] = t;
+    }
+    if (threadIdx.x == 8) {
+        long long t = 0;
+        for (int w = 0; w < BS / 64; ++w) t += ws[w];
+        *spart = t;
+    }
+}
+
+}  // namespace chll

@@ -133,6 +133,46 @@ int chll_lattice_sync(chll_lattice *lat);
 int chll_lattice_set_timing(chll_lattice *lat, int32_t on);
 int chll_lattice_timing(chll_lattice *lat, double *ms, int64_t *launches, int32_t nkinds);
 
+/* Batched replicas with parallel tempering ------------------------------- */
+/* No counterpart in the reference, which scans (kT, K, g) grids by running one
+ * ChLLSim after another (e.g. scripts/phase_diagram_large1.py:30-37). */
+
+#define CHLL_ENS_COLS 16
+/* ensemble row columns: 0 istep | 1 kT | 2 E_true/N | 3 mean spin | 4..9 <n_a n_b>
+ * (xx, yy, zz, xy, xz, yz) | 10 naccept/N | 11 nflip/N | 12 d | 13 temperature
+ * slot | 14 reference etot (binary32) | 15 reserved */
+
+typedef struct chll_replica_desc {
+    int32_t ni, nj, nk;
+    int32_t ladder; /* replicas of one ladder: same ni, nj, nk, g, KK; listed
+                       contiguously in ascending kbt */
+    double g, KK, kbt, d0;
+} chll_replica_desc;
+
+typedef struct chll_ensemble chll_ensemble;
+
+const char *chll_ensemble_last_error(void);
+int chll_ensemble_create(const chll_replica_desc *reps, int32_t nrep, int32_t bc, int32_t arith, chll_ensemble **out);
+int chll_ensemble_destroy(chll_ensemble *e);
+int chll_ensemble_set_stream(chll_ensemble *e, void *hip_stream);
+/* threads per block of the batched kernels: 64, 128, 256 or 512 */
+int chll_ensemble_set_block(chll_ensemble *e, int32_t block_size);
+int32_t chll_ensemble_blocks(chll_ensemble *e);
+int chll_ensemble_upload(chll_ensemble *e, int32_t rep, const double *nx, const double *ny, const double *nz,
+                         const int32_t *s);
+int chll_ensemble_download(chll_ensemble *e, int32_t rep, double *nx, double *ny, double *nz, int32_t *s);
+/* nblocks blocks of K sweeps; each block ends with a measurement row per
+ * replica and a parallel-tempering swap attempt in every ladder.  rows:
+ * [min(nblocks, max_rows)][nrep][CHLL_ENS_COLS]; NULL rows only enqueues.
+ * use_graph: capture one block in a hipGraph and replay it. */
+int chll_ensemble_run(chll_ensemble *e, uint64_t seed, uint32_t run_index, int32_t nblocks, int32_t K, int32_t adapt,
+                      int32_t use_graph, double *rows, int32_t max_rows);
+int chll_ensemble_sync(chll_ensemble *e);
+/* replica -> slot, slot kT and step, swap attempts/acceptances per slot pair
+ * (t, t+1), next sweep index; any pointer may be NULL */
+int chll_ensemble_state(chll_ensemble *e, int32_t *rep_slot, double *slot_kbt, double *slot_d, int64_t *swap_att,
+                        int64_t *swap_acc, uint32_t *istep);
+
 #ifdef __cplusplus
 }
 #endif

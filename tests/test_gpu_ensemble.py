@@ -1,0 +1,153 @@
+"""Batched replicas with parallel tempering (libchll_mod.so chll_ensemble_*).
+
+Parity: each replica's sweeps use the same site-keyed Philox counters as the
+single-lattice path (replica id = its index), so after every block of K sweeps
+each replica must equal the oracle run of those K sweeps at the temperature
+slot (kT, d) it held during the block -- bit-exact spins and counts, directors
+within 1e-12.  Swap decisions are recomputed from the oracle's energies with
+the parallel-tempering rule min(1, exp((1/kT_t - 1/kT_{t+1})(E_a - E_b))).
+
+hipGraph: the captured block is replayed several times with synchronisations
+and host work in between, and every replay is checked against eager launches.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from chllmodel_amd.ensemble import Ensemble, series_by_temperature
+from chllmodel_amd.lattice import Lattice
+from chllmodel_amd.states import isotropic_state
+from oracle.oracle import ARITH_COMPAT, ARITH_FP64, BC_OPEN, RNG_PHILOX, Oracle, copy_state
+
+pytestmark = pytest.mark.gpu
+
+
+def small_replicas():
+    reps = []
+    for kT in (0.3, 0.5, 0.9):
+        reps.append({"ni": 4, "nj": 4, "nk": 4, "ladder": 0, "g": 0.5, "KK": 1.0, "kbt": kT, "d0": 0.3})
+    for kT in (0.4, 0.6, 1.1, 1.6):
+        reps.append({"ni": 6, "nj": 4, "nk": 5, "ladder": 1, "g": 1.0, "KK": 0.7, "kbt": kT, "d0": 0.2})
+    reps.append({"ni": 8, "nj": 6, "nk": 4, "ladder": 2, "g": 0.0, "KK": 1.5, "kbt": 0.8, "d0": 0.25})
+    return reps
+
+
+@pytest.mark.parametrize("arith", ["compat", "fp64"])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_ensemble_matches_oracle_with_swaps(arith, use_graph):
+    orc = Oracle()
+    reps = small_replicas()
+    seed, run_index, K, nblocks = 0xC0FFEE1234, 2, 3, 5
+    n = len(reps)
+    states = [isotropic_state(r["ni"], r["nj"], r["nk"], 500 + i) for i, r in enumerate(reps)]
+    slot_kbt = np.array([r["kbt"] for r in reps])
+    slot_d = np.array([r["d0"] for r in reps])
+    rep_slot = np.arange(n)
+    lad_start = [0, 3, 7, 8]
+    oarith = ARITH_COMPAT if arith == "compat" else ARITH_FP64
+    print(f"ensemble parity arith={arith} graph={use_graph}", flush=True)
+    with Ensemble(reps, bc="open", arith=arith) as ens:
+        for r in range(n):
+            ens.upload(r, states[r])
+        for b in range(nblocks):
+            rows = ens.run(1, K, seed=seed, run_index=run_index, adapt=True, use_graph=use_graph)
+            st = ens.state()
+            # oracle: K sweeps per replica at the slot it held during the block
+            E = np.zeros(n)
+            for r in range(n):
+                s = rep_slot[r]
+                _, _, d_out, _ = orc.run(states[r], 1 + b * K, (b + 1) * K, 10 ** 6, reps[r]["g"], reps[r]["KK"],
+                                         slot_d[s], slot_kbt[s], rng=RNG_PHILOX, seed=seed, run_index=run_index,
+                                         replica=r, bc=BC_OPEN, arith=oarith)
+                slot_d[s] = d_out
+                E[r] = orc.energy_true(states[r], reps[r]["g"], reps[r]["KK"])
+                got = ens.download(r)
+                assert np.array_equal(got["s"], states[r]["s"]), (b, r)
+                err = max(np.abs(got[f] - states[r][f]).max() for f in ("nx", "ny", "nz"))
+                assert err <= 1e-12, (b, r, err)
+                N = reps[r]["ni"] * reps[r]["nj"] * reps[r]["nk"]
+                assert abs(rows[0, r, 2] - E[r] / N) < 1e-12
+                assert rows[0, r, 13] == s and rows[0, r, 1] == slot_kbt[s]
+            assert np.array_equal(st["slot_d"], slot_d)
+            # swap rule, round q = (istep of the block's last sweep) // K
+            rnd = ((b + 1) * K) // K
+            slot_rep = np.argsort(rep_slot)
+            for l in range(len(lad_start) - 1):
+                t0, t1 = lad_start[l], lad_start[l + 1]
+                for t in range(t0 + (rnd & 1), t1 - 1, 2):
+                    a, c = slot_rep[t], slot_rep[t + 1]
+                    delta = (1 / slot_kbt[t] - 1 / slot_kbt[t + 1]) * (E[a] - E[c])
+                    w = orc.philox([t, rnd, run_index, 0x50540000], [seed & 0xFFFFFFFF, seed >> 32])
+                    u = ((int(w[1]) << 32 | int(w[0])) >> 11) * 2.0 ** -53
+                    if delta >= 0 or u < math.exp(delta):
+                        slot_rep[t], slot_rep[t + 1] = c, a
+            rep_slot = np.argsort(slot_rep)
+            assert np.array_equal(st["rep_slot"], rep_slot), (b, st["rep_slot"], rep_slot)
+        assert st["istep"] == nblocks * K + 1
+
+
+def test_graph_replays_equal_eager():
+    reps = small_replicas()
+    outs = {}
+    for use_graph in (False, True):
+        with Ensemble(reps, arith="fp64") as ens:
+            ens.init_isotropic(77)
+            snaps = []
+            for b in range(4):
+                rows = ens.run(1, 4, seed=9, use_graph=use_graph)
+                _ = np.linalg.norm(rows)  # host work between replays
+                snaps.append((rows.copy(), ens.download(3), ens.state()["rep_slot"].copy()))
+            outs[use_graph] = snaps
+    for (ra, sa, pa), (rb, sb, pb) in zip(outs[False], outs[True]):
+        assert np.array_equal(ra, rb)
+        assert np.array_equal(pa, pb)
+        for f in ("nx", "ny", "nz", "s"):
+            assert np.array_equal(sa[f], sb[f])
+
+
+def test_single_replica_equals_lattice_path():
+    r = {"ni": 16, "nj": 12, "nk": 10, "ladder": 0, "g": 0.4, "KK": 1.2, "kbt": 0.7, "d0": 0.3}
+    st = isotropic_state(16, 12, 10, 5)
+    with Ensemble([r], arith="compat") as ens:
+        ens.upload(0, st)
+        ens.run(3, 5, seed=31, run_index=1)
+        a = ens.download(0)
+    with Lattice(16, 12, 10) as lat:
+        lat.upload_state(st)
+        lat.set_step(0.3)
+        lat.sweeps(1, 15, 0, g=0.4, KK=1.2, kbt=0.7, seed=31, run_index=1, replica=0)
+        b = lat.download()
+    for f in ("nx", "ny", "nz", "s"):
+        assert np.array_equal(a[f], b[f])
+
+
+def test_block_sizes_agree():
+    reps = small_replicas()
+    res = []
+    for bs in (64, 256, 512):
+        with Ensemble(reps, arith="compat") as ens:
+            ens.init_isotropic(3)
+            ens.set_block(bs)
+            rows = ens.run(3, 3, seed=4)
+            res.append((rows, ens.download(5)))
+    for rows, st in res[1:]:
+        assert np.array_equal(rows, res[0][0])
+        for f in ("nx", "ny", "nz", "s"):
+            assert np.array_equal(st[f], res[0][1][f])
+
+
+def test_k0_ladder_follows_tanh():
+    """K = 0 decouples spins from directors: <s> = tanh(g/kT) at every rung,
+    through swaps (a check of the swap bookkeeping on the observables)."""
+    reps = [{"ni": 16, "nj": 16, "nk": 16, "ladder": 0, "g": 0.3, "KK": 0.0, "kbt": kT, "d0": 0.3}
+            for kT in (0.5, 0.7, 1.0, 1.4)]
+    with Ensemble(reps, arith="fp64") as ens:
+        ens.init_isotropic(11)
+        rows = ens.run(400, 5, seed=21)
+    ser = series_by_temperature(rows[50:], reps)
+    N = 16 ** 3
+    for s, o in ser.items():
+        want = math.tanh(0.3 / o["kT"])
+        sig = math.sqrt((1 - want ** 2) / N / len(o["eta"]))
+        assert abs(o["eta"].mean() - want) < 5 * sig, (s, o["eta"].mean(), want)
