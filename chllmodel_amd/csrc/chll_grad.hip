@@ -1,0 +1,231 @@
+// chll_grad.hip -- lattice energy and its gradient (gfx950).
+//
+// E(n, s; eps, K, g) = sum over bonds (a, b = a + e_axis) of
+//     eps (1 - c^2) - K c q f,   c = n_a . n_b,  q = (n_a x n_b)_axis,
+//     f = (s_a + s_b) / 2,
+// minus g sum_a s_a -- the Hamiltonian the sweep samples (chll/chll_mod.f90:
+// 369-411 with each bond counted once; the reference has eps = 1).  Open
+// boundaries drop the bonds that leave the lattice, periodic ones wrap.
+//
+// One thread per site gathers the bond terms of its +x, +y, +z bonds (each bond
+// once, for E and the parameter derivatives) and the derivative of all six of
+// its bonds with respect to its own director:
+//   d t / d n_a = -2 eps c n_b - K f (q n_b + c dq/dn_a)       (a lower end)
+//   d t / d n_b = -2 eps c n_a - K f (q n_a + c dq/dn_b)       (b upper end)
+// with, for axis x, dq/dn_a = (0, n_bz, -n_by), dq/dn_b = (0, -n_az, n_ay) and
+// cyclically for y, z.  The gradient returned is the tangent projection
+// (I - n_a n_a^T) dE/dn_a.  No atomics: each block writes partial sums that a
+// second kernel reduces in a fixed order, so results are reproducible.
+//
+// Layout: plain Fortran order per lattice, field c of lattice b at
+// n + (b*3 + c)*N (i fastest), spins int8 at s + b*N.
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/chll_mod.h"
+#include "chll_device.h"
+
+namespace chll {
+
+constexpr int kGradBlock = 256;
+
+struct GradArgs {
+    const double *n;
+    const int8_t *s;
+    const double *K, *g, *eps;  // per lattice
+    double *dEdn;               // B x 3 x N, or null
+    double *part;               // B x nblk x 4: sum eps-term (1-c^2), sum c q f, sum s, unused
+    uint32_t ni, nj, nk, N, nblk;
+    FastDiv fd_ni, fd_nj;
+    int bc;
+};
+
+__device__ inline void dq_axis(int axis, double ax, double ay, double az, double bx, double by, double bz, double &q,
+                               double da[3], double db[3])
+{
+    if (axis == 0) {
+        q = ay * bz - az * by;
+        da[0] = 0.0; da[1] = bz; da[2] = -by;
+        db[0] = 0.0; db[1] = -az; db[2] = ay;
+    } else if (axis == 1) {
+        q = az * bx - ax * bz;
+        da[0] = -bz; da[1] = 0.0; da[2] = bx;
+        db[0] = az; db[1] = 0.0; db[2] = -ax;
+    } else {
+        q = ax * by - ay * bx;
+        da[0] = by; da[1] = -bx; da[2] = 0.0;
+        db[0] = -ay; db[1] = ax; db[2] = 0.0;
+    }
+}
+
+template <int BC>
+__global__ __launch_bounds__(kGradBlock) void k_energy_grad(GradArgs A)
+{
+    const uint32_t b = blockIdx.y;
+    const uint32_t l = blockIdx.x * kGradBlock + threadIdx.x;
+    const double *nx = A.n + (size_t)b * 3 * A.N, *ny = nx + A.N, *nz = ny + A.N;
+    const int8_t *s = A.s + (size_t)b * A.N;
+    const double K = A.K[b], eps = A.eps[b];
+    double sum_e = 0.0, sum_cqf = 0.0, sum_s = 0.0;
+    if (l < A.N) {
+        const uint32_t row = fdiv(l, A.fd_ni);
+        const uint32_t i = l - row * A.ni;
+        const uint32_t k = fdiv(row, A.fd_nj);
+        const uint32_t j = row - k * A.nj;
+        const double ax = nx[l], ay = ny[l], az = nz[l];
+        const int sa = s[l];
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        const uint32_t co[3] = {i, j, k};
+        const uint32_t dim[3] = {A.ni, A.nj, A.nk};
+        const uint32_t stride[3] = {1u, A.ni, A.ni * A.nj};
+#pragma unroll
+        for (int axis = 0; axis < 3; ++axis) {
+#pragma unroll
+            for (int dir = 0; dir < 2; ++dir) {  // dir 0: bond to the + neighbour (a lower), 1: to the - neighbour
+                uint32_t q;
+                bool ok;
+                if (dir == 0) {
+                    ok = co[axis] + 1 < dim[axis];
+                    q = ok ? l + stride[axis] : (BC == BC_PERIODIC ? l - (dim[axis] - 1) * stride[axis] : l);
+                    ok = ok || (BC == BC_PERIODIC);
+                } else {
+                    ok = co[axis] >= 1;
+                    q = ok ? l - stride[axis] : (BC == BC_PERIODIC ? l + (dim[axis] - 1) * stride[axis] : l);
+                    ok = ok || (BC == BC_PERIODIC);
+                }
+                if (!ok) continue;
+                const double bx = nx[q], by = ny[q], bz = nz[q];
+                const double f = 0.5 * (double)(sa + s[q]);
+                double qv, da[3], db[3];
+                double c;
+                if (dir == 0) {
+                    c = fma(az, bz, fma(ay, by, ax * bx));
+                    dq_axis(axis, ax, ay, az, bx, by, bz, qv, da, db);
+                    sum_e += 1.0 - c * c;
+                    sum_cqf += c * qv * f;
+                    const double w = -2.0 * eps * c, kf = -K * f;
+                    gx += w * bx + kf * (qv * bx + c * da[0]);
+                    gy += w * by + kf * (qv * by + c * da[1]);
+                    gz += w * bz + kf * (qv * bz + c * da[2]);
+                } else {
+                    // the bond (q, l): q is the lower end, l the upper end
+                    c = fma(az, bz, fma(ay, by, ax * bx));
+                    dq_axis(axis, bx, by, bz, ax, ay, az, qv, da, db);
+                    const double w = -2.0 * eps * c, kf = -K * f;
+                    gx += w * bx + kf * (qv * bx + c * db[0]);
+                    gy += w * by + kf * (qv * by + c * db[1]);
+                    gz += w * bz + kf * (qv * bz + c * db[2]);
+                }
+            }
+        }
+        sum_s = (double)sa;
+        if (A.dEdn) {
+            const double p = gx * ax + gy * ay + gz * az;
+            double *o = A.dEdn + (size_t)b * 3 * A.N;
+            o[l] = gx - p * ax;
+            o[A.N + l] = gy - p * ay;
+            o[2 * (size_t)A.N + l] = gz - p * az;
+        }
+    }
+    // deterministic block sums
+    __shared__ double red[kGradBlock / 64][3];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        sum_e += __shfl_xor(sum_e, off, 64);
+        sum_cqf += __shfl_xor(sum_cqf, off, 64);
+        sum_s += __shfl_xor(sum_s, off, 64);
+    }
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    if (lane == 0) { red[wid][0] = sum_e; red[wid][1] = sum_cqf; red[wid][2] = sum_s; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int w = 0; w < kGradBlock / 64; ++w) t += red[w][threadIdx.x];
+        A.part[((size_t)b * A.nblk + blockIdx.x) * 4 + threadIdx.x] = t;
+    }
+}
+
+// One block per lattice: E = eps S_e - K S_cqf - g S_s and the parameter
+// derivatives dE/deps = S_e, dE/dK = -S_cqf, dE/dg = -S_s.
+__global__ __launch_bounds__(256) void k_energy_final(GradArgs A, double *E, double *dK, double *dg, double *deps)
+{
+    const uint32_t b = blockIdx.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (uint32_t q = threadIdx.x; q < A.nblk; q += 256) {
+        const double *p = A.part + ((size_t)b * A.nblk + q) * 4;
+        v[0] += p[0]; v[1] += p[1]; v[2] += p[2];
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] += __shfl_xor(v[c], off, 64);
+    __shared__ double red[4][3];
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    if (lane == 0) { red[wid][0] = v[0]; red[wid][1] = v[1]; red[wid][2] = v[2]; }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double t[3];
+    for (int c = 0; c < 3; ++c) t[c] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+    const double K = A.K[b], g = A.g[b], eps = A.eps[b];
+    if (E) E[b] = eps * t[0] - K * t[1] - g * t[2];
+    if (deps) deps[b] = t[0];
+    if (dK) dK[b] = -t[1];
+    if (dg) dg[b] = -t[2];
+}
+
+}  // namespace chll
+
+using namespace chll;
+
+namespace {
+thread_local std::string g_gerr;
+int gfail(const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_gerr = buf;
+    return -1;
+}
+}  // namespace
+
+extern "C" {
+
+const char *chll_energy_last_error(void) { return g_gerr.c_str(); }
+
+size_t chll_energy_workspace_bytes(int32_t batch, int32_t ni, int32_t nj, int32_t nk)
+{
+    const uint64_t N = (uint64_t)ni * nj * nk;
+    const uint64_t nblk = (N + kGradBlock - 1) / kGradBlock;
+    return (size_t)batch * nblk * 4 * sizeof(double);
+}
+
+int chll_energy_grad(const double *n, const int8_t *s, int32_t batch, int32_t ni, int32_t nj, int32_t nk, int32_t bc,
+                     const double *K, const double *g, const double *eps, double *E, double *dEdn, double *dEdK,
+                     double *dEdg, double *dEdeps, void *workspace, size_t workspace_bytes, void *stream)
+{
+    g_gerr.clear();
+    if (!n || !s || !K || !g || !eps || !workspace) return gfail("chll_energy_grad: null argument");
+    if (batch < 1 || batch > 65535 || ni < 1 || nj < 1 || nk < 1) return gfail("chll_energy_grad: bad shape");
+    if (bc != BC_OPEN && bc != BC_PERIODIC) return gfail("chll_energy_grad: unknown boundary condition %d", bc);
+    const uint64_t N = (uint64_t)ni * nj * nk;
+    if (N >= (1ull << 32) / 3) return gfail("chll_energy_grad: lattice too large");
+    if (workspace_bytes < chll_energy_workspace_bytes(batch, ni, nj, nk)) return gfail("chll_energy_grad: workspace too small");
+    GradArgs A;
+    A.n = n; A.s = s; A.K = K; A.g = g; A.eps = eps; A.dEdn = dEdn; A.part = (double *)workspace;
+    A.ni = ni; A.nj = nj; A.nk = nk; A.N = (uint32_t)N; A.nblk = (uint32_t)((N + kGradBlock - 1) / kGradBlock);
+    A.fd_ni = make_fastdiv(ni); A.fd_nj = make_fastdiv(nj); A.bc = bc;
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid(A.nblk, batch);
+    if (bc == BC_OPEN) hipLaunchKernelGGL(k_energy_grad<BC_OPEN>, grid, dim3(kGradBlock), 0, st, A);
+    else hipLaunchKernelGGL(k_energy_grad<BC_PERIODIC>, grid, dim3(kGradBlock), 0, st, A);
+    hipLaunchKernelGGL(k_energy_final, dim3(batch), dim3(256), 0, st, A, E, dEdK, dEdg, dEdeps);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return gfail("chll_energy_grad launch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+}  // extern "C"

@@ -173,6 +173,22 @@ int chll_ensemble_sync(chll_ensemble *e);
 int chll_ensemble_state(chll_ensemble *e, int32_t *rep_slot, double *slot_kbt, double *slot_d, int64_t *swap_att,
                         int64_t *swap_acc, uint32_t *istep);
 
+/* Lattice energy and gradient ------------------------------------------ */
+/* No counterpart in the reference (its Hamiltonian, chll/chll_mod.f90:369-411,
+ * is only evaluated inside the Metropolis test).  Device pointers, batch of B
+ * lattices in plain Fortran order: director component c of lattice b at
+ * n + (b*3 + c)*N, int8 spins at s + b*N, per-lattice eps, K, g.  Computes
+ * E_b = sum_bonds [eps (1 - c^2) - K c q f] - g sum s (each bond once, open or
+ * periodic boundaries) and, for any non-NULL output, dE/dn projected on the
+ * tangent plane of each director (B x 3 x N), dE/dK, dE/dg, dE/deps (B each).
+ * workspace: device memory of chll_energy_workspace_bytes(); stream: HIP
+ * stream or NULL.  Enqueues only. */
+const char *chll_energy_last_error(void);
+size_t chll_energy_workspace_bytes(int32_t batch, int32_t ni, int32_t nj, int32_t nk);
+int chll_energy_grad(const double *n, const int8_t *s, int32_t batch, int32_t ni, int32_t nj, int32_t nk, int32_t bc,
+                     const double *K, const double *g, const double *eps, double *E, double *dEdn, double *dEdK,
+                     double *dEdg, double *dEdeps, void *workspace, size_t workspace_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,7 @@ def test_ensemble_matches_oracle_with_swaps(arith, use_graph):
             rep_slot = np.argsort(slot_rep)
             assert np.array_equal(st["rep_slot"], rep_slot), (b, st["rep_slot"], rep_slot)
         assert st["istep"] == nblocks * K + 1
+        assert st["swap_acc"].sum() > 0 and st["swap_acc"].sum() < st["swap_att"].sum()
 
 
 def test_graph_replays_equal_eager():
