@@ -130,9 +130,17 @@ def main():
         del src, dst
         log(f"bench: calibration copy of {args.copy_calib_mib} MiB done")
 
+    tuned = {}
+
     def time_single(arith, timing):
-        """W warm-up sweeps, then K timed sweeps bracketed by synchronisation."""
+        """Autotune the half-sweep kernels for this shape, then W warm-up sweeps
+        and K timed sweeps bracketed by synchronisation."""
+        from chllmodel_amd.autotune import apply, tune_lattice
+
+        tuned[arith] = tune_lattice(L, L, L, bc=args.bc, arith=arith, use_cache=False)
+        log(f"bench: {arith}: tuned director {tuned[arith]['director']} chirality {tuned[arith]['chirality']}")
         lat = Lattice(L, L, L, bc=args.bc)
+        apply(lat, tuned[arith], arith)
         lat.set_stream(stream.cuda_stream)
         lat.upload_state(st0)
         lat.set_step(args.d0)
@@ -209,6 +217,7 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "variants": variants,
+        "kernel_choice": {a: {k: t[k] for k in ("director", "chirality")} for a, t in tuned.items()},
         "cpu_baseline": None,
     }
     if not args.no_batched:

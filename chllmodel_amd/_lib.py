@@ -106,11 +106,14 @@ def lib() -> ct.CDLL:
     L.chll_lattice_sync.argtypes = [ct.c_void_p]
     L.chll_lattice_set_timing.argtypes = [ct.c_void_p, ct.c_int32]
     L.chll_lattice_timing.argtypes = [ct.c_void_p, _f64p, _i64p, ct.c_int32]
+    L.chll_lattice_set_kernel.argtypes = [ct.c_void_p] + [ct.c_int32] * 5
+    L.chll_lattice_get_kernel.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32, _i32p, _i32p, _i32p]
     for name in ("chll_configure", "chll_set_seed", "chll_set_stream", "chll_set_uniform_buffer",
                  "chll_lattice_create", "chll_lattice_destroy", "chll_lattice_set_stream",
                  "chll_lattice_upload", "chll_lattice_download", "chll_lattice_set_step",
                  "chll_lattice_get_step", "chll_lattice_sweeps", "chll_lattice_measure", "chll_lattice_sync",
-                 "chll_lattice_set_timing", "chll_lattice_timing"):
+                 "chll_lattice_set_timing", "chll_lattice_timing", "chll_lattice_set_kernel",
+                 "chll_lattice_get_kernel"):
         getattr(L, name).restype = ct.c_int
     _lib = L
     return L

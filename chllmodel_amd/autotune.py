@@ -1,0 +1,88 @@
+"""Project autotuner for the single-lattice half-sweep kernels.
+
+For one lattice shape, boundary condition and arithmetic it times every
+candidate kernel (one site per thread, or the k-march with each tile shape and
+a few slab counts) for the director and the chirality half-sweeps separately,
+with HIP events on the launch stream, and applies the fastest.  Sweeping with
+any candidate produces the same lattice (the trial arithmetic and random
+counters do not depend on the kernel shape), so tuning on a scratch lattice of
+the same shape is safe.  Results are cached per (shape, bc, arith) in the same
+JSON file as the ensemble tuner (``CHLL_AUTOTUNE_CACHE``).
+"""
+from __future__ import annotations
+
+import json
+import os
+
+from .ensemble import CACHE
+from .lattice import Lattice
+from .states import isotropic_state
+
+
+def candidates(ni, nj, nk):
+    out = [("site", 0, 1)]
+    half = ni // 2
+    for tile, (th, tj) in enumerate(((64, 4), (32, 8), (16, 16))):
+        if th > 2 * half and tile < 2:
+            continue
+        tiles = -(-half // th) * -(-nj // tj)
+        for target in (512, 1024, 2048, 4096):
+            ns = max(1, min(nk, -(-target // tiles)))
+            if -(-nk // ns) < 4:
+                continue
+            c = ("march", tile, ns)
+            if c not in out:
+                out.append(c)
+    return out
+
+
+def tune_lattice(ni, nj, nk, *, bc="open", arith="fp64", sweeps=20, seed=7, use_cache=True, verbose=False):
+    """Returns {'director': (variant, tile, nslab), 'chirality': ..., 'us': {...}}."""
+    key = json.dumps({"lattice": [ni, nj, nk], "bc": bc, "arith": arith}, sort_keys=True)
+    cache = {}
+    if use_cache and os.path.exists(CACHE):
+        try:
+            with open(CACHE) as f:
+                cache = json.load(f)
+        except (OSError, ValueError):
+            cache = {}
+        if key in cache:
+            return cache[key]
+    res = {"us": {}}
+    with Lattice(ni, nj, nk, bc=bc) as lat:
+        lat.upload_state(isotropic_state(ni, nj, nk, seed))
+        best = {"director": (None, 1e30), "chirality": (None, 1e30)}
+        for cand in candidates(ni, nj, nk):
+            lat.set_kernel(*cand, ariths=arith)
+            lat.set_step(0.3)
+            p = lat.params(g=0.5, KK=1.0, kbt=0.7, arith=arith, seed=seed)
+            lat.sweeps(1, 2, 0, params=p, want_rows=False)
+            lat.set_timing(True)
+            lat.sweeps(3, 2 + sweeps, 0, params=p, want_rows=False)
+            t = lat.timing()
+            lat.set_timing(False)
+            for kind in ("director", "chirality"):
+                ms, n = t[kind]
+                us = 1e3 * ms / max(n, 1)
+                res["us"][f"{kind} {cand[0]} tile={cand[1]} nslab={cand[2]}"] = round(us, 2)
+                if us < best[kind][1]:
+                    best[kind] = (cand, us)
+                if verbose:
+                    print(f"autotune {ni}x{nj}x{nk} {arith} {kind} {cand}: {us:.1f} us", flush=True)
+    res["director"] = list(best["director"][0])
+    res["chirality"] = list(best["chirality"][0])
+    if use_cache:
+        cache[key] = res
+        try:
+            os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+            with open(CACHE, "w") as f:
+                json.dump(cache, f, indent=1)
+        except OSError:
+            pass
+    return res
+
+
+def apply(lat: Lattice, tuned: dict, arith: str) -> None:
+    for kind in ("director", "chirality"):
+        v, t, n = tuned[kind]
+        lat.set_kernel(v, t, n, kinds=kind, ariths=arith)

@@ -89,6 +89,9 @@ struct chll_lattice {
     long long *counts_dev = nullptr;
     size_t counts_cap = 0;
     double *pinned = nullptr;  // host: 8 doubles
+    // half-sweep kernel per [kind: director, chirality][arith: compat, fp64]:
+    // variant 0 = one site per thread, 1 = k-march through LDS; march tile, slabs
+    struct KCfg { int variant, tile; uint32_t nslab; } kcfg[2][2];
     bool timing = false;
     struct Ev { int kind; hipEvent_t a, b; };
     std::vector<Ev> ev;
@@ -190,7 +193,50 @@ int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lat
         return -1;
     }
     L->stream = L->own;
+    // Default kernels (MI355X measurements at L = 256): the k-march wins for both
+    // chirality modes and the FP64 director, the one-site kernel for the
+    // reference-arithmetic director.  March shape: the widest tile the rows
+    // fill, up to ~2048 blocks with slabs of at least 8 planes; lattices that
+    // cannot fill 512 such blocks keep the one-site kernels.
+    {
+        const int tile = G.half >= 64 ? 0 : (G.half >= 32 ? 1 : 2);
+        const uint32_t tiles = march_tiles(tile, G);
+        uint32_t ns = (2048u + tiles - 1) / tiles;
+        const uint32_t ns8 = G.nk / 8 ? G.nk / 8 : 1;
+        ns = ns < ns8 ? ns : ns8;
+        const bool thick = (G.nk + ns - 1) / ns >= 8 && tiles * ns >= 512;
+        for (int kind = 0; kind < 2; ++kind)
+            for (int ar = 0; ar < 2; ++ar) {
+                const bool march = thick && !(kind == 0 && ar == ARITH_COMPAT);
+                L->kcfg[kind][ar] = {march ? 1 : 0, tile, ns};
+            }
+    }
     *out = L;
+    return 0;
+}
+
+int chll_lattice_set_kernel(chll_lattice *L, int32_t kind, int32_t arith, int32_t variant, int32_t tile,
+                            int32_t nslab)
+{
+    if (!L) return fail("null lattice");
+    if (kind != 0 && kind != 1) return fail("kind must be 0 (director) or 1 (chirality)");
+    if (arith != ARITH_COMPAT && arith != ARITH_FP64) return fail("unknown arithmetic mode %d", arith);
+    if (variant != 0 && variant != 1) return fail("kernel variant must be 0 (site) or 1 (march)");
+    if (tile < 0 || tile > 2) return fail("march tile must be 0 (64x4), 1 (32x8) or 2 (16x16)");
+    if (nslab < 1) return fail("nslab must be >= 1");
+    L->kcfg[kind][arith] = {variant, tile, (uint32_t)nslab > L->G.nk ? L->G.nk : (uint32_t)nslab};
+    return 0;
+}
+
+int chll_lattice_get_kernel(chll_lattice *L, int32_t kind, int32_t arith, int32_t *variant, int32_t *tile,
+                            int32_t *nslab)
+{
+    if (!L) return fail("null lattice");
+    if ((kind != 0 && kind != 1) || (arith != ARITH_COMPAT && arith != ARITH_FP64)) return fail("bad kind/arith");
+    const auto &c = L->kcfg[kind][arith];
+    if (variant) *variant = c.variant;
+    if (tile) *tile = c.tile;
+    if (nslab) *nslab = (int32_t)c.nslab;
     return 0;
 }
 
@@ -390,8 +436,15 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
                 else { A.u_phi = nullptr; A.u_acc = L->ustream + (4 + colour) * ns; }
             }
             if (tbegin(L, kind == 0 ? KIND_DIR : KIND_SPIN, &ti)) return -1;
-            launch_phase(kind, p->arith, p->rng == RNG_PHILOX ? RNG_PHILOX : RNG_STREAM, L->bc, G, L->F, colour, A,
-                         L->counts + (size_t)ph * kSlots * kSlotStride, st);
+            const int rngk = p->rng == RNG_PHILOX ? RNG_PHILOX : RNG_STREAM;
+            uint32_t *slots = L->counts + (size_t)ph * kSlots * kSlotStride;
+            const auto &kc = L->kcfg[kind][p->arith];
+            if (kc.variant == 1) {
+                if (launch_phase_march(kind, p->arith, rngk, L->bc, kc.tile, G, L->F, colour, A, slots, kc.nslab, st))
+                    return fail("unknown march tile %d", kc.tile);
+            } else {
+                launch_phase(kind, p->arith, rngk, L->bc, G, L->F, colour, A, slots, st);
+            }
             if (launch_check() || tend(L, ti)) return -1;
         }
         const bool meas = nout > 0 && (istep % nout) == 0 && row < max_rows;

@@ -192,10 +192,20 @@ __host__ __device__ inline double div_by_recip(double x, double b, double y)
 }
 
 // Neighbour data gathered once per trial.
+// Neighbour data gathered once per trial into registers.  The energy
+// functions below take any neighbour accessor with get(b, x, y, z), spin(b)
+// and mask; the k-march kernels pass one that reads LDS at every use.
 struct Nbr {
     double x[6], y[6], z[6];
     int s[6];
     uint32_t mask;
+    __device__ void get(int b, double &mx, double &my, double &mz) const
+    {
+        mx = x[b];
+        my = y[b];
+        mz = z[b];
+    }
+    __device__ int spin(int b) const { return s[b]; }
 };
 
 // Reference-compatible site energy E(n, snew) (chll/chll_mod.f90:363-412):
@@ -203,19 +213,20 @@ struct Nbr {
 //   e = fl32( (e + (1 - c*c)) -/+ ((K*c)*q)*f ),  c = (nx*mx + ny*my) + nz*mz,
 //   f = 0.5 * (snew + s_b);
 // then e = fl32(e - g*snew).  Absent bonds are skipped.
-__device__ inline double energy_compat(const Nbr &N, double nx, double ny, double nz, int snew, double g,
-                                       double KK)
+template <class NB>
+__device__ inline double energy_compat(const NB &N, double nx, double ny, double nz, int snew, double g, double KK)
 {
     double e = 0.0;
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
-        const double mx = N.x[b], my = N.y[b], mz = N.z[b];
+        double mx, my, mz;
+        N.get(b, mx, my, mz);
         const double dott = nx * mx + ny * my + nz * mz;
         double cr;
         if (b < 2) cr = ny * mz - nz * my;
         else if (b < 4) cr = nz * mx - nx * mz;
         else cr = nx * my - ny * mx;
-        const double sfac = 0.5 * (double)(snew + N.s[b]);
+        const double sfac = 0.5 * (double)(snew + N.spin(b));
         double r;
         if ((b & 1) == 0) r = e + (1.0 - dott * dott) - KK * dott * cr * sfac;
         else r = e + (1.0 - dott * dott) + KK * dott * cr * sfac;
@@ -266,12 +277,14 @@ __device__ inline void trial_compat(double nnx, double nny, double nnz, double d
 //   dE = sum_b [ (c_b^2 - c'_b^2) - sgn_b K f_b (c'_b q'_b - c_b q_b) ].
 // Chirality flip sigma -> -sigma at fixed director:
 //   dE = sigma * (K * sum_b sgn_b c_b q_b + 2 g).
-__device__ inline double cq_sum(const Nbr &N, double nx, double ny, double nz)
+template <class NB>
+__device__ inline double cq_sum(const NB &N, double nx, double ny, double nz)
 {
     double acc = 0.0;
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
-        const double mx = N.x[b], my = N.y[b], mz = N.z[b];
+        double mx, my, mz;
+        N.get(b, mx, my, mz);
         const double c = fma(nz, mz, fma(ny, my, nx * mx));
         double q;
         if (b < 2) q = fma(ny, mz, -nz * my);
@@ -282,20 +295,24 @@ __device__ inline double cq_sum(const Nbr &N, double nx, double ny, double nz)
     return acc;
 }
 
-// Weighted sum for the director move: sum_b sgn_b f_b c_b q_b and sum_b c_b^2.
-// Absent bonds read the zero site (c = q = 0) and add exactly nothing.
-__device__ inline double cqf_sum(const Nbr &N, double nx, double ny, double nz, const double f[6], double &csq)
+// Weighted sum for the director move: sum_b sgn_b f_b c_b q_b (f_b = (sigma +
+// s_b)/2) and sum_b c_b^2.  Absent bonds read the zero site (c = q = 0) and add
+// exactly nothing.
+template <class NB>
+__device__ inline double cqf_sum(const NB &N, int sigma, double nx, double ny, double nz, double &csq)
 {
     double acc = 0.0, c2 = 0.0;
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
-        const double mx = N.x[b], my = N.y[b], mz = N.z[b];
+        double mx, my, mz;
+        N.get(b, mx, my, mz);
+        const double fb = (b & 1) ? -0.5 * (double)(sigma + N.spin(b)) : 0.5 * (double)(sigma + N.spin(b));
         const double c = fma(nz, mz, fma(ny, my, nx * mx));
         double q;
         if (b < 2) q = fma(ny, mz, -nz * my);
         else if (b < 4) q = fma(nz, mx, -nx * mz);
         else q = fma(nx, my, -ny * mx);
-        acc = fma(c * q, f[b], acc);
+        acc = fma(c * q, fb, acc);
         c2 = fma(c, c, c2);
     }
     csq = c2;

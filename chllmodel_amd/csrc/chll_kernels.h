@@ -56,6 +56,11 @@ void launch_gather_spin(const Geom &G, const int8_t *src, int32_t *dst, hipStrea
 // kind 0 = director, 1 = chirality
 void launch_phase(int kind, int arith, int rng, int bc, const Geom &G, const Fields &F, int colour,
                   const PhaseArgs &A, uint32_t *counts, hipStream_t s);
+// k-marching half-sweeps through LDS (chll_march.hip); tile 0: 64x4, 1: 32x8,
+// 2: 16x16 packed sites x rows; returns -1 for an unknown tile.
+int launch_phase_march(int kind, int arith, int rng, int bc, int tile, const Geom &G, const Fields &F, int colour,
+                       const PhaseArgs &A, uint32_t *counts, uint32_t nslab, hipStream_t s);
+uint32_t march_tiles(int tile, const Geom &G);
 void launch_measure(int bc, const Geom &G, const Fields &F, double g, double KK, double *part, long long *spart,
                     hipStream_t s);
 void launch_finalize(const FinArgs &A, hipStream_t s);

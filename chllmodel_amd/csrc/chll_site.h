@@ -37,21 +37,26 @@ __device__ inline void load_nbrs(const Fields &F, const uint32_t nb[6], Nbr &N)
     }
 }
 
-// One director trial at packed site m of `colour` (chll/chll_mod.f90:130-200);
-// returns whether it was accepted (and then stores the new director).
-template <int ARITH, int RNG, int BC>
-__device__ inline bool dir_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m)
-{
-    const Site st = decode(G, m, colour);
-    uint32_t nb[6];
+// A trial's inputs: the site's director and spin, its six neighbours (order
+// +x, -x, +y, -y, +z, -z, absent ones zero) and their presence mask.
+struct SiteIn {
     Nbr N;
-    const uint32_t own = colour ? G.nsub + m : m;
-    const uint32_t oth = colour ? 0u : G.nsub;
-    N.mask = neighbours<BC>(G, m, st, oth, nb);
-    load_nbrs(F, nb, N);
-    const double nx = F.nx[own], ny = F.ny[own], nz = F.nz[own];
-    const int sg = F.s[own];
-    const double d = *A.d;
+    double nx, ny, nz;
+    int sg;
+};
+
+// Keeps the compiler from reusing loaded neighbour values across it, so an
+// accessor that reads LDS reloads them instead of holding them in registers
+// through the trial move.
+__device__ inline void reload_barrier() { __asm__ volatile("" ::: "memory"); }
+
+// Director trial (chll/chll_mod.f90:130-200) with neighbour accessor N and the
+// site's director (nx, ny, nz) and spin sg; returns whether it is accepted and
+// the trial director in t[3].
+template <int ARITH, int RNG, class NB>
+__device__ inline bool dir_compute(const PhaseArgs &A, int colour, uint32_t m, const NB &N, double nx, double ny,
+                                   double nz, int sg, double d, double t[3])
+{
     double u_phi, u_acc;
     if (RNG == RNG_PHILOX) {
         const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour, 0);
@@ -61,45 +66,28 @@ __device__ inline bool dir_site(const Geom &G, const Fields &F, int colour, cons
         u_phi = A.u_phi[m];
         u_acc = A.u_acc[m];
     }
-    double tx, ty, tz;
-    bool acc;
     if (ARITH == ARITH_COMPAT) {
         const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
-        trial_compat(nx, ny, nz, d, u_phi, tx, ty, tz);
-        const double enew = energy_compat(N, tx, ty, tz, sg, A.g, A.KK);
-        acc = (enew < eold) || (u_acc <= exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt)));
+        trial_compat(nx, ny, nz, d, u_phi, t[0], t[1], t[2]);
+        reload_barrier();
+        const double enew = energy_compat(N, t[0], t[1], t[2], sg, A.g, A.KK);
+        return (enew < eold) || (u_acc <= exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt)));
     } else {
-        double f[6];
-#pragma unroll
-        for (int b = 0; b < 6; ++b) f[b] = (b & 1) ? -0.5 * (double)(sg + N.s[b]) : 0.5 * (double)(sg + N.s[b]);
         double c2o, c2n;
-        const double ao = cqf_sum(N, nx, ny, nz, f, c2o);
-        trial_fp64(nx, ny, nz, d, u_phi, tx, ty, tz);
-        const double an = cqf_sum(N, tx, ty, tz, f, c2n);
+        const double ao = cqf_sum(N, sg, nx, ny, nz, c2o);
+        trial_fp64(nx, ny, nz, d, u_phi, t[0], t[1], t[2]);
+        reload_barrier();
+        const double an = cqf_sum(N, sg, t[0], t[1], t[2], c2n);
         const double dE = (c2o - c2n) - A.KK * (an - ao);
-        acc = (dE < 0.0) || (u_acc <= exp(-dE * A.inv_kbt));
+        return (dE < 0.0) || (u_acc <= exp(-dE * A.inv_kbt));
     }
-    if (acc) {
-        F.nx[own] = tx;
-        F.ny[own] = ty;
-        F.nz[own] = tz;
-    }
-    return acc;
 }
 
-// One chirality trial at packed site m of `colour` (chll/chll_mod.f90:214-243).
-template <int ARITH, int RNG, int BC>
-__device__ inline bool spin_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m)
+// Chirality trial (chll/chll_mod.f90:214-243).
+template <int ARITH, int RNG, class NB>
+__device__ inline bool spin_compute(const PhaseArgs &A, int colour, uint32_t m, const NB &N, double nx, double ny,
+                                    double nz, int sg)
 {
-    const Site st = decode(G, m, colour);
-    uint32_t nb[6];
-    Nbr N;
-    const uint32_t own = colour ? G.nsub + m : m;
-    const uint32_t oth = colour ? 0u : G.nsub;
-    N.mask = neighbours<BC>(G, m, st, oth, nb);
-    load_nbrs(F, nb, N);
-    const double nx = F.nx[own], ny = F.ny[own], nz = F.nz[own];
-    const int sg = F.s[own];
     double u_acc;
     if (RNG == RNG_PHILOX) {
         const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour, 1);
@@ -107,17 +95,57 @@ __device__ inline bool spin_site(const Geom &G, const Fields &F, int colour, con
     } else {
         u_acc = A.u_acc[m];
     }
-    bool acc;
     if (ARITH == ARITH_COMPAT) {
         const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
         const double enew = energy_compat(N, nx, ny, nz, -sg, A.g, A.KK);
-        acc = (enew < eold) || (u_acc <= exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt)));
+        return (enew < eold) || (u_acc <= exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt)));
     } else {
         const double a = cq_sum(N, nx, ny, nz);
         const double dE = (double)sg * fma(A.KK, a, 2.0 * A.g);
-        acc = (dE < 0.0) || (u_acc <= exp(-dE * A.inv_kbt));
+        return (dE < 0.0) || (u_acc <= exp(-dE * A.inv_kbt));
     }
-    if (acc) F.s[own] = (int8_t)(-sg);
+}
+
+// Gather a site's inputs straight from the colour-split fields.
+template <int BC>
+__device__ inline uint32_t load_site(const Geom &G, const Fields &F, int colour, uint32_t m, SiteIn &S)
+{
+    const Site st = decode(G, m, colour);
+    uint32_t nb[6];
+    const uint32_t own = colour ? G.nsub + m : m;
+    const uint32_t oth = colour ? 0u : G.nsub;
+    S.N.mask = neighbours<BC>(G, m, st, oth, nb);
+    load_nbrs(F, nb, S.N);
+    S.nx = F.nx[own]; S.ny = F.ny[own]; S.nz = F.nz[own];
+    S.sg = F.s[own];
+    return own;
+}
+
+// One director trial at packed site m of `colour`, inputs read from the
+// fields; stores the new director when accepted.
+template <int ARITH, int RNG, int BC>
+__device__ inline bool dir_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m)
+{
+    SiteIn S;
+    const uint32_t own = load_site<BC>(G, F, colour, m, S);
+    double t[3];
+    const bool acc = dir_compute<ARITH, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg, *A.d, t);
+    if (acc) {
+        F.nx[own] = t[0];
+        F.ny[own] = t[1];
+        F.nz[own] = t[2];
+    }
+    return acc;
+}
+
+// One chirality trial at packed site m of `colour`.
+template <int ARITH, int RNG, int BC>
+__device__ inline bool spin_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m)
+{
+    SiteIn S;
+    const uint32_t own = load_site<BC>(G, F, colour, m, S);
+    const bool acc = spin_compute<ARITH, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg);
+    if (acc) F.s[own] = (int8_t)(-S.sg);
     return acc;
 }
 

@@ -66,6 +66,23 @@ class Lattice:
         """Launch on a caller-owned HIP stream (e.g. torch.cuda.Stream().cuda_stream)."""
         check(lib().chll_lattice_set_stream(self._h, ct.c_void_p(hip_stream_ptr or 0)))
 
+    KINDS = ("director", "chirality")
+
+    def set_kernel(self, variant: str = "march", tile: int = 0, nslab: int = 1, *, kinds=KINDS,
+                   ariths=("compat", "fp64")) -> None:
+        """Half-sweep kernel ('site' or 'march'; tile 0/1/2 = 64x4/32x8/16x16,
+        nslab slabs along k) for the given trial kinds and arithmetic modes."""
+        for kind in ([kinds] if isinstance(kinds, str) else kinds):
+            for ar in ([ariths] if isinstance(ariths, str) else ariths):
+                check(lib().chll_lattice_set_kernel(self._h, self.KINDS.index(kind), ARITH[ar],
+                                                    {"site": 0, "march": 1}[variant], int(tile), int(nslab)))
+
+    def get_kernel(self, kind: str = "director", arith: str = "fp64") -> tuple:
+        v, t, n = ct.c_int32(), ct.c_int32(), ct.c_int32()
+        check(lib().chll_lattice_get_kernel(self._h, self.KINDS.index(kind), ARITH[arith], ct.byref(v), ct.byref(t),
+                                            ct.byref(n)))
+        return ("site", "march")[v.value], t.value, n.value
+
     def sync(self) -> None:
         check(lib().chll_lattice_sync(self._h))
 

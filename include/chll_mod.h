@@ -128,6 +128,14 @@ int chll_lattice_sweeps(chll_lattice *lat, chll_params *p, int32_t nstart, int32
 /* Measure one row of the current state without sweeping (row-major, CHLL_ROW_COLS). */
 int chll_lattice_measure(chll_lattice *lat, double g, double KK, double *row);
 int chll_lattice_sync(chll_lattice *lat);
+/* Half-sweep kernel for one trial kind (0 director, 1 chirality) and
+ * arithmetic: variant 0 = one site per thread, 1 = k-march with an LDS ring
+ * of plane tiles; tile 0 = 64x4, 1 = 32x8, 2 = 16x16 packed sites x rows;
+ * nslab = slabs along k.  Results do not depend on the choice. */
+int chll_lattice_set_kernel(chll_lattice *lat, int32_t kind, int32_t arith, int32_t variant, int32_t tile,
+                            int32_t nslab);
+int chll_lattice_get_kernel(chll_lattice *lat, int32_t kind, int32_t arith, int32_t *variant, int32_t *tile,
+                            int32_t *nslab);
 /* Per-kernel HIP-event timing: kinds 0 director, 1 chirality, 2 measure,
  * 3 finalize, 4 uniform fill. */
 int chll_lattice_set_timing(chll_lattice *lat, int32_t on);

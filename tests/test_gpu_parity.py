@@ -223,3 +223,31 @@ def test_dropin_run_philox_run_counter(orc):
             assert_state_match(a, b)
     finally:
         _lib.check(L.chll_configure(0, 0, 0, 1))
+
+
+SHAPES = [((256 // 8, 24, 20), "open"), ((48, 40, 30), "periodic"), ((10, 7, 5), "open"), ((130, 6, 9), "open"),
+          ((18, 34, 4), "periodic")]
+
+
+@pytest.mark.parametrize("dims,bc", SHAPES, ids=[f"{'x'.join(map(str, d))}-{b}" for d, b in SHAPES])
+@pytest.mark.parametrize("arith", ["compat", "fp64"])
+def test_kernel_shapes_identical(dims, bc, arith):
+    """The one-site-per-thread kernels and every k-march tile / slab split give
+    the same lattice, counts and rows (partial tiles, odd open dimensions)."""
+    st0 = isotropic_state(*dims, 2000 + sum(dims))
+    ref = None
+    for variant, tile, nslab in [("site", 0, 1), ("march", 0, 1), ("march", 0, 3), ("march", 1, 2),
+                                 ("march", 2, 1), ("march", 2, 64)]:
+        print(f"kernel shapes dims={dims} bc={bc} {arith} {variant} tile={tile} nslab={nslab}", flush=True)
+        with Lattice(*dims, bc=bc) as lat:
+            lat.set_kernel(variant, tile, nslab)
+            lat.upload_state(st0)
+            lat.set_step(0.3)
+            out = lat.sweeps(1, 6, 3, g=0.4, KK=1.1, kbt=0.8, arith=arith, seed=77, want_counts=True)
+            got = (lat.download(), out["counts"], out["rows"])
+        if ref is None:
+            ref = got
+            continue
+        for f in ("nx", "ny", "nz", "s"):
+            assert np.array_equal(got[0][f], ref[0][f]), (variant, tile, nslab, f)
+        assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2])
