@@ -1,8 +1,8 @@
 """Project autotuner for the single-lattice half-sweep kernels.
 
 For one lattice shape, boundary condition and arithmetic it times every
-candidate kernel (one site per thread, or the k-march with each tile shape and
-a few slab counts) for the director and the chirality half-sweeps separately,
+candidate kernel (one site per thread in lattice order or in XCD row bands, or
+the k-march with each tile shape and a few slab counts) for the director and the chirality half-sweeps separately,
 with HIP events on the launch stream, and applies the fastest.  Sweeping with
 any candidate produces the same lattice (the trial arithmetic and random
 counters do not depend on the kernel shape), so tuning on a scratch lattice of
@@ -20,7 +20,7 @@ from .states import isotropic_state
 
 
 def candidates(ni, nj, nk):
-    out = [("site", 0, 1)]
+    out = [("site", 0, 1), ("site", 1, 1)]
     half = ni // 2
     for tile, (th, tj) in enumerate(((64, 4), (32, 8), (16, 16))):
         if th > 2 * half and tile < 2:
@@ -33,6 +33,8 @@ def candidates(ni, nj, nk):
             c = ("march", tile, ns)
             if c not in out:
                 out.append(c)
+        if th <= 2 * half or tile == 2:
+            out.append(("march", tile + 3, 1))
     return out
 
 
@@ -49,10 +51,13 @@ def tune_lattice(ni, nj, nk, *, bc="open", arith="fp64", sweeps=20, seed=7, use_
         if key in cache:
             return cache[key]
     res = {"us": {}}
+    st0 = isotropic_state(ni, nj, nk, seed)
     with Lattice(ni, nj, nk, bc=bc) as lat:
-        lat.upload_state(isotropic_state(ni, nj, nk, seed))
         best = {"director": (None, 1e30), "chirality": (None, 1e30)}
         for cand in candidates(ni, nj, nk):
+            # every candidate sweeps the same initial state: the cost of a sweep
+            # depends on how many trials are accepted
+            lat.upload_state(st0)
             lat.set_kernel(*cand, ariths=arith)
             lat.set_step(0.3)
             p = lat.params(g=0.5, KK=1.0, kbt=0.7, arith=arith, seed=seed)

@@ -90,7 +90,8 @@ struct chll_lattice {
     size_t counts_cap = 0;
     double *pinned = nullptr;  // host: 8 doubles
     // half-sweep kernel per [kind: director, chirality][arith: compat, fp64]:
-    // variant 0 = one site per thread, 1 = k-march through LDS; march tile, slabs
+    // variant 0 = one site per thread (tile = block order: 0 lattice, 1 XCD
+    // row bands), 1 = k-march through LDS (tile shape, slabs)
     struct KCfg { int variant, tile; uint32_t nslab; } kcfg[2][2];
     bool timing = false;
     struct Ev { int kind; hipEvent_t a, b; };
@@ -193,11 +194,12 @@ int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lat
         return -1;
     }
     L->stream = L->own;
-    // Default kernels (MI355X measurements at L = 256): the k-march wins for both
-    // chirality modes and the FP64 director, the one-site kernel for the
-    // reference-arithmetic director.  March shape: the widest tile the rows
-    // fill, up to ~2048 blocks with slabs of at least 8 planes; lattices that
-    // cannot fill 512 such blocks keep the one-site kernels.
+    // Default kernels (MI355X measurements at L = 256 and 128): the one-site
+    // kernel for the director trials, whose register budget a carried plane
+    // ring would halve; the k-march for the chirality trials.  March shape: the
+    // widest tile the rows fill, up to ~2048 blocks with slabs of at least 8
+    // planes; lattices that cannot fill 512 such blocks keep the one-site
+    // kernels.
     {
         const int tile = G.half >= 64 ? 0 : (G.half >= 32 ? 1 : 2);
         const uint32_t tiles = march_tiles(tile, G);
@@ -207,8 +209,8 @@ int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lat
         const bool thick = (G.nk + ns - 1) / ns >= 8 && tiles * ns >= 512;
         for (int kind = 0; kind < 2; ++kind)
             for (int ar = 0; ar < 2; ++ar) {
-                const bool march = thick && !(kind == 0 && ar == ARITH_COMPAT);
-                L->kcfg[kind][ar] = {march ? 1 : 0, tile, ns};
+                const bool march = thick && kind == 1;
+                L->kcfg[kind][ar] = {march ? 1 : 0, march ? tile : 1, ns};
             }
     }
     *out = L;
@@ -222,7 +224,9 @@ int chll_lattice_set_kernel(chll_lattice *L, int32_t kind, int32_t arith, int32_
     if (kind != 0 && kind != 1) return fail("kind must be 0 (director) or 1 (chirality)");
     if (arith != ARITH_COMPAT && arith != ARITH_FP64) return fail("unknown arithmetic mode %d", arith);
     if (variant != 0 && variant != 1) return fail("kernel variant must be 0 (site) or 1 (march)");
-    if (tile < 0 || tile > 2) return fail("march tile must be 0 (64x4), 1 (32x8) or 2 (16x16)");
+    if (variant == 0 && (tile < 0 || tile > 1))
+        return fail("site kernel order must be 0 (lattice order) or 1 (XCD row bands)");
+    if (tile < 0 || tile > 5) return fail("tile must be 0..2 (k-march 64x4, 32x8, 16x16) or 3..5 (plane tiles)");
     if (nslab < 1) return fail("nslab must be >= 1");
     L->kcfg[kind][arith] = {variant, tile, (uint32_t)nslab > L->G.nk ? L->G.nk : (uint32_t)nslab};
     return 0;
@@ -443,7 +447,7 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
                 if (launch_phase_march(kind, p->arith, rngk, L->bc, kc.tile, G, L->F, colour, A, slots, kc.nslab, st))
                     return fail("unknown march tile %d", kc.tile);
             } else {
-                launch_phase(kind, p->arith, rngk, L->bc, G, L->F, colour, A, slots, st);
+                launch_phase(kind, p->arith, rngk, L->bc, G, L->F, colour, A, slots, kc.tile, st);
             }
             if (launch_check() || tend(L, ti)) return -1;
         }

@@ -53,9 +53,11 @@ void launch_scatter_f64(const Geom &G, const double *src, double *dst, hipStream
 void launch_gather_f64(const Geom &G, const double *src, double *dst, hipStream_t s);
 void launch_scatter_spin(const Geom &G, const int32_t *src, int8_t *dst, int *bad, hipStream_t s);
 void launch_gather_spin(const Geom &G, const int8_t *src, int32_t *dst, hipStream_t s);
-// kind 0 = director, 1 = chirality
+// kind 0 = director, 1 = chirality; one site per thread.  order 0: blocks in
+// lattice order; 1: XCD row bands (xcd_band blocks per band, identity when 0).
 void launch_phase(int kind, int arith, int rng, int bc, const Geom &G, const Fields &F, int colour,
-                  const PhaseArgs &A, uint32_t *counts, hipStream_t s);
+                  const PhaseArgs &A, uint32_t *counts, int order, hipStream_t s);
+uint32_t xcd_band(const Geom &G);
 // k-marching half-sweeps through LDS (chll_march.hip); tile 0: 64x4, 1: 32x8,
 // 2: 16x16 packed sites x rows; returns -1 for an unknown tile.
 int launch_phase_march(int kind, int arith, int rng, int bc, int tile, const Geom &G, const Fields &F, int colour,

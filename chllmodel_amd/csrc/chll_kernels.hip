@@ -75,11 +75,26 @@ __global__ __launch_bounds__(kBlock) void k_gather_spin(Geom G, const int8_t *__
 
 // ---------------------------------------------------------------------------
 // Director half-sweep over one colour (chll/chll_mod.f90:117-202).
+//
+// Workgroups go round-robin to the 8 XCDs (workgroup b to XCD b % 8), each with
+// its own L2.  With band > 0 the plane's blocks are cut into 8 bands of `band`
+// blocks (rows of the sublattice) and XCD x takes band x of every plane, planes
+// in order: a row's y neighbours then sit in the same L2 except at the band
+// edges, and the z neighbours of a band stay in that L2 while its next planes
+// run.  band = 0 keeps the identity order.
+__device__ inline uint32_t band_block(uint32_t b, uint32_t band)
+{
+    if (band == 0u) return b;
+    const uint32_t x = b & 7u, n = b >> 3;
+    const uint32_t p = n / band, r = n - p * band;
+    return (p * 8u + x) * band + r;
+}
+
 template <int ARITH, int RNG, int BC>
 __global__ __launch_bounds__(kBlock) void k_dir(Geom G, Fields F, int colour, PhaseArgs A,
-                                                uint32_t *__restrict__ counts)
+                                                uint32_t *__restrict__ counts, uint32_t band)
 {
-    const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t m = band_block(blockIdx.x, band) * kBlock + threadIdx.x;
     const bool acc = m < G.nsub && dir_site<ARITH, RNG, BC>(G, F, colour, A, m);
     block_count<kBlock>(acc, counts, blockIdx.x);
 }
@@ -87,9 +102,9 @@ __global__ __launch_bounds__(kBlock) void k_dir(Geom G, Fields F, int colour, Ph
 // Chirality half-sweep over one colour (chll/chll_mod.f90:204-245).
 template <int ARITH, int RNG, int BC>
 __global__ __launch_bounds__(kBlock) void k_spin(Geom G, Fields F, int colour, PhaseArgs A,
-                                                 uint32_t *__restrict__ counts)
+                                                 uint32_t *__restrict__ counts, uint32_t band)
 {
-    const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t m = band_block(blockIdx.x, band) * kBlock + threadIdx.x;
     const bool acc = m < G.nsub && spin_site<ARITH, RNG, BC>(G, F, colour, A, m);
     block_count<kBlock>(acc, counts, blockIdx.x);
 }
@@ -251,35 +266,44 @@ void launch_gather_spin(const Geom &G, const int8_t *src, int32_t *dst, hipStrea
     hipLaunchKernelGGL(k_gather_spin, grid1(G.nsub), dim3(kBlock), 0, s, G, src, dst);
 }
 
+uint32_t xcd_band(const Geom &G)
+{
+    // 8 equal bands of whole blocks per plane; every block then lies in one plane.
+    return G.plane % (8u * kBlock) == 0u ? G.plane / (8u * kBlock) : 0u;
+}
+
 template <int ARITH, int RNG, int BC>
 static void launch_phase_t(int kind, const Geom &G, const Fields &F, int colour, const PhaseArgs &A,
-                           uint32_t *counts, hipStream_t s)
+                           uint32_t *counts, uint32_t band, hipStream_t s)
 {
     if (kind == 0)
-        hipLaunchKernelGGL((k_dir<ARITH, RNG, BC>), grid1(G.nsub), dim3(kBlock), 0, s, G, F, colour, A, counts);
+        hipLaunchKernelGGL((k_dir<ARITH, RNG, BC>), grid1(G.nsub), dim3(kBlock), 0, s, G, F, colour, A, counts,
+                           band);
     else
-        hipLaunchKernelGGL((k_spin<ARITH, RNG, BC>), grid1(G.nsub), dim3(kBlock), 0, s, G, F, colour, A, counts);
+        hipLaunchKernelGGL((k_spin<ARITH, RNG, BC>), grid1(G.nsub), dim3(kBlock), 0, s, G, F, colour, A, counts,
+                           band);
 }
 
 void launch_phase(int kind, int arith, int rng, int bc, const Geom &G, const Fields &F, int colour,
-                  const PhaseArgs &A, uint32_t *counts, hipStream_t s)
+                  const PhaseArgs &A, uint32_t *counts, int order, hipStream_t s)
 {
+    const uint32_t band = order == 1 ? xcd_band(G) : 0u;
     const bool philox = (rng == RNG_PHILOX);
     if (arith == ARITH_COMPAT) {
         if (philox) {
-            if (bc == BC_OPEN) launch_phase_t<ARITH_COMPAT, RNG_PHILOX, BC_OPEN>(kind, G, F, colour, A, counts, s);
-            else launch_phase_t<ARITH_COMPAT, RNG_PHILOX, BC_PERIODIC>(kind, G, F, colour, A, counts, s);
+            if (bc == BC_OPEN) launch_phase_t<ARITH_COMPAT, RNG_PHILOX, BC_OPEN>(kind, G, F, colour, A, counts, band, s);
+            else launch_phase_t<ARITH_COMPAT, RNG_PHILOX, BC_PERIODIC>(kind, G, F, colour, A, counts, band, s);
         } else {
-            if (bc == BC_OPEN) launch_phase_t<ARITH_COMPAT, RNG_STREAM, BC_OPEN>(kind, G, F, colour, A, counts, s);
-            else launch_phase_t<ARITH_COMPAT, RNG_STREAM, BC_PERIODIC>(kind, G, F, colour, A, counts, s);
+            if (bc == BC_OPEN) launch_phase_t<ARITH_COMPAT, RNG_STREAM, BC_OPEN>(kind, G, F, colour, A, counts, band, s);
+            else launch_phase_t<ARITH_COMPAT, RNG_STREAM, BC_PERIODIC>(kind, G, F, colour, A, counts, band, s);
         }
     } else {
         if (philox) {
-            if (bc == BC_OPEN) launch_phase_t<ARITH_FP64, RNG_PHILOX, BC_OPEN>(kind, G, F, colour, A, counts, s);
-            else launch_phase_t<ARITH_FP64, RNG_PHILOX, BC_PERIODIC>(kind, G, F, colour, A, counts, s);
+            if (bc == BC_OPEN) launch_phase_t<ARITH_FP64, RNG_PHILOX, BC_OPEN>(kind, G, F, colour, A, counts, band, s);
+            else launch_phase_t<ARITH_FP64, RNG_PHILOX, BC_PERIODIC>(kind, G, F, colour, A, counts, band, s);
         } else {
-            if (bc == BC_OPEN) launch_phase_t<ARITH_FP64, RNG_STREAM, BC_OPEN>(kind, G, F, colour, A, counts, s);
-            else launch_phase_t<ARITH_FP64, RNG_STREAM, BC_PERIODIC>(kind, G, F, colour, A, counts, s);
+            if (bc == BC_OPEN) launch_phase_t<ARITH_FP64, RNG_STREAM, BC_OPEN>(kind, G, F, colour, A, counts, band, s);
+            else launch_phase_t<ARITH_FP64, RNG_STREAM, BC_PERIODIC>(kind, G, F, colour, A, counts, band, s);
         }
     }
 }

@@ -194,6 +194,112 @@ __global__ __launch_bounds__(TH * TJ) void k_march(Geom G, Fields F, int colour,
     }
 }
 
+
+// Single-plane tile variant: a block updates one TH x TJ tile of one plane.
+// The other colour's in-plane neighbours come from an LDS copy of the tile
+// and its one-site halo, loaded once by the block; the two z-neighbours and
+// the site itself are read directly.  No state is carried between planes, so
+// the kernel keeps the one-site kernel's register budget.
+template <int KIND, int ARITH, int RNG, int BC, int TH, int TJ>
+__global__ __launch_bounds__(TH * TJ) void k_tile(Geom G, Fields F, int colour, PhaseArgs A,
+                                                  uint32_t *__restrict__ counts, uint32_t tiles_x)
+{
+    constexpr int NT = TH * TJ;
+    constexpr int W = TH + 2, H = TJ + 2, NS = W * H;
+    constexpr int PER = (NS + NT - 1) / NT;
+    __shared__ double lx[NS], ly[NS], lz[NS];
+    __shared__ int8_t ls[NS];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const int h0 = (int)(tx * TH), j0 = (int)(ty * TJ);
+    const uint32_t k = blockIdx.y;
+    const uint32_t oth = colour ? 0u : G.nsub;
+    const uint32_t ownb = colour ? G.nsub : 0u;
+    const uint32_t hl = tid % TH, jl = tid / TH;
+    const uint32_t h = (uint32_t)h0 + hl, j = (uint32_t)j0 + jl;
+    const bool in = (h < G.half) && (j < G.nj);
+    const int half = (int)G.half, nj = (int)G.nj;
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int q = (int)tid + r * NT;
+        if (q < NS) {
+            int hh = h0 - 1 + q % W, jj = j0 - 1 + q / W;
+            bool ok;
+            if (BC == BC_PERIODIC) {
+                if (hh < 0) hh += half; else if (hh >= half) hh -= half;
+                if (hh >= half) hh = 0;
+                if (jj < 0) jj += nj; else if (jj >= nj) jj -= nj;
+                if (jj >= nj) jj = 0;
+                ok = true;
+            } else {
+                ok = hh >= 0 && hh < half && jj >= 0 && jj < nj;
+            }
+            const uint32_t idx = ok ? oth + (uint32_t)hh + G.half * ((uint32_t)jj + G.nj * k) : G.zero;
+            lx[q] = F.nx[idx];
+            ly[q] = F.ny[idx];
+            lz[q] = F.nz[idx];
+            ls[q] = F.s[idx];
+        }
+    }
+    __syncthreads();
+    bool acc = false;
+    if (in) {
+        const uint32_t p = (j + k + (uint32_t)colour) & 1u;
+        const int c0 = ((int)jl + 1) * W + (int)hl + 1, xp = c0 + (int)p;
+        const int q[4] = {xp, xp - 1, c0 + W, c0 - W};
+        Nbr N;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            N.x[b] = lx[q[b]];
+            N.y[b] = ly[q[b]];
+            N.z[b] = lz[q[b]];
+            N.s[b] = ls[q[b]];
+        }
+        const uint32_t m = h + G.half * (j + G.nj * k);
+        const uint32_t i = 2u * h + p;
+        // z-neighbours: the other colour at (h, j) of planes k+1 and k-1
+        uint32_t zp, zm;
+        if (BC == BC_PERIODIC) {
+            zp = oth + (k + 1 < G.nk ? m + G.plane : m - (G.nk - 1) * G.plane);
+            zm = oth + (k >= 1 ? m - G.plane : m + (G.nk - 1) * G.plane);
+            N.mask = 63u;
+        } else {
+            zp = k + 1 < G.nk ? oth + m + G.plane : G.zero;
+            zm = k >= 1 ? oth + m - G.plane : G.zero;
+            N.mask = (i + 1 < G.ni ? 1u : 0u) | (i >= 1 ? 2u : 0u) | (j + 1 < G.nj ? 4u : 0u) | (j >= 1 ? 8u : 0u) |
+                     (k + 1 < G.nk ? 16u : 0u) | (k >= 1 ? 32u : 0u);
+        }
+        N.x[4] = F.nx[zp]; N.y[4] = F.ny[zp]; N.z[4] = F.nz[zp]; N.s[4] = F.s[zp];
+        N.x[5] = F.nx[zm]; N.y[5] = F.ny[zm]; N.z[5] = F.nz[zm]; N.s[5] = F.s[zm];
+        const uint32_t own = ownb + m;
+        const double nx = F.nx[own], ny = F.ny[own], nz = F.nz[own];
+        const int sg = F.s[own];
+        if (KIND == 0) {
+            double t[3];
+            acc = dir_compute<ARITH, RNG>(A, colour, m, N, nx, ny, nz, sg, *A.d, t);
+            if (acc) {
+                F.nx[own] = t[0];
+                F.ny[own] = t[1];
+                F.nz[own] = t[2];
+            }
+        } else {
+            acc = spin_compute<ARITH, RNG>(A, colour, m, N, nx, ny, nz, sg);
+            if (acc) F.s[own] = (int8_t)(-sg);
+        }
+    }
+    block_count<NT>(acc, counts, blockIdx.x + blockIdx.y * gridDim.x);
+}
+
+template <int KIND, int ARITH, int RNG, int BC, int TH, int TJ>
+static void launch_tile_t(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t *counts,
+                          hipStream_t s)
+{
+    const uint32_t tiles_x = (G.half + TH - 1) / TH, tiles_y = (G.nj + TJ - 1) / TJ;
+    hipLaunchKernelGGL((k_tile<KIND, ARITH, RNG, BC, TH, TJ>), dim3(tiles_x * tiles_y, G.nk), dim3(TH * TJ), 0, s, G,
+                       F, colour, A, counts, tiles_x);
+}
+
 template <int KIND, int ARITH, int RNG, int BC, int TH, int TJ>
 static void launch_march_t(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t *counts,
                            uint32_t nslab, hipStream_t s)
@@ -215,6 +321,9 @@ static int launch_march_tile(int tile, const Geom &G, const Fields &F, int colou
     case 0: launch_march_t<KIND, ARITH, RNG, BC, 64, 4>(G, F, colour, A, counts, nslab, s); return 0;
     case 1: launch_march_t<KIND, ARITH, RNG, BC, 32, 8>(G, F, colour, A, counts, nslab, s); return 0;
     case 2: launch_march_t<KIND, ARITH, RNG, BC, 16, 16>(G, F, colour, A, counts, nslab, s); return 0;
+    case 3: launch_tile_t<KIND, ARITH, RNG, BC, 64, 4>(G, F, colour, A, counts, s); return 0;
+    case 4: launch_tile_t<KIND, ARITH, RNG, BC, 32, 8>(G, F, colour, A, counts, s); return 0;
+    case 5: launch_tile_t<KIND, ARITH, RNG, BC, 16, 16>(G, F, colour, A, counts, s); return 0;
     default: return -1;
     }
 }
@@ -245,7 +354,7 @@ int launch_phase_march(int kind, int arith, int rng, int bc, int tile, const Geo
 
 uint32_t march_tiles(int tile, const Geom &G)
 {
-    static const uint32_t TH[3] = {64, 32, 16}, TJ[3] = {4, 8, 16};
+    static const uint32_t TH[6] = {64, 32, 16, 64, 32, 16}, TJ[6] = {4, 8, 16, 4, 8, 16};
     return ((G.half + TH[tile] - 1) / TH[tile]) * ((G.nj + TJ[tile] - 1) / TJ[tile]);
 }
 

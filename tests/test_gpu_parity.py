@@ -226,18 +226,21 @@ def test_dropin_run_philox_run_counter(orc):
 
 
 SHAPES = [((256 // 8, 24, 20), "open"), ((48, 40, 30), "periodic"), ((10, 7, 5), "open"), ((130, 6, 9), "open"),
-          ((18, 34, 4), "periodic")]
+          ((18, 34, 4), "periodic"), ((64, 64, 8), "open"), ((128, 64, 6), "periodic")]
 
 
 @pytest.mark.parametrize("dims,bc", SHAPES, ids=[f"{'x'.join(map(str, d))}-{b}" for d, b in SHAPES])
 @pytest.mark.parametrize("arith", ["compat", "fp64"])
 def test_kernel_shapes_identical(dims, bc, arith):
-    """The one-site-per-thread kernels and every k-march tile / slab split give
-    the same lattice, counts and rows (partial tiles, odd open dimensions)."""
+    """The one-site-per-thread kernels in both block orders and every k-march
+    tile / slab split give the same lattice, counts and rows (partial tiles,
+    odd open dimensions; 64x64 and 128x64 planes split into XCD bands of 1 and
+    2 blocks)."""
     st0 = isotropic_state(*dims, 2000 + sum(dims))
     ref = None
-    for variant, tile, nslab in [("site", 0, 1), ("march", 0, 1), ("march", 0, 3), ("march", 1, 2),
-                                 ("march", 2, 1), ("march", 2, 64)]:
+    for variant, tile, nslab in [("site", 0, 1), ("site", 1, 1), ("march", 0, 1), ("march", 0, 3), ("march", 1, 2),
+                                 ("march", 2, 1), ("march", 2, 64), ("march", 3, 1), ("march", 4, 1),
+                                 ("march", 5, 1)]:
         print(f"kernel shapes dims={dims} bc={bc} {arith} {variant} tile={tile} nslab={nslab}", flush=True)
         with Lattice(*dims, bc=bc) as lat:
             lat.set_kernel(variant, tile, nslab)
