@@ -13,10 +13,11 @@ numbers; the reference-compatible arithmetic (binary32-rounded site energies)
 is timed beside it under "variants".  A second section times the 64-replica
 batched phase-diagram scan against the same replicas run one after another.
 
-Prints ONE JSON line.  The roofline object is for the dominant kernel, the
-director half-sweep, timed with HIP events on the stream it runs on; its
-algorithmic traffic is 37 B per site per launch (read all directors 24 B and
-all spins 1 B, write the active half's directors 12 B).  The CPU baseline is
+Prints ONE JSON line.  The roofline object is for the dominant kernel (the
+one with the largest total time: the director half-sweep, or with FP64 the
+fused colour-1 director + chirality launch), timed with HIP events on the
+stream it runs on, priced at its algorithmic traffic per site (BYTES below).
+The CPU baseline is
 the reference's own run() compiled from source (serial), timed on a bounded
 sample of the same workload.
 """
@@ -33,9 +34,17 @@ sys.path.insert(0, ROOT)
 
 METRIC = "site-updates/s at L=256 FP64 + 64-replica batched scan; % of HBM roofline"
 HBM_PEAK_GBS = 8000.0
-BYTES_DIR = 37.0    # director half-sweep, per lattice site per launch
-BYTES_SPIN = 25.5   # chirality half-sweep
-BYTES_SWEEP = 125.0  # per site-update, four half-sweeps
+# Algorithmic HBM bytes per lattice site per launch of each sweep kernel:
+# director half-sweep: read all directors 24 B and all spins 1 B, write the
+# active half's directors 12 B; chirality half-sweep: read all directors 24 B
+# and the active half's spins 0.5 B (FP64; the reference-compatible energies
+# also read the other half's spins, 0.5 B more), write those spins 0.5 B (upper
+# bound: every flip written); fused colour-1 director + chirality: the director
+# traffic plus the active half's spin writes 0.5 B.
+BYTES = {"director": 37.0, "chirality": 25.0, "fused": 37.5}
+BYTES_CHIRALITY_COMPAT = 25.5
+KERNEL_NAMES = {"director": "k_dir (director half-sweep)", "chirality": "chirality half-sweep",
+                "fused": "k_dir_spin (colour-1 director + chirality, fused)"}
 
 
 def log(msg):
@@ -175,20 +184,25 @@ def main():
     W, K = args.warmup, args.steps
     sups, wall, kernels = time_single(args.arith, not args.no_timing)
     roof = None
-    if "director" in kernels:
-        avg_s = kernels["director"]["avg_us"] * 1e-6
-        ach = BYTES_DIR * N / avg_s / 1e9
+    if args.arith == "compat":
+        BYTES["chirality"] = BYTES_CHIRALITY_COMPAT
+    sweep_bytes = None
+    sweep_kinds = [k for k in BYTES if k in kernels]
+    if sweep_kinds:
+        for k in sweep_kinds:
+            kernels[k]["GBps"] = BYTES[k] * N / (kernels[k]["avg_us"] * 1e-6) / 1e9
+        # bytes per site-update from the launches actually made
+        sweep_bytes = sum(BYTES[k] * kernels[k]["launches"] for k in sweep_kinds) / K
+        dom = max(sweep_kinds, key=lambda k: kernels[k]["total_ms"])
+        ach = kernels[dom]["GBps"]
         traffic = None
-        tf = os.path.join(ROOT, "profiles", f"director_L{L}_{args.bc}_{args.arith}_traffic.json")
+        tf = os.path.join(ROOT, "profiles", f"{dom}_L{L}_{args.bc}_{args.arith}_traffic.json")
         if os.path.exists(tf):
             with open(tf) as f:
                 traffic = json.load(f).get("hbm_bytes_per_launch")
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                "traffic": traffic, "kernel": "k_dir (director half-sweep)",
-                "algorithmic_bytes_per_launch": BYTES_DIR * N}
-        if "chirality" in kernels:
-            kernels["chirality"]["GBps"] = BYTES_SPIN * N / (kernels["chirality"]["avg_us"] * 1e-6) / 1e9
-        kernels["director"]["GBps"] = ach
+                "traffic": traffic, "kernel": KERNEL_NAMES[dom],
+                "algorithmic_bytes_per_launch": BYTES[dom] * N}
     variants = {}
     if not args.no_variants:
         other = "compat" if args.arith == "fp64" else "fp64"
@@ -213,11 +227,12 @@ def main():
                                f"d0={args.d0} nout={args.nout}",
                    "L": L, "sites": N, "bc": args.bc, "arith": args.arith, "rng": args.rng, "nout": args.nout},
         "sweeps_per_s": K / wall,
-        "sweep_hbm_frac": sups * BYTES_SWEEP / (HBM_PEAK_GBS * 1e9),
+        "sweep_bytes_per_site_update": sweep_bytes,
+        "sweep_hbm_frac": sups * sweep_bytes / (HBM_PEAK_GBS * 1e9) if sweep_bytes else None,
         "roofline": roof,
         "kernels": kernels,
         "variants": variants,
-        "kernel_choice": {a: {k: t[k] for k in ("director", "chirality")} for a, t in tuned.items()},
+        "kernel_choice": {a: {k: t.get(k) for k in ("director", "chirality", "fuse")} for a, t in tuned.items()},
         "cpu_baseline": None,
     }
     if not args.no_batched:

@@ -23,7 +23,7 @@ BC = {"open": 0, "periodic": 1}
 ROW_COLS = 16
 ROW_NAMES = ("istep", "E", "eta", "paccept", "faccept", "d", "E_true", "eta_exact",
              "Qxx", "Qyy", "Qzz", "Qxy", "Qxz", "Qyz", "naccept", "nflip")
-TIMING_KINDS = ("director", "chirality", "measure", "finalize", "fill")
+TIMING_KINDS = ("director", "chirality", "measure", "finalize", "fill", "fused")
 
 _f64p = ct.POINTER(ct.c_double)
 _i32p = ct.POINTER(ct.c_int32)
@@ -108,12 +108,14 @@ def lib() -> ct.CDLL:
     L.chll_lattice_timing.argtypes = [ct.c_void_p, _f64p, _i64p, ct.c_int32]
     L.chll_lattice_set_kernel.argtypes = [ct.c_void_p] + [ct.c_int32] * 5
     L.chll_lattice_get_kernel.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32, _i32p, _i32p, _i32p]
+    L.chll_lattice_set_fusion.argtypes = [ct.c_void_p, ct.c_int32]
+    L.chll_lattice_get_fusion.argtypes = [ct.c_void_p, _i32p]
     for name in ("chll_configure", "chll_set_seed", "chll_set_stream", "chll_set_uniform_buffer",
                  "chll_lattice_create", "chll_lattice_destroy", "chll_lattice_set_stream",
                  "chll_lattice_upload", "chll_lattice_download", "chll_lattice_set_step",
                  "chll_lattice_get_step", "chll_lattice_sweeps", "chll_lattice_measure", "chll_lattice_sync",
                  "chll_lattice_set_timing", "chll_lattice_timing", "chll_lattice_set_kernel",
-                 "chll_lattice_get_kernel"):
+                 "chll_lattice_get_kernel", "chll_lattice_set_fusion", "chll_lattice_get_fusion"):
         getattr(L, name).restype = ct.c_int
     _lib = L
     return L

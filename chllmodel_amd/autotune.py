@@ -2,8 +2,11 @@
 
 For one lattice shape, boundary condition and arithmetic it times every
 candidate kernel (one site per thread in lattice order or in XCD row bands, or
-the k-march with each tile shape and a few slab counts) for the director and the chirality half-sweeps separately,
-with HIP events on the launch stream, and applies the fastest.  Sweeping with
+the k-march with each tile shape and a few slab counts) for the director and
+the chirality half-sweeps separately, with HIP events on the launch stream, and
+applies the fastest.  For FP64 it then times whole sweeps of those kernels
+against sweeps with the fused colour-1 director + chirality launch and keeps
+the faster.  Sweeping with
 any candidate produces the same lattice (the trial arithmetic and random
 counters do not depend on the kernel shape), so tuning on a scratch lattice of
 the same shape is safe.  Results are cached per (shape, bc, arith) in the same
@@ -39,7 +42,8 @@ def candidates(ni, nj, nk):
 
 
 def tune_lattice(ni, nj, nk, *, bc="open", arith="fp64", sweeps=20, seed=7, use_cache=True, verbose=False):
-    """Returns {'director': (variant, tile, nslab), 'chirality': ..., 'us': {...}}."""
+    """Returns {'director': (variant, tile, nslab), 'chirality': ..., 'fuse':
+    bool (FP64: fused colour-1 director + chirality launch), 'us': {...}}."""
     key = json.dumps({"lattice": [ni, nj, nk], "bc": bc, "arith": arith}, sort_keys=True)
     cache = {}
     if use_cache and os.path.exists(CACHE):
@@ -53,7 +57,10 @@ def tune_lattice(ni, nj, nk, *, bc="open", arith="fp64", sweeps=20, seed=7, use_
     res = {"us": {}}
     st0 = isotropic_state(ni, nj, nk, seed)
     with Lattice(ni, nj, nk, bc=bc) as lat:
+        # each half-sweep kernel on its own first: no fused launches
+        lat.set_fusion(False)
         best = {"director": (None, 1e30), "chirality": (None, 1e30)}
+        best_site = (None, 1e30)
         for cand in candidates(ni, nj, nk):
             # every candidate sweeps the same initial state: the cost of a sweep
             # depends on how many trials are accepted
@@ -72,10 +79,34 @@ def tune_lattice(ni, nj, nk, *, bc="open", arith="fp64", sweeps=20, seed=7, use_
                 res["us"][f"{kind} {cand[0]} tile={cand[1]} nslab={cand[2]}"] = round(us, 2)
                 if us < best[kind][1]:
                     best[kind] = (cand, us)
+                if kind == "director" and cand[0] == "site" and us < best_site[1]:
+                    best_site = (cand, us)
                 if verbose:
                     print(f"autotune {ni}x{nj}x{nk} {arith} {kind} {cand}: {us:.1f} us", flush=True)
-    res["director"] = list(best["director"][0])
-    res["chirality"] = list(best["chirality"][0])
+        res["director"] = list(best["director"][0])
+        res["chirality"] = list(best["chirality"][0])
+        res["fuse"] = False
+        if arith == "fp64":
+            # whole sweeps: the fastest kernels in four half-sweeps against the
+            # fused colour-1 launch, which needs the one-site director kernel
+            sweep_us = {}
+            for fuse, dcand in ((False, best["director"][0]), (True, best_site[0])):
+                lat.upload_state(st0)
+                lat.set_kernel(*dcand, kinds="director", ariths=arith)
+                lat.set_kernel(*best["chirality"][0], kinds="chirality", ariths=arith)
+                lat.set_fusion(fuse)
+                lat.set_step(0.3)
+                p = lat.params(g=0.5, KK=1.0, kbt=0.7, arith=arith, seed=seed)
+                lat.sweeps(1, 2, 0, params=p, want_rows=False)
+                lat.set_timing(True)
+                lat.sweeps(3, 2 + sweeps, 0, params=p, want_rows=False)
+                t = lat.timing()
+                lat.set_timing(False)
+                sweep_us[fuse] = 1e3 * sum(ms for ms, _ in t.values()) / sweeps
+                res["us"][f"sweep fuse={fuse}"] = round(sweep_us[fuse], 2)
+            if sweep_us[True] < sweep_us[False]:
+                res["fuse"] = True
+                res["director"] = list(best_site[0])
     if use_cache:
         cache[key] = res
         try:
@@ -91,3 +122,5 @@ def apply(lat: Lattice, tuned: dict, arith: str) -> None:
     for kind in ("director", "chirality"):
         v, t, n = tuned[kind]
         lat.set_kernel(v, t, n, kinds=kind, ariths=arith)
+    if arith == "fp64":
+        lat.set_fusion(bool(tuned.get("fuse", True)))

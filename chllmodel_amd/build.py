@@ -50,5 +50,20 @@ def build(out: str = LIB, extra: list[str] | None = None, quiet: bool = True) ->
     return out
 
 
+def build_selftest() -> str:
+    """tests/libdevice_selftest.so: GPU checks of hardware-dependent device
+    helpers (tests/device_selftest.hip), built with the library's flags."""
+    src = os.path.join(ROOT, "tests", "device_selftest.hip")
+    out = os.path.join(ROOT, "tests", "libdevice_selftest.so")
+    cmd = [HIPCC, *FLAGS, f"-I{CSRC}", src, "-o", out + ".tmp"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc failed ({res.returncode}): {' '.join(cmd)}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(quiet=False))
+    print(build_selftest())

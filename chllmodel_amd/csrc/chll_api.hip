@@ -68,7 +68,7 @@ bool g_cfg_env_read = false;
 
 }  // namespace
 
-enum { KIND_DIR = 0, KIND_SPIN = 1, KIND_MEASURE = 2, KIND_FINAL = 3, KIND_FILL = 4, KIND_COUNT = 5 };
+enum { KIND_DIR = 0, KIND_SPIN = 1, KIND_MEASURE = 2, KIND_FINAL = 3, KIND_FILL = 4, KIND_FUSED = 5, KIND_COUNT = 6 };
 
 struct chll_lattice {
     Geom G{};
@@ -93,6 +93,9 @@ struct chll_lattice {
     // variant 0 = one site per thread (tile = block order: 0 lattice, 1 XCD
     // row bands), 1 = k-march through LDS (tile shape, slabs)
     struct KCfg { int variant, tile; uint32_t nslab; } kcfg[2][2];
+    // FP64 sweeps: run each colour-1 site's chirality trial inside the colour-1
+    // director launch (needs the one-site director kernel)
+    bool fuse = true;
     bool timing = false;
     struct Ev { int kind; hipEvent_t a, b; };
     std::vector<Ev> ev;
@@ -229,6 +232,20 @@ int chll_lattice_set_kernel(chll_lattice *L, int32_t kind, int32_t arith, int32_
     if (tile < 0 || tile > 5) return fail("tile must be 0..2 (k-march 64x4, 32x8, 16x16) or 3..5 (plane tiles)");
     if (nslab < 1) return fail("nslab must be >= 1");
     L->kcfg[kind][arith] = {variant, tile, (uint32_t)nslab > L->G.nk ? L->G.nk : (uint32_t)nslab};
+    return 0;
+}
+
+int chll_lattice_set_fusion(chll_lattice *L, int32_t on)
+{
+    if (!L) return fail("null lattice");
+    L->fuse = on != 0;
+    return 0;
+}
+
+int chll_lattice_get_fusion(chll_lattice *L, int32_t *on)
+{
+    if (!L || !on) return fail("null argument");
+    *on = L->fuse ? 1 : 0;
     return 0;
 }
 
@@ -382,6 +399,11 @@ static int ensure_rows(chll_lattice *L, int32_t rows)
     return 0;
 }
 
+static bool fused_sweep(const chll_lattice *L, int arith)
+{
+    return arith == ARITH_FP64 && L->fuse && L->kcfg[0][ARITH_FP64].variant == 0;
+}
+
 int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t nstop, int32_t nout,
                         double *rows, int32_t max_rows, int64_t *counts)
 {
@@ -433,16 +455,28 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
             p->stream_pos += 10 * ns;
         }
         A.istep = (uint32_t)istep;
+        // Phases: director colour 0, director colour 1, chirality colour 0,
+        // chirality colour 1; fused, the last runs inside the second.
+        const bool fused = fused_sweep(L, p->arith);
         for (int ph = 0; ph < 4; ++ph) {
             const int kind = ph >> 1, colour = ph & 1;
+            if (fused && ph == 3) continue;
             if (p->rng != RNG_PHILOX) {
                 if (kind == 0) { A.u_phi = L->ustream + (2 * colour) * ns; A.u_acc = L->ustream + (2 * colour + 1) * ns; }
                 else { A.u_phi = nullptr; A.u_acc = L->ustream + (4 + colour) * ns; }
+                A.u_spin = L->ustream + (4 + colour) * ns;
             }
-            if (tbegin(L, kind == 0 ? KIND_DIR : KIND_SPIN, &ti)) return -1;
             const int rngk = p->rng == RNG_PHILOX ? RNG_PHILOX : RNG_STREAM;
             uint32_t *slots = L->counts + (size_t)ph * kSlots * kSlotStride;
             const auto &kc = L->kcfg[kind][p->arith];
+            if (fused && ph == 1) {
+                if (tbegin(L, KIND_FUSED, &ti)) return -1;
+                launch_phase_fused(rngk, L->bc, G, L->F, colour, A, slots, L->counts + (size_t)3 * kSlots * kSlotStride,
+                                   kc.tile, st);
+                if (launch_check() || tend(L, ti)) return -1;
+                continue;
+            }
+            if (tbegin(L, kind == 0 ? KIND_DIR : KIND_SPIN, &ti)) return -1;
             if (kc.variant == 1) {
                 if (launch_phase_march(kind, p->arith, rngk, L->bc, kc.tile, G, L->F, colour, A, slots, kc.nslab, st))
                     return fail("unknown march tile %d", kc.tile);

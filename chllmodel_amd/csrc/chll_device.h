@@ -180,6 +180,34 @@ __host__ __device__ inline U4 site_block(uint64_t seed, uint32_t run_index, uint
 // ---------------------------------------------------------------------------
 __device__ inline double fl32(double x) { return (double)(float)x; }
 
+// Metropolis test u <= exp(x) for x <= 0, with the same result as evaluating
+// exp(x) in binary64 but usually without doing so.  e = 2^RN32(RN32(x) *
+// RN32(log2 e)) from the hardware exp2 (v_exp_f32, 1 ulp) has
+//   |e / exp(x) - 1| <= |x| 1.5 2^-23 + 2^-23 + O(2^-46)
+// for -80 <= x <= 0 (argument rounding |x| 2^-24, constant and product
+// rounding |x| log2e 1.5 2^-24, times ln 2 when moved into the result).  The
+// bracket below uses r = |x| 2^-20 + 2^-17, over 5x that bound, so u < e(1-r)
+// and u > e(1+r) decide the test; u inside the bracket (probability ~2 e r)
+// takes the binary64 exp.  For x < -80 the bracket at -80 bounds exp(x) from
+// above, so only a rejection is decided there.
+__device__ inline void exp_bracket(double x, double &lo, double &hi)
+{
+    const float xf = (float)fmax(x, -80.0);
+    const double e = (double)__builtin_amdgcn_exp2f(xf * 1.44269504f);
+    const double r = fma(-(double)xf, 0x1.0p-20, 0x1.0p-17);
+    lo = x >= -80.0 ? e * (1.0 - r) : 0.0;
+    hi = e * (1.0 + r);
+}
+
+__device__ inline bool accept_exp(double x, double u)
+{
+    double lo, hi;
+    exp_bracket(x, lo, hi);
+    if (u < lo) return true;
+    if (u > hi) return false;
+    return u <= exp(x);
+}
+
 // x / b from y = RN(1/b): q = RN(x y), r = x - b q (exact with an FMA),
 // q' = RN(q + r y) = RN(x / b) (Markstein, IBM J. Res. Dev. 34(1), 1990, for
 // binary64 without underflow or overflow).  Three FMA-unit operations replace

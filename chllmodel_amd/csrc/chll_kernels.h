@@ -17,6 +17,7 @@ struct PhaseArgs {
     const double *d;      // device cone step (updated by the finalize kernel)
     const double *u_phi;  // deterministic mode: uniforms indexed by packed site
     const double *u_acc;
+    const double *u_spin; // deterministic mode, fused director + chirality launch: chirality uniforms
     uint64_t seed;
     uint32_t run_index, replica, istep;
 };
@@ -58,6 +59,13 @@ void launch_gather_spin(const Geom &G, const int8_t *src, int32_t *dst, hipStrea
 void launch_phase(int kind, int arith, int rng, int bc, const Geom &G, const Fields &F, int colour,
                   const PhaseArgs &A, uint32_t *counts, int order, hipStream_t s);
 uint32_t xcd_band(const Geom &G);
+// FP64 only: director trials of `colour` followed at each site by its
+// chirality trial, accepts counted into counts_dir, flips into counts_spin.
+void launch_phase_fused(int rng, int bc, const Geom &G, const Fields &F, int colour, const PhaseArgs &A,
+                        uint32_t *counts_dir, uint32_t *counts_spin, int order, hipStream_t s);
+// Whether every field (2 nsub + kFieldPad elements of at most 8 bytes) spans
+// less than 4 GiB, so that kernels may address it with 32-bit byte offsets.
+bool field_off32(const Geom &G);
 // k-marching half-sweeps through LDS (chll_march.hip); tile 0: 64x4, 1: 32x8,
 // 2: 16x16 packed sites x rows; returns -1 for an unknown tile.
 int launch_phase_march(int kind, int arith, int rng, int bc, int tile, const Geom &G, const Fields &F, int colour,

@@ -90,23 +90,36 @@ __device__ inline uint32_t band_block(uint32_t b, uint32_t band)
     return (p * 8u + x) * band + r;
 }
 
-template <int ARITH, int RNG, int BC>
+template <int ARITH, int RNG, int BC, bool OFF32>
 __global__ __launch_bounds__(kBlock) void k_dir(Geom G, Fields F, int colour, PhaseArgs A,
                                                 uint32_t *__restrict__ counts, uint32_t band)
 {
     const uint32_t m = band_block(blockIdx.x, band) * kBlock + threadIdx.x;
-    const bool acc = m < G.nsub && dir_site<ARITH, RNG, BC>(G, F, colour, A, m);
+    const bool acc = m < G.nsub && dir_site<ARITH, RNG, BC, OFF32>(G, F, colour, A, m);
     block_count<kBlock>(acc, counts, blockIdx.x);
 }
 
 // Chirality half-sweep over one colour (chll/chll_mod.f90:204-245).
-template <int ARITH, int RNG, int BC>
+template <int ARITH, int RNG, int BC, bool OFF32>
 __global__ __launch_bounds__(kBlock) void k_spin(Geom G, Fields F, int colour, PhaseArgs A,
                                                  uint32_t *__restrict__ counts, uint32_t band)
 {
     const uint32_t m = band_block(blockIdx.x, band) * kBlock + threadIdx.x;
-    const bool acc = m < G.nsub && spin_site<ARITH, RNG, BC>(G, F, colour, A, m);
+    const bool acc = m < G.nsub && spin_site<ARITH, RNG, BC, OFF32>(G, F, colour, A, m);
     block_count<kBlock>(acc, counts, blockIdx.x);
+}
+
+// Director half-sweep over one colour with each site's chirality trial fused
+// behind its director trial (FP64 arithmetic; see dir_spin_site).
+template <int RNG, int BC, bool OFF32>
+__global__ __launch_bounds__(kBlock) void k_dir_spin(Geom G, Fields F, int colour, PhaseArgs A,
+                                                     uint32_t *__restrict__ counts_dir,
+                                                     uint32_t *__restrict__ counts_spin, uint32_t band)
+{
+    const uint32_t m = band_block(blockIdx.x, band) * kBlock + threadIdx.x;
+    bool flip = false;
+    const bool acc = m < G.nsub && dir_spin_site<RNG, BC, OFF32>(G, F, colour, A, m, flip);
+    block_count2<kBlock>(acc, flip, counts_dir, counts_spin, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -272,16 +285,45 @@ uint32_t xcd_band(const Geom &G)
     return G.plane % (8u * kBlock) == 0u ? G.plane / (8u * kBlock) : 0u;
 }
 
+bool field_off32(const Geom &G)
+{
+    return (2ull * G.nsub + kFieldPad) * sizeof(double) <= 0xffffffffull;
+}
+
 template <int ARITH, int RNG, int BC>
 static void launch_phase_t(int kind, const Geom &G, const Fields &F, int colour, const PhaseArgs &A,
                            uint32_t *counts, uint32_t band, hipStream_t s)
 {
-    if (kind == 0)
-        hipLaunchKernelGGL((k_dir<ARITH, RNG, BC>), grid1(G.nsub), dim3(kBlock), 0, s, G, F, colour, A, counts,
-                           band);
-    else
-        hipLaunchKernelGGL((k_spin<ARITH, RNG, BC>), grid1(G.nsub), dim3(kBlock), 0, s, G, F, colour, A, counts,
-                           band);
+    const dim3 g = grid1(G.nsub), b(kBlock);
+    if (field_off32(G)) {
+        if (kind == 0) hipLaunchKernelGGL((k_dir<ARITH, RNG, BC, true>), g, b, 0, s, G, F, colour, A, counts, band);
+        else hipLaunchKernelGGL((k_spin<ARITH, RNG, BC, true>), g, b, 0, s, G, F, colour, A, counts, band);
+    } else {
+        if (kind == 0) hipLaunchKernelGGL((k_dir<ARITH, RNG, BC, false>), g, b, 0, s, G, F, colour, A, counts, band);
+        else hipLaunchKernelGGL((k_spin<ARITH, RNG, BC, false>), g, b, 0, s, G, F, colour, A, counts, band);
+    }
+}
+
+template <int RNG, int BC>
+static void launch_fused_t(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t *cd,
+                           uint32_t *cs, uint32_t band, hipStream_t s)
+{
+    const dim3 g = grid1(G.nsub), b(kBlock);
+    if (field_off32(G)) hipLaunchKernelGGL((k_dir_spin<RNG, BC, true>), g, b, 0, s, G, F, colour, A, cd, cs, band);
+    else hipLaunchKernelGGL((k_dir_spin<RNG, BC, false>), g, b, 0, s, G, F, colour, A, cd, cs, band);
+}
+
+void launch_phase_fused(int rng, int bc, const Geom &G, const Fields &F, int colour, const PhaseArgs &A,
+                        uint32_t *counts_dir, uint32_t *counts_spin, int order, hipStream_t s)
+{
+    const uint32_t band = order == 1 ? xcd_band(G) : 0u;
+    if (rng == RNG_PHILOX) {
+        if (bc == BC_OPEN) launch_fused_t<RNG_PHILOX, BC_OPEN>(G, F, colour, A, counts_dir, counts_spin, band, s);
+        else launch_fused_t<RNG_PHILOX, BC_PERIODIC>(G, F, colour, A, counts_dir, counts_spin, band, s);
+    } else {
+        if (bc == BC_OPEN) launch_fused_t<RNG_STREAM, BC_OPEN>(G, F, colour, A, counts_dir, counts_spin, band, s);
+        else launch_fused_t<RNG_STREAM, BC_PERIODIC>(G, F, colour, A, counts_dir, counts_spin, band, s);
+    }
 }
 
 void launch_phase(int kind, int arith, int rng, int bc, const Geom &G, const Fields &F, int colour,

@@ -25,15 +25,48 @@ __device__ inline void block_count(bool flag, uint32_t *slots, uint32_t blk)
     }
 }
 
+// Element q of a field.  OFF32: addressed by a 32-bit byte offset from the
+// field's base, which compiles to one global load or store with a scalar base
+// and a 32-bit vector offset instead of 64-bit address arithmetic per field;
+// valid while a field spans less than 4 GiB (field_off32).
+template <bool OFF32, class T>
+__device__ inline T &fld(T *base, uint32_t q)
+{
+    if (OFF32) return *(T *)((char *)base + q * (uint32_t)sizeof(T));
+    return base[q];
+}
+
+// The same for two flags, into two slot arrays (thread 0 and thread 1 add).
+template <int BS>
+__device__ inline void block_count2(bool f0, bool f1, uint32_t *slots0, uint32_t *slots1, uint32_t blk)
+{
+    __shared__ uint32_t wsum[2][BS / 64];
+    const uint64_t b0 = __ballot(f0), b1 = __ballot(f1);
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        wsum[0][wid] = (uint32_t)__popcll(b0);
+        wsum[1][wid] = (uint32_t)__popcll(b1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < BS / 64; ++w) t += wsum[threadIdx.x][w];
+        uint32_t *slots = threadIdx.x ? slots1 : slots0;
+        if (t) atomicAdd(&slots[(blk & (kSlots - 1)) * kSlotStride], t);
+    }
+}
+
+template <bool OFF32>
 __device__ inline void load_nbrs(const Fields &F, const uint32_t nb[6], Nbr &N)
 {
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
         const uint32_t q = nb[b];
-        N.x[b] = F.nx[q];
-        N.y[b] = F.ny[q];
-        N.z[b] = F.nz[q];
-        N.s[b] = F.s[q];
+        N.x[b] = fld<OFF32>(F.nx, q);
+        N.y[b] = fld<OFF32>(F.ny, q);
+        N.z[b] = fld<OFF32>(F.nz, q);
+        N.s[b] = fld<OFF32>(F.s, q);
     }
 }
 
@@ -71,7 +104,7 @@ __device__ inline bool dir_compute(const PhaseArgs &A, int colour, uint32_t m, c
         trial_compat(nx, ny, nz, d, u_phi, t[0], t[1], t[2]);
         reload_barrier();
         const double enew = energy_compat(N, t[0], t[1], t[2], sg, A.g, A.KK);
-        return (enew < eold) || (u_acc <= exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt)));
+        return (enew < eold) || accept_exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt), u_acc);
     } else {
         double c2o, c2n;
         const double ao = cqf_sum(N, sg, nx, ny, nz, c2o);
@@ -79,7 +112,7 @@ __device__ inline bool dir_compute(const PhaseArgs &A, int colour, uint32_t m, c
         reload_barrier();
         const double an = cqf_sum(N, sg, t[0], t[1], t[2], c2n);
         const double dE = (c2o - c2n) - A.KK * (an - ao);
-        return (dE < 0.0) || (u_acc <= exp(-dE * A.inv_kbt));
+        return (dE < 0.0) || accept_exp(-dE * A.inv_kbt, u_acc);
     }
 }
 
@@ -98,16 +131,16 @@ __device__ inline bool spin_compute(const PhaseArgs &A, int colour, uint32_t m, 
     if (ARITH == ARITH_COMPAT) {
         const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
         const double enew = energy_compat(N, nx, ny, nz, -sg, A.g, A.KK);
-        return (enew < eold) || (u_acc <= exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt)));
+        return (enew < eold) || accept_exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt), u_acc);
     } else {
         const double a = cq_sum(N, nx, ny, nz);
         const double dE = (double)sg * fma(A.KK, a, 2.0 * A.g);
-        return (dE < 0.0) || (u_acc <= exp(-dE * A.inv_kbt));
+        return (dE < 0.0) || accept_exp(-dE * A.inv_kbt, u_acc);
     }
 }
 
 // Gather a site's inputs straight from the colour-split fields.
-template <int BC>
+template <int BC, bool OFF32>
 __device__ inline uint32_t load_site(const Geom &G, const Fields &F, int colour, uint32_t m, SiteIn &S)
 {
     const Site st = decode(G, m, colour);
@@ -115,37 +148,68 @@ __device__ inline uint32_t load_site(const Geom &G, const Fields &F, int colour,
     const uint32_t own = colour ? G.nsub + m : m;
     const uint32_t oth = colour ? 0u : G.nsub;
     S.N.mask = neighbours<BC>(G, m, st, oth, nb);
-    load_nbrs(F, nb, S.N);
-    S.nx = F.nx[own]; S.ny = F.ny[own]; S.nz = F.nz[own];
-    S.sg = F.s[own];
+    load_nbrs<OFF32>(F, nb, S.N);
+    S.nx = fld<OFF32>(F.nx, own); S.ny = fld<OFF32>(F.ny, own); S.nz = fld<OFF32>(F.nz, own);
+    S.sg = fld<OFF32>(F.s, own);
     return own;
 }
 
 // One director trial at packed site m of `colour`, inputs read from the
 // fields; stores the new director when accepted.
-template <int ARITH, int RNG, int BC>
+template <int ARITH, int RNG, int BC, bool OFF32 = false>
 __device__ inline bool dir_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m)
 {
     SiteIn S;
-    const uint32_t own = load_site<BC>(G, F, colour, m, S);
+    const uint32_t own = load_site<BC, OFF32>(G, F, colour, m, S);
     double t[3];
     const bool acc = dir_compute<ARITH, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg, *A.d, t);
     if (acc) {
-        F.nx[own] = t[0];
-        F.ny[own] = t[1];
-        F.nz[own] = t[2];
+        fld<OFF32>(F.nx, own) = t[0];
+        fld<OFF32>(F.ny, own) = t[1];
+        fld<OFF32>(F.nz, own) = t[2];
     }
     return acc;
 }
 
 // One chirality trial at packed site m of `colour`.
-template <int ARITH, int RNG, int BC>
+template <int ARITH, int RNG, int BC, bool OFF32 = false>
 __device__ inline bool spin_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m)
 {
     SiteIn S;
-    const uint32_t own = load_site<BC>(G, F, colour, m, S);
+    const uint32_t own = load_site<BC, OFF32>(G, F, colour, m, S);
     const bool acc = spin_compute<ARITH, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg);
-    if (acc) F.s[own] = (int8_t)(-S.sg);
+    if (acc) fld<OFF32>(F.s, own) = (int8_t)(-S.sg);
+    return acc;
+}
+
+// Director trial, then chirality trial, at packed site m of `colour`; FP64
+// arithmetic only.  The reference runs director colour 0, director colour 1,
+// chirality colour 0, chirality colour 1 (chll/chll_mod.f90:70-81).  In FP64
+// arithmetic the chirality energy change sigma (K sum_b sgn_b c_b q_b + 2 g)
+// reads the directors and the site's own spin but no neighbour spin, so once
+// both colours' directors are final a site's chirality trial has the same
+// outcome before or after the other colour's chirality trials.  Run right after
+// this site's director trial it reuses the neighbour directors in registers and
+// saves one pass over the lattice.  Returns the director decision; `flip` gets
+// the chirality decision.
+template <int RNG, int BC, bool OFF32>
+__device__ inline bool dir_spin_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m,
+                                     bool &flip)
+{
+    SiteIn S;
+    const uint32_t own = load_site<BC, OFF32>(G, F, colour, m, S);
+    double t[3];
+    const bool acc = dir_compute<ARITH_FP64, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg, *A.d, t);
+    if (acc) {
+        fld<OFF32>(F.nx, own) = t[0];
+        fld<OFF32>(F.ny, own) = t[1];
+        fld<OFF32>(F.nz, own) = t[2];
+    }
+    PhaseArgs As = A;
+    As.u_acc = A.u_spin;
+    flip = spin_compute<ARITH_FP64, RNG>(As, colour, m, S.N, acc ? t[0] : S.nx, acc ? t[1] : S.ny,
+                                         acc ? t[2] : S.nz, S.sg);
+    if (flip) fld<OFF32>(F.s, own) = (int8_t)(-S.sg);
     return acc;
 }
 

@@ -86,6 +86,16 @@ class Lattice:
                                             ct.byref(n)))
         return ("site", "march")[v.value], t.value, n.value
 
+    def set_fusion(self, on: bool) -> None:
+        """FP64 sweeps: run the colour-1 chirality trials inside the colour-1
+        director launch (same results, one lattice pass fewer)."""
+        check(lib().chll_lattice_set_fusion(self._h, 1 if on else 0))
+
+    def get_fusion(self) -> bool:
+        v = ct.c_int32()
+        check(lib().chll_lattice_get_fusion(self._h, ct.byref(v)))
+        return bool(v.value)
+
     def sync(self) -> None:
         check(lib().chll_lattice_sync(self._h))
 

@@ -140,8 +140,16 @@ int chll_lattice_set_kernel(chll_lattice *lat, int32_t kind, int32_t arith, int3
                             int32_t nslab);
 int chll_lattice_get_kernel(chll_lattice *lat, int32_t kind, int32_t arith, int32_t *variant, int32_t *tile,
                             int32_t *nslab);
+/* FP64 sweeps (on by default): each colour-1 site's chirality trial runs in
+ * the colour-1 director launch, right after that site's director trial, when
+ * the director kernel is the one-site kernel (variant 0).  FP64 chirality
+ * energy changes do not read neighbour spins, so the lattice, counts and rows
+ * are the same as with four separate half-sweeps.  No counterpart in the
+ * reference (chll/chll_mod.f90:70-81 runs the four half-sweeps in turn). */
+int chll_lattice_set_fusion(chll_lattice *lat, int32_t on);
+int chll_lattice_get_fusion(chll_lattice *lat, int32_t *on);
 /* Per-kernel HIP-event timing: kinds 0 director, 1 chirality, 2 measure,
- * 3 finalize, 4 uniform fill. */
+ * 3 finalize, 4 uniform fill, 5 fused director + chirality. */
 int chll_lattice_set_timing(chll_lattice *lat, int32_t on);
 int chll_lattice_timing(chll_lattice *lat, double *ms, int64_t *launches, int32_t nkinds);
 

@@ -97,6 +97,7 @@ CASES = [
     ((16, 16, 16), "open", "philox", "fp64", 6, 3),
     ((12, 8, 10), "periodic", "philox", "fp64", 6, 2),
     ((16, 8, 8), "open", "stream", "fp64", 5, 5),
+    ((10, 6, 8), "periodic", "stream", "fp64", 6, 3),
     ((64, 64, 64), "open", "philox", "compat", 3, 3),
 ]
 
@@ -232,18 +233,22 @@ SHAPES = [((256 // 8, 24, 20), "open"), ((48, 40, 30), "periodic"), ((10, 7, 5),
 @pytest.mark.parametrize("dims,bc", SHAPES, ids=[f"{'x'.join(map(str, d))}-{b}" for d, b in SHAPES])
 @pytest.mark.parametrize("arith", ["compat", "fp64"])
 def test_kernel_shapes_identical(dims, bc, arith):
-    """The one-site-per-thread kernels in both block orders and every k-march
+    """The one-site-per-thread kernels in both block orders, with and without
+    the fused colour-1 director + chirality launch (FP64), and every k-march
     tile / slab split give the same lattice, counts and rows (partial tiles,
     odd open dimensions; 64x64 and 128x64 planes split into XCD bands of 1 and
     2 blocks)."""
     st0 = isotropic_state(*dims, 2000 + sum(dims))
     ref = None
-    for variant, tile, nslab in [("site", 0, 1), ("site", 1, 1), ("march", 0, 1), ("march", 0, 3), ("march", 1, 2),
-                                 ("march", 2, 1), ("march", 2, 64), ("march", 3, 1), ("march", 4, 1),
-                                 ("march", 5, 1)]:
-        print(f"kernel shapes dims={dims} bc={bc} {arith} {variant} tile={tile} nslab={nslab}", flush=True)
+    for variant, tile, nslab, fuse in [("site", 0, 1, False), ("site", 0, 1, True), ("site", 1, 1, True),
+                                       ("site", 1, 1, False), ("march", 0, 1, True), ("march", 0, 3, True),
+                                       ("march", 1, 2, True), ("march", 2, 1, True), ("march", 2, 64, True),
+                                       ("march", 3, 1, True), ("march", 4, 1, True), ("march", 5, 1, True)]:
+        print(f"kernel shapes dims={dims} bc={bc} {arith} {variant} tile={tile} nslab={nslab} fuse={fuse}",
+              flush=True)
         with Lattice(*dims, bc=bc) as lat:
             lat.set_kernel(variant, tile, nslab)
+            lat.set_fusion(fuse)
             lat.upload_state(st0)
             lat.set_step(0.3)
             out = lat.sweeps(1, 6, 3, g=0.4, KK=1.1, kbt=0.8, arith=arith, seed=77, want_counts=True)
@@ -252,5 +257,38 @@ def test_kernel_shapes_identical(dims, bc, arith):
             ref = got
             continue
         for f in ("nx", "ny", "nz", "s"):
-            assert np.array_equal(got[0][f], ref[0][f]), (variant, tile, nslab, f)
+            assert np.array_equal(got[0][f], ref[0][f]), (variant, tile, nslab, fuse, f)
         assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2])
+
+
+@pytest.mark.parametrize("rng", ["philox", "stream"])
+def test_fused_sweep_counts_and_timing(rng):
+    """FP64 sweeps launch the fused colour-1 kernel once per sweep (and one
+    chirality launch), and give the same lattice, counts and rows as the four
+    separate half-sweeps; compat sweeps never fuse."""
+    dims = (64, 32, 16)
+    st0 = isotropic_state(*dims, 31)
+    res = {}
+    for fuse in (True, False):
+        print(f"fused sweep rng={rng} fuse={fuse}", flush=True)
+        with Lattice(*dims) as lat:
+            lat.set_kernel("site", 1, 1, kinds="director")
+            lat.set_fusion(fuse)
+            lat.upload_state(st0)
+            lat.set_step(0.3)
+            lat.set_timing(True)
+            out = lat.sweeps(1, 5, 5, g=0.5, KK=1.0, kbt=0.7, arith="fp64", rng=rng, seed=11, want_counts=True)
+            t = lat.timing()
+            res[fuse] = (lat.download(), out["counts"], out["rows"], t)
+    t_on, t_off = res[True][3], res[False][3]
+    assert t_on["fused"][1] == 5 and t_on["director"][1] == 5 and t_on["chirality"][1] == 5
+    assert t_off["fused"][1] == 0 and t_off["director"][1] == 10 and t_off["chirality"][1] == 10
+    for f in ("nx", "ny", "nz", "s"):
+        assert np.array_equal(res[True][0][f], res[False][0][f]), f
+    assert np.array_equal(res[True][1], res[False][1]) and np.array_equal(res[True][2], res[False][2])
+    with Lattice(*dims) as lat:
+        lat.upload_state(st0)
+        lat.set_timing(True)
+        lat.sweeps(1, 2, 0, g=0.5, KK=1.0, kbt=0.7, arith="compat", rng=rng, seed=11, want_rows=False)
+        t = lat.timing()
+    assert t["fused"][1] == 0 and t["director"][1] == 4 and t["chirality"][1] == 4
