@@ -39,12 +39,14 @@ HBM_PEAK_GBS = 8000.0
 # active half's directors 12 B; chirality half-sweep: read all directors 24 B
 # and the active half's spins 0.5 B (FP64; the reference-compatible energies
 # also read the other half's spins, 0.5 B more), write those spins 0.5 B (upper
-# bound: every flip written); fused colour-1 director + chirality: the director
-# traffic plus the active half's spin writes 0.5 B.
-BYTES = {"director": 37.0, "chirality": 25.0, "fused": 37.5}
+# bound: every flip written); fused colour-1 director + chirality, and fused
+# colour-0 chirality + next sweep's colour-0 director: the director traffic
+# plus the active half's spin writes 0.5 B.
+BYTES = {"director": 37.0, "chirality": 25.0, "fused": 37.5, "fused_next": 37.5}
 BYTES_CHIRALITY_COMPAT = 25.5
 KERNEL_NAMES = {"director": "k_dir (director half-sweep)", "chirality": "chirality half-sweep",
-                "fused": "k_dir_spin (colour-1 director + chirality, fused)"}
+                "fused": "k_dir_spin (colour-1 director + chirality, fused)",
+                "fused_next": "k_spin_dir (colour-0 chirality + next sweep's colour-0 director, fused)"}
 
 
 def log(msg):

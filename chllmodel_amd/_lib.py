@@ -23,7 +23,7 @@ BC = {"open": 0, "periodic": 1}
 ROW_COLS = 16
 ROW_NAMES = ("istep", "E", "eta", "paccept", "faccept", "d", "E_true", "eta_exact",
              "Qxx", "Qyy", "Qzz", "Qxy", "Qxz", "Qyz", "naccept", "nflip")
-TIMING_KINDS = ("director", "chirality", "measure", "finalize", "fill", "fused")
+TIMING_KINDS = ("director", "chirality", "measure", "finalize", "fill", "fused", "fused_next")
 
 _f64p = ct.POINTER(ct.c_double)
 _i32p = ct.POINTER(ct.c_int32)

@@ -68,7 +68,10 @@ bool g_cfg_env_read = false;
 
 }  // namespace
 
-enum { KIND_DIR = 0, KIND_SPIN = 1, KIND_MEASURE = 2, KIND_FINAL = 3, KIND_FILL = 4, KIND_FUSED = 5, KIND_COUNT = 6 };
+enum {
+    KIND_DIR = 0, KIND_SPIN = 1, KIND_MEASURE = 2, KIND_FINAL = 3, KIND_FILL = 4, KIND_FUSED = 5, KIND_FUSED_NEXT = 6,
+    KIND_COUNT = 7
+};
 
 struct chll_lattice {
     Geom G{};
@@ -85,6 +88,7 @@ struct chll_lattice {
     double *rows_dev = nullptr;
     int32_t rows_cap = 0;
     double *d_dev = nullptr;
+    long long *pend = nullptr;  // device: 2 director accept totals awaiting their flips
     double *ustream = nullptr;
     long long *counts_dev = nullptr;
     size_t counts_cap = 0;
@@ -188,6 +192,7 @@ int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lat
     rc |= dalloc(&L->spart, (size_t)L->nblk_meas);
     if (!rc && hipMemset(L->counts, 0, kCountWords * sizeof(uint32_t)) != hipSuccess) rc = fail("hipMemset failed");
     rc |= dalloc(&L->d_dev, 1);
+    rc |= dalloc(&L->pend, 2);
     if (!rc && hipHostMalloc((void **)&L->pinned, 8 * sizeof(double)) != hipSuccess) rc = fail("hipHostMalloc failed");
     if (!rc && hipStreamCreate(&L->own) != hipSuccess) rc = fail("hipStreamCreate failed");
     if (rc) {
@@ -266,7 +271,7 @@ int chll_lattice_destroy(chll_lattice *L)
     if (!L) return 0;
     if (L->stream) (void)hipStreamSynchronize(L->stream);
     dfree(L->F.nx); dfree(L->F.ny); dfree(L->F.nz); dfree(L->F.s);
-    dfree(L->stage); dfree(L->stage_s); dfree(L->bad); dfree(L->counts);
+    dfree(L->stage); dfree(L->stage_s); dfree(L->bad); dfree(L->counts); dfree(L->pend);
     dfree(L->part); dfree(L->spart); dfree(L->rows_dev); dfree(L->d_dev);
     dfree(L->ustream); dfree(L->counts_dev);
     if (L->pinned) (void)hipHostFree(L->pinned);
@@ -435,11 +440,24 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
     A.g = p->g; A.KK = p->KK; A.kbt = p->kbt; A.inv_kbt = 1.0 / p->kbt;
     A.d = L->d_dev; A.seed = p->seed; A.run_index = p->run_index; A.replica = p->replica;
     FinArgs FA{};
-    FA.counts = L->counts; FA.part = L->part; FA.spart = L->spart;
+    FA.counts = L->counts; FA.pend = L->pend; FA.part = L->part; FA.spart = L->spart;
     FA.nblk_meas = L->nblk_meas; FA.n3 = 2 * ns; FA.d = L->d_dev; FA.rows = L->rows_dev;
     FA.counts_out = counts ? L->counts_dev : nullptr; FA.adapt = p->adapt;
     int32_t row = 0;
     size_t ti = 0;
+    const int rngk = p->rng == RNG_PHILOX ? RNG_PHILOX : RNG_STREAM;
+    // Sweep schedule.  Reference order per sweep: director colour 0 (D0),
+    // director colour 1 (D1), chirality colour 0 (C0), chirality colour 1 (C1),
+    // then the finalize kernel.  FP64 with the one-site director kernel:
+    // C1 runs inside the D1 launch; with Philox draws also C0 of sweep t and
+    // D0 of sweep t + 1 share a launch, after a finalize that closes sweep t's
+    // director accepts (and so its cone-step update).  Sweep t's flips are then
+    // closed by the next finalize.  Measurement sweeps and the last sweep of
+    // the call run their C0 alone, so the state they leave is complete.
+    const bool fuse1 = fused_sweep(L, p->arith);
+    const bool fuse2 = fuse1 && p->rng == RNG_PHILOX;
+    const auto grp = [&](int g) { return L->counts + (size_t)g * kSlots * kSlotStride; };
+    bool d0_done = false, prev_open = false;
     for (int64_t t = 0; t < nsw; ++t) {
         const int32_t istep = nstart + (int32_t)t;
         if (p->rng == RNG_STREAM) {
@@ -455,27 +473,15 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
             p->stream_pos += 10 * ns;
         }
         A.istep = (uint32_t)istep;
-        // Phases: director colour 0, director colour 1, chirality colour 0,
-        // chirality colour 1; fused, the last runs inside the second.
-        const bool fused = fused_sweep(L, p->arith);
-        for (int ph = 0; ph < 4; ++ph) {
-            const int kind = ph >> 1, colour = ph & 1;
-            if (fused && ph == 3) continue;
-            if (p->rng != RNG_PHILOX) {
-                if (kind == 0) { A.u_phi = L->ustream + (2 * colour) * ns; A.u_acc = L->ustream + (2 * colour + 1) * ns; }
-                else { A.u_phi = nullptr; A.u_acc = L->ustream + (4 + colour) * ns; }
-                A.u_spin = L->ustream + (4 + colour) * ns;
-            }
-            const int rngk = p->rng == RNG_PHILOX ? RNG_PHILOX : RNG_STREAM;
-            uint32_t *slots = L->counts + (size_t)ph * kSlots * kSlotStride;
+        const auto set_u = [&](int kind, int colour) {
+            if (p->rng == RNG_PHILOX) return;
+            if (kind == 0) { A.u_phi = L->ustream + (2 * colour) * ns; A.u_acc = L->ustream + (2 * colour + 1) * ns; }
+            else { A.u_phi = nullptr; A.u_acc = L->ustream + (4 + colour) * ns; }
+            A.u_spin = L->ustream + (4 + colour) * ns;
+        };
+        const auto half = [&](int kind, int colour, uint32_t *slots) -> int {
+            set_u(kind, colour);
             const auto &kc = L->kcfg[kind][p->arith];
-            if (fused && ph == 1) {
-                if (tbegin(L, KIND_FUSED, &ti)) return -1;
-                launch_phase_fused(rngk, L->bc, G, L->F, colour, A, slots, L->counts + (size_t)3 * kSlots * kSlotStride,
-                                   kc.tile, st);
-                if (launch_check() || tend(L, ti)) return -1;
-                continue;
-            }
             if (tbegin(L, kind == 0 ? KIND_DIR : KIND_SPIN, &ti)) return -1;
             if (kc.variant == 1) {
                 if (launch_phase_march(kind, p->arith, rngk, L->bc, kc.tile, G, L->F, colour, A, slots, kc.nslab, st))
@@ -483,19 +489,54 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
             } else {
                 launch_phase(kind, p->arith, rngk, L->bc, G, L->F, colour, A, slots, kc.tile, st);
             }
+            return (launch_check() || tend(L, ti)) ? -1 : 0;
+        };
+        if (!d0_done && half(0, 0, grp(kGroupAcc0))) return -1;
+        if (fuse1) {
+            set_u(0, 1);
+            if (tbegin(L, KIND_FUSED, &ti)) return -1;
+            launch_phase_fused(rngk, L->bc, G, L->F, 1, A, grp(kGroupAcc1), grp(flip_group(istep, 1)),
+                               L->kcfg[0][p->arith].tile, st);
             if (launch_check() || tend(L, ti)) return -1;
+        } else if (half(0, 1, grp(kGroupAcc1))) {
+            return -1;
         }
         const bool meas = nout > 0 && (istep % nout) == 0 && row < max_rows;
+        const bool chain = fuse2 && !meas && t + 1 < nsw;
+        FA.istep = istep; FA.measure = meas ? 1 : 0; FA.row = row; FA.sweep_index = t;
+        FA.close_prev = prev_open ? 1 : 0;
+        if (chain) {
+            // close sweep t's director accepts (cone step of sweep t + 1), then
+            // C0 of sweep t with D0 of sweep t + 1
+            FA.close_this = 0;
+            if (tbegin(L, KIND_FINAL, &ti)) return -1;
+            launch_finalize(FA, st);
+            if (launch_check() || tend(L, ti)) return -1;
+            PhaseArgs A2 = A;
+            A2.istep = (uint32_t)(istep + 1);
+            A2.istep_spin = (uint32_t)istep;
+            if (tbegin(L, KIND_FUSED_NEXT, &ti)) return -1;
+            launch_phase_fused_next(L->bc, G, L->F, A2, grp(kGroupAcc0), grp(flip_group(istep, 0)),
+                                    L->kcfg[0][p->arith].tile, st);
+            if (launch_check() || tend(L, ti)) return -1;
+            d0_done = true;
+            prev_open = true;
+            continue;
+        }
+        if (half(1, 0, grp(flip_group(istep, 0)))) return -1;
+        if (!fuse1 && half(1, 1, grp(flip_group(istep, 1)))) return -1;
         if (meas) {
             if (tbegin(L, KIND_MEASURE, &ti)) return -1;
             launch_measure(L->bc, G, L->F, p->g, p->KK, L->part, L->spart, st);
             if (launch_check() || tend(L, ti)) return -1;
         }
-        FA.istep = istep; FA.measure = meas ? 1 : 0; FA.row = row; FA.sweep_index = t;
+        FA.close_this = 1;
         if (tbegin(L, KIND_FINAL, &ti)) return -1;
         launch_finalize(FA, st);
         if (launch_check() || tend(L, ti)) return -1;
         if (meas) ++row;
+        d0_done = false;
+        prev_open = false;
     }
     if (rows && row > 0)
         HIPCHECK(hipMemcpyAsync(rows, L->rows_dev, (size_t)row * kRowCols * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -514,9 +555,9 @@ int chll_lattice_measure(chll_lattice *L, double g, double KK, double *row)
     launch_measure(L->bc, L->G, L->F, g, KK, L->part, L->spart, st);
     if (launch_check()) return -1;
     FinArgs FA{};
-    FA.counts = L->counts; FA.part = L->part; FA.spart = L->spart;
+    FA.counts = L->counts; FA.pend = L->pend; FA.part = L->part; FA.spart = L->spart;
     FA.nblk_meas = L->nblk_meas; FA.n3 = 2 * (uint64_t)L->G.nsub; FA.d = L->d_dev; FA.rows = L->rows_dev;
-    FA.istep = 0; FA.measure = 1; FA.row = 0; FA.adapt = 0;
+    FA.istep = 0; FA.measure = 1; FA.row = 0; FA.adapt = 0; FA.close_prev = 0; FA.close_this = 1;
     launch_finalize(FA, st);
     if (launch_check()) return -1;
     HIPCHECK(hipMemcpyAsync(row, L->rows_dev, kRowCols * sizeof(double), hipMemcpyDeviceToHost, st));

@@ -20,22 +20,36 @@ struct PhaseArgs {
     const double *u_spin; // deterministic mode, fused director + chirality launch: chirality uniforms
     uint64_t seed;
     uint32_t run_index, replica, istep;
+    uint32_t istep_spin;  // fused chirality colour 0 + next director colour 0: the chirality trial's sweep
 };
 
 constexpr int kRowCols = 16;
 
 // Accept/flip counters: each block adds its count atomically into one of
-// kSlots slots per phase (slot = block % kSlots, slots kSlotStride words apart
-// so they sit on separate cache lines); the finalize kernel sums and re-zeroes
-// them.
+// kSlots slots of its group (slot = block % kSlots, slots kSlotStride words
+// apart so they sit on separate cache lines); the finalize kernel sums and
+// re-zeroes them.  Single-lattice groups: director accepts of colours 0 and 1,
+// then chirality flips of colours 0 and 1 of even sweeps, then of odd sweeps,
+// so that a sweep's colour-0 flips can still be open while the next sweep's
+// director trials count (the replica ensemble uses the first four groups as
+// director 0, director 1, chirality 0, chirality 1).
 constexpr int kSlots = 64;
 constexpr int kSlotStride = 32;
-constexpr int kCountWords = 4 * kSlots * kSlotStride;
+constexpr int kGroups = 6;
+constexpr int kGroupAcc0 = 0, kGroupAcc1 = 1;
+constexpr int kCountWords = kGroups * kSlots * kSlotStride;
+__host__ __device__ constexpr int flip_group(int32_t istep, int colour) { return 2 + 2 * (istep & 1) + colour; }
 // Cap on measurement blocks (grid-stride), bounding the finalize kernel's sum.
 constexpr uint32_t kMeasBlocksMax = 2048;
 
+// The finalize kernel always closes the director trials of sweep istep (sums
+// the accept groups, adapts d, keeps the count in pend[istep & 1]); with
+// close_prev it also closes the chirality trials of sweep istep - 1 (writes
+// its counts_out entry from pend), with close_this those of sweep istep
+// (counts_out entry and, with measure, the row).
 struct FinArgs {
-    uint32_t *counts;        // 4 phases x kSlots slots, zeroed after reading
+    uint32_t *counts;        // kGroups x kSlots slots, zeroed after reading
+    long long *pend;         // 2 director accept totals awaiting their sweep's flips
     const double *part;      // nblk_meas x 8 measurement partials
     const long long *spart;  // nblk_meas spin partials
     uint32_t nblk_meas;
@@ -44,7 +58,8 @@ struct FinArgs {
     double *rows;            // rows x kRowCols
     long long *counts_out;   // optional per-sweep (naccept, nflip)
     int32_t istep, measure, row, adapt;
-    int64_t sweep_index;
+    int32_t close_prev, close_this;
+    int64_t sweep_index;     // of sweep istep in counts_out
 };
 
 uint32_t phase_blocks(const Geom &G);
@@ -63,6 +78,10 @@ uint32_t xcd_band(const Geom &G);
 // chirality trial, accepts counted into counts_dir, flips into counts_spin.
 void launch_phase_fused(int rng, int bc, const Geom &G, const Fields &F, int colour, const PhaseArgs &A,
                         uint32_t *counts_dir, uint32_t *counts_spin, int order, hipStream_t s);
+// FP64, Philox: chirality trials of colour 0 for sweep A.istep_spin, each
+// followed by the same site's director trial for sweep A.istep.
+void launch_phase_fused_next(int bc, const Geom &G, const Fields &F, const PhaseArgs &A, uint32_t *counts_dir,
+                             uint32_t *counts_spin, int order, hipStream_t s);
 // Whether every field (2 nsub + kFieldPad elements of at most 8 bytes) spans
 // less than 4 GiB, so that kernels may address it with 32-bit byte offsets.
 bool field_off32(const Geom &G);

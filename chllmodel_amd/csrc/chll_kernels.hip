@@ -122,6 +122,20 @@ __global__ __launch_bounds__(kBlock) void k_dir_spin(Geom G, Fields F, int colou
     block_count2<kBlock>(acc, flip, counts_dir, counts_spin, blockIdx.x);
 }
 
+// Chirality half-sweep of colour 0 (sweep A.istep_spin) with each site's
+// director trial of the next sweep (A.istep) fused behind it (FP64, Philox;
+// see spin_dir_site).
+template <int BC, bool OFF32>
+__global__ __launch_bounds__(kBlock) void k_spin_dir(Geom G, Fields F, PhaseArgs A,
+                                                     uint32_t *__restrict__ counts_dir,
+                                                     uint32_t *__restrict__ counts_spin, uint32_t band)
+{
+    const uint32_t m = band_block(blockIdx.x, band) * kBlock + threadIdx.x;
+    bool flip = false;
+    const bool acc = m < G.nsub && spin_dir_site<BC, OFF32>(G, F, A, m, flip);
+    block_count2<kBlock>(acc, flip, counts_dir, counts_spin, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Observables over the whole lattice (grid-stride), one partial per block.
 template <int BC>
@@ -143,15 +157,22 @@ constexpr int kFinBlock = 256;
 
 __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs A)
 {
-    __shared__ double red[kFinBlock / 64][10];
+    __shared__ double red[kFinBlock / 64][11];
     const uint32_t t = threadIdx.x;
-    unsigned long long ca = 0, cf = 0;
-    for (uint32_t q = t; q < 4 * kSlots; q += kFinBlock) {
-        uint32_t *slot = A.counts + (q / kSlots) * (kSlots * kSlotStride) + (q % kSlots) * kSlotStride;
+    unsigned long long ca = 0, cf = 0, cp = 0;
+    const int gt = flip_group(A.istep, 0), gp = flip_group(A.istep - 1, 0);
+    for (uint32_t q = t; q < kGroups * kSlots; q += kFinBlock) {
+        const int g = (int)(q / kSlots);
+        const bool ia = g < 2;
+        const bool it = A.close_this && (g == gt || g == gt + 1);
+        const bool ip = A.close_prev && (g == gp || g == gp + 1);
+        if (!(ia || it || ip)) continue;
+        uint32_t *slot = A.counts + g * (kSlots * kSlotStride) + (q % kSlots) * kSlotStride;
         const uint32_t c = *slot;
         *slot = 0u;
-        if (q < 2 * kSlots) ca += c;
-        else cf += c;
+        if (ia) ca += c;
+        else if (it) cf += c;
+        else cp += c;
     }
     double v[8];
 #pragma unroll
@@ -164,13 +185,14 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs A)
             ss += A.spart[b];
         }
     }
-    double cad = (double)ca, cfd = (double)cf, ssd = (double)ss;  // exact: < 2^53
+    double cad = (double)ca, cfd = (double)cf, cpd = (double)cp, ssd = (double)ss;  // exact: < 2^53
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) v[q] += __shfl_xor(v[q], off, 64);
         cad += __shfl_xor(cad, off, 64);
         cfd += __shfl_xor(cfd, off, 64);
+        cpd += __shfl_xor(cpd, off, 64);
         ssd += __shfl_xor(ssd, off, 64);
     }
     const uint32_t lane = t & 63u, wid = t >> 6;
@@ -179,14 +201,15 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs A)
         for (int q = 0; q < 8; ++q) red[wid][q] = v[q];
         red[wid][8] = cad;
         red[wid][9] = cfd;
+        red[wid][10] = cpd;
     }
     __shared__ double sred[kFinBlock / 64];
     if (lane == 0) sred[wid] = ssd;
     __syncthreads();
     if (t != 0) return;
-    double tot[10];
+    double tot[11];
 #pragma unroll
-    for (int q = 0; q < 10; ++q) {
+    for (int q = 0; q < 11; ++q) {
         double acc = 0.0;
 #pragma unroll
         for (int w = 0; w < kFinBlock / 64; ++w) acc += red[w][q];
@@ -208,11 +231,16 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs A)
         else if (paccept > 0.6) d = d / 0.99500000476837158203125;
     }
     *A.d = d;
-    if (A.counts_out) {
+    if (A.close_prev && A.counts_out) {
+        A.counts_out[2 * (size_t)(A.sweep_index - 1)] = A.pend[(A.istep - 1) & 1];
+        A.counts_out[2 * (size_t)(A.sweep_index - 1) + 1] = (long long)tot[10];
+    }
+    A.pend[A.istep & 1] = (long long)tot[8];
+    if (A.close_this && A.counts_out) {
         A.counts_out[2 * (size_t)A.sweep_index] = (long long)tot[8];
         A.counts_out[2 * (size_t)A.sweep_index + 1] = (long long)tot[9];
     }
-    if (A.measure) {
+    if (A.close_this && A.measure) {
         double *r = A.rows + (size_t)A.row * kRowCols;
         const double faccept = (double)(float)((double)(float)nflp / n3f);
         const long long S = (long long)stot;
@@ -323,6 +351,23 @@ void launch_phase_fused(int rng, int bc, const Geom &G, const Fields &F, int col
     } else {
         if (bc == BC_OPEN) launch_fused_t<RNG_STREAM, BC_OPEN>(G, F, colour, A, counts_dir, counts_spin, band, s);
         else launch_fused_t<RNG_STREAM, BC_PERIODIC>(G, F, colour, A, counts_dir, counts_spin, band, s);
+    }
+}
+
+void launch_phase_fused_next(int bc, const Geom &G, const Fields &F, const PhaseArgs &A, uint32_t *counts_dir,
+                             uint32_t *counts_spin, int order, hipStream_t s)
+{
+    const uint32_t band = order == 1 ? xcd_band(G) : 0u;
+    const dim3 g = grid1(G.nsub), b(kBlock);
+    const bool o32 = field_off32(G);
+    if (bc == BC_OPEN) {
+        if (o32) hipLaunchKernelGGL((k_spin_dir<BC_OPEN, true>), g, b, 0, s, G, F, A, counts_dir, counts_spin, band);
+        else hipLaunchKernelGGL((k_spin_dir<BC_OPEN, false>), g, b, 0, s, G, F, A, counts_dir, counts_spin, band);
+    } else {
+        if (o32)
+            hipLaunchKernelGGL((k_spin_dir<BC_PERIODIC, true>), g, b, 0, s, G, F, A, counts_dir, counts_spin, band);
+        else
+            hipLaunchKernelGGL((k_spin_dir<BC_PERIODIC, false>), g, b, 0, s, G, F, A, counts_dir, counts_spin, band);
     }
 }
 

@@ -213,6 +213,36 @@ __device__ inline bool dir_spin_site(const Geom &G, const Fields &F, int colour,
     return acc;
 }
 
+// Chirality trial of sweep A.istep_spin, then director trial of sweep A.istep,
+// at packed site m of colour 0; FP64 arithmetic, Philox draws.  After sweep
+// t's colour-1 director and chirality trials every input of colour-0 site m
+// for its chirality trial in sweep t (own director, colour-1 directors) and
+// for its director trial in sweep t + 1 (own director and spin after the flip,
+// colour-1 directors and spins) is final, and by the argument of
+// dir_spin_site no colour-0 chirality trial reads another colour-0 site.  The
+// pair therefore gives the sequential outcome; the cone step of sweep t + 1 is
+// fixed before this launch, since it depends on sweep t's director accepts
+// only (chll/chll_mod.f90:83-90).
+template <int BC, bool OFF32>
+__device__ inline bool spin_dir_site(const Geom &G, const Fields &F, const PhaseArgs &A, uint32_t m, bool &flip)
+{
+    SiteIn S;
+    const uint32_t own = load_site<BC, OFF32>(G, F, 0, m, S);
+    PhaseArgs As = A;
+    As.istep = A.istep_spin;
+    flip = spin_compute<ARITH_FP64, RNG_PHILOX>(As, 0, m, S.N, S.nx, S.ny, S.nz, S.sg);
+    const int sg = flip ? -S.sg : S.sg;
+    double t[3];
+    const bool acc = dir_compute<ARITH_FP64, RNG_PHILOX>(A, 0, m, S.N, S.nx, S.ny, S.nz, sg, *A.d, t);
+    if (flip) fld<OFF32>(F.s, own) = (int8_t)sg;
+    if (acc) {
+        fld<OFF32>(F.nx, own) = t[0];
+        fld<OFF32>(F.ny, own) = t[1];
+        fld<OFF32>(F.nz, own) = t[2];
+    }
+    return acc;
+}
+
 // ---------------------------------------------------------------------------
 // Observables.  Per site: the reference's reported energy term (periodic i
 // and j, open k, +y bond x-component from the diagonal site (i+1, j+1, k),

@@ -142,14 +142,18 @@ int chll_lattice_get_kernel(chll_lattice *lat, int32_t kind, int32_t arith, int3
                             int32_t *nslab);
 /* FP64 sweeps (on by default): each colour-1 site's chirality trial runs in
  * the colour-1 director launch, right after that site's director trial, when
- * the director kernel is the one-site kernel (variant 0).  FP64 chirality
- * energy changes do not read neighbour spins, so the lattice, counts and rows
- * are the same as with four separate half-sweeps.  No counterpart in the
- * reference (chll/chll_mod.f90:70-81 runs the four half-sweeps in turn). */
+ * the director kernel is the one-site kernel (variant 0); with Philox draws
+ * each colour-0 chirality trial also shares a launch with the same site's
+ * director trial of the next sweep (not across a measurement sweep or the end
+ * of the call).  FP64 chirality energy changes do not read neighbour spins,
+ * so the lattice, counts and rows are the same as with four separate
+ * half-sweeps.  No counterpart in the reference (chll/chll_mod.f90:70-81 runs
+ * the four half-sweeps in turn). */
 int chll_lattice_set_fusion(chll_lattice *lat, int32_t on);
 int chll_lattice_get_fusion(chll_lattice *lat, int32_t *on);
 /* Per-kernel HIP-event timing: kinds 0 director, 1 chirality, 2 measure,
- * 3 finalize, 4 uniform fill, 5 fused director + chirality. */
+ * 3 finalize, 4 uniform fill, 5 fused colour-1 director + chirality, 6 fused
+ * colour-0 chirality + next sweep's colour-0 director. */
 int chll_lattice_set_timing(chll_lattice *lat, int32_t on);
 int chll_lattice_timing(chll_lattice *lat, double *ms, int64_t *launches, int32_t nkinds);
 

@@ -263,9 +263,11 @@ def test_kernel_shapes_identical(dims, bc, arith):
 
 @pytest.mark.parametrize("rng", ["philox", "stream"])
 def test_fused_sweep_counts_and_timing(rng):
-    """FP64 sweeps launch the fused colour-1 kernel once per sweep (and one
-    chirality launch), and give the same lattice, counts and rows as the four
-    separate half-sweeps; compat sweeps never fuse."""
+    """FP64 sweeps launch the fused colour-1 kernel once per sweep; with Philox
+    draws the colour-0 chirality trials of sweeps 1-4 share launches with the
+    director trials of sweeps 2-5 (sweep 5 is measured and last, so its
+    chirality colour 0 runs alone).  The lattice, per-sweep counts and rows
+    equal those of four separate half-sweeps; compat sweeps never fuse."""
     dims = (64, 32, 16)
     st0 = isotropic_state(*dims, 31)
     res = {}
@@ -281,8 +283,13 @@ def test_fused_sweep_counts_and_timing(rng):
             t = lat.timing()
             res[fuse] = (lat.download(), out["counts"], out["rows"], t)
     t_on, t_off = res[True][3], res[False][3]
-    assert t_on["fused"][1] == 5 and t_on["director"][1] == 5 and t_on["chirality"][1] == 5
-    assert t_off["fused"][1] == 0 and t_off["director"][1] == 10 and t_off["chirality"][1] == 10
+    launches = {k: t_on[k][1] for k in ("director", "chirality", "fused", "fused_next")}
+    if rng == "philox":
+        assert launches == {"director": 1, "chirality": 1, "fused": 5, "fused_next": 4}, launches
+    else:
+        assert launches == {"director": 5, "chirality": 5, "fused": 5, "fused_next": 0}, launches
+    assert t_off["fused"][1] == 0 and t_off["fused_next"][1] == 0
+    assert t_off["director"][1] == 10 and t_off["chirality"][1] == 10
     for f in ("nx", "ny", "nz", "s"):
         assert np.array_equal(res[True][0][f], res[False][0][f]), f
     assert np.array_equal(res[True][1], res[False][1]) and np.array_equal(res[True][2], res[False][2])
@@ -291,4 +298,33 @@ def test_fused_sweep_counts_and_timing(rng):
         lat.set_timing(True)
         lat.sweeps(1, 2, 0, g=0.5, KK=1.0, kbt=0.7, arith="compat", rng=rng, seed=11, want_rows=False)
         t = lat.timing()
-    assert t["fused"][1] == 0 and t["director"][1] == 4 and t["chirality"][1] == 4
+    assert t["fused"][1] == 0 and t["fused_next"][1] == 0 and t["director"][1] == 4 and t["chirality"][1] == 4
+
+
+def test_pipelined_counts_across_calls():
+    """Sweeps split over several calls (each call closes its last sweep) give
+    the same lattice and per-sweep counts as one call, with and without the
+    fused launches, measurement rows at every third sweep included."""
+    dims = (32, 32, 16)
+    st0 = isotropic_state(*dims, 77)
+    kw = dict(g=0.3, KK=1.2, kbt=0.9, arith="fp64", rng="philox", seed=5)
+    outs = []
+    for fuse, splits in ((True, [(1, 10)]), (True, [(1, 4), (5, 5), (6, 10)]), (False, [(1, 10)])):
+        print(f"pipelined counts fuse={fuse} splits={splits}", flush=True)
+        with Lattice(*dims) as lat:
+            lat.set_kernel("site", 1, 1, kinds="director")
+            lat.set_fusion(fuse)
+            lat.upload_state(st0)
+            lat.set_step(0.3)
+            counts, rows = [], []
+            for a, b in splits:
+                o = lat.sweeps(a, b, 3, want_counts=True, **kw)
+                counts.append(o["counts"])
+                rows.append(o["rows"])
+            outs.append((lat.download(), np.concatenate(counts), np.concatenate(rows), lat.get_step()))
+    for o in outs[1:]:
+        for f in ("nx", "ny", "nz", "s"):
+            assert np.array_equal(o[0][f], outs[0][0][f]), f
+        assert np.array_equal(o[1], outs[0][1])
+        assert np.array_equal(o[2], outs[0][2])
+        assert o[3] == outs[0][3]
