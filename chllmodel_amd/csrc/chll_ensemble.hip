@@ -41,7 +41,7 @@ struct RepDesc {
     uint32_t replica;          // Philox counter word
     uint32_t pad;
     double g, KK;
-    uint32_t *counts;          // 4 phases x kSlots x kSlotStride
+    uint32_t *counts;          // kGroups x kSlots x kSlotStride (groups as the single lattice)
 };
 
 // Ensemble-wide device state.
@@ -65,6 +65,10 @@ struct EnsDev {
     uint32_t run_index, K;
 };
 
+// kind 0: director trials of `colour`, 1: chirality trials of `colour`; FP64
+// only, 2: colour-1 director trials each followed by the site's chirality
+// trial, 3: colour-0 chirality trials of sweep t each followed by the site's
+// director trial of sweep t + 1 (the single-lattice fused launches).
 template <int ARITH, int BC, int BS>
 __global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int kind, int colour, uint32_t istep_off)
 {
@@ -74,15 +78,31 @@ __global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int kind, int colour
     const int s = E.rep_slot[r];
     PhaseArgs A;
     A.g = D.g; A.KK = D.KK; A.kbt = E.kbt[s]; A.inv_kbt = E.inv_kbt[s]; A.d = E.d + s;
-    A.u_phi = nullptr; A.u_acc = nullptr;
+    A.u_phi = nullptr; A.u_acc = nullptr; A.u_spin = nullptr;
     A.seed = E.seed; A.run_index = E.run_index; A.replica = D.replica; A.istep = *E.istep + istep_off;
+    A.istep_spin = A.istep;
     const uint32_t m = lb * BS + threadIdx.x;
-    bool acc = false;
+    const size_t gs = (size_t)kSlots * kSlotStride;
+    bool acc = false, flip = false;
+    if constexpr (ARITH == ARITH_FP64) {
+        if (kind == 2) {
+            if (m < D.G.nsub) acc = dir_spin_site<RNG_PHILOX, BC, false>(D.G, D.F, 1, A, m, flip);
+            block_count2<BS>(acc, flip, D.counts + kGroupAcc1 * gs, D.counts + flip_group(A.istep, 1) * gs, lb);
+            return;
+        }
+        if (kind == 3) {
+            A.istep = A.istep_spin + 1;
+            if (m < D.G.nsub) acc = spin_dir_site<BC, false>(D.G, D.F, A, m, flip);
+            block_count2<BS>(acc, flip, D.counts + kGroupAcc0 * gs, D.counts + flip_group(A.istep_spin, 0) * gs,
+                             lb);
+            return;
+        }
+    }
     if (m < D.G.nsub) {
         if (kind == 0) acc = dir_site<ARITH, RNG_PHILOX, BC>(D.G, D.F, colour, A, m);
         else acc = spin_site<ARITH, RNG_PHILOX, BC>(D.G, D.F, colour, A, m);
     }
-    block_count<BS>(acc, D.counts + (size_t)(2 * kind + colour) * kSlots * kSlotStride, lb);
+    block_count<BS>(acc, D.counts + (kind == 0 ? colour : flip_group(A.istep, colour)) * gs, lb);
 }
 
 template <int BC, int BS>
@@ -102,18 +122,29 @@ __global__ __launch_bounds__(BS) void k_measure_ens(EnsDev E)
 // on measuring sweeps the observable row and the replica's energy for the swap.
 constexpr int kEnsFin = 256;
 
-__global__ __launch_bounds__(kEnsFin) void k_finalize_ens(EnsDev E, uint32_t istep_off, int measure)
+// The director accepts of sweep istep are always closed (cone-step update);
+// close_prev zeroes the flips of sweep istep - 1, close_this counts those of
+// sweep istep (for the row) and zeroes them.
+__global__ __launch_bounds__(kEnsFin) void k_finalize_ens(EnsDev E, uint32_t istep_off, int measure, int close_prev,
+                                                          int close_this)
 {
     const int r = blockIdx.x;
     const RepDesc &D = E.R[r];
     const uint32_t t = threadIdx.x;
+    const uint32_t istep = *E.istep + istep_off;
+    const int gt = flip_group((int32_t)istep, 0), gp = flip_group((int32_t)istep - 1, 0);
     unsigned long long ca = 0, cf = 0;
-    for (uint32_t q = t; q < 4 * kSlots; q += kEnsFin) {
-        uint32_t *slot = D.counts + (q / kSlots) * (kSlots * kSlotStride) + (q % kSlots) * kSlotStride;
+    for (uint32_t q = t; q < kGroups * kSlots; q += kEnsFin) {
+        const int g = (int)(q / kSlots);
+        const bool ia = g < 2;
+        const bool it = close_this && (g == gt || g == gt + 1);
+        const bool ip = close_prev && (g == gp || g == gp + 1);
+        if (!(ia || it || ip)) continue;
+        uint32_t *slot = D.counts + g * (kSlots * kSlotStride) + (q % kSlots) * kSlotStride;
         const uint32_t c = *slot;
         *slot = 0u;
-        if (q < 2 * kSlots) ca += c;
-        else cf += c;
+        if (ia) ca += c;
+        else if (it) cf += c;
     }
     double v[8];
 #pragma unroll
@@ -155,7 +186,6 @@ __global__ __launch_bounds__(kEnsFin) void k_finalize_ens(EnsDev E, uint32_t ist
         for (int w = 0; w < kEnsFin / 64; ++w) a += red[w][q];
         tot[q] = a;
     }
-    const uint32_t istep = *E.istep + istep_off;
     const int s = E.rep_slot[r];
     const double n3 = 2.0 * (double)D.G.nsub;
     const double n3f = (double)(float)n3;
@@ -269,7 +299,8 @@ struct chll_ensemble {
     int32_t rows_cap = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int graph_K = 0, graph_bs = 0, graph_adapt = -1;
+    int graph_K = 0, graph_bs = 0, graph_adapt = -1, graph_fuse = -1;
+    bool fuse = true;  // FP64: the single lattice's fused launches
     uint64_t graph_seed = 0;
     uint32_t graph_run = 0;
     double *pinned = nullptr;
@@ -326,16 +357,38 @@ int build_tables(chll_ensemble *X)
     return 0;
 }
 
+// One block of K sweeps.  FP64 with fusion on, as the single lattice: D0 of
+// the first sweep alone, then per sweep t the fused D1 + C1 launch and either
+// (t < K - 1) the epilogue closing t's director accepts and the fused C0(t) +
+// D0(t + 1) launch, or (last sweep, which is measured) C0 alone, the
+// observable reduction and the closing epilogue.  The swap kernel then moves
+// replicas between temperature slots, so no fused launch crosses it.
 template <int ARITH, int BC, int BS>
 void launch_block_t(chll_ensemble *X, int K, hipStream_t st)
 {
     const EnsDev &E = X->E;
+    const dim3 g(X->nblocks), b(BS), gf(X->nrep), bf(kEnsFin);
+    const bool fuse = ARITH == ARITH_FP64 && X->fuse;
     for (int t = 0; t < K; ++t) {
-        for (int ph = 0; ph < 4; ++ph)
-            hipLaunchKernelGGL((k_phase_ens<ARITH, BC, BS>), dim3(X->nblocks), dim3(BS), 0, st, E, ph >> 1, ph & 1,
-                               (uint32_t)t);
-        if (t + 1 == K) hipLaunchKernelGGL((k_measure_ens<BC, BS>), dim3(X->nblocks), dim3(BS), 0, st, E);
-        hipLaunchKernelGGL(k_finalize_ens, dim3(X->nrep), dim3(kEnsFin), 0, st, E, (uint32_t)t, t + 1 == K ? 1 : 0);
+        const uint32_t off = (uint32_t)t;
+        const bool last = t + 1 == K;
+        if (!fuse) {
+            for (int ph = 0; ph < 4; ++ph)
+                hipLaunchKernelGGL((k_phase_ens<ARITH, BC, BS>), g, b, 0, st, E, ph >> 1, ph & 1, off);
+            if (last) hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E);
+            hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, last ? 1 : 0, 0, 1);
+            continue;
+        }
+        if (t == 0) hipLaunchKernelGGL((k_phase_ens<ARITH, BC, BS>), g, b, 0, st, E, 0, 0, off);
+        hipLaunchKernelGGL((k_phase_ens<ARITH, BC, BS>), g, b, 0, st, E, 2, 1, off);
+        if (!last) {
+            hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 0, t > 0 ? 1 : 0, 0);
+            hipLaunchKernelGGL((k_phase_ens<ARITH, BC, BS>), g, b, 0, st, E, 3, 0, off);
+        } else {
+            hipLaunchKernelGGL((k_phase_ens<ARITH, BC, BS>), g, b, 0, st, E, 1, 0, off);
+            hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E);
+            hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 1, t > 0 ? 1 : 0, 1);
+        }
     }
     hipLaunchKernelGGL(k_swap_ens, dim3(1), dim3(256), 0, st, E, 1);
 }
@@ -516,6 +569,13 @@ int chll_ensemble_set_stream(chll_ensemble *X, void *s)
     return 0;
 }
 
+int chll_ensemble_set_fusion(chll_ensemble *X, int32_t on)
+{
+    if (!X) return efail("null ensemble");
+    X->fuse = on != 0;
+    return 0;
+}
+
 int chll_ensemble_set_block(chll_ensemble *X, int32_t bs)
 {
     if (!X) return efail("null ensemble");
@@ -602,7 +662,7 @@ int chll_ensemble_run(chll_ensemble *X, uint64_t seed, uint32_t run_index, int32
     }
     // a captured graph bakes in the kernel arguments: rebuild it when they change
     if (X->graph && (X->graph_K != K || X->graph_bs != X->bs || X->graph_adapt != adapt || X->graph_seed != seed ||
-                     X->graph_run != run_index || E.max_rows != max_rows))
+                     X->graph_run != run_index || E.max_rows != max_rows || X->graph_fuse != (int)X->fuse))
         drop_graph(X);
     E.seed = seed;
     E.run_index = run_index;
@@ -629,6 +689,7 @@ int chll_ensemble_run(chll_ensemble *X, uint64_t seed, uint32_t run_index, int32
         X->graph_adapt = adapt;
         X->graph_seed = seed;
         X->graph_run = run_index;
+        X->graph_fuse = (int)X->fuse;
     }
     for (int b = 0; b < nblocks; ++b) {
         if (use_graph) ECHECK(hipGraphLaunch(X->gexec, st));

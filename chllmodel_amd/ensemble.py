@@ -44,6 +44,7 @@ def _declare(L):
     L.chll_ensemble_destroy.argtypes = [ct.c_void_p]
     L.chll_ensemble_set_stream.argtypes = [ct.c_void_p, ct.c_void_p]
     L.chll_ensemble_set_block.argtypes = [ct.c_void_p, ct.c_int32]
+    L.chll_ensemble_set_fusion.argtypes = [ct.c_void_p, ct.c_int32]
     L.chll_ensemble_blocks.argtypes = [ct.c_void_p]
     L.chll_ensemble_upload.argtypes = [ct.c_void_p, ct.c_int32, _f64p, _f64p, _f64p, _i32p]
     L.chll_ensemble_download.argtypes = [ct.c_void_p, ct.c_int32, _f64p, _f64p, _f64p, _i32p]
@@ -52,7 +53,7 @@ def _declare(L):
     L.chll_ensemble_sync.argtypes = [ct.c_void_p]
     L.chll_ensemble_state.argtypes = [ct.c_void_p, _i32p, _f64p, _f64p, _i64p, _i64p, ct.POINTER(ct.c_uint32)]
     for n in ("chll_ensemble_create", "chll_ensemble_destroy", "chll_ensemble_set_stream", "chll_ensemble_set_block",
-              "chll_ensemble_blocks", "chll_ensemble_upload", "chll_ensemble_download", "chll_ensemble_run",
+              "chll_ensemble_set_fusion", "chll_ensemble_blocks", "chll_ensemble_upload", "chll_ensemble_download", "chll_ensemble_run",
               "chll_ensemble_sync", "chll_ensemble_state"):
         getattr(L, n).restype = ct.c_int
     L._ens_declared = True
@@ -109,6 +110,11 @@ class Ensemble:
     def set_block(self, bs: int):
         _check(lib().chll_ensemble_set_block(self._h, int(bs)))
         self.block_size = int(bs)
+
+    def set_fusion(self, on: bool):
+        """FP64: fused director + chirality launches within each block of
+        sweeps (same results, fewer passes over the replicas)."""
+        _check(lib().chll_ensemble_set_fusion(self._h, 1 if on else 0))
 
     def set_stream(self, ptr):
         _check(lib().chll_ensemble_set_stream(self._h, ct.c_void_p(ptr or 0)))

@@ -181,6 +181,9 @@ int chll_ensemble_destroy(chll_ensemble *e);
 int chll_ensemble_set_stream(chll_ensemble *e, void *hip_stream);
 /* threads per block of the batched kernels: 64, 128, 256 or 512 */
 int chll_ensemble_set_block(chll_ensemble *e, int32_t block_size);
+/* FP64 ensembles (on by default): the fused launches of chll_lattice_set_fusion
+ * inside each block of K sweeps; results are unchanged. */
+int chll_ensemble_set_fusion(chll_ensemble *e, int32_t on);
 int32_t chll_ensemble_blocks(chll_ensemble *e);
 int chll_ensemble_upload(chll_ensemble *e, int32_t rep, const double *nx, const double *ny, const double *nz,
                          const int32_t *s);

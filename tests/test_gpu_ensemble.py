@@ -107,6 +107,30 @@ def test_graph_replays_equal_eager():
             assert np.array_equal(sa[f], sb[f])
 
 
+@pytest.mark.parametrize("K", [1, 2, 5])
+def test_fused_blocks_equal_unfused(K):
+    """FP64 blocks with the fused launches (D1 + C1 in one launch, C0 of
+    sweep t with D0 of sweep t + 1 in another) give the same replicas, rows,
+    slots and swap counts as four separate half-sweeps per sweep."""
+    reps = small_replicas()
+    outs = []
+    for fuse in (False, True):
+        print(f"ensemble fusion K={K} fuse={fuse}", flush=True)
+        with Ensemble(reps, arith="fp64") as ens:
+            ens.set_fusion(fuse)
+            ens.init_isotropic(21)
+            rows = ens.run(4, K, seed=13, use_graph=True)
+            st = ens.state()
+            outs.append((rows, [ens.download(r) for r in range(len(reps))], st))
+    (ra, sa, ta), (rb, sb, tb) = outs
+    assert np.array_equal(ra, rb)
+    for a, b in zip(sa, sb):
+        for f in ("nx", "ny", "nz", "s"):
+            assert np.array_equal(a[f], b[f])
+    for k in ("rep_slot", "slot_d", "swap_att", "swap_acc"):
+        assert np.array_equal(ta[k], tb[k]), k
+
+
 def test_single_replica_equals_lattice_path():
     r = {"ni": 16, "nj": 12, "nk": 10, "ladder": 0, "g": 0.4, "KK": 1.2, "kbt": 0.7, "d0": 0.3}
     st = isotropic_state(16, 12, 10, 5)
