@@ -1,8 +1,8 @@
 """Project autotuner for the single-lattice half-sweep kernels.
 
 For one lattice shape, boundary condition and arithmetic it times every
-candidate kernel (one site per thread in lattice order or in XCD row bands, or
-the k-march with each tile shape and a few slab counts) for the director and
+candidate kernel (one or two sites per thread, in lattice order or in XCD row
+bands, or the k-march with each tile shape and a few slab counts) for the director and
 the chirality half-sweeps separately, with HIP events on the launch stream, and
 applies the fastest.  For FP64 it then times whole sweeps of those kernels
 against sweeps with the fused colour-1 director + chirality launch and keeps
@@ -24,6 +24,8 @@ from .states import isotropic_state
 
 def candidates(ni, nj, nk):
     out = [("site", 0, 1), ("site", 1, 1)]
+    if ni % 4 == 0:
+        out += [("site", 2, 1), ("site", 3, 1)]
     half = ni // 2
     for tile, (th, tj) in enumerate(((64, 4), (32, 8), (16, 16))):
         if th > 2 * half and tile < 2:

@@ -94,8 +94,9 @@ struct chll_lattice {
     size_t counts_cap = 0;
     double *pinned = nullptr;  // host: 8 doubles
     // half-sweep kernel per [kind: director, chirality][arith: compat, fp64]:
-    // variant 0 = one site per thread (tile = block order: 0 lattice, 1 XCD
-    // row bands), 1 = k-march through LDS (tile shape, slabs)
+    // variant 0 = one site per thread (tile bit 0: XCD row bands, bit 1: two
+    // sites per thread with pair loads), 1 = k-march through LDS (tile shape,
+    // slabs)
     struct KCfg { int variant, tile; uint32_t nslab; } kcfg[2][2];
     // FP64 sweeps: run each colour-1 site's chirality trial inside the colour-1
     // director launch (needs the one-site director kernel)
@@ -232,8 +233,8 @@ int chll_lattice_set_kernel(chll_lattice *L, int32_t kind, int32_t arith, int32_
     if (kind != 0 && kind != 1) return fail("kind must be 0 (director) or 1 (chirality)");
     if (arith != ARITH_COMPAT && arith != ARITH_FP64) return fail("unknown arithmetic mode %d", arith);
     if (variant != 0 && variant != 1) return fail("kernel variant must be 0 (site) or 1 (march)");
-    if (variant == 0 && (tile < 0 || tile > 1))
-        return fail("site kernel order must be 0 (lattice order) or 1 (XCD row bands)");
+    if (variant == 0 && (tile < 0 || tile > 3))
+        return fail("site kernel order must be 0..3 (bit 0: XCD row bands, bit 1: site pairs)");
     if (tile < 0 || tile > 5) return fail("tile must be 0..2 (k-march 64x4, 32x8, 16x16) or 3..5 (plane tiles)");
     if (nslab < 1) return fail("nslab must be >= 1");
     L->kcfg[kind][arith] = {variant, tile, (uint32_t)nslab > L->G.nk ? L->G.nk : (uint32_t)nslab};

@@ -136,6 +136,41 @@ __global__ __launch_bounds__(kBlock) void k_spin_dir(Geom G, Fields F, PhaseArgs
     block_count2<kBlock>(acc, flip, counts_dir, counts_spin, blockIdx.x);
 }
 
+// Two sites per thread (load_pair: 16-byte loads of site pairs).  OP 0:
+// director trials, 1: chirality trials, 2: director then chirality trials
+// (FP64), 3: chirality trials of sweep A.istep_spin then director trials of
+// sweep A.istep (FP64, Philox, colour 0).  Counts: OP 0 accepts and OP 1 flips
+// into c0; OP 2 and 3 director accepts into c0, flips into c1.  Needs an even
+// half (ni % 4 == 0) and fields below 4 GiB.
+template <int OP, int ARITH, int RNG, int BC>
+__global__ __launch_bounds__(kBlock) void k_pair(Geom G, Fields F, int colour, PhaseArgs A,
+                                                 uint32_t *__restrict__ c0, uint32_t *__restrict__ c1,
+                                                 uint32_t band)
+{
+    const uint32_t m0 = 2u * (band_block(blockIdx.x, band) * kBlock + threadIdx.x);
+    bool a0 = false, a1 = false, f0 = false, f1 = false;
+    if (m0 < G.nsub) {
+        SiteIn S0, S1;
+        const uint32_t own = load_pair<BC, true>(G, F, colour, m0, S0, S1);
+        if constexpr (OP == 0) {
+            a0 = dir_apply<ARITH, RNG, true>(F, colour, A, m0, own, S0);
+            a1 = dir_apply<ARITH, RNG, true>(F, colour, A, m0 + 1u, own + 1u, S1);
+        } else if constexpr (OP == 1) {
+            a0 = spin_apply<ARITH, RNG, true>(F, colour, A, m0, own, S0);
+            a1 = spin_apply<ARITH, RNG, true>(F, colour, A, m0 + 1u, own + 1u, S1);
+        } else if constexpr (OP == 2) {
+            a0 = dir_spin_apply<RNG, true>(F, colour, A, m0, own, S0, f0);
+            a1 = dir_spin_apply<RNG, true>(F, colour, A, m0 + 1u, own + 1u, S1, f1);
+        } else {
+            a0 = spin_dir_apply<true>(F, A, m0, own, S0, f0);
+            a1 = spin_dir_apply<true>(F, A, m0 + 1u, own + 1u, S1, f1);
+        }
+    }
+    const uint32_t wa = (uint32_t)(__popcll(__ballot(a0)) + __popcll(__ballot(a1)));
+    const uint32_t wf = (uint32_t)(__popcll(__ballot(f0)) + __popcll(__ballot(f1)));
+    block_sum2<kBlock>(wa, wf, c0, OP < 2 ? c0 : c1, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Observables over the whole lattice (grid-stride), one partial per block.
 template <int BC>
@@ -318,11 +353,33 @@ bool field_off32(const Geom &G)
     return (2ull * G.nsub + kFieldPad) * sizeof(double) <= 0xffffffffull;
 }
 
+bool pairs_ok(const Geom &G) { return (G.half & 1u) == 0u && field_off32(G); }
+
+// Pair kernels: nsub / 2 threads; XCD bands of whole blocks of pairs.
+static inline dim3 grid_pairs(const Geom &G) { return grid1(G.nsub / 2u); }
+static uint32_t pair_band(const Geom &G, int order)
+{
+    const uint32_t per = 16u * kBlock;  // 8 bands x kBlock pairs
+    return (order & 1) && G.plane % per == 0u ? G.plane / per : 0u;
+}
+static uint32_t site_band(const Geom &G, int order) { return (order & 1) ? xcd_band(G) : 0u; }
+
 template <int ARITH, int RNG, int BC>
 static void launch_phase_t(int kind, const Geom &G, const Fields &F, int colour, const PhaseArgs &A,
-                           uint32_t *counts, uint32_t band, hipStream_t s)
+                           uint32_t *counts, int order, hipStream_t s)
 {
-    const dim3 g = grid1(G.nsub), b(kBlock);
+    const dim3 b(kBlock);
+    if ((order & 2) && pairs_ok(G)) {
+        const dim3 g = grid_pairs(G);
+        const uint32_t band = pair_band(G, order);
+        if (kind == 0)
+            hipLaunchKernelGGL((k_pair<0, ARITH, RNG, BC>), g, b, 0, s, G, F, colour, A, counts, counts, band);
+        else
+            hipLaunchKernelGGL((k_pair<1, ARITH, RNG, BC>), g, b, 0, s, G, F, colour, A, counts, counts, band);
+        return;
+    }
+    const dim3 g = grid1(G.nsub);
+    const uint32_t band = site_band(G, order);
     if (field_off32(G)) {
         if (kind == 0) hipLaunchKernelGGL((k_dir<ARITH, RNG, BC, true>), g, b, 0, s, G, F, colour, A, counts, band);
         else hipLaunchKernelGGL((k_spin<ARITH, RNG, BC, true>), g, b, 0, s, G, F, colour, A, counts, band);
@@ -334,9 +391,16 @@ static void launch_phase_t(int kind, const Geom &G, const Fields &F, int colour,
 
 template <int RNG, int BC>
 static void launch_fused_t(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t *cd,
-                           uint32_t *cs, uint32_t band, hipStream_t s)
+                           uint32_t *cs, int order, hipStream_t s)
 {
-    const dim3 g = grid1(G.nsub), b(kBlock);
+    const dim3 b(kBlock);
+    if ((order & 2) && pairs_ok(G)) {
+        hipLaunchKernelGGL((k_pair<2, ARITH_FP64, RNG, BC>), grid_pairs(G), b, 0, s, G, F, colour, A, cd, cs,
+                           pair_band(G, order));
+        return;
+    }
+    const dim3 g = grid1(G.nsub);
+    const uint32_t band = site_band(G, order);
     if (field_off32(G)) hipLaunchKernelGGL((k_dir_spin<RNG, BC, true>), g, b, 0, s, G, F, colour, A, cd, cs, band);
     else hipLaunchKernelGGL((k_dir_spin<RNG, BC, false>), g, b, 0, s, G, F, colour, A, cd, cs, band);
 }
@@ -344,21 +408,32 @@ static void launch_fused_t(const Geom &G, const Fields &F, int colour, const Pha
 void launch_phase_fused(int rng, int bc, const Geom &G, const Fields &F, int colour, const PhaseArgs &A,
                         uint32_t *counts_dir, uint32_t *counts_spin, int order, hipStream_t s)
 {
-    const uint32_t band = order == 1 ? xcd_band(G) : 0u;
     if (rng == RNG_PHILOX) {
-        if (bc == BC_OPEN) launch_fused_t<RNG_PHILOX, BC_OPEN>(G, F, colour, A, counts_dir, counts_spin, band, s);
-        else launch_fused_t<RNG_PHILOX, BC_PERIODIC>(G, F, colour, A, counts_dir, counts_spin, band, s);
+        if (bc == BC_OPEN) launch_fused_t<RNG_PHILOX, BC_OPEN>(G, F, colour, A, counts_dir, counts_spin, order, s);
+        else launch_fused_t<RNG_PHILOX, BC_PERIODIC>(G, F, colour, A, counts_dir, counts_spin, order, s);
     } else {
-        if (bc == BC_OPEN) launch_fused_t<RNG_STREAM, BC_OPEN>(G, F, colour, A, counts_dir, counts_spin, band, s);
-        else launch_fused_t<RNG_STREAM, BC_PERIODIC>(G, F, colour, A, counts_dir, counts_spin, band, s);
+        if (bc == BC_OPEN) launch_fused_t<RNG_STREAM, BC_OPEN>(G, F, colour, A, counts_dir, counts_spin, order, s);
+        else launch_fused_t<RNG_STREAM, BC_PERIODIC>(G, F, colour, A, counts_dir, counts_spin, order, s);
     }
 }
 
 void launch_phase_fused_next(int bc, const Geom &G, const Fields &F, const PhaseArgs &A, uint32_t *counts_dir,
                              uint32_t *counts_spin, int order, hipStream_t s)
 {
-    const uint32_t band = order == 1 ? xcd_band(G) : 0u;
-    const dim3 g = grid1(G.nsub), b(kBlock);
+    const dim3 b(kBlock);
+    if ((order & 2) && pairs_ok(G)) {
+        const dim3 g = grid_pairs(G);
+        const uint32_t band = pair_band(G, order);
+        if (bc == BC_OPEN)
+            hipLaunchKernelGGL((k_pair<3, ARITH_FP64, RNG_PHILOX, BC_OPEN>), g, b, 0, s, G, F, 0, A, counts_dir,
+                               counts_spin, band);
+        else
+            hipLaunchKernelGGL((k_pair<3, ARITH_FP64, RNG_PHILOX, BC_PERIODIC>), g, b, 0, s, G, F, 0, A, counts_dir,
+                               counts_spin, band);
+        return;
+    }
+    const uint32_t band = site_band(G, order);
+    const dim3 g = grid1(G.nsub);
     const bool o32 = field_off32(G);
     if (bc == BC_OPEN) {
         if (o32) hipLaunchKernelGGL((k_spin_dir<BC_OPEN, true>), g, b, 0, s, G, F, A, counts_dir, counts_spin, band);
@@ -374,23 +449,22 @@ void launch_phase_fused_next(int bc, const Geom &G, const Fields &F, const Phase
 void launch_phase(int kind, int arith, int rng, int bc, const Geom &G, const Fields &F, int colour,
                   const PhaseArgs &A, uint32_t *counts, int order, hipStream_t s)
 {
-    const uint32_t band = order == 1 ? xcd_band(G) : 0u;
     const bool philox = (rng == RNG_PHILOX);
     if (arith == ARITH_COMPAT) {
         if (philox) {
-            if (bc == BC_OPEN) launch_phase_t<ARITH_COMPAT, RNG_PHILOX, BC_OPEN>(kind, G, F, colour, A, counts, band, s);
-            else launch_phase_t<ARITH_COMPAT, RNG_PHILOX, BC_PERIODIC>(kind, G, F, colour, A, counts, band, s);
+            if (bc == BC_OPEN) launch_phase_t<ARITH_COMPAT, RNG_PHILOX, BC_OPEN>(kind, G, F, colour, A, counts, order, s);
+            else launch_phase_t<ARITH_COMPAT, RNG_PHILOX, BC_PERIODIC>(kind, G, F, colour, A, counts, order, s);
         } else {
-            if (bc == BC_OPEN) launch_phase_t<ARITH_COMPAT, RNG_STREAM, BC_OPEN>(kind, G, F, colour, A, counts, band, s);
-            else launch_phase_t<ARITH_COMPAT, RNG_STREAM, BC_PERIODIC>(kind, G, F, colour, A, counts, band, s);
+            if (bc == BC_OPEN) launch_phase_t<ARITH_COMPAT, RNG_STREAM, BC_OPEN>(kind, G, F, colour, A, counts, order, s);
+            else launch_phase_t<ARITH_COMPAT, RNG_STREAM, BC_PERIODIC>(kind, G, F, colour, A, counts, order, s);
         }
     } else {
         if (philox) {
-            if (bc == BC_OPEN) launch_phase_t<ARITH_FP64, RNG_PHILOX, BC_OPEN>(kind, G, F, colour, A, counts, band, s);
-            else launch_phase_t<ARITH_FP64, RNG_PHILOX, BC_PERIODIC>(kind, G, F, colour, A, counts, band, s);
+            if (bc == BC_OPEN) launch_phase_t<ARITH_FP64, RNG_PHILOX, BC_OPEN>(kind, G, F, colour, A, counts, order, s);
+            else launch_phase_t<ARITH_FP64, RNG_PHILOX, BC_PERIODIC>(kind, G, F, colour, A, counts, order, s);
         } else {
-            if (bc == BC_OPEN) launch_phase_t<ARITH_FP64, RNG_STREAM, BC_OPEN>(kind, G, F, colour, A, counts, band, s);
-            else launch_phase_t<ARITH_FP64, RNG_STREAM, BC_PERIODIC>(kind, G, F, colour, A, counts, band, s);
+            if (bc == BC_OPEN) launch_phase_t<ARITH_FP64, RNG_STREAM, BC_OPEN>(kind, G, F, colour, A, counts, order, s);
+            else launch_phase_t<ARITH_FP64, RNG_STREAM, BC_PERIODIC>(kind, G, F, colour, A, counts, order, s);
         }
     }
 }

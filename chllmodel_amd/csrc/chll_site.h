@@ -36,16 +36,17 @@ __device__ inline T &fld(T *base, uint32_t q)
     return base[q];
 }
 
-// The same for two flags, into two slot arrays (thread 0 and thread 1 add).
+// Block-wide sums of two per-wave counts w0, w1 (the same in every lane of a
+// wave), added to slot blk % kSlots of slots0 and slots1 (thread 0 and thread
+// 1 add).
 template <int BS>
-__device__ inline void block_count2(bool f0, bool f1, uint32_t *slots0, uint32_t *slots1, uint32_t blk)
+__device__ inline void block_sum2(uint32_t w0, uint32_t w1, uint32_t *slots0, uint32_t *slots1, uint32_t blk)
 {
     __shared__ uint32_t wsum[2][BS / 64];
-    const uint64_t b0 = __ballot(f0), b1 = __ballot(f1);
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     if (lane == 0) {
-        wsum[0][wid] = (uint32_t)__popcll(b0);
-        wsum[1][wid] = (uint32_t)__popcll(b1);
+        wsum[0][wid] = w0;
+        wsum[1][wid] = w1;
     }
     __syncthreads();
     if (threadIdx.x < 2) {
@@ -55,6 +56,13 @@ __device__ inline void block_count2(bool f0, bool f1, uint32_t *slots0, uint32_t
         uint32_t *slots = threadIdx.x ? slots1 : slots0;
         if (t) atomicAdd(&slots[(blk & (kSlots - 1)) * kSlotStride], t);
     }
+}
+
+// Block-wide sums of two 0/1 flags per thread.
+template <int BS>
+__device__ inline void block_count2(bool f0, bool f1, uint32_t *slots0, uint32_t *slots1, uint32_t blk)
+{
+    block_sum2<BS>((uint32_t)__popcll(__ballot(f0)), (uint32_t)__popcll(__ballot(f1)), slots0, slots1, blk);
 }
 
 template <bool OFF32>
@@ -154,13 +162,14 @@ __device__ inline uint32_t load_site(const Geom &G, const Fields &F, int colour,
     return own;
 }
 
-// One director trial at packed site m of `colour`, inputs read from the
-// fields; stores the new director when accepted.
-template <int ARITH, int RNG, int BC, bool OFF32 = false>
-__device__ inline bool dir_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m)
+// Trials on loaded inputs S of packed site m of `colour` whose field index is
+// `own`; accepted moves are stored.
+//
+// One director trial; returns whether it is accepted.
+template <int ARITH, int RNG, bool OFF32>
+__device__ inline bool dir_apply(const Fields &F, int colour, const PhaseArgs &A, uint32_t m, uint32_t own,
+                                 const SiteIn &S)
 {
-    SiteIn S;
-    const uint32_t own = load_site<BC, OFF32>(G, F, colour, m, S);
     double t[3];
     const bool acc = dir_compute<ARITH, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg, *A.d, t);
     if (acc) {
@@ -171,33 +180,29 @@ __device__ inline bool dir_site(const Geom &G, const Fields &F, int colour, cons
     return acc;
 }
 
-// One chirality trial at packed site m of `colour`.
-template <int ARITH, int RNG, int BC, bool OFF32 = false>
-__device__ inline bool spin_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m)
+// One chirality trial; returns whether the spin flips.
+template <int ARITH, int RNG, bool OFF32>
+__device__ inline bool spin_apply(const Fields &F, int colour, const PhaseArgs &A, uint32_t m, uint32_t own,
+                                  const SiteIn &S)
 {
-    SiteIn S;
-    const uint32_t own = load_site<BC, OFF32>(G, F, colour, m, S);
     const bool acc = spin_compute<ARITH, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg);
     if (acc) fld<OFF32>(F.s, own) = (int8_t)(-S.sg);
     return acc;
 }
 
-// Director trial, then chirality trial, at packed site m of `colour`; FP64
-// arithmetic only.  The reference runs director colour 0, director colour 1,
-// chirality colour 0, chirality colour 1 (chll/chll_mod.f90:70-81).  In FP64
-// arithmetic the chirality energy change sigma (K sum_b sgn_b c_b q_b + 2 g)
-// reads the directors and the site's own spin but no neighbour spin, so once
-// both colours' directors are final a site's chirality trial has the same
-// outcome before or after the other colour's chirality trials.  Run right after
-// this site's director trial it reuses the neighbour directors in registers and
-// saves one pass over the lattice.  Returns the director decision; `flip` gets
-// the chirality decision.
-template <int RNG, int BC, bool OFF32>
-__device__ inline bool dir_spin_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m,
-                                     bool &flip)
+// Director trial, then chirality trial; FP64 arithmetic only.  The reference
+// runs director colour 0, director colour 1, chirality colour 0, chirality
+// colour 1 (chll/chll_mod.f90:70-81).  In FP64 arithmetic the chirality energy
+// change sigma (K sum_b sgn_b c_b q_b + 2 g) reads the directors and the
+// site's own spin but no neighbour spin, so once both colours' directors are
+// final a site's chirality trial has the same outcome before or after the
+// other colour's chirality trials.  Run right after this site's director trial
+// it reuses the neighbour directors in registers and saves one pass over the
+// lattice.  Returns the director decision; `flip` gets the chirality decision.
+template <int RNG, bool OFF32>
+__device__ inline bool dir_spin_apply(const Fields &F, int colour, const PhaseArgs &A, uint32_t m, uint32_t own,
+                                      const SiteIn &S, bool &flip)
 {
-    SiteIn S;
-    const uint32_t own = load_site<BC, OFF32>(G, F, colour, m, S);
     double t[3];
     const bool acc = dir_compute<ARITH_FP64, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg, *A.d, t);
     if (acc) {
@@ -214,20 +219,19 @@ __device__ inline bool dir_spin_site(const Geom &G, const Fields &F, int colour,
 }
 
 // Chirality trial of sweep A.istep_spin, then director trial of sweep A.istep,
-// at packed site m of colour 0; FP64 arithmetic, Philox draws.  After sweep
-// t's colour-1 director and chirality trials every input of colour-0 site m
-// for its chirality trial in sweep t (own director, colour-1 directors) and
-// for its director trial in sweep t + 1 (own director and spin after the flip,
+// at a colour-0 site; FP64 arithmetic, Philox draws.  After sweep t's colour-1
+// director and chirality trials every input of a colour-0 site for its
+// chirality trial in sweep t (own director, colour-1 directors) and for its
+// director trial in sweep t + 1 (own director and spin after the flip,
 // colour-1 directors and spins) is final, and by the argument of
-// dir_spin_site no colour-0 chirality trial reads another colour-0 site.  The
+// dir_spin_apply no colour-0 chirality trial reads another colour-0 site.  The
 // pair therefore gives the sequential outcome; the cone step of sweep t + 1 is
 // fixed before this launch, since it depends on sweep t's director accepts
-// only (chll/chll_mod.f90:83-90).
-template <int BC, bool OFF32>
-__device__ inline bool spin_dir_site(const Geom &G, const Fields &F, const PhaseArgs &A, uint32_t m, bool &flip)
+// only (chll/chll_mod.f90:83-90).  Returns the director decision.
+template <bool OFF32>
+__device__ inline bool spin_dir_apply(const Fields &F, const PhaseArgs &A, uint32_t m, uint32_t own,
+                                      const SiteIn &S, bool &flip)
 {
-    SiteIn S;
-    const uint32_t own = load_site<BC, OFF32>(G, F, 0, m, S);
     PhaseArgs As = A;
     As.istep = A.istep_spin;
     flip = spin_compute<ARITH_FP64, RNG_PHILOX>(As, 0, m, S.N, S.nx, S.ny, S.nz, S.sg);
@@ -241,6 +245,113 @@ __device__ inline bool spin_dir_site(const Geom &G, const Fields &F, const Phase
         fld<OFF32>(F.nz, own) = t[2];
     }
     return acc;
+}
+
+// Single sites: load, then apply.
+template <int ARITH, int RNG, int BC, bool OFF32 = false>
+__device__ inline bool dir_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m)
+{
+    SiteIn S;
+    const uint32_t own = load_site<BC, OFF32>(G, F, colour, m, S);
+    return dir_apply<ARITH, RNG, OFF32>(F, colour, A, m, own, S);
+}
+
+template <int ARITH, int RNG, int BC, bool OFF32 = false>
+__device__ inline bool spin_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m)
+{
+    SiteIn S;
+    const uint32_t own = load_site<BC, OFF32>(G, F, colour, m, S);
+    return spin_apply<ARITH, RNG, OFF32>(F, colour, A, m, own, S);
+}
+
+template <int RNG, int BC, bool OFF32>
+__device__ inline bool dir_spin_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m,
+                                     bool &flip)
+{
+    SiteIn S;
+    const uint32_t own = load_site<BC, OFF32>(G, F, colour, m, S);
+    return dir_spin_apply<RNG, OFF32>(F, colour, A, m, own, S, flip);
+}
+
+template <int BC, bool OFF32>
+__device__ inline bool spin_dir_site(const Geom &G, const Fields &F, const PhaseArgs &A, uint32_t m, bool &flip)
+{
+    SiteIn S;
+    const uint32_t own = load_site<BC, OFF32>(G, F, 0, m, S);
+    return spin_dir_apply<OFF32>(F, A, m, own, S, flip);
+}
+
+// Site pairs.  With half = ni / 2 even, packed sites m0 = 2 P and m0 + 1 of one
+// colour lie in one row, and every index below is even: the pair's own fields
+// and its y and z neighbours (q, q + 1) are read with 16-byte loads (spins
+// with 2-byte loads), an absent y or z bond of both sites with the zero pair.
+// Along x the two sites' four bonds touch three other-colour sites q - 1 + p ..
+// q + 1 + p (q = oth + m0, p = row parity): the pair (q, q + 1), whose bonds
+// always exist, and one more site, the -x neighbour of m0 (p = 0) or the +x
+// neighbour of m1 (p = 1), absent, wrapped or at q - 1 / q + 2 as
+// neighbours() gives it.  Per pair: 15 field loads of 16 bytes and 3 of 8
+// bytes instead of 42 of 8 bytes.
+template <bool OFF32>
+__device__ inline double2 fld2(const double *base, uint32_t q)
+{
+    if (OFF32) return *(const double2 *)((const char *)base + q * 8u);
+    return *(const double2 *)(base + q);
+}
+
+template <bool OFF32>
+__device__ inline uint32_t spin2(const int8_t *base, uint32_t q)
+{
+    if (OFF32) return *(const uint16_t *)((const char *)base + q);
+    return *(const uint16_t *)(base + q);
+}
+
+__device__ inline int lo_spin(uint32_t v) { return (int)(int8_t)(v & 0xffu); }
+__device__ inline int hi_spin(uint32_t v) { return (int)(int8_t)(v >> 8); }
+
+template <int BC, bool OFF32>
+__device__ inline uint32_t load_pair(const Geom &G, const Fields &F, int colour, uint32_t m0, SiteIn &S0, SiteIn &S1)
+{
+    const Site st0 = decode(G, m0, colour);
+    const Site st1{st0.i + 2u, st0.j, st0.k};
+    const uint32_t own = (colour ? G.nsub : 0u) + m0;
+    const uint32_t oth = colour ? 0u : G.nsub;
+    uint32_t n0[6], n1[6];
+    S0.N.mask = neighbours<BC>(G, m0, st0, oth, n0);
+    S1.N.mask = neighbours<BC>(G, m0 + 1u, st1, oth, n1);
+    const uint32_t p = st0.i & 1u;
+    const double2 ox = fld2<OFF32>(F.nx, own), oy = fld2<OFF32>(F.ny, own), oz = fld2<OFF32>(F.nz, own);
+    const uint32_t os = spin2<OFF32>(F.s, own);
+    S0.nx = ox.x; S0.ny = oy.x; S0.nz = oz.x; S0.sg = lo_spin(os);
+    S1.nx = ox.y; S1.ny = oy.y; S1.nz = oz.y; S1.sg = hi_spin(os);
+    // x: pair (q, q + 1) and the extra site e
+    const uint32_t q = oth + m0;
+    const uint32_t e = p ? n1[0] : n0[1];
+    const double2 xx = fld2<OFF32>(F.nx, q), xy = fld2<OFF32>(F.ny, q), xz = fld2<OFF32>(F.nz, q);
+    const uint32_t xs = spin2<OFF32>(F.s, q);
+    const double ex = fld<OFF32>(F.nx, e), ey = fld<OFF32>(F.ny, e), ez = fld<OFF32>(F.nz, e);
+    const int es = fld<OFF32>(F.s, e);
+    if (p == 0u) {
+        S0.N.x[0] = xx.x; S0.N.y[0] = xy.x; S0.N.z[0] = xz.x; S0.N.s[0] = lo_spin(xs);
+        S0.N.x[1] = ex;   S0.N.y[1] = ey;   S0.N.z[1] = ez;   S0.N.s[1] = es;
+        S1.N.x[0] = xx.y; S1.N.y[0] = xy.y; S1.N.z[0] = xz.y; S1.N.s[0] = hi_spin(xs);
+        S1.N.x[1] = xx.x; S1.N.y[1] = xy.x; S1.N.z[1] = xz.x; S1.N.s[1] = lo_spin(xs);
+    } else {
+        S0.N.x[0] = xx.y; S0.N.y[0] = xy.y; S0.N.z[0] = xz.y; S0.N.s[0] = hi_spin(xs);
+        S0.N.x[1] = xx.x; S0.N.y[1] = xy.x; S0.N.z[1] = xz.x; S0.N.s[1] = lo_spin(xs);
+        S1.N.x[0] = ex;   S1.N.y[0] = ey;   S1.N.z[0] = ez;   S1.N.s[0] = es;
+        S1.N.x[1] = xx.y; S1.N.y[1] = xy.y; S1.N.z[1] = xz.y; S1.N.s[1] = hi_spin(xs);
+    }
+    // y and z: pairs at site m0's neighbour (its bond and site m1's exist
+    // together; absent ones read the zero pair)
+#pragma unroll
+    for (int b = 2; b < 6; ++b) {
+        const uint32_t r = n0[b];
+        const double2 vx = fld2<OFF32>(F.nx, r), vy = fld2<OFF32>(F.ny, r), vz = fld2<OFF32>(F.nz, r);
+        const uint32_t vs = spin2<OFF32>(F.s, r);
+        S0.N.x[b] = vx.x; S0.N.y[b] = vy.x; S0.N.z[b] = vz.x; S0.N.s[b] = lo_spin(vs);
+        S1.N.x[b] = vx.y; S1.N.y[b] = vy.y; S1.N.z[b] = vz.y; S1.N.s[b] = hi_spin(vs);
+    }
+    return own;
 }
 
 // ---------------------------------------------------------------------------

@@ -129,9 +129,11 @@ int chll_lattice_sweeps(chll_lattice *lat, chll_params *p, int32_t nstart, int32
 int chll_lattice_measure(chll_lattice *lat, double g, double KK, double *row);
 int chll_lattice_sync(chll_lattice *lat);
 /* Half-sweep kernel for one trial kind (0 director, 1 chirality) and
- * arithmetic: variant 0 = one site per thread, tile 0 = blocks in lattice
- * order, 1 = each of the 8 XCDs takes one band of rows of every plane (lattice
- * order when a plane does not split into 8 bands of 256-site blocks);
+ * arithmetic: variant 0 = one site per thread, tile bit 0 = each of the 8 XCDs
+ * takes one band of rows of every plane (else blocks in lattice order, also
+ * when a plane does not split into 8 bands of whole blocks), tile bit 1 = two
+ * neighbouring sites per thread read with 16-byte pair loads (when ni % 4 == 0
+ * and a field spans less than 4 GiB; else one site per thread);
  * variant 1 = tiled through LDS: tile
  * 0 = 64x4, 1 = 32x8, 2 = 16x16 packed sites x rows marching along k in nslab
  * slabs, 3..5 = the same tile shapes one plane per block (nslab unused).
