@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events")
     ap.add_argument("--no-variants", action="store_true", help="skip the other arithmetic mode")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-replica scan")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the library's default kernels instead of autotuning (clean profiles)")
     ap.add_argument("--batched-blocks", type=int, default=20, help="timed blocks of 10 sweeps")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run one plain elementwise copy of this many MiB (HBM counter calibration)")
@@ -148,10 +150,15 @@ def main():
         and K timed sweeps bracketed by synchronisation."""
         from chllmodel_amd.autotune import apply, tune_lattice
 
-        tuned[arith] = tune_lattice(L, L, L, bc=args.bc, arith=arith, use_cache=False)
-        log(f"bench: {arith}: tuned director {tuned[arith]['director']} chirality {tuned[arith]['chirality']}")
         lat = Lattice(L, L, L, bc=args.bc)
-        apply(lat, tuned[arith], arith)
+        if args.no_tune:
+            tuned[arith] = {k: list(lat.get_kernel(k, arith)) for k in ("director", "chirality")}
+            tuned[arith]["fuse"] = lat.get_fusion() if arith == "fp64" else False
+            log(f"bench: {arith}: default kernels {tuned[arith]}")
+        else:
+            tuned[arith] = tune_lattice(L, L, L, bc=args.bc, arith=arith, use_cache=False)
+            log(f"bench: {arith}: tuned director {tuned[arith]['director']} chirality {tuned[arith]['chirality']}")
+            apply(lat, tuned[arith], arith)
         lat.set_stream(stream.cuda_stream)
         lat.upload_state(st0)
         lat.set_step(args.d0)
