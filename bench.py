@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events")
     ap.add_argument("--no-variants", action="store_true", help="skip the other arithmetic mode")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-replica scan")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in run() timing")
+    ap.add_argument("--dropin-sweeps", type=int, default=20, help="sweeps per drop-in run() call")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default kernels instead of autotuning (clean profiles)")
     ap.add_argument("--batched-blocks", type=int, default=20, help="timed blocks of 10 sweeps")
@@ -244,6 +246,8 @@ def main():
         "kernel_choice": {a: {k: t.get(k) for k in ("director", "chirality", "fuse")} for a, t in tuned.items()},
         "cpu_baseline": None,
     }
+    if not args.no_dropin:
+        out["dropin_run"] = dropin_run(L, st0, args)
     if not args.no_batched:
         out["batched"] = batched_scan(args, stream)
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
@@ -251,6 +255,53 @@ def main():
         out["cpu_baseline"] = cpu_baseline(L, st0, args)
         out["speedup_vs_cpu"] = sups / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
+
+
+def dropin_run(L, st0, args):
+    """The reference's own entry point run() (chll/chll_mod.f90:40) on host
+    arrays: each call uploads the Fortran-ordered fields over PCIe, sweeps on
+    the device and downloads them again, as chll/chll.py:160-173 drives it.
+    Reported beside value, never as value."""
+    import ctypes as ct
+
+    import numpy as np
+
+    from chllmodel_amd import _lib
+    from oracle.oracle import copy_state
+
+    lib = _lib.lib()
+    st = copy_state(st0)
+    n = args.dropin_sweeps
+    nsub = L ** 3 // 2
+    msteps = np.zeros(1, dtype=np.int32)
+    m = np.zeros((1, 5), dtype=np.float64, order="F")
+    sl1 = np.zeros(nsub, np.int32)
+    sl2 = np.zeros(nsub, np.int32)
+    arith = {"compat": 0, "fp64": 1}[args.arith]
+    _lib.check(lib.chll_configure(0, arith, {"open": 0, "periodic": 1}[args.bc], 0))
+    _lib.check(lib.chll_set_seed(20261203, 0))
+
+    def call(dd):
+        lib.run(ct.c_int(1), ct.c_int(n), ct.c_int(n), st["nx"], st["ny"], st["nz"], st["s"], st["dope"], sl1, sl2,
+                ct.c_double(args.g), ct.c_double(args.K), dd, ct.c_double(args.kT), ct.c_int(L), ct.c_int(L),
+                ct.c_int(L), ct.c_int(0), msteps, m)
+        if _lib.last_error():
+            raise RuntimeError(_lib.last_error())
+
+    try:
+        call(ct.c_double(args.d0))  # first call allocates and builds the lists
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            call(ct.c_double(args.d0))
+            times.append(time.perf_counter() - t0)
+    finally:
+        _lib.check(lib.chll_configure(0, 0, 0, 1))
+    wall = min(times)
+    log(f"bench: drop-in run(): {n} sweeps per call in {wall:.4f} s")
+    return {"entry": "run (chll/chll_mod.f90:40), host arrays in and out per call",
+            "sweeps_per_call": n, "ms_per_call": 1e3 * wall, "site_updates_per_s": L ** 3 * n / wall,
+            "arith": args.arith, "calls_timed": len(times), "statistic": "best of 3 calls"}
 
 
 def batched_scan(args, stream):

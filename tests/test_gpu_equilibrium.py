@@ -1,0 +1,52 @@
+"""Equilibrium parity with the reference's own chains.
+
+tests/golden/make_equilibrium.py ran the reference's chll/chll_mod.f90
+(compiled from source, its own random_number) from seeded isotropic states at
+sweeps 10001..30000, where the step adaptation is off, and kept the rows
+(reported energy, eta, paccept, faccept) every 10 sweeps after the first 4000
+sweeps.  The GPU runs the same chains with its site-keyed Philox draws in both
+arithmetic modes; every mean must agree with the reference's within 3 combined
+standard errors (Flyvbjerg-Petersen blocking on both series).  The random
+streams differ, so this is the statistical bar; the deterministic modes are
+tested bit-exactly in test_gpu_parity.py."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from chllmodel_amd.lattice import Lattice
+from chllmodel_amd.observables import blocking_error
+from chllmodel_amd.states import isotropic_state
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "eq_*.npz")))
+NAMES = ("E", "eta", "paccept", "faccept")
+
+
+@pytest.mark.parametrize("arith", ["compat", "fp64"])
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p)[:-4] for p in FIXTURES])
+def test_equilibrium_means_match_reference(path, arith):
+    z = np.load(path, allow_pickle=False)
+    ref = z["series"]
+    nstart, nsweeps, nout, discard, g, KK, d, kbt, seed = z["params"]
+    nstart, nsweeps, nout, discard = int(nstart), int(nsweeps), int(nout), int(discard)
+    dims = tuple(int(v) for v in z["dims"])
+    print(f"equilibrium {os.path.basename(path)} {arith}: {nsweeps} sweeps of {dims}", flush=True)
+    with Lattice(*dims) as lat:
+        lat.upload_state(isotropic_state(*dims, int(seed)))
+        lat.set_step(d)
+        out = lat.sweeps(nstart, nstart + nsweeps - 1, nout, g=g, KK=KK, kbt=kbt, arith=arith, rng="philox",
+                         seed=0x5EED + int(seed), adapt=True)
+        assert lat.get_step() == d  # no adaptation past sweep 10000
+    rows = out["rows"][discard:]
+    assert rows.shape[0] == ref.shape[0]
+    gpu = rows[:, 1:5]
+    for c, name in enumerate(NAMES):
+        mg, mr = gpu[:, c].mean(), ref[:, c].mean()
+        eg, er = blocking_error(gpu[:, c]), blocking_error(ref[:, c])
+        sig = np.hypot(eg, er)
+        print(f"  {name}: gpu {mg:.5f} +- {eg:.5f}  reference {mr:.5f} +- {er:.5f}  ({(mg - mr) / sig:+.2f} sigma)")
+        assert abs(mg - mr) <= 3.0 * sig, (name, mg, mr, sig)
