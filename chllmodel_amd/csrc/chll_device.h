@@ -347,6 +347,16 @@ __device__ inline double cqf_sum(const NB &N, int sigma, double nx, double ny, d
     return acc;
 }
 
+// a * b + c rounded once, with c taken from a scalar register pair: for a
+// constant c this is one v_fma_f64 fed by two scalar moves, where the compiler
+// otherwise rebuilds c in vector registers (two v_mov_b32) for a v_fmac_f64.
+__device__ inline double fma_sc(double a, double b, double c)
+{
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+    return r;
+}
+
 // sin and cos of 2 pi u for u in [0, 1): w = 4u and f = w - rint(w) are exact,
 // so 2 pi u = n pi/2 + x with x = f pi/2 in [-pi/4, pi/4]; sin x and cos x
 // from their Taylor series through x^17 and x^18 (truncation below 1e-19 on
@@ -358,22 +368,22 @@ __device__ inline void sincos_2pi(double u, double &s, double &c)
     const double x = (w - n) * 1.5707963267948966;
     const double z = x * x;
     double ps = -1.0 / 355687428096000.0;
-    ps = fma(ps, z, 1.0 / 1307674368000.0);
-    ps = fma(ps, z, -1.0 / 6227020800.0);
-    ps = fma(ps, z, 1.0 / 39916800.0);
-    ps = fma(ps, z, -1.0 / 362880.0);
-    ps = fma(ps, z, 1.0 / 5040.0);
-    ps = fma(ps, z, -1.0 / 120.0);
-    ps = fma(ps, z, 1.0 / 6.0);
+    ps = fma_sc(ps, z, 1.0 / 1307674368000.0);
+    ps = fma_sc(ps, z, -1.0 / 6227020800.0);
+    ps = fma_sc(ps, z, 1.0 / 39916800.0);
+    ps = fma_sc(ps, z, -1.0 / 362880.0);
+    ps = fma_sc(ps, z, 1.0 / 5040.0);
+    ps = fma_sc(ps, z, -1.0 / 120.0);
+    ps = fma_sc(ps, z, 1.0 / 6.0);
     const double sx = fma(-x * z, ps, x);
     double pc = 1.0 / 6402373705728000.0;
-    pc = fma(pc, z, -1.0 / 20922789888000.0);
-    pc = fma(pc, z, 1.0 / 87178291200.0);
-    pc = fma(pc, z, -1.0 / 479001600.0);
-    pc = fma(pc, z, 1.0 / 3628800.0);
-    pc = fma(pc, z, -1.0 / 40320.0);
-    pc = fma(pc, z, 1.0 / 720.0);
-    pc = fma(pc, z, -1.0 / 24.0);
+    pc = fma_sc(pc, z, -1.0 / 20922789888000.0);
+    pc = fma_sc(pc, z, 1.0 / 87178291200.0);
+    pc = fma_sc(pc, z, -1.0 / 479001600.0);
+    pc = fma_sc(pc, z, 1.0 / 3628800.0);
+    pc = fma_sc(pc, z, -1.0 / 40320.0);
+    pc = fma_sc(pc, z, 1.0 / 720.0);
+    pc = fma_sc(pc, z, -1.0 / 24.0);
     pc = fma(pc, z, 0.5);
     const double cx = fma(-z, pc, 1.0);
     const int q = (int)n;
