@@ -65,13 +65,15 @@ struct EnsDev {
     uint32_t run_index, K;
 };
 
-// kind 0: director trials of `colour`, 1: chirality trials of `colour`; FP64
+// KIND 0: director trials of `colour`, 1: chirality trials of `colour`; FP64
 // only, 2: colour-1 director trials each followed by the site's chirality
 // trial, 3: colour-0 chirality trials of sweep t each followed by the site's
-// director trial of sweep t + 1 (the single-lattice fused launches).
-template <int ARITH, int BC, int BS>
-__global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int kind, int colour, uint32_t istep_off)
+// director trial of sweep t + 1 (the single-lattice fused launches).  One
+// kernel per kind, so each gets the register budget of its own trial.
+template <int KIND, int ARITH, int BC, int BS>
+__global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int colour, uint32_t istep_off)
 {
+    constexpr int kind = KIND;
     const uint32_t code = E.blk_rep[blockIdx.x];
     const uint32_t r = code >> 24, lb = code & 0xFFFFFFu;
     const RepDesc &D = E.R[r];
@@ -84,7 +86,7 @@ __global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int kind, int colour
     const uint32_t m = lb * BS + threadIdx.x;
     const size_t gs = (size_t)kSlots * kSlotStride;
     bool acc = false, flip = false;
-    if constexpr (ARITH == ARITH_FP64) {
+    if constexpr (ARITH == ARITH_FP64 && KIND >= 2) {
         if (kind == 2) {
             if (m < D.G.nsub) acc = dir_spin_site<RNG_PHILOX, BC, false>(D.G, D.F, 1, A, m, flip);
             block_count2<BS>(acc, flip, D.counts + kGroupAcc1 * gs, D.counts + flip_group(A.istep, 1) * gs, lb);
@@ -373,19 +375,21 @@ void launch_block_t(chll_ensemble *X, int K, hipStream_t st)
         const uint32_t off = (uint32_t)t;
         const bool last = t + 1 == K;
         if (!fuse) {
-            for (int ph = 0; ph < 4; ++ph)
-                hipLaunchKernelGGL((k_phase_ens<ARITH, BC, BS>), g, b, 0, st, E, ph >> 1, ph & 1, off);
+            hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off);
+            hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 1, off);
+            hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off);
+            hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 1, off);
             if (last) hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E);
             hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, last ? 1 : 0, 0, 1);
             continue;
         }
-        if (t == 0) hipLaunchKernelGGL((k_phase_ens<ARITH, BC, BS>), g, b, 0, st, E, 0, 0, off);
-        hipLaunchKernelGGL((k_phase_ens<ARITH, BC, BS>), g, b, 0, st, E, 2, 1, off);
+        if (t == 0) hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off);
+        hipLaunchKernelGGL((k_phase_ens<2, ARITH, BC, BS>), g, b, 0, st, E, 1, off);
         if (!last) {
             hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 0, t > 0 ? 1 : 0, 0);
-            hipLaunchKernelGGL((k_phase_ens<ARITH, BC, BS>), g, b, 0, st, E, 3, 0, off);
+            hipLaunchKernelGGL((k_phase_ens<3, ARITH, BC, BS>), g, b, 0, st, E, 0, off);
         } else {
-            hipLaunchKernelGGL((k_phase_ens<ARITH, BC, BS>), g, b, 0, st, E, 1, 0, off);
+            hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off);
             hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E);
             hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 1, t > 0 ? 1 : 0, 1);
         }
