@@ -8,6 +8,15 @@
 
 namespace chll {
 
+// Device-scope atomic add through a global-address-space pointer: a
+// global_atomic even where the compiler cannot tell that the pointer
+// addresses global memory (see fst below).
+__device__ inline void atomic_add_global(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_fetch_add((__attribute__((address_space(1))) uint32_t *)p, v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Block-wide sum of one 0/1 flag per thread, added to slot blk % kSlots.
 template <int BS>
 __device__ inline void block_count(bool flag, uint32_t *slots, uint32_t blk)
@@ -21,7 +30,7 @@ __device__ inline void block_count(bool flag, uint32_t *slots, uint32_t blk)
         uint32_t t = 0;
 #pragma unroll
         for (int w = 0; w < BS / 64; ++w) t += wsum[w];
-        if (t) atomicAdd(&slots[(blk & (kSlots - 1)) * kSlotStride], t);
+        if (t) atomic_add_global(&slots[(blk & (kSlots - 1)) * kSlotStride], t);
     }
 }
 
@@ -54,7 +63,7 @@ __device__ inline void block_sum2(uint32_t w0, uint32_t w1, uint32_t *slots0, ui
 #pragma unroll
         for (int w = 0; w < BS / 64; ++w) t += wsum[threadIdx.x][w];
         uint32_t *slots = threadIdx.x ? slots1 : slots0;
-        if (t) atomicAdd(&slots[(blk & (kSlots - 1)) * kSlotStride], t);
+        if (t) atomic_add_global(&slots[(blk & (kSlots - 1)) * kSlotStride], t);
     }
 }
 
@@ -63,6 +72,18 @@ template <int BS>
 __device__ inline void block_count2(bool f0, bool f1, uint32_t *slots0, uint32_t *slots1, uint32_t blk)
 {
     block_sum2<BS>((uint32_t)__popcll(__ballot(f0)), (uint32_t)__popcll(__ballot(f1)), slots0, slots1, blk);
+}
+
+// Store v to element q of a field through a global-address-space pointer, so
+// that the store is a global_store even where the compiler cannot tell that
+// the field pointer addresses global memory (pointers read from device
+// memory, as the replica descriptors are); it would otherwise emit a flat
+// store, which also counts against lgkmcnt.
+template <bool OFF32, class T>
+__device__ inline void fst(T *base, uint32_t q, T v)
+{
+    T *p = &fld<OFF32>(base, q);
+    *(__attribute__((address_space(1))) T *)p = v;
 }
 
 template <bool OFF32>
@@ -173,9 +194,9 @@ __device__ inline bool dir_apply(const Fields &F, int colour, const PhaseArgs &A
     double t[3];
     const bool acc = dir_compute<ARITH, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg, *A.d, t);
     if (acc) {
-        fld<OFF32>(F.nx, own) = t[0];
-        fld<OFF32>(F.ny, own) = t[1];
-        fld<OFF32>(F.nz, own) = t[2];
+        fst<OFF32>(F.nx, own, t[0]);
+        fst<OFF32>(F.ny, own, t[1]);
+        fst<OFF32>(F.nz, own, t[2]);
     }
     return acc;
 }
@@ -186,7 +207,7 @@ __device__ inline bool spin_apply(const Fields &F, int colour, const PhaseArgs &
                                   const SiteIn &S)
 {
     const bool acc = spin_compute<ARITH, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg);
-    if (acc) fld<OFF32>(F.s, own) = (int8_t)(-S.sg);
+    if (acc) fst<OFF32>(F.s, own, (int8_t)(-S.sg));
     return acc;
 }
 
@@ -206,15 +227,15 @@ __device__ inline bool dir_spin_apply(const Fields &F, int colour, const PhaseAr
     double t[3];
     const bool acc = dir_compute<ARITH_FP64, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg, *A.d, t);
     if (acc) {
-        fld<OFF32>(F.nx, own) = t[0];
-        fld<OFF32>(F.ny, own) = t[1];
-        fld<OFF32>(F.nz, own) = t[2];
+        fst<OFF32>(F.nx, own, t[0]);
+        fst<OFF32>(F.ny, own, t[1]);
+        fst<OFF32>(F.nz, own, t[2]);
     }
     PhaseArgs As = A;
     As.u_acc = A.u_spin;
     flip = spin_compute<ARITH_FP64, RNG>(As, colour, m, S.N, acc ? t[0] : S.nx, acc ? t[1] : S.ny,
                                          acc ? t[2] : S.nz, S.sg);
-    if (flip) fld<OFF32>(F.s, own) = (int8_t)(-S.sg);
+    if (flip) fst<OFF32>(F.s, own, (int8_t)(-S.sg));
     return acc;
 }
 
@@ -238,11 +259,11 @@ __device__ inline bool spin_dir_apply(const Fields &F, const PhaseArgs &A, uint3
     const int sg = flip ? -S.sg : S.sg;
     double t[3];
     const bool acc = dir_compute<ARITH_FP64, RNG_PHILOX>(A, 0, m, S.N, S.nx, S.ny, S.nz, sg, *A.d, t);
-    if (flip) fld<OFF32>(F.s, own) = (int8_t)sg;
+    if (flip) fst<OFF32>(F.s, own, (int8_t)sg);
     if (acc) {
-        fld<OFF32>(F.nx, own) = t[0];
-        fld<OFF32>(F.ny, own) = t[1];
-        fld<OFF32>(F.nz, own) = t[2];
+        fst<OFF32>(F.nx, own, t[0]);
+        fst<OFF32>(F.ny, own, t[1]);
+        fst<OFF32>(F.nz, own, t[2]);
     }
     return acc;
 }
