@@ -172,14 +172,21 @@ __global__ __launch_bounds__(kBlock) void k_pair(Geom G, Fields F, int colour, P
 }
 
 // ---------------------------------------------------------------------------
-// Observables over the whole lattice (grid-stride), one partial per block.
+// Observables over the whole lattice, one partial per block.  Block b walks
+// its own contiguous range of `chunk` packed indices (a multiple of kBlock):
+// a row's +y neighbours are then read by the same block one step later, and
+// the +z plane is read at the same time by block b + 8 on the same XCD
+// (workgroups go round-robin to the 8 XCDs), so both come from L2.
 template <int BC>
 __global__ __launch_bounds__(kBlock) void k_measure(Geom G, Fields F, double g, double KK,
-                                                    double *__restrict__ part, long long *__restrict__ spart)
+                                                    double *__restrict__ part, long long *__restrict__ spart,
+                                                    uint32_t chunk)
 {
     MeasureAcc acc;
-    for (uint32_t m = blockIdx.x * kBlock + threadIdx.x; m < G.nsub; m += gridDim.x * kBlock)
-        measure_site<BC>(G, F, g, KK, m, acc);
+    const uint64_t beg = (uint64_t)blockIdx.x * chunk;
+    const uint64_t end = beg + chunk < G.nsub ? beg + chunk : G.nsub;
+    for (uint64_t m = beg + threadIdx.x; m < end; m += kBlock)
+        measure_site<BC>(G, F, g, KK, (uint32_t)m, acc);
     block_reduce_measure<kBlock>(acc, part + (size_t)blockIdx.x * 8, spart + blockIdx.x);
 }
 
@@ -472,10 +479,15 @@ void launch_phase(int kind, int arith, int rng, int bc, const Geom &G, const Fie
 void launch_measure(int bc, const Geom &G, const Fields &F, double g, double KK, double *part, long long *spart,
                     hipStream_t s)
 {
+    // measure_blocks(G) partials (the finalize kernel reads that many), each
+    // block a contiguous range; trailing blocks may get an empty one
+    const uint32_t nb = measure_blocks(G);
+    uint32_t chunk = (G.nsub + nb - 1) / nb;
+    chunk = (chunk + kBlock - 1) / kBlock * kBlock;
     if (bc == BC_OPEN)
-        hipLaunchKernelGGL(k_measure<BC_OPEN>, dim3(measure_blocks(G)), dim3(kBlock), 0, s, G, F, g, KK, part, spart);
+        hipLaunchKernelGGL(k_measure<BC_OPEN>, dim3(nb), dim3(kBlock), 0, s, G, F, g, KK, part, spart, chunk);
     else
-        hipLaunchKernelGGL(k_measure<BC_PERIODIC>, dim3(measure_blocks(G)), dim3(kBlock), 0, s, G, F, g, KK, part, spart);
+        hipLaunchKernelGGL(k_measure<BC_PERIODIC>, dim3(nb), dim3(kBlock), 0, s, G, F, g, KK, part, spart, chunk);
 }
 
 void launch_finalize(const FinArgs &A, hipStream_t s)
