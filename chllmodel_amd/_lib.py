@@ -50,6 +50,17 @@ class ChllError(RuntimeError):
 _lib = None
 
 
+def _load_torch_first() -> None:
+    """Import torch, when installed, before this library is loaded.  Both link
+    the HIP runtime under one soname, so whichever loads first serves the whole
+    process; torch's ROCm build finds no GPU ("No HIP GPUs are available") on
+    a runtime of another ROCm release, while libchll_mod.so runs on torch's."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib() -> ct.CDLL:
     """Load libchll_mod.so once and declare every exported symbol."""
     global _lib
@@ -57,6 +68,7 @@ def lib() -> ct.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ChllError(f"{LIB_PATH} not built: run `python -m chllmodel_amd.build` (hipcc, gfx950)")
+    _load_torch_first()
     L = ct.CDLL(LIB_PATH)
     # drop-in symbols, argtypes as chll/chll.py:9-36
     L.init.argtypes = [ctypeslib.ndpointer(dtype=np.float64, ndim=3, flags="F_CONTIGUOUS"),

@@ -1,0 +1,58 @@
+"""ChLLSim end to end on the GPU, as a reference user runs it
+(chll/chll.py:154-262): initial spins from the global numpy stream, one
+run() through the drop-in entry point, then the CSV of measurement rows,
+the npz of the final configuration and the parameter file.  In the
+deterministic stream mode the rows and the final state must equal the
+oracle's run of the same sweeps; the files must hold exactly what the run
+returned."""
+import os
+
+import numpy as np
+import pytest
+
+from chllmodel_amd.chll import ChLLSim
+from oracle.oracle import RNG_STREAM, Oracle, copy_state
+
+pytestmark = pytest.mark.gpu
+
+
+def f32_ulp(x):
+    return float(np.spacing(np.float32(abs(x))))
+
+
+def test_chllsim_run_and_files(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    os.makedirs("data")
+    np.random.seed(11)
+    print("ChLLSim 8x6x4 stream mode, 12 sweeps", flush=True)
+    sim = ChLLSim(8, 6, 4, kbt=0.7, d=0.3, KK=1.0, g=0.5, rho=0.0, name="e2e", rng="stream", seed=77,
+                  verbose=False)
+    sim.init_rand_s()
+    st0 = {k: getattr(sim, k).copy(order="F") for k in ("nx", "ny", "nz", "s", "dope")}
+    sim.run(12, 4, save=True)
+
+    st = copy_state(st0)
+    ms, m, d_out, _ = Oracle().run(st, 1, 12, 4, 0.5, 1.0, 0.3, 0.7, rng=RNG_STREAM, seed=77, stream_pos=0)
+    assert np.array_equal(sim.msteps, ms)
+    for r in range(len(ms)):
+        assert abs(sim.m[r, 0] - m[r, 0]) <= f32_ulp(m[r, 0])
+    assert np.array_equal(sim.m[:, 1:], m[:, 1:])
+    assert sim.d_out == d_out
+    assert np.array_equal(sim.s, st["s"])
+    for f in ("nx", "ny", "nz"):
+        assert np.abs(getattr(sim, f) - st[f]).max() <= 1e-12
+
+    path = os.path.join("data", "e2e")
+    lines = open(os.path.join(path, "e2e.csv")).read().splitlines()
+    assert lines[0] == "Step,TotalEnergy,Eexcess,Paccept,Faccept,D"
+    vals = np.array([[float(x) for x in ln.split(",")] for ln in lines[1:]])
+    assert vals.shape == (3, 6)
+    assert np.array_equal(vals[:, 0].astype(np.int32), sim.msteps)
+    assert np.array_equal(vals[:, 1:], sim.m)
+    z = np.load(os.path.join(path, "e2e_data.npz"), allow_pickle=False)
+    for f in ("nx", "ny", "nz", "s", "dope"):
+        assert np.array_equal(z[f], getattr(sim, f))
+    assert os.path.isfile(os.path.join(path, "e2ee2e_params.txt"))
+    # the full measurement rows of the run match the reference columns
+    assert np.array_equal(sim.ext["istep"].astype(np.int32), sim.msteps)
+    assert np.array_equal(sim.ext["paccept"], sim.m[:, 2])
