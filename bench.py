@@ -110,6 +110,7 @@ def main():
     ap.add_argument("--no-variants", action="store_true", help="skip the other arithmetic mode")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-replica scan")
     ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in run() timing")
+    ap.add_argument("--no-grad", action="store_true", help="skip the energy-gradient timing")
     ap.add_argument("--dropin-sweeps", type=int, default=20, help="sweeps per drop-in run() call")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default kernels instead of autotuning (clean profiles)")
@@ -252,6 +253,8 @@ def main():
     }
     if not args.no_dropin:
         out["dropin_run"] = dropin_run(L, st0, args)
+    if not args.no_grad:
+        out["gradient"] = gradient_rate(args)
     if not args.no_batched:
         out["batched"] = batched_scan(args, stream)
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
@@ -306,6 +309,42 @@ def dropin_run(L, st0, args):
     return {"entry": "run (chll/chll_mod.f90:40), host arrays in and out per call",
             "sweeps_per_call": n, "ms_per_call": 1e3 * wall, "site_updates_per_s": L ** 3 * n / wall,
             "arith": args.arith, "calls_timed": len(times), "statistic": "best of 3 calls"}
+
+
+def gradient_rate(args):
+    """Energy and tangent director gradient (chll_energy_grad) of a batch of
+    8 L=128 lattices (16.8 M sites, as L=256), timed with HIP events on
+    torch's current stream.  Algorithmic traffic 49 B per site: read the
+    directors 24 B and the spin 1 B, write the gradient 24 B."""
+    import torch
+
+    from chllmodel_amd.autograd import energy_and_grad
+
+    B, Lg = 8, 128
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    n = torch.randn(B, 3, Lg, Lg, Lg, device="cuda", dtype=torch.float64, generator=gen)
+    n = n / n.norm(dim=1, keepdim=True)
+    s = (torch.randint(0, 2, (B, Lg, Lg, Lg), device="cuda", generator=gen) * 2 - 1).to(torch.int8)
+    K = torch.full((B,), args.K, device="cuda", dtype=torch.float64)
+    g = torch.full((B,), args.g, device="cuda", dtype=torch.float64)
+    eps = torch.ones(B, device="cuda", dtype=torch.float64)
+    for _ in range(3):
+        energy_and_grad(n, s, K, g, eps)
+    reps = 20
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        energy_and_grad(n, s, K, g, eps)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / reps
+    sites = B * Lg ** 3
+    gbps = 49.0 * sites / sec / 1e9
+    log(f"bench: energy + gradient of {B} x {Lg}^3: {1e6 * sec:.1f} us per call")
+    return {"workload": f"{B} lattices of {Lg}^3, open boundaries, energy + tangent gradient + parameter sums",
+            "us_per_call": 1e6 * sec, "sites_per_s": sites / sec, "algorithmic_bytes_per_site": 49.0,
+            "GBps": gbps, "hbm_frac": gbps / HBM_PEAK_GBS}
 
 
 def batched_scan(args, stream):

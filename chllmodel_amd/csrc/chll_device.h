@@ -127,6 +127,20 @@ __device__ inline uint32_t neighbours(const Geom &G, uint32_t m, const Site &s, 
     return mask;
 }
 
+// Block order for one-site-per-thread launches.  Workgroups go round-robin to
+// the 8 XCDs (workgroup b to XCD b % 8), each with its own L2.  With band > 0
+// a plane's blocks are cut into 8 bands of `band` blocks (rows) and XCD x
+// takes band x of every plane, planes in order: a row's y neighbours then sit
+// in the same L2 except at the band edges, and the z neighbours of a band stay
+// in that L2 while its next planes run.  band = 0 keeps the identity order.
+__device__ inline uint32_t band_block(uint32_t b, uint32_t band)
+{
+    if (band == 0u) return b;
+    const uint32_t x = b & 7u, n = b >> 3;
+    const uint32_t p = n / band, r = n - p * band;
+    return (p * 8u + x) * band + r;
+}
+
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
 struct U4 {

@@ -7,9 +7,10 @@
 // 369-411 with each bond counted once; the reference has eps = 1).  Open
 // boundaries drop the bonds that leave the lattice, periodic ones wrap.
 //
-// One thread per site gathers the bond terms of its +x, +y, +z bonds (each bond
-// once, for E and the parameter derivatives) and the derivative of all six of
-// its bonds with respect to its own director:
+// One thread per site (two adjacent sites of a row when ni is even) gathers
+// the bond terms of its +x, +y, +z bonds (each bond once, for E and the
+// parameter derivatives) and the derivative of all six of its bonds with
+// respect to its own director:
 //   d t / d n_a = -2 eps c n_b - K f (q n_b + c dq/dn_a)       (a lower end)
 //   d t / d n_b = -2 eps c n_a - K f (q n_a + c dq/dn_b)       (b upper end)
 // with, for axis x, dq/dn_a = (0, n_bz, -n_by), dq/dn_b = (0, -n_az, n_ay) and
@@ -37,6 +38,7 @@ struct GradArgs {
     double *dEdn;               // B x 3 x N, or null
     double *part;               // B x nblk x 4: sum eps-term (1-c^2), sum c q f, sum s, unused
     uint32_t ni, nj, nk, N, nblk;
+    uint32_t band;              // XCD row bands (band_block), 0 = lattice order
     FastDiv fd_ni, fd_nj;
     int bc;
 };
@@ -45,28 +47,38 @@ __device__ inline void dq_axis(int axis, double ax, double ay, double az, double
                                double da[3], double db[3])
 {
     if (axis == 0) {
-        q = ay * bz - az * by;
+        q = fma(ay, bz, -az * by);
         da[0] = 0.0; da[1] = bz; da[2] = -by;
         db[0] = 0.0; db[1] = -az; db[2] = ay;
     } else if (axis == 1) {
-        q = az * bx - ax * bz;
+        q = fma(az, bx, -ax * bz);
         da[0] = -bz; da[1] = 0.0; da[2] = bx;
         db[0] = az; db[1] = 0.0; db[2] = -ax;
     } else {
-        q = ax * by - ay * bx;
+        q = fma(ax, by, -ay * bx);
         da[0] = by; da[1] = -bx; da[2] = 0.0;
         db[0] = -ay; db[1] = ax; db[2] = 0.0;
     }
+}
+
+// g += w m + kf (q m + c dq) with fused multiply-adds; components whose dq is
+// identically zero reduce to products.
+__device__ inline void grad_acc(double &gx, double &gy, double &gz, double w, double kf, double q, double c,
+                                double mx, double my, double mz, const double dq[3])
+{
+    gx = fma(kf, fma(c, dq[0], q * mx), fma(w, mx, gx));
+    gy = fma(kf, fma(c, dq[1], q * my), fma(w, my, gy));
+    gz = fma(kf, fma(c, dq[2], q * mz), fma(w, mz, gz));
 }
 
 template <int BC>
 __global__ __launch_bounds__(kGradBlock) void k_energy_grad(GradArgs A)
 {
     const uint32_t b = blockIdx.y;
-    const uint32_t l = blockIdx.x * kGradBlock + threadIdx.x;
+    const uint32_t l = band_block(blockIdx.x, A.band) * kGradBlock + threadIdx.x;
     const double *nx = A.n + (size_t)b * 3 * A.N, *ny = nx + A.N, *nz = ny + A.N;
     const int8_t *s = A.s + (size_t)b * A.N;
-    const double K = A.K[b], eps = A.eps[b];
+    const double K = A.K[b], m2eps = -2.0 * A.eps[b];
     double sum_e = 0.0, sum_cqf = 0.0, sum_s = 0.0;
     if (l < A.N) {
         const uint32_t row = fdiv(l, A.fd_ni);
@@ -102,20 +114,14 @@ __global__ __launch_bounds__(kGradBlock) void k_energy_grad(GradArgs A)
                 if (dir == 0) {
                     c = fma(az, bz, fma(ay, by, ax * bx));
                     dq_axis(axis, ax, ay, az, bx, by, bz, qv, da, db);
-                    sum_e += 1.0 - c * c;
-                    sum_cqf += c * qv * f;
-                    const double w = -2.0 * eps * c, kf = -K * f;
-                    gx += w * bx + kf * (qv * bx + c * da[0]);
-                    gy += w * by + kf * (qv * by + c * da[1]);
-                    gz += w * bz + kf * (qv * bz + c * da[2]);
+                    sum_e += fma(-c, c, 1.0);
+                    sum_cqf = fma(c * qv, f, sum_cqf);
+                    grad_acc(gx, gy, gz, m2eps * c, -K * f, qv, c, bx, by, bz, da);
                 } else {
                     // the bond (q, l): q is the lower end, l the upper end
                     c = fma(az, bz, fma(ay, by, ax * bx));
                     dq_axis(axis, bx, by, bz, ax, ay, az, qv, da, db);
-                    const double w = -2.0 * eps * c, kf = -K * f;
-                    gx += w * bx + kf * (qv * bx + c * db[0]);
-                    gy += w * by + kf * (qv * by + c * db[1]);
-                    gz += w * bz + kf * (qv * bz + c * db[2]);
+                    grad_acc(gx, gy, gz, m2eps * c, -K * f, qv, c, bx, by, bz, db);
                 }
             }
         }
@@ -129,6 +135,138 @@ __global__ __launch_bounds__(kGradBlock) void k_energy_grad(GradArgs A)
         }
     }
     // deterministic block sums
+    __shared__ double red[kGradBlock / 64][3];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        sum_e += __shfl_xor(sum_e, off, 64);
+        sum_cqf += __shfl_xor(sum_cqf, off, 64);
+        sum_s += __shfl_xor(sum_s, off, 64);
+    }
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    if (lane == 0) { red[wid][0] = sum_e; red[wid][1] = sum_cqf; red[wid][2] = sum_s; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int w = 0; w < kGradBlock / 64; ++w) t += red[w][threadIdx.x];
+        A.part[((size_t)b * A.nblk + blockIdx.x) * 4 + threadIdx.x] = t;
+    }
+}
+
+// Bond (lo, up = lo + e_axis) with spin factor f: adds its energy terms when
+// COUNT, its derivative with respect to lo into glo when GLO, and with respect
+// to up into gup when GUP.
+template <bool COUNT, bool GLO, bool GUP>
+__device__ inline void bond(int axis, double lx, double ly, double lz, double ux, double uy, double uz, double f,
+                            double m2eps, double K, double &se, double &scqf, double glo[3], double gup[3])
+{
+    double q, dl[3], du[3];
+    const double c = fma(lz, uz, fma(ly, uy, lx * ux));
+    dq_axis(axis, lx, ly, lz, ux, uy, uz, q, dl, du);
+    if (COUNT) {
+        se += fma(-c, c, 1.0);
+        scqf = fma(c * q, f, scqf);
+    }
+    const double w = m2eps * c, kf = -K * f;
+    if (GLO) grad_acc(glo[0], glo[1], glo[2], w, kf, q, c, ux, uy, uz, dl);
+    if (GUP) grad_acc(gup[0], gup[1], gup[2], w, kf, q, c, lx, ly, lz, du);
+}
+
+__device__ inline double2 ld2(const double *p) { return *reinterpret_cast<const double2 *>(p); }
+__device__ inline void spin_pair(const int8_t *s, uint32_t q, int &lo, int &hi)
+{
+    const uint32_t v = *reinterpret_cast<const uint16_t *>(s + q);
+    lo = (int)(int8_t)(v & 0xffu);
+    hi = (int)(int8_t)(v >> 8);
+}
+
+// Two sites (l, l + 1), l even, per thread when ni is even: both lie in one
+// row, so their directors, their +-y and +-z neighbours and their gradient
+// move as 16-byte words and their spins as 2-byte words; the x bond between
+// them is evaluated once for both ends.  Requires 16-byte aligned director
+// and gradient arrays and 2-byte aligned spins (checked on the host).
+template <int BC>
+__global__ __launch_bounds__(kGradBlock) void k_energy_grad2(GradArgs A)
+{
+    const uint32_t b = blockIdx.y;
+    const uint32_t l = 2u * (band_block(blockIdx.x, A.band) * kGradBlock + threadIdx.x);
+    const double *nx = A.n + (size_t)b * 3 * A.N, *ny = nx + A.N, *nz = ny + A.N;
+    const int8_t *s = A.s + (size_t)b * A.N;
+    const double K = A.K[b], m2eps = -2.0 * A.eps[b];
+    double sum_e = 0.0, sum_cqf = 0.0, sum_s = 0.0;
+    if (l < A.N) {
+        const uint32_t row = fdiv(l, A.fd_ni);
+        const uint32_t i = l - row * A.ni;
+        const uint32_t k = fdiv(row, A.fd_nj);
+        const uint32_t j = row - k * A.nj;
+        const uint32_t plane = A.ni * A.nj;
+        const double2 vx = ld2(nx + l), vy = ld2(ny + l), vz = ld2(nz + l);
+        int sa, sb;
+        spin_pair(s, l, sa, sb);
+        double ga[3] = {0.0, 0.0, 0.0}, gb[3] = {0.0, 0.0, 0.0};
+        // x: a -> b inside the pair, b -> the site after it, the site before a -> a
+        bond<true, true, true>(0, vx.x, vy.x, vz.x, vx.y, vy.y, vz.y, 0.5 * (double)(sa + sb), m2eps, K, sum_e,
+                               sum_cqf, ga, gb);
+        {
+            const bool ok = BC == BC_PERIODIC || i + 2u < A.ni;
+            if (ok) {
+                const uint32_t q = i + 2u < A.ni ? l + 2u : l - i;
+                const double f = 0.5 * (double)(sb + s[q]);
+                bond<true, true, false>(0, vx.y, vy.y, vz.y, nx[q], ny[q], nz[q], f, m2eps, K, sum_e, sum_cqf, gb,
+                                        gb);
+            }
+        }
+        {
+            const bool ok = BC == BC_PERIODIC || i >= 1u;
+            if (ok) {
+                const uint32_t q = i >= 1u ? l - 1u : l + A.ni - 1u;
+                const double f = 0.5 * (double)(s[q] + sa);
+                bond<false, false, true>(0, nx[q], ny[q], nz[q], vx.x, vy.x, vz.x, f, m2eps, K, sum_e, sum_cqf, ga,
+                                         ga);
+            }
+        }
+        // y and z: the pair above (a, b lower ends) and the pair below (a, b upper ends)
+        const uint32_t co[2] = {j, k}, dim[2] = {A.nj, A.nk}, stride[2] = {A.ni, plane};
+#pragma unroll
+        for (int ax = 0; ax < 2; ++ax) {
+            const int axis = ax + 1;
+            {
+                const bool in = co[ax] + 1u < dim[ax];
+                if (in || BC == BC_PERIODIC) {
+                    const uint32_t q = in ? l + stride[ax] : l - (dim[ax] - 1u) * stride[ax];
+                    const double2 ux = ld2(nx + q), uy = ld2(ny + q), uz = ld2(nz + q);
+                    int ta, tb;
+                    spin_pair(s, q, ta, tb);
+                    bond<true, true, false>(axis, vx.x, vy.x, vz.x, ux.x, uy.x, uz.x, 0.5 * (double)(sa + ta), m2eps,
+                                            K, sum_e, sum_cqf, ga, ga);
+                    bond<true, true, false>(axis, vx.y, vy.y, vz.y, ux.y, uy.y, uz.y, 0.5 * (double)(sb + tb), m2eps,
+                                            K, sum_e, sum_cqf, gb, gb);
+                }
+            }
+            {
+                const bool in = co[ax] >= 1u;
+                if (in || BC == BC_PERIODIC) {
+                    const uint32_t q = in ? l - stride[ax] : l + (dim[ax] - 1u) * stride[ax];
+                    const double2 ux = ld2(nx + q), uy = ld2(ny + q), uz = ld2(nz + q);
+                    int ta, tb;
+                    spin_pair(s, q, ta, tb);
+                    bond<false, false, true>(axis, ux.x, uy.x, uz.x, vx.x, vy.x, vz.x, 0.5 * (double)(ta + sa), m2eps,
+                                             K, sum_e, sum_cqf, ga, ga);
+                    bond<false, false, true>(axis, ux.y, uy.y, uz.y, vx.y, vy.y, vz.y, 0.5 * (double)(tb + sb), m2eps,
+                                             K, sum_e, sum_cqf, gb, gb);
+                }
+            }
+        }
+        sum_s = (double)(sa + sb);
+        if (A.dEdn) {
+            const double pa = fma(ga[2], vz.x, fma(ga[1], vy.x, ga[0] * vx.x));
+            const double pb = fma(gb[2], vz.y, fma(gb[1], vy.y, gb[0] * vx.y));
+            double *o = A.dEdn + (size_t)b * 3 * A.N + l;
+            *reinterpret_cast<double2 *>(o) = make_double2(fma(-pa, vx.x, ga[0]), fma(-pb, vx.y, gb[0]));
+            *reinterpret_cast<double2 *>(o + A.N) = make_double2(fma(-pa, vy.x, ga[1]), fma(-pb, vy.y, gb[1]));
+            *reinterpret_cast<double2 *>(o + 2 * (size_t)A.N) =
+                make_double2(fma(-pa, vz.x, ga[2]), fma(-pb, vz.y, gb[2]));
+        }
+    }
     __shared__ double red[kGradBlock / 64][3];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
@@ -216,12 +354,25 @@ int chll_energy_grad(const double *n, const int8_t *s, int32_t batch, int32_t ni
     if (workspace_bytes < chll_energy_workspace_bytes(batch, ni, nj, nk)) return gfail("chll_energy_grad: workspace too small");
     GradArgs A;
     A.n = n; A.s = s; A.K = K; A.g = g; A.eps = eps; A.dEdn = dEdn; A.part = (double *)workspace;
-    A.ni = ni; A.nj = nj; A.nk = nk; A.N = (uint32_t)N; A.nblk = (uint32_t)((N + kGradBlock - 1) / kGradBlock);
+    A.ni = ni; A.nj = nj; A.nk = nk; A.N = (uint32_t)N;
     A.fd_ni = make_fastdiv(ni); A.fd_nj = make_fastdiv(nj); A.bc = bc;
+    // two sites per thread when every row splits into aligned pairs
+    const bool pairs = ni % 2 == 0 && (uintptr_t)n % 16 == 0 && (uintptr_t)dEdn % 16 == 0 && (uintptr_t)s % 2 == 0;
+    const uint64_t per_thread = pairs ? 2 : 1;
+    A.nblk = (uint32_t)((N / per_thread + kGradBlock - 1) / kGradBlock);
+    // XCD row bands when a plane is 8 bands of whole blocks (then every
+    // lattice's block count is a multiple of 8 and block (x, b) runs on XCD x % 8)
+    const uint64_t plane = (uint64_t)ni * nj / per_thread;
+    A.band = plane % (8u * kGradBlock) == 0 ? (uint32_t)(plane / (8u * kGradBlock)) : 0u;
     hipStream_t st = (hipStream_t)stream;
     dim3 grid(A.nblk, batch);
-    if (bc == BC_OPEN) hipLaunchKernelGGL(k_energy_grad<BC_OPEN>, grid, dim3(kGradBlock), 0, st, A);
-    else hipLaunchKernelGGL(k_energy_grad<BC_PERIODIC>, grid, dim3(kGradBlock), 0, st, A);
+    if (pairs) {
+        if (bc == BC_OPEN) hipLaunchKernelGGL(k_energy_grad2<BC_OPEN>, grid, dim3(kGradBlock), 0, st, A);
+        else hipLaunchKernelGGL(k_energy_grad2<BC_PERIODIC>, grid, dim3(kGradBlock), 0, st, A);
+    } else {
+        if (bc == BC_OPEN) hipLaunchKernelGGL(k_energy_grad<BC_OPEN>, grid, dim3(kGradBlock), 0, st, A);
+        else hipLaunchKernelGGL(k_energy_grad<BC_PERIODIC>, grid, dim3(kGradBlock), 0, st, A);
+    }
     hipLaunchKernelGGL(k_energy_final, dim3(batch), dim3(256), 0, st, A, E, dEdK, dEdg, dEdeps);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return gfail("chll_energy_grad launch: %s", hipGetErrorString(e));

@@ -74,21 +74,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_spin(Geom G, const int8_t *__
 }
 
 // ---------------------------------------------------------------------------
-// Director half-sweep over one colour (chll/chll_mod.f90:117-202).
-//
-// Workgroups go round-robin to the 8 XCDs (workgroup b to XCD b % 8), each with
-// its own L2.  With band > 0 the plane's blocks are cut into 8 bands of `band`
-// blocks (rows of the sublattice) and XCD x takes band x of every plane, planes
-// in order: a row's y neighbours then sit in the same L2 except at the band
-// edges, and the z neighbours of a band stay in that L2 while its next planes
-// run.  band = 0 keeps the identity order.
-__device__ inline uint32_t band_block(uint32_t b, uint32_t band)
-{
-    if (band == 0u) return b;
-    const uint32_t x = b & 7u, n = b >> 3;
-    const uint32_t p = n / band, r = n - p * band;
-    return (p * 8u + x) * band + r;
-}
+// Director half-sweep over one colour (chll/chll_mod.f90:117-202); block order
+// band_block (chll_device.h).
 
 template <int ARITH, int RNG, int BC, bool OFF32>
 __global__ __launch_bounds__(kBlock) void k_dir(Geom G, Fields F, int colour, PhaseArgs A,

@@ -47,7 +47,7 @@ def ref_energy(n, s, K, g, eps, bc):
 
 
 @pytest.mark.parametrize("bc", ["open", "periodic"])
-@pytest.mark.parametrize("shape", [(2, 3, 4), (6, 5, 4), (8, 8, 8), (1, 4, 6)])
+@pytest.mark.parametrize("shape", [(2, 3, 4), (6, 5, 4), (8, 8, 8), (1, 4, 6), (3, 4, 5), (2, 3, 2), (4, 1, 7)])
 def test_against_cpu_autograd(bc, shape):
     B = 3
     n, s = random_batch(B, *shape, seed=sum(shape))
