@@ -16,7 +16,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libchll_mod.so")
-SOURCES = ["chll_kernels.hip", "chll_api.hip", "chll_ensemble.hip", "chll_grad.hip", "chll_march.hip"]
+SOURCES = ["chll_kernels.hip", "chll_api.hip", "chll_ensemble.hip", "chll_grad.hip", "chll_march.hip",
+           "chll_analysis.hip"]
 HEADERS = ["chll_device.h", "chll_kernels.h", "chll_site.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

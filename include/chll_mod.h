@@ -218,6 +218,42 @@ int chll_energy_grad(const double *n, const int8_t *s, int32_t batch, int32_t ni
                      const double *K, const double *g, const double *eps, double *E, double *dEdn, double *dEdK,
                      double *dEdg, double *dEdeps, void *workspace, size_t workspace_bytes, void *stream);
 
+/* Analysis ------------------------------------------------------------- */
+/* Device pointers, one lattice in plain Fortran order (i fastest), float64
+ * director components, int8 spins.  Functions return 0 or -1 with
+ * chll_analysis_last_error() set; the device work is enqueued on `stream`
+ * (HIP stream or NULL), after a wait for the small host-to-device table
+ * copies each call makes. */
+const char *chll_analysis_last_error(void);
+
+/* Replaces calc_n_corr and calc_s_corr (notebooks/paper_plots2.ipynb, cell 0;
+ * numba loops).  For shells sh = 0 .. nshells-1 (1 <= nshells <= 127) writes
+ * g_nem[sh] = mean of (n_r . n_{r+d})^2 and g_chi[sh] = mean of
+ * s_r s_{r+d} - <s>^2 over the ordered pairs (r, r + d) inside the lattice
+ * with floor(|d|) = sh and |d| < nshells; either output may be NULL (then its
+ * inputs may be too).  Fails when a shell has no pairs (the reference divides
+ * by zero there).  workspace: chll_corr_workspace_bytes(). */
+size_t chll_corr_workspace_bytes(int32_t ni, int32_t nj, int32_t nk, int32_t nshells);
+int chll_shell_corr(const double *nx, const double *ny, const double *nz, const int8_t *s, int32_t ni, int32_t nj,
+                    int32_t nk, int32_t nshells, double *g_nem, double *g_chi, void *workspace,
+                    size_t workspace_bytes, void *stream);
+/* Host only: the pair counts per shell (the reference's second return value). */
+int chll_corr_counts(int32_t ni, int32_t nj, int32_t nk, int32_t nshells, int64_t *counts);
+
+/* Replaces get_sc / structure_factor
+ * (notebooks/.ipynb_checkpoints/hero_stripe-checkpoint.py:10-29; numba loop).
+ * planes: host array of 3 ints per plane (1 <= nplanes <= 16); kv: host array
+ * of `bins` wave numbers.  Writes sf[p*bins + b] =
+ * |sum_r s_r exp(-i kv[b] planes[p] . (r + 1))|^2 / N^2 (device, nplanes x bins),
+ * r = (i, j, k) 0-based.  The planes' projections p . (r + 1) may span at most
+ * 12288 distinct values in total per call.
+ * workspace: chll_sf_workspace_bytes() (0 for invalid planes). */
+size_t chll_sf_workspace_bytes(int32_t ni, int32_t nj, int32_t nk, const int32_t *planes, int32_t nplanes,
+                               int32_t bins);
+int chll_structure_factor(const int8_t *s, int32_t ni, int32_t nj, int32_t nk, const int32_t *planes,
+                          int32_t nplanes, const double *kv, int32_t bins, double *sf, void *workspace,
+                          size_t workspace_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
