@@ -111,6 +111,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-replica scan")
     ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in run() timing")
     ap.add_argument("--no-grad", action="store_true", help="skip the energy-gradient timing")
+    ap.add_argument("--no-analysis", action="store_true", help="skip the analysis-kernel timing")
+    ap.add_argument("--analysis-only", action="store_true",
+                    help="run only the analysis section (and its CPU figures unless --no-cpu)")
     ap.add_argument("--dropin-sweeps", type=int, default=20, help="sweeps per drop-in run() call")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default kernels instead of autotuning (clean profiles)")
@@ -129,6 +132,14 @@ def main():
 
     from chllmodel_amd.lattice import Lattice
     from chllmodel_amd.states import isotropic_state
+
+    if args.analysis_only:
+        torch.cuda.init()
+        res = analysis_rates(args)
+        if not args.no_cpu:
+            analysis_cpu_baseline(res)
+        print(json.dumps({"analysis": res}), flush=True)
+        return
 
     L = args.L
     N = L ** 3
@@ -255,12 +266,16 @@ def main():
         out["dropin_run"] = dropin_run(L, st0, args)
     if not args.no_grad:
         out["gradient"] = gradient_rate(args)
+    if not args.no_analysis:
+        out["analysis"] = analysis_rates(args)
     if not args.no_batched:
         out["batched"] = batched_scan(args, stream)
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         log(f"bench: CPU baseline, {args.cpu_sweeps} sweeps at L={L}")
         out["cpu_baseline"] = cpu_baseline(L, st0, args)
         out["speedup_vs_cpu"] = sups / out["cpu_baseline"]["value"]
+        if "analysis" in out:
+            analysis_cpu_baseline(out["analysis"])
     print(json.dumps(out), flush=True)
 
 
@@ -274,10 +289,9 @@ def dropin_run(L, st0, args):
     import numpy as np
 
     from chllmodel_amd import _lib
-    from oracle.oracle import copy_state
 
     lib = _lib.lib()
-    st = copy_state(st0)
+    st = {k: np.asfortranarray(v.copy()) for k, v in st0.items()}
     n = args.dropin_sweeps
     nsub = L ** 3 // 2
     msteps = np.zeros(1, dtype=np.int32)
@@ -345,6 +359,107 @@ def gradient_rate(args):
     return {"workload": f"{B} lattices of {Lg}^3, open boundaries, energy + tangent gradient + parameter sums",
             "us_per_call": 1e6 * sec, "sites_per_s": sites / sec, "algorithmic_bytes_per_site": 49.0,
             "GBps": gbps, "hbm_frac": gbps / HBM_PEAK_GBS}
+
+
+def analysis_rates(args):
+    """Analysis kernels on device-resident inputs, timed per call with HIP
+    events on torch's current stream (each call also copies its small host
+    table to the device and waits for it):
+    * chirality structure factor of a 480^3 spin field on 7 planes (faces and
+      body diagonals) x 500 kv in [0, 0.5], the stripe analysis of the L=480
+      runs; the reference's form evaluates planes x kv x N phases;
+    * nematic + chiral shell correlations of a 128^3 state, 8 shells; the
+      reference's form visits every ordered pair (r, r + d) once.
+    The CPU figures come from analysis_cpu_baseline (the CPU-baseline leg)."""
+    import numpy as np
+    import torch
+
+    from chllmodel_amd.analysis import corr_counts, shell_correlations_device, structure_factors_device
+
+    planes = [[1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 1], [1, 1, -1], [1, -1, 1], [1, -1, -1]]
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    res = {}
+
+    Ls, bins = 480, 500
+    N = Ls ** 3
+    ds = torch.randint(0, 2, (N,), device="cuda", generator=gen, dtype=torch.int8) * 2 - 1
+    kvs = np.linspace(0.0, 0.5, bins)
+    for _ in range(2):
+        structure_factors_device(ds, (Ls, Ls, Ls), kvs, planes)
+    reps = 5
+    e0, e1 = ev(), ev()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        structure_factors_device(ds, (Ls, Ls, Ls), kvs, planes)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / reps
+    terms = len(planes) * bins * N
+    log(f"bench: structure factor {Ls}^3 x {len(planes)} planes x {bins} kv: {1e3 * sec:.2f} ms per call")
+    res["structure_factor"] = {
+        "workload": f"{Ls}^3 spins, {len(planes)} planes x {bins} kv in [0, 0.5]",
+        "ms_per_call": 1e3 * sec, "reference_phase_terms_per_s": terms / sec,
+        "phase_terms": terms, "histogram_pass_GBps": N / sec / 1e9}
+    del ds
+
+    Lr, ns = 128, 8
+    N = Lr ** 3
+    n = torch.randn(3, N, device="cuda", dtype=torch.float64, generator=gen)
+    n /= n.norm(dim=0, keepdim=True)
+    dn = [n[0].contiguous(), n[1].contiguous(), n[2].contiguous()]
+    dsp = torch.randint(0, 2, (N,), device="cuda", generator=gen, dtype=torch.int8) * 2 - 1
+    shell_correlations_device((Lr, Lr, Lr), ns, dn, dsp)
+    reps = 3
+    e0, e1 = ev(), ev()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        shell_correlations_device((Lr, Lr, Lr), ns, dn, dsp)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / reps
+    pairs = int(corr_counts((Lr, Lr, Lr), ns).sum())
+    log(f"bench: shell correlations {Lr}^3, {ns} shells: {1e3 * sec:.2f} ms per call")
+    res["shell_correlations"] = {
+        "workload": f"{Lr}^3 directors + spins, {ns} shells, nematic and chiral",
+        "nshells": ns, "ms_per_call": 1e3 * sec, "ordered_pairs": pairs, "pairs_per_s": pairs / sec}
+    return res
+
+
+def analysis_cpu_baseline(res):
+    """CPU figures for the analysis section: the oracle's vectorised numpy sums
+    (one core) on bounded samples, per unit of the same work (phase terms of
+    the structure factor, ordered pairs of the correlations)."""
+    import numpy as np
+
+    from oracle import analysis as ora
+
+    sf = res["structure_factor"]
+    Lc = 96
+    sc = np.where(np.random.default_rng(1).random((Lc, Lc, Lc)) < 0.5, -1, 1).astype(np.int32)
+    t0 = time.perf_counter()
+    for kv in (0.1, 0.2, 0.3):
+        ora.get_sc(sc, kv * np.array([1.0, 1.0, 1.0]))
+    rate = 3 * Lc ** 3 / (time.perf_counter() - t0)
+    sf["cpu_oracle"] = {"phase_terms_per_s": rate, "cores": 1, "kind": "port",
+                        "sample": f"numpy direct sum, {Lc}^3 spins, 3 kv, plane (1,1,1)",
+                        "est_seconds_for_workload": sf["phase_terms"] / rate}
+    sf["speedup_vs_cpu"] = sf["reference_phase_terms_per_s"] / rate
+    cr = res["shell_correlations"]
+    ns, Lc = cr["nshells"], 24
+    rng = np.random.default_rng(2)
+    v = rng.normal(size=(3, Lc, Lc, Lc))
+    v /= np.linalg.norm(v, axis=0)
+    scc = np.where(rng.random((Lc, Lc, Lc)) < 0.5, -1, 1).astype(np.int32)
+    t0 = time.perf_counter()
+    ora.calc_n_corr(v[0], v[1], v[2], ns)
+    ora.calc_s_corr(scc, ns)
+    rate = int(ora.corr_counts((Lc, Lc, Lc), ns).sum()) / (time.perf_counter() - t0)
+    cr["cpu_oracle"] = {"pairs_per_s": rate, "cores": 1, "kind": "port",
+                        "sample": f"numpy per-offset sums, {Lc}^3, {ns} shells, nematic and chiral"}
+    cr["speedup_vs_cpu"] = cr["pairs_per_s"] / rate
 
 
 def batched_scan(args, stream):

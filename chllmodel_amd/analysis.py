@@ -100,34 +100,66 @@ def corr_counts(shape, nshells: int) -> np.ndarray:
     return out
 
 
-def shell_correlations(nx=None, ny=None, nz=None, s=None, *, nshells: int):
-    """(g_nem or None, g_chi or None, counts as float64) in one kernel pass."""
+def prepare_directors(nx, ny, nz):
+    """(ni, nj, nk) director components -> three flat Fortran-order float64
+    tensors on the GPU (the layout the kernels read)."""
+    import torch
+
+    shape = _shape3(nx)
+    if tuple(ny.shape) != shape or tuple(nz.shape) != shape:
+        raise ValueError("arrays must share one shape")
+    return [_device(a, torch.float64) for a in (nx, ny, nz)]
+
+
+def prepare_spins(s):
+    """(ni, nj, nk) spins -> flat Fortran-order int8 tensor on the GPU."""
+    _shape3(s)
+    return _spins_i8(s)
+
+
+def shell_correlations_device(shape, nshells, dn=None, ds=None, workspace=None):
+    """Device-resident form: dn = prepare_directors(...) or None, ds =
+    prepare_spins(...) or None.  Returns (g_nem, g_chi) as GPU tensors (None
+    where not asked), enqueued on torch's current stream."""
     import torch
 
     L = _declare(lib())
-    nem = nx is not None
-    chi = s is not None
-    if not nem and not chi:
+    if dn is None and ds is None:
         raise ValueError("pass the directors, the spins, or both")
-    shape = _shape3(nx if nem else s)
-    for a in (ny, nz, s) if nem else ():
-        if a is not None and tuple(a.shape) != shape:
-            raise ValueError("arrays must share one shape")
     ni, nj, nk = shape
-    dx = [_device(a, torch.float64) for a in (nx, ny, nz)] if nem else [None] * 3
-    ds = _spins_i8(s) if chi else None
-    gn = torch.empty(nshells, dtype=torch.float64, device="cuda") if nem else None
-    gs = torch.empty(nshells, dtype=torch.float64, device="cuda") if chi else None
+    N = ni * nj * nk
+    for t in (dn or []) + ([ds] if ds is not None else []):
+        if t.numel() != N or not t.is_cuda:
+            raise ValueError("device arrays must hold ni*nj*nk elements on the GPU")
+    gn = torch.empty(nshells, dtype=torch.float64, device="cuda") if dn is not None else None
+    gs = torch.empty(nshells, dtype=torch.float64, device="cuda") if ds is not None else None
     wsb = L.chll_corr_workspace_bytes(ni, nj, nk, nshells)
     if wsb == 0:
         raise ValueError("nshells must be in 1..127 and the shape positive")
-    ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+    ws = workspace if workspace is not None and workspace.numel() >= wsb else \
+        torch.empty(wsb, dtype=torch.uint8, device="cuda")
     ptr = lambda t: _vp(t.data_ptr()) if t is not None else None  # noqa: E731
+    dx = dn if dn is not None else [None] * 3
     stream = torch.cuda.current_stream().cuda_stream
     rc = L.chll_shell_corr(ptr(dx[0]), ptr(dx[1]), ptr(dx[2]), ptr(ds), ni, nj, nk, nshells, ptr(gn), ptr(gs),
                            ptr(ws), wsb, _vp(stream))
     if rc != 0:
         _fail(L, "chll_shell_corr")
+    return gn, gs
+
+
+def shell_correlations(nx=None, ny=None, nz=None, s=None, *, nshells: int):
+    """(g_nem or None, g_chi or None, counts as float64) in one kernel pass."""
+    nem = nx is not None
+    chi = s is not None
+    if not nem and not chi:
+        raise ValueError("pass the directors, the spins, or both")
+    shape = _shape3(nx if nem else s)
+    if chi and tuple(s.shape) != shape:
+        raise ValueError("arrays must share one shape")
+    dn = prepare_directors(nx, ny, nz) if nem else None
+    ds = prepare_spins(s) if chi else None
+    gn, gs = shell_correlations_device(shape, nshells, dn, ds)
     c = corr_counts(shape, nshells).astype(np.float64)
     return (gn.cpu().numpy() if nem else None), (gs.cpu().numpy() if chi else None), c
 
@@ -168,18 +200,19 @@ def fit_corr_exp2(g):
     return popt[0], popt[1], popt[2]
 
 
-def structure_factors(data, kvs, planes) -> np.ndarray:
-    """S(kv) for every plane vector in `planes` and every kv: array (P, bins).
-    Planes are grouped so that each call's projection histograms fit."""
+def structure_factors_device(ds, shape, kvs, planes):
+    """Device-resident form: ds = prepare_spins(...).  Returns an array
+    (P, bins) on the GPU; planes are grouped so that each call's projection
+    histograms fit."""
     import torch
 
     L = _declare(lib())
-    shape = _shape3(data)
     ni, nj, nk = shape
+    if ds.numel() != ni * nj * nk or ds.dtype != torch.int8 or not ds.is_cuda:
+        raise ValueError("ds must be the flat int8 spins on the GPU")
     kv = np.ascontiguousarray(np.asarray(kvs, dtype=np.float64))
     pl = np.ascontiguousarray(np.asarray(planes, dtype=np.int32).reshape(-1, 3))
-    ds = _spins_i8(data)
-    out = np.empty((len(pl), len(kv)), dtype=np.float64)
+    out = torch.empty((len(pl), len(kv)), dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
     p0 = 0
     while p0 < len(pl):
@@ -191,15 +224,19 @@ def structure_factors(data, kvs, planes) -> np.ndarray:
         if wsb == 0:
             _fail(L, f"plane {pl[p0].tolist()}")
         ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
-        sf = torch.empty((len(group), len(kv)), dtype=torch.float64, device="cuda")
         rc = L.chll_structure_factor(_vp(ds.data_ptr()), ni, nj, nk, group.ctypes.data_as(ct.POINTER(ct.c_int32)),
                                      len(group), kv.ctypes.data_as(ct.POINTER(ct.c_double)), len(kv),
-                                     _vp(sf.data_ptr()), _vp(ws.data_ptr()), wsb, _vp(stream))
+                                     _vp(out[p0:p1].data_ptr()), _vp(ws.data_ptr()), wsb, _vp(stream))
         if rc != 0:
             _fail(L, "chll_structure_factor")
-        out[p0:p1] = sf.cpu().numpy()
         p0 = p1
     return out
+
+
+def structure_factors(data, kvs, planes) -> np.ndarray:
+    """S(kv) for every plane vector in `planes` and every kv: array (P, bins)."""
+    shape = _shape3(data)
+    return structure_factors_device(prepare_spins(data), shape, kvs, planes).cpu().numpy()
 
 
 def _sf_ws(L, shape, planes, bins):

@@ -39,8 +39,9 @@ namespace chll {
 
 constexpr int kCorrBlock = 256;
 constexpr int kSfBlock = 256;
+constexpr int kSfHistBlock = 1024;
 constexpr int kSfMaxBins = 12288;    // LDS histogram entries per pass (48 KiB)
-constexpr int kSfMaxBlocks = 1024;
+constexpr int kSfMaxBlocks = 512;
 
 struct CorrArgs {
     const double *nx, *ny, *nz;
@@ -187,56 +188,94 @@ struct SfArgs {
     FastDiv fd_nj;
 };
 
-// One wave per row (j, k) at a time: lanes stride i, so the loads are
-// contiguous runs of bytes.  A plane with p0 != 0 sends each lane's site to its
-// own histogram entry (distinct entries within the row); a plane with p0 == 0
-// maps the whole row to one entry, which gets the wave's row sum.
-__global__ __launch_bounds__(kSfBlock) void k_sf_hist(SfArgs A)
+// One wave per row (j, k) at a time, 16 waves of a block sharing one LDS
+// histogram.  Each pass of the wave takes 64 U consecutive sites of the row:
+// lane l holds sites ib + l + 64 u (U byte loads in flight), so every atomic
+// round of a plane with |p0| = 1 hits 64 consecutive entries (no bank
+// conflicts).  A plane with p0 != 0 sends each site to its own entry,
+// p0 (i + 1) + c_row; a plane with p0 == 0 maps the whole row to one entry,
+// which gets the wave's row sum.  Lane p holds plane p's constants (and, per
+// row, its c_row); the plane loop broadcasts them with readlane, in
+// wave-uniform control flow.
+template <int U>
+__global__ __launch_bounds__(kSfHistBlock) void k_sf_hist(SfArgs A)
 {
     extern __shared__ int h[];
-    for (int t = threadIdx.x; t < A.ntot; t += kSfBlock) h[t] = 0;
+    for (int t = threadIdx.x; t < A.ntot; t += kSfHistBlock) h[t] = 0;
     __syncthreads();
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t waves = A.nblk * (kSfBlock / 64);
-    for (uint32_t row = blockIdx.x * (kSfBlock / 64) + (threadIdx.x >> 6); row < A.rows; row += waves) {
+    const int np = A.nplanes;
+    const int lane = (int)(threadIdx.x & 63u);
+    int q0 = 0, q1 = 0, q2 = 0, qb = 0;
+    for (int p = 0; p < np; ++p) {
+        if (lane == p) {
+            q0 = A.pl[p].p0; q1 = A.pl[p].p1; q2 = A.pl[p].p2;
+            qb = A.pl[p].base - A.pl[p].tmin;
+        }
+    }
+    const uint32_t waves = A.nblk * (kSfHistBlock / 64);
+    // the row is one per wave: keep it and its coordinates in scalar registers
+    const uint32_t row0 =
+        (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kSfHistBlock / 64) + (threadIdx.x >> 6)));
+    for (uint32_t row = row0; row < A.rows; row += waves) {
         const uint32_t k = fdiv(row, A.fd_nj), j = row - k * A.nj;
+        const int cst = q1 * (int)(j + 1) + q2 * (int)(k + 1) + qb;
         const int8_t *sr = A.s + (size_t)row * A.ni;
         int rsum = 0;
-        for (uint32_t i = lane; i < A.ni; i += 64) {
-            const int v = sr[i];
-            rsum += v;
-            for (int p = 0; p < A.nplanes; ++p) {
-                const SfPlane P = A.pl[p];
-                if (P.p0 != 0) {
-                    const int t = P.p0 * (int)(i + 1) + P.p1 * (int)(j + 1) + P.p2 * (int)(k + 1);
-                    atomicAdd(&h[P.base + t - P.tmin], v);
+        for (uint32_t ib = 0; ib < A.ni; ib += 64u * U) {
+            int v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t i = ib + 64u * u + (uint32_t)lane;
+                v[u] = i < A.ni ? (int)sr[i] : 0;
+                rsum += v[u];
+            }
+            for (int p = 0; p < np; ++p) {
+                const int a = __builtin_amdgcn_readlane(q0, p);
+                const int c = __builtin_amdgcn_readlane(cst, p);
+                if (a != 0) {
+                    const int t0 = a * (int)(ib + (uint32_t)lane + 1u) + c;
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (ib + 64u * u + (uint32_t)lane < A.ni) atomicAdd(&h[t0 + a * 64 * u], v[u]);
                 }
             }
         }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) rsum += __shfl_xor(rsum, off, 64);
-        if (lane == 0) {
-            for (int p = 0; p < A.nplanes; ++p) {
-                const SfPlane P = A.pl[p];
-                if (P.p0 == 0) {
-                    const int t = P.p1 * (int)(j + 1) + P.p2 * (int)(k + 1);
-                    atomicAdd(&h[P.base + t - P.tmin], rsum);
-                }
-            }
+        for (int p = 0; p < np; ++p) {
+            const int a = __builtin_amdgcn_readlane(q0, p);
+            const int c = __builtin_amdgcn_readlane(cst, p);
+            if (a == 0 && lane == 0) atomicAdd(&h[c], rsum);
         }
     }
     __syncthreads();
     int *out = A.part + (size_t)blockIdx.x * A.ntot;
-    for (int t = threadIdx.x; t < A.ntot; t += kSfBlock) out[t] = h[t];
+    for (int t = threadIdx.x; t < A.ntot; t += kSfHistBlock) out[t] = h[t];
 }
 
+// Block of 64 entries x 4 waves: wave w sums the block histograms w, w + 4,
+// ... with four independent accumulators; the 4 partial sums are combined in
+// LDS (integer sums: exact in any order).
 __global__ __launch_bounds__(kSfBlock) void k_sf_reduce(SfArgs A)
 {
-    const int t = blockIdx.x * kSfBlock + threadIdx.x;
-    if (t >= A.ntot) return;
-    int v = 0;
-    for (uint32_t b = 0; b < A.nblk; ++b) v += A.part[(size_t)b * A.ntot + t];
-    A.hist[t] = v;
+    __shared__ int red[kSfBlock / 64][64];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const int t = blockIdx.x * 64 + (int)lane;
+    int a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    if (t < A.ntot) {
+        const size_t stride = (size_t)A.ntot;
+        uint32_t b = w;
+        for (; b + 12u < A.nblk; b += 16u) {
+            a0 += A.part[(size_t)b * stride + t];
+            a1 += A.part[(size_t)(b + 4u) * stride + t];
+            a2 += A.part[(size_t)(b + 8u) * stride + t];
+            a3 += A.part[(size_t)(b + 12u) * stride + t];
+        }
+        for (; b < A.nblk; b += 4u) a0 += A.part[(size_t)b * stride + t];
+    }
+    red[w][lane] = (a0 + a1) + (a2 + a3);
+    __syncthreads();
+    if (w == 0 && t < A.ntot) A.hist[t] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
 // Block (bin, plane): sum_t h[t] exp(-i kv t) in a fixed order, then
@@ -401,7 +440,7 @@ SfLayout sf_layout(int32_t ni, int32_t nj, int32_t nk, int ntot, int32_t bins)
 {
     SfLayout L{};
     const uint64_t rows = (uint64_t)nj * nk;
-    const uint64_t want = (rows + kSfBlock / 64 - 1) / (kSfBlock / 64);
+    const uint64_t want = (rows + kSfHistBlock / 64 - 1) / (kSfHistBlock / 64);
     L.nblk = (uint32_t)(want < kSfMaxBlocks ? want : kSfMaxBlocks);
     L.ntot = ntot;
     size_t off = 0;
@@ -520,8 +559,8 @@ int chll_structure_factor(const int8_t *s, int32_t ni, int32_t nj, int32_t nk, c
     A.ni = ni; A.nj = nj; A.nk = nk; A.N = (uint32_t)N; A.rows = (uint32_t)((uint64_t)nj * nk); A.nblk = L.nblk;
     A.nplanes = nplanes; A.ntot = ntot;
     A.fd_nj = make_fastdiv(nj);
-    hipLaunchKernelGGL(k_sf_hist, dim3(L.nblk), dim3(kSfBlock), (size_t)ntot * sizeof(int), st, A);
-    hipLaunchKernelGGL(k_sf_reduce, dim3((ntot + kSfBlock - 1) / kSfBlock), dim3(kSfBlock), 0, st, A);
+    hipLaunchKernelGGL(k_sf_hist<4>, dim3(L.nblk), dim3(kSfHistBlock), (size_t)ntot * sizeof(int), st, A);
+    hipLaunchKernelGGL(k_sf_reduce, dim3((ntot + 63) / 64), dim3(kSfBlock), 0, st, A);
     hipLaunchKernelGGL(k_sf_phase, dim3(bins, nplanes), dim3(kSfBlock), 0, st, A, (const double *)(ws + L.kv), bins, sf);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return afail("chll_structure_factor launch: %s", hipGetErrorString(e));
