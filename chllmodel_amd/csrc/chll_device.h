@@ -141,6 +141,16 @@ __device__ inline uint32_t band_block(uint32_t b, uint32_t band)
     return (p * 8u + x) * band + r;
 }
 
+// Block of a sweep launch: bits 0..30 of `band` as band_block, bit 31 = walk
+// the grid last to first (a launch then starts on the planes the launch before
+// it finished with, which the memory-side cache still holds; the grid must be
+// a multiple of 8 blocks when band > 0, as it is whenever band_block bands).
+__device__ inline uint32_t sweep_block(uint32_t band)
+{
+    const uint32_t b = (band >> 31) ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
+    return band_block(b, band & 0x7fffffffu);
+}
+
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
 struct U4 {

@@ -70,11 +70,13 @@ struct EnsDev {
 // trial, 3: colour-0 chirality trials of sweep t each followed by the site's
 // director trial of sweep t + 1 (the single-lattice fused launches).  One
 // kernel per kind, so each gets the register budget of its own trial.
+// rev: walk the block table last to first, so that a launch starts on the
+// replicas the launch before it finished with (still in the memory-side cache).
 template <int KIND, int ARITH, int BC, int BS>
-__global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int colour, uint32_t istep_off)
+__global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int colour, uint32_t istep_off, int rev)
 {
     constexpr int kind = KIND;
-    const uint32_t code = E.blk_rep[blockIdx.x];
+    const uint32_t code = E.blk_rep[rev ? gridDim.x - 1u - blockIdx.x : blockIdx.x];
     const uint32_t r = code >> 24, lb = code & 0xFFFFFFu;
     const RepDesc &D = E.R[r];
     const int s = E.rep_slot[r];
@@ -374,22 +376,23 @@ void launch_block_t(chll_ensemble *X, int K, hipStream_t st)
     for (int t = 0; t < K; ++t) {
         const uint32_t off = (uint32_t)t;
         const bool last = t + 1 == K;
+        // consecutive sweep launches alternate their block order
         if (!fuse) {
-            hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off);
-            hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 1, off);
-            hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off);
-            hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 1, off);
+            hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 0);
+            hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 1);
+            hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 0);
+            hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 1);
             if (last) hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E);
             hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, last ? 1 : 0, 0, 1);
             continue;
         }
-        if (t == 0) hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off);
-        hipLaunchKernelGGL((k_phase_ens<2, ARITH, BC, BS>), g, b, 0, st, E, 1, off);
+        if (t == 0) hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1);
+        hipLaunchKernelGGL((k_phase_ens<2, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 0);
         if (!last) {
             hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 0, t > 0 ? 1 : 0, 0);
-            hipLaunchKernelGGL((k_phase_ens<3, ARITH, BC, BS>), g, b, 0, st, E, 0, off);
+            hipLaunchKernelGGL((k_phase_ens<3, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1);
         } else {
-            hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off);
+            hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1);
             hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E);
             hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 1, t > 0 ? 1 : 0, 1);
         }

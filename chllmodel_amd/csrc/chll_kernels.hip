@@ -81,7 +81,7 @@ template <int ARITH, int RNG, int BC, bool OFF32>
 __global__ __launch_bounds__(kBlock) void k_dir(Geom G, Fields F, int colour, PhaseArgs A,
                                                 uint32_t *__restrict__ counts, uint32_t band)
 {
-    const uint32_t m = band_block(blockIdx.x, band) * kBlock + threadIdx.x;
+    const uint32_t m = sweep_block(band) * kBlock + threadIdx.x;
     const bool acc = m < G.nsub && dir_site<ARITH, RNG, BC, OFF32>(G, F, colour, A, m);
     block_count<kBlock>(acc, counts, blockIdx.x);
 }
@@ -91,7 +91,7 @@ template <int ARITH, int RNG, int BC, bool OFF32>
 __global__ __launch_bounds__(kBlock) void k_spin(Geom G, Fields F, int colour, PhaseArgs A,
                                                  uint32_t *__restrict__ counts, uint32_t band)
 {
-    const uint32_t m = band_block(blockIdx.x, band) * kBlock + threadIdx.x;
+    const uint32_t m = sweep_block(band) * kBlock + threadIdx.x;
     const bool acc = m < G.nsub && spin_site<ARITH, RNG, BC, OFF32>(G, F, colour, A, m);
     block_count<kBlock>(acc, counts, blockIdx.x);
 }
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_dir_spin(Geom G, Fields F, int colou
                                                      uint32_t *__restrict__ counts_dir,
                                                      uint32_t *__restrict__ counts_spin, uint32_t band)
 {
-    const uint32_t m = band_block(blockIdx.x, band) * kBlock + threadIdx.x;
+    const uint32_t m = sweep_block(band) * kBlock + threadIdx.x;
     bool flip = false;
     const bool acc = m < G.nsub && dir_spin_site<RNG, BC, OFF32>(G, F, colour, A, m, flip);
     block_count2<kBlock>(acc, flip, counts_dir, counts_spin, blockIdx.x);
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_spin_dir(Geom G, Fields F, PhaseArgs
                                                      uint32_t *__restrict__ counts_dir,
                                                      uint32_t *__restrict__ counts_spin, uint32_t band)
 {
-    const uint32_t m = band_block(blockIdx.x, band) * kBlock + threadIdx.x;
+    const uint32_t m = sweep_block(band) * kBlock + threadIdx.x;
     bool flip = false;
     const bool acc = m < G.nsub && spin_dir_site<BC, OFF32>(G, F, A, m, flip);
     block_count2<kBlock>(acc, flip, counts_dir, counts_spin, blockIdx.x);
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(Geom G, Fields F, int colour, P
                                                  uint32_t *__restrict__ c0, uint32_t *__restrict__ c1,
                                                  uint32_t band)
 {
-    const uint32_t m0 = 2u * (band_block(blockIdx.x, band) * kBlock + threadIdx.x);
+    const uint32_t m0 = 2u * (sweep_block(band) * kBlock + threadIdx.x);
     bool a0 = false, a1 = false, f0 = false, f1 = false;
     if (m0 < G.nsub) {
         SiteIn S0, S1;
@@ -358,14 +358,16 @@ static uint32_t pair_band(const Geom &G, int order)
 }
 static uint32_t site_band(const Geom &G, int order) { return (order & 1) ? xcd_band(G) : 0u; }
 
+// order bit 0: XCD row bands, bit 1: site pairs, bit 2: reverse block order.
 template <int ARITH, int RNG, int BC>
 static void launch_phase_t(int kind, const Geom &G, const Fields &F, int colour, const PhaseArgs &A,
                            uint32_t *counts, int order, hipStream_t s)
 {
     const dim3 b(kBlock);
+    const uint32_t rev = (order & 4) ? 0x80000000u : 0u;
     if ((order & 2) && pairs_ok(G)) {
         const dim3 g = grid_pairs(G);
-        const uint32_t band = pair_band(G, order);
+        const uint32_t band = pair_band(G, order) | rev;
         if (kind == 0)
             hipLaunchKernelGGL((k_pair<0, ARITH, RNG, BC>), g, b, 0, s, G, F, colour, A, counts, counts, band);
         else
@@ -373,7 +375,7 @@ static void launch_phase_t(int kind, const Geom &G, const Fields &F, int colour,
         return;
     }
     const dim3 g = grid1(G.nsub);
-    const uint32_t band = site_band(G, order);
+    const uint32_t band = site_band(G, order) | rev;
     if (field_off32(G)) {
         if (kind == 0) hipLaunchKernelGGL((k_dir<ARITH, RNG, BC, true>), g, b, 0, s, G, F, colour, A, counts, band);
         else hipLaunchKernelGGL((k_spin<ARITH, RNG, BC, true>), g, b, 0, s, G, F, colour, A, counts, band);
@@ -417,7 +419,9 @@ void launch_phase_fused_next(int bc, const Geom &G, const Fields &F, const Phase
     const dim3 b(kBlock);
     if ((order & 2) && pairs_ok(G)) {
         const dim3 g = grid_pairs(G);
-        const uint32_t band = pair_band(G, order);
+        // order bit 2: this launch walks its blocks in reverse, so it starts on
+        // the planes the colour-1 launch before it finished with
+        const uint32_t band = pair_band(G, order) | ((order & 4) ? 0x80000000u : 0u);
         if (bc == BC_OPEN)
             hipLaunchKernelGGL((k_pair<3, ARITH_FP64, RNG_PHILOX, BC_OPEN>), g, b, 0, s, G, F, 0, A, counts_dir,
                                counts_spin, band);
@@ -426,7 +430,7 @@ void launch_phase_fused_next(int bc, const Geom &G, const Fields &F, const Phase
                                counts_spin, band);
         return;
     }
-    const uint32_t band = site_band(G, order);
+    const uint32_t band = site_band(G, order) | ((order & 4) ? 0x80000000u : 0u);
     const dim3 g = grid1(G.nsub);
     const bool o32 = field_off32(G);
     if (bc == BC_OPEN) {

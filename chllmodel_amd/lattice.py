@@ -71,8 +71,10 @@ class Lattice:
     def set_kernel(self, variant: str = "march", tile: int = 0, nslab: int = 1, *, kinds=KINDS,
                    ariths=("compat", "fp64")) -> None:
         """Half-sweep kernel for the given trial kinds and arithmetic modes.
-        'site': one site per thread, tile 0 = blocks in lattice order, 1 = one
-        band of rows per XCD.  'march': tile 0/1/2 = 64x4/32x8/16x16 packed
+        'site': one site per thread, tile bit 0 = one band of rows per XCD
+        (else blocks in lattice order), bit 1 = two sites per thread with pair
+        loads, bit 2 = the fused colour-0 FP64 launch walks its blocks in
+        reverse.  'march': tile 0/1/2 = 64x4/32x8/16x16 packed
         sites x rows marching along k in nslab slabs, 3/4/5 = the same tiles one
         plane per block."""
         for kind in ([kinds] if isinstance(kinds, str) else kinds):

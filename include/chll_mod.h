@@ -133,7 +133,9 @@ int chll_lattice_sync(chll_lattice *lat);
  * takes one band of rows of every plane (else blocks in lattice order, also
  * when a plane does not split into 8 bands of whole blocks), tile bit 1 = two
  * neighbouring sites per thread read with 16-byte pair loads (when ni % 4 == 0
- * and a field spans less than 4 GiB; else one site per thread);
+ * and a field spans less than 4 GiB; else one site per thread), tile bit 2
+ * (director kind, FP64 fused sweeps with site pairs) = the fused colour-0
+ * launch walks its blocks in reverse order;
  * variant 1 = tiled through LDS: tile
  * 0 = 64x4, 1 = 32x8, 2 = 16x16 packed sites x rows marching along k in nslab
  * slabs, 3..5 = the same tile shapes one plane per block (nslab unused).

@@ -234,7 +234,8 @@ SHAPES = [((256 // 8, 24, 20), "open"), ((48, 40, 30), "periodic"), ((10, 7, 5),
 @pytest.mark.parametrize("arith", ["compat", "fp64"])
 def test_kernel_shapes_identical(dims, bc, arith):
     """The one-site-per-thread kernels in both block orders, the two-sites-
-    per-thread pair kernels (ni % 4 == 0; other shapes fall back to one site),
+    per-thread pair kernels (ni % 4 == 0; other shapes fall back to one site;
+    tiles 6 and 7 walk the fused colour-0 launch's blocks in reverse),
     each with and without the fused launches (FP64), and every k-march tile /
     slab split give the same lattice, counts and rows (partial tiles, odd open
     dimensions; 64x64 and 128x64 planes split into XCD bands of 1 and 2
@@ -243,7 +244,8 @@ def test_kernel_shapes_identical(dims, bc, arith):
     ref = None
     for variant, tile, nslab, fuse in [("site", 0, 1, False), ("site", 0, 1, True), ("site", 1, 1, True),
                                        ("site", 1, 1, False), ("site", 2, 1, True), ("site", 3, 1, True),
-                                       ("site", 3, 1, False), ("march", 0, 1, True), ("march", 0, 3, True),
+                                       ("site", 3, 1, False), ("site", 7, 1, True), ("site", 6, 1, True),
+                                       ("site", 7, 1, False), ("march", 0, 1, True), ("march", 0, 3, True),
                                        ("march", 1, 2, True), ("march", 2, 1, True), ("march", 2, 64, True),
                                        ("march", 3, 1, True), ("march", 4, 1, True), ("march", 5, 1, True)]:
         print(f"kernel shapes dims={dims} bc={bc} {arith} {variant} tile={tile} nslab={nslab} fuse={fuse}",
