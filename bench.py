@@ -481,6 +481,7 @@ def batched_scan(args, stream):
     ens = Ensemble(reps, bc="open", arith=args.arith)
     ens.set_stream(stream.cuda_stream)
     ens.set_block(tune["best"])
+    ens.set_chunk(tune["chunk"])
     ens.init_isotropic(20261203)
     ens.run(2, Kb, seed=20261203, want_rows=False, use_graph=True)
     ens.sync()
@@ -491,6 +492,7 @@ def batched_scan(args, stream):
     torch.cuda.synchronize()
     t_b = time.perf_counter() - t0
     stt = ens.state()
+    nchunks = ens.chunks()
     ens.close()
     nsw = nb * Kb
     log(f"bench: batched {nsw} sweeps in {t_b:.4f} s")
@@ -517,7 +519,8 @@ def batched_scan(args, stream):
             "site_updates_per_s": sites * nsw / t_b, "ms_per_sweep": 1e3 * t_b / nsw,
             "sequential_site_updates_per_s": sites * nsw / t_seq, "sequential_ms_per_sweep": 1e3 * t_seq / nsw,
             "speedup_vs_sequential": t_seq / t_b, "block_size": tune["best"],
-            "autotune_ms_per_sweep": tune["ms_per_sweep"],
+            "chunk_bytes": tune["chunk"], "chunks": nchunks,
+            "autotune_ms_per_sweep": tune["ms_per_sweep"], "autotune_ms_per_sweep_chunk": tune["ms_per_sweep_chunk"],
             "swap_acceptance": float(acc.sum() / max(att.sum(), 1))}
 
 

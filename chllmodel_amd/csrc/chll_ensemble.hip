@@ -48,6 +48,8 @@ struct RepDesc {
 struct EnsDev {
     const RepDesc *R;
     const uint32_t *blk_rep;  // per launch block: replica << 24 | local block
+    const uint32_t *blk_rep2; // the same for two sites per thread (pair kernels), or null
+    const uint32_t *order;    // replicas in block-table order (largest first)
     double *kbt, *inv_kbt;    // per temperature slot
     double *d;                // per temperature slot (adaptive cone step)
     int32_t *rep_slot;        // replica -> slot
@@ -70,13 +72,15 @@ struct EnsDev {
 // trial, 3: colour-0 chirality trials of sweep t each followed by the site's
 // director trial of sweep t + 1 (the single-lattice fused launches).  One
 // kernel per kind, so each gets the register budget of its own trial.
-// rev: walk the block table last to first, so that a launch starts on the
-// replicas the launch before it finished with (still in the memory-side cache).
+// A launch covers the blocks base .. base + gridDim.x - 1 of the table (one
+// chunk of replicas).  rev: walk them last to first, so that a launch starts on
+// the replicas the launch before it finished with (still in the memory-side
+// cache).
 template <int KIND, int ARITH, int BC, int BS>
-__global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int colour, uint32_t istep_off, int rev)
+__global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int colour, uint32_t istep_off, int rev, uint32_t base)
 {
     constexpr int kind = KIND;
-    const uint32_t code = E.blk_rep[rev ? gridDim.x - 1u - blockIdx.x : blockIdx.x];
+    const uint32_t code = E.blk_rep[base + (rev ? gridDim.x - 1u - blockIdx.x : blockIdx.x)];
     const uint32_t r = code >> 24, lb = code & 0xFFFFFFu;
     const RepDesc &D = E.R[r];
     const int s = E.rep_slot[r];
@@ -109,16 +113,55 @@ __global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int colour, uint32_t
     block_count<BS>(acc, D.counts + (kind == 0 ? colour : flip_group(A.istep, colour)) * gs, lb);
 }
 
-template <int BC, int BS>
-__global__ __launch_bounds__(BS) void k_measure_ens(EnsDev E)
+// FP64 fused kinds 2 and 3 (as k_phase_ens) with two sites per thread and
+// 16-byte pair loads (the single lattice's k_pair), blocks from blk_rep2.
+// Needs every replica to satisfy pairs_ok.
+template <int KIND, int BC, int BS>
+__global__ __launch_bounds__(BS) void k_pair_ens(EnsDev E, uint32_t istep_off, int rev, uint32_t base)
 {
-    const uint32_t code = E.blk_rep[blockIdx.x];
+    const uint32_t code = E.blk_rep2[base + (rev ? gridDim.x - 1u - blockIdx.x : blockIdx.x)];
+    const uint32_t r = code >> 24, lb = code & 0xFFFFFFu;
+    const RepDesc &D = E.R[r];
+    const int s = E.rep_slot[r];
+    PhaseArgs A;
+    A.g = D.g; A.KK = D.KK; A.kbt = E.kbt[s]; A.inv_kbt = E.inv_kbt[s]; A.d = E.d + s;
+    A.u_phi = nullptr; A.u_acc = nullptr; A.u_spin = nullptr;
+    A.seed = E.seed; A.run_index = E.run_index; A.replica = D.replica; A.istep = *E.istep + istep_off;
+    A.istep_spin = A.istep;
+    const uint32_t m0 = 2u * (lb * BS + threadIdx.x);
+    const size_t gs = (size_t)kSlots * kSlotStride;
+    bool a0 = false, a1 = false, f0 = false, f1 = false;
+    if constexpr (KIND == 3) A.istep = A.istep_spin + 1;
+    if (m0 < D.G.nsub) {
+        SiteIn S0, S1;
+        const uint32_t own = load_pair<BC, true>(D.G, D.F, KIND == 2 ? 1 : 0, m0, S0, S1);
+        if constexpr (KIND == 2) {
+            a0 = dir_spin_apply<RNG_PHILOX, true>(D.F, 1, A, m0, own, S0, f0);
+            a1 = dir_spin_apply<RNG_PHILOX, true>(D.F, 1, A, m0 + 1u, own + 1u, S1, f1);
+        } else {
+            a0 = spin_dir_apply<true>(D.F, A, m0, own, S0, f0);
+            a1 = spin_dir_apply<true>(D.F, A, m0 + 1u, own + 1u, S1, f1);
+        }
+    }
+    const uint32_t wa = (uint32_t)(__popcll(__ballot(a0)) + __popcll(__ballot(a1)));
+    const uint32_t wf = (uint32_t)(__popcll(__ballot(f0)) + __popcll(__ballot(f1)));
+    if constexpr (KIND == 2)
+        block_sum2<BS>(wa, wf, D.counts + kGroupAcc1 * gs, D.counts + flip_group(A.istep, 1) * gs, lb);
+    else
+        block_sum2<BS>(wa, wf, D.counts + kGroupAcc0 * gs, D.counts + flip_group(A.istep_spin, 0) * gs, lb);
+}
+
+template <int BC, int BS>
+__global__ __launch_bounds__(BS) void k_measure_ens(EnsDev E, uint32_t base)
+{
+    const uint32_t gb = base + blockIdx.x;
+    const uint32_t code = E.blk_rep[gb];
     const uint32_t r = code >> 24, lb = code & 0xFFFFFFu;
     const RepDesc &D = E.R[r];
     MeasureAcc acc;
     const uint32_t m = lb * BS + threadIdx.x;
     if (m < D.G.nsub) measure_site<BC>(D.G, D.F, D.g, D.KK, m, acc);
-    block_reduce_measure<BS>(acc, E.part + (size_t)blockIdx.x * 8, E.spart + blockIdx.x);
+    block_reduce_measure<BS>(acc, E.part + (size_t)gb * 8, E.spart + gb);
 }
 
 // Per-replica epilogue (one block per replica): accept/flip totals, step
@@ -130,9 +173,9 @@ constexpr int kEnsFin = 256;
 // close_prev zeroes the flips of sweep istep - 1, close_this counts those of
 // sweep istep (for the row) and zeroes them.
 __global__ __launch_bounds__(kEnsFin) void k_finalize_ens(EnsDev E, uint32_t istep_off, int measure, int close_prev,
-                                                          int close_this)
+                                                          int close_this, uint32_t fbase)
 {
-    const int r = blockIdx.x;
+    const int r = (int)E.order[fbase + blockIdx.x];
     const RepDesc &D = E.R[r];
     const uint32_t t = threadIdx.x;
     const uint32_t istep = *E.istep + istep_off;
@@ -285,6 +328,13 @@ int efail(const char *fmt, ...)
 
 }  // namespace
 
+// One chunk of replicas (consecutive in the block-table order): its blocks in
+// the one-site table [t1, t1 + n1) and in the pair table [t2, t2 + n2), its
+// replicas order[f .. f + nf).
+struct Chunk {
+    uint32_t t1, n1, t2, n2, f, nf;
+};
+
 struct chll_ensemble {
     int nrep = 0, nlad = 0, bc = BC_OPEN, arith = ARITH_COMPAT, bs = 256;
     std::vector<chll_replica_desc> rd;
@@ -295,6 +345,15 @@ struct chll_ensemble {
     RepDesc *R = nullptr;
     uint32_t *blk_rep = nullptr;
     uint32_t nblocks = 0;
+    uint32_t *blk_rep2 = nullptr;  // pair-kernel blocks (when every replica allows site pairs)
+    uint32_t nblocks2 = 0;
+    bool pairs = false;
+    uint32_t *order_dev = nullptr;
+    // Replicas are independent between swaps, so a block of K sweeps runs
+    // chunk by chunk: all K sweeps of one chunk of at most chunk_bytes of state
+    // (0: one chunk), which then stays in the memory-side cache.
+    size_t chunk_bytes = 0;
+    std::vector<Chunk> chunks;
     EnsDev E{};
     double *stage = nullptr;
     int32_t *stage_s = nullptr;
@@ -304,7 +363,7 @@ struct chll_ensemble {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_K = 0, graph_bs = 0, graph_adapt = -1, graph_fuse = -1;
-    bool fuse = true;  // FP64: the single lattice's fused launches
+    int fuse = 1;  // FP64: 0 four half-sweeps, 1 fused launches (site pairs where allowed), 2 fused, one site
     uint64_t graph_seed = 0;
     uint32_t graph_run = 0;
     double *pinned = nullptr;
@@ -330,6 +389,34 @@ void drop_graph(chll_ensemble *X)
     X->graph_K = 0;
 }
 
+// Chunks of consecutive replicas in block-table order, each holding at most
+// X->chunk_bytes of lattice state (a replica larger than that is a chunk alone).
+void build_chunks(chll_ensemble *X)
+{
+    X->chunks.clear();
+    Chunk c{0, 0, 0, 0, 0, 0};
+    size_t bytes = 0;
+    uint32_t p1 = 0, p2 = 0;
+    for (uint32_t i = 0; i < (uint32_t)X->order.size(); ++i) {
+        const RepDesc &D = X->rep[X->order[i]];
+        const size_t rb = (2 * (size_t)D.G.nsub + kFieldPad) * (3 * sizeof(double) + 1);
+        const uint32_t nb1 = D.nblk;
+        const uint32_t nb2 = X->pairs ? (D.G.nsub / 2 + X->bs - 1) / X->bs : 0u;
+        if (c.nf > 0 && X->chunk_bytes > 0 && bytes + rb > X->chunk_bytes) {
+            X->chunks.push_back(c);
+            c = Chunk{p1, 0, p2, 0, i, 0};
+            bytes = 0;
+        }
+        c.n1 += nb1;
+        c.n2 += nb2;
+        c.nf += 1;
+        bytes += rb;
+        p1 += nb1;
+        p2 += nb2;
+    }
+    if (c.nf) X->chunks.push_back(c);
+}
+
 // (Re)build the block table and descriptors for block size X->bs.
 int build_tables(chll_ensemble *X)
 {
@@ -344,19 +431,54 @@ int build_tables(chll_ensemble *X)
         start += D.nblk;
     }
     X->nblocks = start;
+    // Pair-kernel blocks.  A replica whose pair-plane splits into 8 bands of
+    // whole blocks, starting at a table position that is a multiple of 8, gets
+    // XCD row bands: table position p runs on XCD p % 8 (reversed launches
+    // permute the XCDs the same way for every block), and the band_block order
+    // gives that XCD one band of rows of every plane, so the y neighbours of a
+    // row stay in one L2.  Only the integer accept / flip counts depend on
+    // these blocks.
+    std::vector<uint32_t> table2;
+    if (X->pairs) {
+        for (uint32_t r : X->order) {
+            const Geom &G = X->rep[r].G;
+            const uint32_t nb = (G.nsub / 2 + X->bs - 1) / X->bs;
+            const uint32_t per = 8u * (uint32_t)X->bs;
+            const uint32_t band =
+                (G.plane / 2) % per == 0 && table2.size() % 8 == 0 ? (G.plane / 2) / per : 0u;
+            for (uint32_t b = 0; b < nb; ++b) {
+                uint32_t lb = b;
+                if (band) {
+                    const uint32_t x = b & 7u, n = b >> 3, q = n / band, rr = n - q * band;
+                    lb = (q * 8u + x) * band + rr;
+                }
+                table2.push_back((r << 24) | lb);
+            }
+        }
+    }
+    X->nblocks2 = (uint32_t)table2.size();
     if (X->blk_rep) (void)hipFree(X->blk_rep);
+    if (X->blk_rep2) (void)hipFree(X->blk_rep2);
     X->blk_rep = nullptr;
+    X->blk_rep2 = nullptr;
     if (X->E.part) (void)hipFree(X->E.part);
     if (X->E.spart) (void)hipFree(X->E.spart);
     X->E.part = nullptr;
     X->E.spart = nullptr;
     if (ealloc(&X->blk_rep, table.size()) || ealloc(&X->E.part, (size_t)X->nblocks * 8) ||
-        ealloc(&X->E.spart, X->nblocks))
+        ealloc(&X->E.spart, X->nblocks) || (X->pairs && ealloc(&X->blk_rep2, table2.size())))
         return -1;
     ECHECK(hipMemcpy(X->blk_rep, table.data(), table.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (X->pairs)
+        ECHECK(hipMemcpy(X->blk_rep2, table2.data(), table2.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    X->E.blk_rep2 = X->blk_rep2;
     ECHECK(hipMemcpy(X->R, X->rep.data(), X->rep.size() * sizeof(RepDesc), hipMemcpyHostToDevice));
+    if (!X->order_dev && ealloc(&X->order_dev, X->order.size())) return -1;
+    ECHECK(hipMemcpy(X->order_dev, X->order.data(), X->order.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     X->E.R = X->R;
     X->E.blk_rep = X->blk_rep;
+    X->E.order = X->order_dev;
+    build_chunks(X);
     drop_graph(X);
     return 0;
 }
@@ -371,30 +493,36 @@ template <int ARITH, int BC, int BS>
 void launch_block_t(chll_ensemble *X, int K, hipStream_t st)
 {
     const EnsDev &E = X->E;
-    const dim3 g(X->nblocks), b(BS), gf(X->nrep), bf(kEnsFin);
-    const bool fuse = ARITH == ARITH_FP64 && X->fuse;
-    for (int t = 0; t < K; ++t) {
-        const uint32_t off = (uint32_t)t;
-        const bool last = t + 1 == K;
-        // consecutive sweep launches alternate their block order
-        if (!fuse) {
-            hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 0);
-            hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 1);
-            hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 0);
-            hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 1);
-            if (last) hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E);
-            hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, last ? 1 : 0, 0, 1);
-            continue;
-        }
-        if (t == 0) hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1);
-        hipLaunchKernelGGL((k_phase_ens<2, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 0);
-        if (!last) {
-            hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 0, t > 0 ? 1 : 0, 0);
-            hipLaunchKernelGGL((k_phase_ens<3, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1);
-        } else {
-            hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1);
-            hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E);
-            hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 1, t > 0 ? 1 : 0, 1);
+    const dim3 b(BS), bf(kEnsFin);
+    const bool fuse = ARITH == ARITH_FP64 && X->fuse != 0;
+    const bool pairs = fuse && X->fuse == 1 && X->pairs;
+    for (const Chunk &c : X->chunks) {
+        const dim3 g(c.n1), g2(c.n2), gf(c.nf);
+        for (int t = 0; t < K; ++t) {
+            const uint32_t off = (uint32_t)t;
+            const bool last = t + 1 == K;
+            // consecutive sweep launches alternate their block order
+            if (!fuse) {
+                hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 0, c.t1);
+                hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 1, c.t1);
+                hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 0, c.t1);
+                hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 1, c.t1);
+                if (last) hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E, c.t1);
+                hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, last ? 1 : 0, 0, 1, c.f);
+                continue;
+            }
+            if (t == 0) hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, c.t1);
+            if (pairs) hipLaunchKernelGGL((k_pair_ens<2, BC, BS>), g2, b, 0, st, E, off, 0, c.t2);
+            else hipLaunchKernelGGL((k_phase_ens<2, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 0, c.t1);
+            if (!last) {
+                hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 0, t > 0 ? 1 : 0, 0, c.f);
+                if (pairs) hipLaunchKernelGGL((k_pair_ens<3, BC, BS>), g2, b, 0, st, E, off, 1, c.t2);
+                else hipLaunchKernelGGL((k_phase_ens<3, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, c.t1);
+            } else {
+                hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, c.t1);
+                hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E, c.t1);
+                hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 1, t > 0 ? 1 : 0, 1, c.f);
+            }
         }
     }
     hipLaunchKernelGGL(k_swap_ens, dim3(1), dim3(256), 0, st, E, 1);
@@ -497,6 +625,9 @@ int chll_ensemble_create(const chll_replica_desc *reps, int32_t nrep, int32_t bc
         D.g = reps[r].g;
         D.KK = reps[r].KK;
     }
+    // two sites per thread in the fused FP64 launches when every replica allows it
+    X->pairs = arith == ARITH_FP64;
+    for (int r = 0; r < nrep; ++r) X->pairs = X->pairs && pairs_ok(X->rep[r].G);
     // block order: largest replicas first
     X->order.resize(nrep);
     for (int r = 0; r < nrep; ++r) X->order[r] = r;
@@ -558,7 +689,7 @@ int chll_ensemble_destroy(chll_ensemble *X)
         (void)hipFree(D.counts);
     }
     EnsDev &E = X->E;
-    void *ptrs[] = {X->R, X->blk_rep, E.kbt, E.inv_kbt, E.d, E.rep_slot, E.slot_rep, (void *)E.lad_start, E.E,
+    void *ptrs[] = {X->R, X->blk_rep, X->blk_rep2, X->order_dev, E.kbt, E.inv_kbt, E.d, E.rep_slot, E.slot_rep, (void *)E.lad_start, E.E,
                     E.istep, E.row, E.swap_att, E.swap_acc, E.part, E.spart, E.rows, X->stage, X->stage_s, X->bad};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -579,9 +710,23 @@ int chll_ensemble_set_stream(chll_ensemble *X, void *s)
 int chll_ensemble_set_fusion(chll_ensemble *X, int32_t on)
 {
     if (!X) return efail("null ensemble");
-    X->fuse = on != 0;
+    if (on < 0 || on > 2) return efail("fusion must be 0 (off), 1 (on) or 2 (on, one site per thread)");
+    X->fuse = on;
     return 0;
 }
+
+int chll_ensemble_set_chunk(chll_ensemble *X, int64_t bytes)
+{
+    if (!X) return efail("null ensemble");
+    if (bytes < 0) return efail("chunk size must be >= 0 bytes (0: one chunk)");
+    ECHECK(hipStreamSynchronize(X->stream));
+    X->chunk_bytes = (size_t)bytes;
+    build_chunks(X);
+    drop_graph(X);
+    return 0;
+}
+
+int32_t chll_ensemble_chunks(chll_ensemble *X) { return X ? (int32_t)X->chunks.size() : -1; }
 
 int chll_ensemble_set_block(chll_ensemble *X, int32_t bs)
 {
@@ -669,7 +814,7 @@ int chll_ensemble_run(chll_ensemble *X, uint64_t seed, uint32_t run_index, int32
     }
     // a captured graph bakes in the kernel arguments: rebuild it when they change
     if (X->graph && (X->graph_K != K || X->graph_bs != X->bs || X->graph_adapt != adapt || X->graph_seed != seed ||
-                     X->graph_run != run_index || E.max_rows != max_rows || X->graph_fuse != (int)X->fuse))
+                     X->graph_run != run_index || E.max_rows != max_rows || X->graph_fuse != X->fuse))
         drop_graph(X);
     E.seed = seed;
     E.run_index = run_index;
@@ -696,7 +841,7 @@ int chll_ensemble_run(chll_ensemble *X, uint64_t seed, uint32_t run_index, int32
         X->graph_adapt = adapt;
         X->graph_seed = seed;
         X->graph_run = run_index;
-        X->graph_fuse = (int)X->fuse;
+        X->graph_fuse = X->fuse;
     }
     for (int b = 0; b < nblocks; ++b) {
         if (use_graph) ECHECK(hipGraphLaunch(X->gexec, st));

@@ -45,6 +45,9 @@ def _declare(L):
     L.chll_ensemble_set_stream.argtypes = [ct.c_void_p, ct.c_void_p]
     L.chll_ensemble_set_block.argtypes = [ct.c_void_p, ct.c_int32]
     L.chll_ensemble_set_fusion.argtypes = [ct.c_void_p, ct.c_int32]
+    L.chll_ensemble_set_chunk.argtypes = [ct.c_void_p, ct.c_int64]
+    L.chll_ensemble_chunks.argtypes = [ct.c_void_p]
+    L.chll_ensemble_chunks.restype = ct.c_int32
     L.chll_ensemble_blocks.argtypes = [ct.c_void_p]
     L.chll_ensemble_upload.argtypes = [ct.c_void_p, ct.c_int32, _f64p, _f64p, _f64p, _i32p]
     L.chll_ensemble_download.argtypes = [ct.c_void_p, ct.c_int32, _f64p, _f64p, _f64p, _i32p]
@@ -53,7 +56,7 @@ def _declare(L):
     L.chll_ensemble_sync.argtypes = [ct.c_void_p]
     L.chll_ensemble_state.argtypes = [ct.c_void_p, _i32p, _f64p, _f64p, _i64p, _i64p, ct.POINTER(ct.c_uint32)]
     for n in ("chll_ensemble_create", "chll_ensemble_destroy", "chll_ensemble_set_stream", "chll_ensemble_set_block",
-              "chll_ensemble_set_fusion", "chll_ensemble_blocks", "chll_ensemble_upload", "chll_ensemble_download", "chll_ensemble_run",
+              "chll_ensemble_set_fusion", "chll_ensemble_set_chunk", "chll_ensemble_blocks", "chll_ensemble_upload", "chll_ensemble_download", "chll_ensemble_run",
               "chll_ensemble_sync", "chll_ensemble_state"):
         getattr(L, n).restype = ct.c_int
     L._ens_declared = True
@@ -111,10 +114,20 @@ class Ensemble:
         _check(lib().chll_ensemble_set_block(self._h, int(bs)))
         self.block_size = int(bs)
 
-    def set_fusion(self, on: bool):
+    def set_fusion(self, on: bool, pairs: bool = True):
         """FP64: fused director + chirality launches within each block of
-        sweeps (same results, fewer passes over the replicas)."""
-        _check(lib().chll_ensemble_set_fusion(self._h, 1 if on else 0))
+        sweeps (same results, fewer passes over the replicas); with `pairs`,
+        two sites per thread in those launches when every replica's ni is a
+        multiple of 4."""
+        _check(lib().chll_ensemble_set_fusion(self._h, (1 if pairs else 2) if on else 0))
+
+    def set_chunk(self, nbytes: int):
+        """Run each block of sweeps chunk by chunk, at most `nbytes` of lattice
+        state per chunk (0: all replicas at once); results are unchanged."""
+        _check(lib().chll_ensemble_set_chunk(self._h, int(nbytes)))
+
+    def chunks(self) -> int:
+        return int(lib().chll_ensemble_chunks(self._h))
 
     def set_stream(self, ptr):
         _check(lib().chll_ensemble_set_stream(self._h, ct.c_void_p(ptr or 0)))
@@ -209,10 +222,12 @@ def _signature(replicas, bc, arith, K):
 
 
 def autotune(replicas, *, bc="open", arith="compat", K=10, blocks=3, candidates=(64, 128, 256, 512),
-             seed=12345, use_cache=True, verbose=False) -> dict:
-    """Time each candidate block size on a scratch ensemble of the same shape
-    (so the caller's ensemble is not advanced) and return
-    {'best': bs, 'ms_per_sweep': {bs: ms}}; cached per shape in CACHE."""
+             chunks=(0, 64 << 20, 128 << 20, 192 << 20), seed=12345, use_cache=True, verbose=False) -> dict:
+    """Time each candidate block size, then each chunk budget (bytes of state
+    per chunk, 0 = all replicas at once) with the best block size, on a scratch
+    ensemble of the same shape (so the caller's ensemble is not advanced).
+    Returns {'best': bs, 'chunk': bytes, 'ms_per_sweep': {...}}; cached per
+    shape in CACHE.  Apply with set_block(res['best']), set_chunk(res['chunk'])."""
     key = _signature(replicas, bc, arith, K)
     cache = {}
     if use_cache and os.path.exists(CACHE):
@@ -221,23 +236,36 @@ def autotune(replicas, *, bc="open", arith="compat", K=10, blocks=3, candidates=
                 cache = json.load(f)
         except (OSError, ValueError):
             cache = {}
-        if key in cache:
+        if key in cache and "chunk" in cache[key]:
             return cache[key]
-    times = {}
+    times, ctimes = {}, {}
     with Ensemble(replicas, bc=bc, arith=arith) as ens:
         ens.init_isotropic(seed)
-        for bs in candidates:
-            ens.set_block(bs)
+
+        def time_blocks():
             ens.run(1, K, seed=seed, want_rows=False, use_graph=True)  # capture + warm up
             ens.sync()
             t0 = time.perf_counter()
             ens.run(blocks, K, seed=seed, want_rows=False, use_graph=True)
             ens.sync()
-            times[bs] = 1e3 * (time.perf_counter() - t0) / (blocks * K)
+            return 1e3 * (time.perf_counter() - t0) / (blocks * K)
+
+        for bs in candidates:
+            ens.set_block(bs)
+            times[bs] = time_blocks()
             if verbose:
                 print(f"autotune bs={bs}: {times[bs]:.3f} ms/sweep", flush=True)
-    best = min(times, key=times.get)
-    res = {"best": int(best), "ms_per_sweep": {str(k): v for k, v in times.items()}}
+        best = min(times, key=times.get)
+        ens.set_block(best)
+        for cb in chunks:
+            ens.set_chunk(cb)
+            ctimes[cb] = time_blocks()
+            if verbose:
+                print(f"autotune bs={best} chunk={cb >> 20} MiB ({ens.chunks()} chunks): {ctimes[cb]:.3f} ms/sweep",
+                      flush=True)
+    chunk = min(ctimes, key=ctimes.get)
+    res = {"best": int(best), "chunk": int(chunk), "ms_per_sweep": {str(k): v for k, v in times.items()},
+           "ms_per_sweep_chunk": {str(k): v for k, v in ctimes.items()}}
     if use_cache:
         cache[key] = res
         try:

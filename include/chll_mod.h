@@ -185,9 +185,18 @@ int chll_ensemble_destroy(chll_ensemble *e);
 int chll_ensemble_set_stream(chll_ensemble *e, void *hip_stream);
 /* threads per block of the batched kernels: 64, 128, 256 or 512 */
 int chll_ensemble_set_block(chll_ensemble *e, int32_t block_size);
-/* FP64 ensembles (on by default): the fused launches of chll_lattice_set_fusion
- * inside each block of K sweeps; results are unchanged. */
+/* FP64 ensembles: on = 1 (default) runs the fused launches of
+ * chll_lattice_set_fusion inside each block of K sweeps, two sites per thread
+ * when every replica has ni % 4 == 0 (the single lattice's site pairs); 2 the
+ * same with one site per thread; 0 four half-sweeps.  Results are unchanged. */
 int chll_ensemble_set_fusion(chll_ensemble *e, int32_t on);
+/* Replicas are independent between swaps, so a block of K sweeps can run chunk
+ * by chunk: all K sweeps of consecutive replicas (largest first) holding at
+ * most `bytes` of lattice state (25 B per site), then the next chunk, so that
+ * a chunk stays in the memory-side cache for its K sweeps.  0 (default): one
+ * chunk.  Results are unchanged.  chll_ensemble_chunks: the number of chunks. */
+int chll_ensemble_set_chunk(chll_ensemble *e, int64_t bytes);
+int32_t chll_ensemble_chunks(chll_ensemble *e);
 int32_t chll_ensemble_blocks(chll_ensemble *e);
 int chll_ensemble_upload(chll_ensemble *e, int32_t rep, const double *nx, const double *ny, const double *nz,
                          const int32_t *s);

@@ -131,6 +131,72 @@ def test_fused_blocks_equal_unfused(K):
         assert np.array_equal(ta[k], tb[k]), k
 
 
+@pytest.mark.parametrize("K", [1, 3])
+@pytest.mark.parametrize("layout", ["small", "banded"])
+def test_pair_kernels_equal_one_site(K, layout):
+    """Replicas whose ni are all multiples of 4 run the fused FP64 launches with
+    two sites per thread (reverse block order on the colour-0 launch; the
+    32x64 planes at 64 threads per block also split into XCD row bands): same
+    replicas, rows, slots and swap counts as one site per thread and as four
+    separate half-sweeps."""
+    ladders = [((8, 6, 4), 0.5, 1.0, (0.3, 0.5, 0.9), 0.3), ((12, 5, 7), 1.0, 0.7, (0.4, 0.7), 0.2),
+               ((4, 4, 6), 0.0, 1.5, (0.8,), 0.25)]
+    if layout == "banded":
+        ladders.insert(0, ((32, 64, 6), 0.5, 1.0, (0.5, 0.8), 0.3))
+    reps = [{"ni": d[0], "nj": d[1], "nk": d[2], "ladder": lad, "g": g, "KK": KK, "kbt": kT, "d0": d0}
+            for lad, (d, g, KK, kTs, d0) in enumerate(ladders) for kT in kTs]
+    outs = []
+    for fuse, pairs in ((True, True), (True, False), (False, True)):
+        print(f"ensemble pairs {layout} K={K} fuse={fuse} pairs={pairs}", flush=True)
+        with Ensemble(reps, arith="fp64") as ens:
+            if layout == "banded":
+                ens.set_block(64)
+            ens.set_fusion(fuse, pairs=pairs)
+            ens.init_isotropic(5)
+            rows = ens.run(4, K, seed=17, use_graph=True)
+            st = ens.state()
+            outs.append((rows, [ens.download(r) for r in range(len(reps))], st))
+    ra, sa, ta = outs[0]
+    for rb, sb, tb in outs[1:]:
+        assert np.array_equal(ra, rb)
+        for a, b in zip(sa, sb):
+            for f in ("nx", "ny", "nz", "s"):
+                assert np.array_equal(a[f], b[f])
+        for k in ("rep_slot", "slot_d", "swap_att", "swap_acc"):
+            assert np.array_equal(ta[k], tb[k]), k
+
+
+@pytest.mark.parametrize("arith,fuse", [("fp64", True), ("fp64", False), ("compat", False)])
+def test_chunks_equal_one_launch(arith, fuse):
+    """A block of sweeps run chunk by chunk (a few replicas at a time, all K
+    sweeps each) gives the same replicas, rows, slots and swap counts as every
+    replica in one launch per half-sweep."""
+    ladders = [((16, 12, 10), 0.5, 1.0, (0.3, 0.6, 1.0), 0.3), ((8, 8, 8), 1.0, 0.7, (0.4, 0.7), 0.2),
+               ((12, 4, 6), 0.0, 1.5, (0.5, 0.8, 1.2), 0.25)]
+    reps = [{"ni": d[0], "nj": d[1], "nk": d[2], "ladder": lad, "g": g, "KK": KK, "kbt": kT, "d0": d0}
+            for lad, (d, g, KK, kTs, d0) in enumerate(ladders) for kT in kTs]
+    outs = []
+    for chunk in (0, 60000, 1):
+        print(f"ensemble chunks {arith} fuse={fuse} chunk={chunk}", flush=True)
+        with Ensemble(reps, arith=arith) as ens:
+            ens.set_fusion(fuse)
+            ens.set_chunk(chunk)
+            n = ens.chunks()
+            assert n == {0: 1, 1: len(reps)}.get(chunk, n) and (chunk != 60000 or 1 < n < len(reps))
+            ens.init_isotropic(9)
+            rows = ens.run(4, 3, seed=23, use_graph=True)
+            st = ens.state()
+            outs.append((rows, [ens.download(r) for r in range(len(reps))], st))
+    ra, sa, ta = outs[0]
+    for rb, sb, tb in outs[1:]:
+        assert np.array_equal(ra, rb)
+        for a, b in zip(sa, sb):
+            for f in ("nx", "ny", "nz", "s"):
+                assert np.array_equal(a[f], b[f])
+        for k in ("rep_slot", "slot_d", "swap_att", "swap_acc"):
+            assert np.array_equal(ta[k], tb[k]), k
+
+
 def test_single_replica_equals_lattice_path():
     r = {"ni": 16, "nj": 12, "nk": 10, "ladder": 0, "g": 0.4, "KK": 1.2, "kbt": 0.7, "d0": 0.3}
     st = isotropic_state(16, 12, 10, 5)
