@@ -160,6 +160,17 @@ struct U4 {
 // Each round's two 32x32->64 products are written as 64-bit multiplies so
 // they lower to one v_mad_u64_u32 each instead of a v_mul_hi_u32 /
 // v_mul_lo_u32 pair.
+// a ^ b ^ c; on the device one v_bitop3_b32 (truth table 0x96) instead of two
+// v_xor_b32.
+__host__ __device__ inline uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 __host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
 {
 #pragma unroll
@@ -167,16 +178,25 @@ __host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
         if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
         const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
-        c = U4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0};
+        // round 0 mixes counter words that are often wave-uniform (scalar
+        // instructions); from round 1 on every word varies per lane
+        if (r == 0)
+            c = U4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0};
+        else
+            c = U4{xor3((uint32_t)(p1 >> 32), c.y, k0), (uint32_t)p1, xor3((uint32_t)(p0 >> 32), c.w, k1),
+                   (uint32_t)p0};
     }
     return c;
 }
 
-// 53-bit uniform in [0,1) from two words (hi first).
+// 53-bit uniform in [0,1) from two words (hi first): X = (hi:lo) >> 11 is
+// A 2^32 + B with A = hi >> 11 (21 bits) and B the next 32 bits, so
+// X 2^-53 = A 2^-21 + B 2^-53, both terms and their sum exact in binary64
+// (two 32-bit conversions and one fma instead of a 64-bit conversion).
 __host__ __device__ inline double u53(uint32_t hi, uint32_t lo)
 {
-    uint64_t x = ((uint64_t)hi << 32) | lo;
-    return (double)(x >> 11) * 0x1.0p-53;
+    const uint32_t a = hi >> 11, b = (hi << 21) | (lo >> 11);
+    return fma((double)a, 0x1.0p-21, (double)b * 0x1.0p-53);
 }
 
 // Deterministic-mode stream u_n: block n/2 with counter (q_lo, q_hi, 0,
@@ -371,6 +391,69 @@ __device__ inline double cqf_sum(const NB &N, int sigma, double nx, double ny, d
     return acc;
 }
 
+// cqf_sum and cq_sum of one director in one pass over its bonds: returns the
+// weighted director sum (cqf_sum), the c^2 sum in csq and the chirality sum
+// sum_b sgn_b c_b q_b (cq_sum) in cq.  Each sum takes the same operands in the
+// same order as the separate functions, so the results are bit-identical.
+template <class NB>
+__device__ inline double cqf_cq_sum(const NB &N, int sigma, double nx, double ny, double nz, double &csq, double &cq)
+{
+    double acc = 0.0, c2 = 0.0, a = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        double mx, my, mz;
+        N.get(b, mx, my, mz);
+        const double fb = (b & 1) ? -0.5 * (double)(sigma + N.spin(b)) : 0.5 * (double)(sigma + N.spin(b));
+        const double c = fma(nz, mz, fma(ny, my, nx * mx));
+        double q;
+        if (b < 2) q = fma(ny, mz, -nz * my);
+        else if (b < 4) q = fma(nz, mx, -nx * mz);
+        else q = fma(nx, my, -ny * mx);
+        acc = fma(c * q, fb, acc);
+        c2 = fma(c, c, c2);
+        a = (b & 1) == 0 ? fma(c, q, a) : fma(-c, q, a);
+    }
+    csq = c2;
+    cq = a;
+    return acc;
+}
+
+// The chirality sum of director n (cq_sum, returned), its c^2 sum and the
+// bond products c_b q_b, which weight_cq turns into cqf_sum for any spin.
+// Same operands and order as the separate functions: bit-identical sums.
+template <class NB>
+__device__ inline double cq_products(const NB &N, double nx, double ny, double nz, double &csq, double cqb[6])
+{
+    double c2 = 0.0, a = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        double mx, my, mz;
+        N.get(b, mx, my, mz);
+        const double c = fma(nz, mz, fma(ny, my, nx * mx));
+        double q;
+        if (b < 2) q = fma(ny, mz, -nz * my);
+        else if (b < 4) q = fma(nz, mx, -nx * mz);
+        else q = fma(nx, my, -ny * mx);
+        cqb[b] = c * q;
+        c2 = fma(c, c, c2);
+        a = (b & 1) == 0 ? fma(c, q, a) : fma(-c, q, a);
+    }
+    csq = c2;
+    return a;
+}
+
+template <class NB>
+__device__ inline double weight_cq(const NB &N, int sigma, const double cqb[6])
+{
+    double acc = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const double fb = (b & 1) ? -0.5 * (double)(sigma + N.spin(b)) : 0.5 * (double)(sigma + N.spin(b));
+        acc = fma(cqb[b], fb, acc);
+    }
+    return acc;
+}
+
 // a * b + c rounded once, with c taken from a scalar register pair: for a
 // constant c this is one v_fma_f64 fed by two scalar moves, where the compiler
 // otherwise rebuilds c in vector registers (two v_mov_b32) for a v_fmac_f64.
@@ -418,10 +501,24 @@ __device__ inline void sincos_2pi(double u, double &s, double &c)
     c = ((q + 1) & 2) ? -cc : cc;
 }
 
+// 1/sqrt(x) for a normal x > 0: the hardware estimate (v_rsq_f64) refined
+// by one step y + y e (1/2 + 3/8 e), e = 1 - x y^2, which leaves an error of
+// order e^3 (below the rounding of the result).  No scaling for denormal or
+// infinite x, which the cone trial never forms from unit directors.
+__device__ inline double rsqrt_normal(double x)
+{
+    const double y0 = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y0, y0, 1.0);
+    return fma(y0 * e, fma(e, 0.375, 0.5), y0);
+}
+
 // Cone trial in binary64 with reciprocal square roots: the same geometry
 // (tangent basis u = z x n / |.|, v = n x u, displacement d on a cone of
 // half-angle atan d at azimuth 2 pi u_phi), without the compat operation order.
-// Same branch as the reference for |nz| > 0.999 (chll/chll_mod.f90:149-157).
+// For a unit director |n x u| = |n| = 1, so v = (-nz nx, -nz ny, rho^2) / rho
+// shares u's normalisation (rounding leaves |v| within a few ulp of 1; the
+// trial vector is renormalised at the end).  Same branch as the reference for
+// |nz| > 0.999 (chll/chll_mod.f90:149-157).
 __device__ inline void trial_fp64(double nnx, double nny, double nnz, double d, double u_phi, double &ox,
                                   double &oy, double &oz)
 {
@@ -434,16 +531,16 @@ __device__ inline void trial_fp64(double nnx, double nny, double nnz, double d, 
         tz = nnz;
     } else {
         const double rho2 = fma(nnx, nnx, nny * nny);
-        const double ir = rsqrt(rho2);
+        const double ir = rsqrt_normal(rho2);
         const double ux = -nny * ir, uy = nnx * ir;
-        const double ivn = rsqrt(fma(nnz * nnz, rho2, rho2 * rho2));
-        const double vx = -nnz * nnx * ivn, vy = -nnz * nny * ivn, vz = rho2 * ivn;
+        const double nzr = nnz * ir;
+        const double vx = -nzr * nnx, vy = -nzr * nny, vz = rho2 * ir;
         const double dc = d * cphi, ds = d * sphi;
         tx = fma(ds, vx, fma(dc, ux, nnx));
         ty = fma(ds, vy, fma(dc, uy, nny));
         tz = fma(ds, vz, nnz);
     }
-    const double inv = rsqrt(fma(tz, tz, fma(ty, ty, tx * tx)));
+    const double inv = rsqrt_normal(fma(tz, tz, fma(ty, ty, tx * tx)));
     ox = tx * inv; oy = ty * inv; oz = tz * inv;
 }
 

@@ -220,21 +220,51 @@ __device__ inline bool spin_apply(const Fields &F, int colour, const PhaseArgs &
 // other colour's chirality trials.  Run right after this site's director trial
 // it reuses the neighbour directors in registers and saves one pass over the
 // lattice.  Returns the director decision; `flip` gets the chirality decision.
+// FP64 chirality decision (spin_compute) from the site's chirality sum a.
+template <int RNG>
+__device__ inline bool spin_decide_fp64(const PhaseArgs &A, int colour, uint32_t m, double a, int sg,
+                                        const double *u_det)
+{
+    double u_acc;
+    if (RNG == RNG_PHILOX) {
+        const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour, 1);
+        u_acc = u53(w.y, w.x);
+    } else {
+        u_acc = u_det[m];
+    }
+    const double dE = (double)sg * fma(A.KK, a, 2.0 * A.g);
+    return (dE < 0.0) || accept_exp(-dE * A.inv_kbt, u_acc);
+}
+
+// The director trial (dir_compute, FP64) then the chirality trial on the
+// accepted director (spin_compute, FP64); the bond passes of the director
+// trial also give the chirality sums of the old and the trial director.
 template <int RNG, bool OFF32>
 __device__ inline bool dir_spin_apply(const Fields &F, int colour, const PhaseArgs &A, uint32_t m, uint32_t own,
                                       const SiteIn &S, bool &flip)
 {
-    double t[3];
-    const bool acc = dir_compute<ARITH_FP64, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg, *A.d, t);
+    double u_phi, u_acc;
+    if (RNG == RNG_PHILOX) {
+        const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour, 0);
+        u_phi = u53(w.y, w.x);
+        u_acc = u53(w.w, w.z);
+    } else {
+        u_phi = A.u_phi[m];
+        u_acc = A.u_acc[m];
+    }
+    double t[3], c2o, c2n, cqo, cqn;
+    const double ao = cqf_cq_sum(S.N, S.sg, S.nx, S.ny, S.nz, c2o, cqo);
+    trial_fp64(S.nx, S.ny, S.nz, *A.d, u_phi, t[0], t[1], t[2]);
+    reload_barrier();
+    const double an = cqf_cq_sum(S.N, S.sg, t[0], t[1], t[2], c2n, cqn);
+    const double dE = (c2o - c2n) - A.KK * (an - ao);
+    const bool acc = (dE < 0.0) || accept_exp(-dE * A.inv_kbt, u_acc);
     if (acc) {
         fst<OFF32>(F.nx, own, t[0]);
         fst<OFF32>(F.ny, own, t[1]);
         fst<OFF32>(F.nz, own, t[2]);
     }
-    PhaseArgs As = A;
-    As.u_acc = A.u_spin;
-    flip = spin_compute<ARITH_FP64, RNG>(As, colour, m, S.N, acc ? t[0] : S.nx, acc ? t[1] : S.ny,
-                                         acc ? t[2] : S.nz, S.sg);
+    flip = spin_decide_fp64<RNG>(A, colour, m, acc ? cqn : cqo, S.sg, A.u_spin);
     if (flip) fst<OFF32>(F.s, own, (int8_t)(-S.sg));
     return acc;
 }
@@ -253,12 +283,25 @@ template <bool OFF32>
 __device__ inline bool spin_dir_apply(const Fields &F, const PhaseArgs &A, uint32_t m, uint32_t own,
                                       const SiteIn &S, bool &flip)
 {
+    // one pass over the old director's bonds: the chirality sum, then the
+    // director's old sum from the same bond products with the spin the
+    // chirality trial leaves
+    double c2o, cqb[6];
+    const double cqo = cq_products(S.N, S.nx, S.ny, S.nz, c2o, cqb);
     PhaseArgs As = A;
     As.istep = A.istep_spin;
-    flip = spin_compute<ARITH_FP64, RNG_PHILOX>(As, 0, m, S.N, S.nx, S.ny, S.nz, S.sg);
+    flip = spin_decide_fp64<RNG_PHILOX>(As, 0, m, cqo, S.sg, nullptr);
     const int sg = flip ? -S.sg : S.sg;
-    double t[3];
-    const bool acc = dir_compute<ARITH_FP64, RNG_PHILOX>(A, 0, m, S.N, S.nx, S.ny, S.nz, sg, *A.d, t);
+    // director trial of sweep A.istep (dir_compute, FP64)
+    const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, 0, 0);
+    const double u_phi = u53(w.y, w.x), u_acc = u53(w.w, w.z);
+    double t[3], c2n;
+    const double ao = weight_cq(S.N, sg, cqb);
+    trial_fp64(S.nx, S.ny, S.nz, *A.d, u_phi, t[0], t[1], t[2]);
+    reload_barrier();
+    const double an = cqf_sum(S.N, sg, t[0], t[1], t[2], c2n);
+    const double dE = (c2o - c2n) - A.KK * (an - ao);
+    const bool acc = (dE < 0.0) || accept_exp(-dE * A.inv_kbt, u_acc);
     if (flip) fst<OFF32>(F.s, own, (int8_t)sg);
     if (acc) {
         fst<OFF32>(F.nx, own, t[0]);
