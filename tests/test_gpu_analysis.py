@@ -127,3 +127,21 @@ def test_structure_factor_plane_groups_and_limits():
         assert np.array_equal(together[p], alone)
     with pytest.raises(RuntimeError, match="histogram entries"):
         structure_factors(np.ones((200, 2, 2), dtype=np.int32), kvs, [[100, 0, 0]])
+
+
+def test_structure_factor_general_planes():
+    """Planes whose rows' level sets run along general directions (|a|, |b| > 1),
+    p0 = 0 with both p1, p2 set, p0 = 2, and a row of 1100 sites (past the
+    line-walk limit of 1024: the per-site histogram kernel) against the oracle."""
+    _, _, _, s = rand_state((12, 9, 7), 31)
+    planes = [[2, 3, -1], [1, 0, 2], [0, 2, 1], [3, 1, 1], [-1, 2, 5], [2, 0, 0], [0, 0, -3], [1, -4, 2]]
+    kvs = np.array([0.0, 0.07, 0.21, 0.5, 1.7])
+    sf = structure_factors(s, kvs, planes)
+    for p, plane in enumerate(planes):
+        ref = np.array([abs(ora.get_sc(s, kv * np.array(plane, dtype=float)) / s.size) ** 2 for kv in kvs])
+        assert np.all(np.abs(sf[p] - ref) <= 1e-10 * ref + 1e-15), plane
+    _, _, _, w = rand_state((1100, 3, 2), 32)
+    for plane in ([1, 0, 0], [1, 1, -1], [0, 1, 1]):
+        got = structure_factors(w, kvs, [plane])[0]
+        ref = np.array([abs(ora.get_sc(w, kv * np.array(plane, dtype=float)) / w.size) ** 2 for kv in kvs])
+        assert np.all(np.abs(got - ref) <= 1e-10 * ref + 1e-15), plane
