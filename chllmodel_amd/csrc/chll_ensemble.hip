@@ -48,7 +48,7 @@ struct RepDesc {
 struct EnsDev {
     const RepDesc *R;
     const uint32_t *blk_rep;  // per launch block: replica << 24 | local block
-    const uint32_t *blk_rep2; // the same for two sites per thread (pair kernels), or null
+    const uint2 *blk_rep2;    // pair kernels: (replica << 24 | local block, live threads), or null
     const uint32_t *order;    // replicas in block-table order (largest first)
     double *kbt, *inv_kbt;    // per temperature slot
     double *d;                // per temperature slot (adaptive cone step)
@@ -72,15 +72,17 @@ struct EnsDev {
 // trial, 3: colour-0 chirality trials of sweep t each followed by the site's
 // director trial of sweep t + 1 (the single-lattice fused launches).  One
 // kernel per kind, so each gets the register budget of its own trial.
-// A launch covers the blocks base .. base + gridDim.x - 1 of the table (one
-// chunk of replicas).  rev: walk them last to first, so that a launch starts on
-// the replicas the launch before it finished with (still in the memory-side
-// cache).
+// A launch covers gridDim.x consecutive blocks of the table (one chunk of
+// replicas).  rev = 0: block b takes entry base + b; rev = 1: entry base - b,
+// base being the chunk's last entry, so that the launch walks them last to
+// first and starts on the replicas the launch before it finished with (still
+// in the memory-side cache).  The host forms base, so the kernel needs no
+// grid-size load.
 template <int KIND, int ARITH, int BC, int BS>
 __global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int colour, uint32_t istep_off, int rev, uint32_t base)
 {
     constexpr int kind = KIND;
-    const uint32_t code = E.blk_rep[base + (rev ? gridDim.x - 1u - blockIdx.x : blockIdx.x)];
+    const uint32_t code = E.blk_rep[rev ? base - blockIdx.x : base + blockIdx.x];
     const uint32_t r = code >> 24, lb = code & 0xFFFFFFu;
     const RepDesc &D = E.R[r];
     const int s = E.rep_slot[r];
@@ -119,8 +121,10 @@ __global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int colour, uint32_t
 template <int KIND, int BC, int BS>
 __global__ __launch_bounds__(BS) void k_pair_ens(EnsDev E, uint32_t istep_off, int rev, uint32_t base)
 {
-    const uint32_t code = E.blk_rep2[base + (rev ? gridDim.x - 1u - blockIdx.x : blockIdx.x)];
-    const uint32_t r = code >> 24, lb = code & 0xFFFFFFu;
+    // the entry also holds the block's count of live threads, so the bounds
+    // test needs no descriptor load and the descriptor loads issue together
+    const uint2 ent = E.blk_rep2[rev ? base - blockIdx.x : base + blockIdx.x];
+    const uint32_t r = ent.x >> 24, lb = ent.x & 0xFFFFFFu;
     const RepDesc &D = E.R[r];
     const int s = E.rep_slot[r];
     PhaseArgs A;
@@ -132,7 +136,7 @@ __global__ __launch_bounds__(BS) void k_pair_ens(EnsDev E, uint32_t istep_off, i
     const size_t gs = (size_t)kSlots * kSlotStride;
     bool a0 = false, a1 = false, f0 = false, f1 = false;
     if constexpr (KIND == 3) A.istep = A.istep_spin + 1;
-    if (m0 < D.G.nsub) {
+    if (threadIdx.x < ent.y) {
         SiteIn S0, S1;
         const uint32_t own = load_pair<BC, true>(D.G, D.F, KIND == 2 ? 1 : 0, m0, S0, S1);
         if constexpr (KIND == 2) {
@@ -345,7 +349,7 @@ struct chll_ensemble {
     RepDesc *R = nullptr;
     uint32_t *blk_rep = nullptr;
     uint32_t nblocks = 0;
-    uint32_t *blk_rep2 = nullptr;  // pair-kernel blocks (when every replica allows site pairs)
+    uint2 *blk_rep2 = nullptr;  // pair-kernel blocks (when every replica allows site pairs)
     uint32_t nblocks2 = 0;
     bool pairs = false;
     uint32_t *order_dev = nullptr;
@@ -438,7 +442,7 @@ int build_tables(chll_ensemble *X)
     // gives that XCD one band of rows of every plane, so the y neighbours of a
     // row stay in one L2.  Only the integer accept / flip counts depend on
     // these blocks.
-    std::vector<uint32_t> table2;
+    std::vector<uint2> table2;
     if (X->pairs) {
         for (uint32_t r : X->order) {
             const Geom &G = X->rep[r].G;
@@ -452,7 +456,8 @@ int build_tables(chll_ensemble *X)
                     const uint32_t x = b & 7u, n = b >> 3, q = n / band, rr = n - q * band;
                     lb = (q * 8u + x) * band + rr;
                 }
-                table2.push_back((r << 24) | lb);
+                const uint32_t live = std::min<uint32_t>((uint32_t)X->bs, G.nsub / 2 - lb * (uint32_t)X->bs);
+                table2.push_back(make_uint2((r << 24) | lb, live));
             }
         }
     }
@@ -470,7 +475,7 @@ int build_tables(chll_ensemble *X)
         return -1;
     ECHECK(hipMemcpy(X->blk_rep, table.data(), table.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     if (X->pairs)
-        ECHECK(hipMemcpy(X->blk_rep2, table2.data(), table2.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        ECHECK(hipMemcpy(X->blk_rep2, table2.data(), table2.size() * sizeof(uint2), hipMemcpyHostToDevice));
     X->E.blk_rep2 = X->blk_rep2;
     ECHECK(hipMemcpy(X->R, X->rep.data(), X->rep.size() * sizeof(RepDesc), hipMemcpyHostToDevice));
     if (!X->order_dev && ealloc(&X->order_dev, X->order.size())) return -1;
@@ -498,28 +503,29 @@ void launch_block_t(chll_ensemble *X, int K, hipStream_t st)
     const bool pairs = fuse && X->fuse == 1 && X->pairs;
     for (const Chunk &c : X->chunks) {
         const dim3 g(c.n1), g2(c.n2), gf(c.nf);
+        const uint32_t r1 = c.t1 + c.n1 - 1u, r2 = c.t2 + c.n2 - 1u;  // last entries, for reversed walks
         for (int t = 0; t < K; ++t) {
             const uint32_t off = (uint32_t)t;
             const bool last = t + 1 == K;
             // consecutive sweep launches alternate their block order
             if (!fuse) {
                 hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 0, c.t1);
-                hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 1, c.t1);
+                hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 1, r1);
                 hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 0, c.t1);
-                hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 1, c.t1);
+                hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 1, r1);
                 if (last) hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E, c.t1);
                 hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, last ? 1 : 0, 0, 1, c.f);
                 continue;
             }
-            if (t == 0) hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, c.t1);
+            if (t == 0) hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, r1);
             if (pairs) hipLaunchKernelGGL((k_pair_ens<2, BC, BS>), g2, b, 0, st, E, off, 0, c.t2);
             else hipLaunchKernelGGL((k_phase_ens<2, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 0, c.t1);
             if (!last) {
                 hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 0, t > 0 ? 1 : 0, 0, c.f);
-                if (pairs) hipLaunchKernelGGL((k_pair_ens<3, BC, BS>), g2, b, 0, st, E, off, 1, c.t2);
-                else hipLaunchKernelGGL((k_phase_ens<3, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, c.t1);
+                if (pairs) hipLaunchKernelGGL((k_pair_ens<3, BC, BS>), g2, b, 0, st, E, off, 1, r2);
+                else hipLaunchKernelGGL((k_phase_ens<3, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, r1);
             } else {
-                hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, c.t1);
+                hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, r1);
                 hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E, c.t1);
                 hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 1, t > 0 ? 1 : 0, 1, c.f);
             }
