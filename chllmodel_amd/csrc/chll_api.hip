@@ -218,13 +218,24 @@ int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lat
         const bool thick = (G.nk + ns - 1) / ns >= 8 && tiles * ns >= 512;
         for (int kind = 0; kind < 2; ++kind)
             for (int ar = 0; ar < 2; ++ar) {
-                const bool march = thick && kind == 1;
-                // one-site kernels in XCD row bands; FP64 with two sites per
-                // thread where the rows allow it (2-5 % faster fused sweeps)
-                // and the fused colour-0 launch in reverse block order (it
-                // starts on the planes the colour-1 launch finished with:
-                // 5 % faster sweeps at L = 256)
-                const int order = (ar == ARITH_FP64 && pairs_ok(G)) ? 7 : 1;
+                // FP64: two sites per thread in XCD row bands where the rows
+                // allow it (2-5 % faster fused sweeps), the fused colour-0
+                // launch in reverse block order (it starts on the planes the
+                // colour-1 launch finished with: 5 % faster sweeps at L = 256),
+                // the k-march for the chirality launches outside the fused
+                // chain.  Compat: site pairs in bands for both kinds where the
+                // rows allow it (5 % faster sweeps at L = 256 than the k-march
+                // chirality), else one-site kernels in bands and the k-march.
+                const bool pairs = pairs_ok(G);
+                bool march;
+                int order;
+                if (ar == ARITH_FP64) {
+                    march = thick && kind == 1;
+                    order = pairs ? 7 : 1;
+                } else {
+                    march = !pairs && thick && kind == 1;
+                    order = pairs ? 3 : 1;
+                }
                 L->kcfg[kind][ar] = {march ? 1 : 0, march ? tile : order, ns};
             }
     }
