@@ -9,8 +9,9 @@ resident in HBM.  The metric is site-updates/s = L^3 * sweeps / s.
 Default workload: L = 256 (16,777,216 sites), seeded isotropic directors and
 +-1 spins, eps = 1, K = 1.0, g = 0.5, J = 0, kT = 0.7, d0 = 0.3, nout = 50, open
 boundaries (the reference's), binary64 energies, site-keyed Philox random
-numbers; the reference-compatible arithmetic (binary32-rounded site energies)
-is timed beside it under "variants".  A second section times the 64-replica
+numbers; the reference-compatible arithmetic (binary32-rounded site energies),
+periodic boundaries and a run without measurement sweeps are timed beside it
+under "variants".  A second section times the 64-replica
 batched phase-diagram scan against the same replicas run one after another.
 
 Prints ONE JSON line.  The roofline object is for the dominant kernel (the
@@ -159,13 +160,20 @@ def main():
 
     tuned = {}
 
-    def time_single(arith, timing):
-        """Autotune the half-sweep kernels for this shape, then W warm-up sweeps
-        and K timed sweeps bracketed by synchronisation."""
+    def time_single(arith, timing, bc=None, nout=None):
+        """Autotune the half-sweep kernels for this shape (once per arithmetic;
+        runs with another boundary condition or measurement interval reuse the
+        choice), then W warm-up sweeps and K timed sweeps bracketed by
+        synchronisation."""
         from chllmodel_amd.autotune import apply, tune_lattice
 
-        lat = Lattice(L, L, L, bc=args.bc)
-        if args.no_tune:
+        bc = args.bc if bc is None else bc
+        nout = args.nout if nout is None else nout
+        lat = Lattice(L, L, L, bc=bc)
+        if arith in tuned:
+            if not args.no_tune:
+                apply(lat, tuned[arith], arith)
+        elif args.no_tune:
             tuned[arith] = {k: list(lat.get_kernel(k, arith)) for k in ("director", "chirality")}
             tuned[arith]["fuse"] = lat.get_fusion() if arith == "fp64" else False
             log(f"bench: {arith}: default kernels {tuned[arith]}")
@@ -178,26 +186,26 @@ def main():
         lat.set_step(args.d0)
         p = lat.params(g=args.g, KK=args.K, kbt=args.kT, rng=args.rng, arith=arith, seed=20261203, adapt=True)
         W, K = args.warmup, args.steps
-        lat.sweeps(1, W, args.nout, params=p, want_rows=False)
+        lat.sweeps(1, W, nout, params=p, want_rows=False)
         lat.sync()
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ev0.record(stream)
-        lat.sweeps(W + 1, W + K, args.nout, params=p, want_rows=False)
+        lat.sweeps(W + 1, W + K, nout, params=p, want_rows=False)
         ev1.record(stream)
         lat.sync()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        log(f"bench: {arith}: {K} sweeps in {wall:.4f} s (events {ev0.elapsed_time(ev1):.3f} ms)")
+        log(f"bench: {arith} bc={bc} nout={nout}: {K} sweeps in {wall:.4f} s (events {ev0.elapsed_time(ev1):.3f} ms)")
         kernels = {}
         if timing:
             # per-kernel HIP events in a pass of their own, after the timed
             # region: the events around every launch add gaps between kernels
             KT = min(K, 100)
             lat.set_timing(True)
-            lat.sweeps(W + K + 1, W + K + KT, args.nout, params=p, want_rows=False)
+            lat.sweeps(W + K + 1, W + K + KT, nout, params=p, want_rows=False)
             kt = lat.timing()
             lat.set_timing(False)
             for name, (ms, cnt) in kt.items():
@@ -236,6 +244,12 @@ def main():
         v_sups, v_wall, v_k = time_single(other, not args.no_timing)
         variants[other] = {"site_updates_per_s": v_sups, "ms_per_step": 1e3 * v_wall / K,
                            "kernels": {k: round(v["avg_us"], 2) for k, v in v_k.items()}}
+        # the same workload with periodic boundaries, and without measurement sweeps
+        other_bc = "periodic" if args.bc == "open" else "open"
+        for name, kw in ((f"{args.arith}_{other_bc}", {"bc": other_bc}), (f"{args.arith}_no_measure", {"nout": 0})):
+            v_sups, v_wall, _ = time_single(args.arith, False, **kw)
+            variants[name] = {"site_updates_per_s": v_sups, "ms_per_step": 1e3 * v_wall / K,
+                              "bc": kw.get("bc", args.bc), "nout": kw.get("nout", args.nout)}
     out = {
         "metric": METRIC,
         "value": sups,
