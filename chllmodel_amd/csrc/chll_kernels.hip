@@ -6,8 +6,12 @@
 // 1; chll/chll_mod.f90:70-81), then a one-block finalize kernel that sums the
 // per-block accept counters, adapts the cone step d on the device
 // (chll/chll_mod.f90:83-90) and, on measurement sweeps, closes the
-// observable reductions into one row (chll/chll_mod.f90:91-105).  Nothing in a
-// run of sweeps waits for the host.
+// observable reductions into one row (chll/chll_mod.f90:91-105).  In FP64
+// arithmetic the half-sweeps pair up into fused launches (k_dir_spin /
+// k_pair<2>: director + chirality of colour 1; k_spin_dir / k_pair<3>:
+// chirality of colour 0 + the next sweep's director of colour 0).  Block order
+// (sweep_block): XCD row bands, and consecutive launches alternate direction.
+// Nothing in a run of sweeps waits for the host.
 #include "chll_kernels.h"
 #include "chll_site.h"
 
@@ -75,7 +79,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_spin(Geom G, const int8_t *__
 
 // ---------------------------------------------------------------------------
 // Director half-sweep over one colour (chll/chll_mod.f90:117-202); block order
-// band_block (chll_device.h).
+// sweep_block (chll_device.h).
 
 template <int ARITH, int RNG, int BC, bool OFF32>
 __global__ __launch_bounds__(kBlock) void k_dir(Geom G, Fields F, int colour, PhaseArgs A,
