@@ -265,6 +265,7 @@ typedef struct {
     const double *buf;
     int64_t buf_len;
     uint32_t run_index, replica;
+    uint32_t site_offset;  /* added to the packed index m of the Philox counter */
     int overflow;
 } usrc_t;
 
@@ -291,7 +292,7 @@ static int64_t evolve_dir(lat_t *L, int colour, usrc_t *U, uint64_t base_phi, ui
                 int64_t r = l >> 1;
                 double u_phi, u_acc;
                 if (U->mode == ORACLE_RNG_PHILOX)
-                    philox_site(U->seed, U->run_index, U->replica, (uint32_t)r, istep, colour, 0, &u_phi, &u_acc);
+                    philox_site(U->seed, U->run_index, U->replica, (uint32_t)r + U->site_offset, istep, colour, 0, &u_phi, &u_acc);
                 else {
                     u_phi = src_at(U, base_phi + (uint64_t)r);
                     u_acc = src_at(U, base_acc + (uint64_t)r);
@@ -323,7 +324,7 @@ static int64_t evolve_s(lat_t *L, int colour, usrc_t *U, uint64_t base_acc,
                 int64_t r = l >> 1;
                 double u_acc, unused;
                 if (U->mode == ORACLE_RNG_PHILOX)
-                    philox_site(U->seed, U->run_index, U->replica, (uint32_t)r, istep, colour, 1, &u_acc, &unused);
+                    philox_site(U->seed, U->run_index, U->replica, (uint32_t)r + U->site_offset, istep, colour, 1, &u_acc, &unused);
                 else
                     u_acc = src_at(U, base_acc + (uint64_t)r);
                 double nnx = L->nx[l], nny = L->ny[l], nnz = L->nz[l];
@@ -412,10 +413,13 @@ double oracle_energy_true(const double *nx, const double *ny, const double *nz, 
  * Phase order per sweep: director(sl1), director(sl2), chirality(sl1),
  * chirality(sl2) (:70-81).  Step adaptation (:83-90) and measurement rows
  * (:91-105) as the reference computes them; m is column-major [rows,5].
- * Rows beyond rows = (1+nstop-nstart)/nout are not written.
+ * Rows beyond rows = (1+nstop-nstart)/nout are not written.  site_offset is
+ * added to every Philox site counter: the lattice is then the slab of planes
+ * k0.. of a larger lattice with the same ni, nj, whose packed indices start at
+ * site_offset = ni nj k0 / 2 (k0 even keeps the colours).
  * Returns 0, or -1 when a buffer source ran out. */
 int oracle_run(int rng_mode, uint64_t seed, uint64_t *stream_pos, const double *buf, int64_t buf_len,
-               uint32_t run_index, uint32_t replica, int bc, int arith,
+               uint32_t run_index, uint32_t replica, uint32_t site_offset, int bc, int arith,
                int nstart, int nstop, int nout,
                double *nx, double *ny, double *nz, int32_t *s,
                double g, double KK, double *d, double kbt,
@@ -427,6 +431,7 @@ int oracle_run(int rng_mode, uint64_t seed, uint64_t *stream_pos, const double *
     memset(&U, 0, sizeof(U));
     U.mode = rng_mode; U.seed = seed; U.pos = stream_pos ? *stream_pos : 0;
     U.buf = buf; U.buf_len = buf_len; U.run_index = run_index; U.replica = replica;
+    U.site_offset = site_offset;
     int64_t n3 = (int64_t)ni * nj * nk;
     int64_t nsub = n3 / 2;
     int rows = (1 + nstop - nstart) / nout;

@@ -78,7 +78,7 @@ class Oracle:
                                          ct.c_int, ct.c_double, ct.c_double]
         L.oracle_energy_true.restype = ct.c_double
         L.oracle_run.argtypes = [ct.c_int, ct.c_uint64, _u64p, _f64p, ct.c_int64,
-                                 ct.c_uint32, ct.c_uint32, ct.c_int, ct.c_int,
+                                 ct.c_uint32, ct.c_uint32, ct.c_uint32, ct.c_int, ct.c_int,
                                  ct.c_int, ct.c_int, ct.c_int,
                                  _f64p, _f64p, _f64p, _i32p,
                                  ct.c_double, ct.c_double, _f64p, ct.c_double,
@@ -134,8 +134,13 @@ class Oracle:
                                            _ptr(st["s"], _i32p), ni, nj, nk, bc, float(g), float(KK))
 
     def run(self, st, nstart, nstop, nout, g, KK, d, kbt, *, rng=RNG_STREAM, seed=0, stream_pos=0,
-            buf=None, run_index=0, replica=0, bc=BC_OPEN, arith=ARITH_COMPAT, want_counts=False):
+            buf=None, run_index=0, replica=0, bc=BC_OPEN, arith=ARITH_COMPAT, want_counts=False,
+            site_offset=0):
         """Sweep driver (chll/chll_mod.f90:40-115).  Mutates st in place.
+
+        site_offset (Philox draws only) is added to the packed site index of
+        every draw's counter, so st can be the slab of planes k0.. of a larger
+        lattice with the same ni, nj (site_offset = ni nj k0 / 2, k0 even).
 
         Returns (msteps, m, d_out, stream_pos_out[, counts])."""
         ni, nj, nk = st["nx"].shape
@@ -151,7 +156,7 @@ class Oracle:
             bptr, blen = _ptr(buf, _f64p), buf.size
         else:
             bptr, blen = None, 0
-        rc = self.lib.oracle_run(rng, seed, ct.byref(pos), bptr, blen, run_index, replica, bc, arith,
+        rc = self.lib.oracle_run(rng, seed, ct.byref(pos), bptr, blen, run_index, replica, site_offset, bc, arith,
                                  nstart, nstop, nout,
                                  _ptr(st["nx"], _f64p), _ptr(st["ny"], _f64p), _ptr(st["nz"], _f64p),
                                  _ptr(st["s"], _i32p), float(g), float(KK), ct.byref(dd), float(kbt),

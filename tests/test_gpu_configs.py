@@ -12,6 +12,9 @@ their full sizes:
   measurement rows agrees with the independent energy kernel
   (chll_energy_grad) on the final state, directors stay unit vectors and
   spins +-1, and the accept rate stays inside the adaptation band.
+* 1024 x 1024 x 520 (545 M sites): every director field spans 4.36 GB, past
+  what a 32-bit byte offset reaches, so the sweep runs the 64-bit addressed
+  kernels.  Planes 508.. are checked against the oracle on a slab.
 """
 import numpy as np
 import pytest
@@ -20,7 +23,7 @@ import torch
 from chllmodel_amd.autograd import energy_and_grad
 from chllmodel_amd.lattice import Lattice
 from chllmodel_amd.states import isotropic_state
-from oracle.oracle import ARITH_COMPAT, RNG_STREAM, Oracle, copy_state
+from oracle.oracle import ARITH_COMPAT, ARITH_FP64, RNG_PHILOX, RNG_STREAM, Oracle, copy_state
 
 pytestmark = pytest.mark.gpu
 
@@ -94,3 +97,61 @@ def test_l480_full_size():
     E, *_ = energy_and_grad(n, s, K, g, 1.0, want_grad=False)
     e_kernel = float(E[0]) / N
     assert abs(rows[-1][6] - e_kernel) <= 1e-12 * max(1.0, abs(e_kernel)), (rows[-1][6], e_kernel)
+
+
+@pytest.mark.timeout(900)
+def test_fields_beyond_4gib_slab_matches_oracle():
+    """ni = nj = 1024, nk = 520, open, FP64, Philox.  Each director field holds
+    2 nsub + pad doubles = 4.36 GB > 4 GiB, so the library takes the one-site
+    kernels with 64-bit addressing (the site-pair kernels need 32-bit offsets);
+    the colour-1 directors of planes 504.. lie past the first 4 GiB of their
+    fields.  Planes 500..519 start random, the rest aligned along z.  Two sweeps
+    at istep 20001-20002 (no step adaptation), then:
+    * planes 508..519 against the oracle run on the slab of planes 500..519
+      with Philox site offset ni nj 500 / 2 (its open bottom face reaches at
+      most four planes per sweep): spins exact, directors within 1e-12;
+    * the row's binary64 Hamiltonian against the energy kernel on the final
+      state, unit directors and +-1 spins everywhere."""
+    ni, nj, nk, k0 = 1024, 1024, 520, 500
+    N = ni * nj * nk
+    assert (N + 64) * 8 > 2 ** 32
+    g, K, kT, d0, seed, run = 0.5, 1.0, 0.7, 0.3, 11, 3
+    print("fields > 4 GiB: building the state", flush=True)
+    slab = isotropic_state(ni, nj, nk - k0, 5200)
+    del slab["dope"]
+    full = {f: np.zeros((ni, nj, nk), dtype=np.float64, order="F") for f in ("nx", "ny", "nz")}
+    full["nz"][...] = 1.0
+    full["s"] = np.ones((ni, nj, nk), dtype=np.int32, order="F")
+    for f in full:
+        full[f][:, :, k0:] = slab[f]
+    with Lattice(ni, nj, nk) as lat:
+        lat.upload_state(full)
+        del full
+        lat.set_step(d0)
+        print("fields > 4 GiB: sweeps", flush=True)
+        out = lat.sweeps(20001, 20002, 2, g=g, KK=K, kbt=kT, rng="philox", arith="fp64", seed=seed,
+                         run_index=run, want_counts=True)
+        st = lat.download()
+    print("fields > 4 GiB: oracle on the slab", flush=True)
+    Oracle().run(slab, 20001, 20002, 2, g, K, d0, kT, rng=RNG_PHILOX, seed=seed, run_index=run,
+                 arith=ARITH_FP64, site_offset=ni * nj * k0 // 2)
+    keep = 8
+    assert np.array_equal(st["s"][:, :, k0 + keep:], slab["s"][:, :, keep:])
+    for f in ("nx", "ny", "nz"):
+        assert np.abs(st[f][:, :, k0 + keep:] - slab[f][:, :, keep:]).max() <= 1e-12, f
+    del slab
+    rows, counts = out["rows"], out["counts"]
+    assert rows.shape[0] == 1 and counts.shape == (2, 2) and np.all(counts > 0)
+    assert np.all(np.isin(st["s"], (-1, 1)))
+    print("fields > 4 GiB: energy kernel", flush=True)
+    n = torch.empty((1, 3, nk, nj, ni), dtype=torch.float64, device="cuda")
+    for c, f in enumerate(("nx", "ny", "nz")):
+        n[0, c].copy_(torch.from_numpy(st[f].transpose(2, 1, 0)))
+    s = torch.from_numpy(st["s"].transpose(2, 1, 0).astype(np.int8)[None]).cuda()
+    E, *_ = energy_and_grad(n, s, K, g, 1.0, want_grad=False)
+    del n, s
+    e_kernel = float(E[0]) / N
+    assert abs(rows[-1][6] - e_kernel) <= 1e-12 * max(1.0, abs(e_kernel)), (rows[-1][6], e_kernel)
+    for f in ("nx", "ny", "nz"):  # squared in place to bound host memory
+        st[f] *= st[f]
+    assert np.abs(st["nx"] + st["ny"] + st["nz"] - 1.0).max() < 1e-12

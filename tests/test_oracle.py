@@ -291,3 +291,32 @@ def test_k0_spins_follow_tanh(orc):
     # per-row std of eta = sqrt((1-want^2)/N); rows are nearly independent at K=0
     sig = math.sqrt((1 - want ** 2) / 512) / math.sqrt(len(eta))
     assert abs(eta.mean() - want) < 5 * sig
+
+
+@pytest.mark.parametrize("arith", [ARITH_FP64, ARITH_COMPAT])
+def test_slab_with_site_offset_matches_whole_lattice(orc, arith):
+    """A slab of planes k0.. run with site_offset = ni nj k0 / 2 draws the same
+    Philox uniforms as those planes of the whole lattice.  Its open bottom face
+    misleads the planes next to it, by at most four planes per sweep (D0, D1,
+    C0 and C1 each reach one plane further), so planes k0 + 4 t and above agree
+    bit for bit after t sweeps.  Sweeps at istep >= 10000 keep
+    the cone step fixed (the adaptation, chll/chll_mod.f90:83-90, would see the
+    slab's own accept counts)."""
+    ni, nj, nk, k0 = 8, 6, 20, 8
+    whole = isotropic_state(ni, nj, nk, 31)
+    slab = {f: np.array(whole[f][:, :, k0:], order="F") for f in whole}
+    for st, off in ((whole, 0), (slab, ni * nj * k0 // 2)):
+        orc.run(st, 20001, 20002, 1, 0.5, 1.0, 0.3, 0.7, rng=RNG_PHILOX, seed=9, run_index=2,
+                arith=arith, site_offset=off)
+    keep = 8
+    for f in ("nx", "ny", "nz", "s"):
+        assert np.array_equal(slab[f][:, :, keep:], whole[f][:, :, k0 + keep:]), f
+    # the bottom planes do differ, so the comparison above is not vacuous
+    assert not np.array_equal(slab["nx"][:, :, :2], whole["nx"][:, :, k0:k0 + 2])
+    # and without the offset the draws are another site's
+    other = {f: np.array(whole[f][:, :, k0:], order="F") for f in whole}
+    orc.run(other, 20003, 20003, 1, 0.5, 1.0, 0.3, 0.7, rng=RNG_PHILOX, seed=9, run_index=2, arith=arith)
+    ref = {f: np.array(whole[f][:, :, k0:], order="F") for f in whole}
+    orc.run(ref, 20003, 20003, 1, 0.5, 1.0, 0.3, 0.7, rng=RNG_PHILOX, seed=9, run_index=2, arith=arith,
+            site_offset=ni * nj * k0 // 2)
+    assert not np.array_equal(other["nx"], ref["nx"])
