@@ -9,6 +9,14 @@ chain is a plain Metropolis chain at fixed d, and keeps the reference's rows
 `discard` rows.  tests/test_gpu_equilibrium.py compares the GPU's Philox
 chains with these series within 3 standard errors.
 
+The reference records no order parameter, and its run() calls random_seed()
+on entry, which in this flang build restarts the same sequence on every call,
+so a chain cannot be cut into calls to read its directors.  For <P2> each case
+also runs the reference built with the documented uniform stream
+(oracle/_ref/libchll_ref_stream.so), one call per nout sweeps continuing the
+stream, and keeps the means <n_a n_b> (xx, yy, zz, xy, xz, yz) of the state
+after each call (q_series) beside that chain's rows (stream_series).
+
 Run in the build container only (needs /root/reference):
     python tests/golden/make_equilibrium.py
 """
@@ -32,6 +40,28 @@ CASES = [
     ("eq_12c_K15_g1_T10", (12, 12, 12), 23, 1.0, 1.5, 0.3, 1.0),
 ]
 NSTART, NSWEEPS, NOUT, DISCARD = 10001, 20000, 10, 400
+STREAM_SEED = 0x51E4
+
+
+def q_sums(st):
+    nx, ny, nz = st["nx"], st["ny"], st["nz"]
+    return [np.mean(nx * nx), np.mean(ny * ny), np.mean(nz * nz), np.mean(nx * ny), np.mean(nx * nz),
+            np.mean(ny * nz)]
+
+
+def stream_chain(dims, seed, g, KK, d, kbt):
+    """The reference on the documented stream, one call per NOUT sweeps."""
+    ref = Reference(stream=True)
+    st = isotropic_state(*dims, seed)
+    pos, qs, rows = 0, [], []
+    for c in range(NSWEEPS // NOUT):
+        a = NSTART + c * NOUT
+        _, m, d_out, pos = ref.run(st, a, a + NOUT - 1, NOUT, g, KK, d, kbt, seed=STREAM_SEED + seed,
+                                   stream_pos=pos)
+        assert d_out == d
+        qs.append(q_sums(st))
+        rows.append(m[0, :4])
+    return np.array(qs)[DISCARD:], np.array(rows)[DISCARD:]
 
 
 def main():
@@ -41,10 +71,11 @@ def main():
         msteps, m, d_out, _ = ref.run(st, NSTART, NSTART + NSWEEPS - 1, NOUT, g, KK, d, kbt)
         assert d_out == d, (name, d_out)
         series = m[DISCARD:, :4].copy()
+        q_series, stream_series = stream_chain(dims, seed, g, KK, d, kbt)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), series=series, msteps=msteps[DISCARD:],
                             params=np.array([NSTART, NSWEEPS, NOUT, DISCARD, g, KK, d, kbt, seed], dtype=np.float64),
-                            dims=np.array(dims, dtype=np.int32))
-        print(name, "rows", series.shape[0], "means", series.mean(axis=0))
+                            dims=np.array(dims, dtype=np.int32), q_series=q_series, stream_series=stream_series)
+        print(name, "rows", series.shape[0], "means", series.mean(axis=0), "stream chain", stream_series.mean(axis=0))
 
 
 if __name__ == "__main__":

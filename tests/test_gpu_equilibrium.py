@@ -6,9 +6,15 @@ sweeps 10001..30000, where the step adaptation is off, and kept the rows
 (reported energy, eta, paccept, faccept) every 10 sweeps after the first 4000
 sweeps.  The GPU runs the same chains with its site-keyed Philox draws in both
 arithmetic modes; every mean must agree with the reference's within 3 combined
-standard errors (Flyvbjerg-Petersen blocking on both series).  The random
-streams differ, so this is the statistical bar; the deterministic modes are
-tested bit-exactly in test_gpu_parity.py."""
+standard errors (Flyvbjerg-Petersen blocking on both series).  The specific
+heat and the chiral susceptibility (N var(e)/kT^2 and N var(eta)/kT on the
+same rows) must agree within 3 combined jackknife errors.  <P2>, the largest
+eigenvalue of Q = (3 <n n> - I)/2, is checked against the reference built
+with the documented uniform stream (the fixture's q_series: the reference
+records no order parameter, and its run() restarts its own random_number
+sequence on every call, so its own chain cannot be read between calls).  The
+random streams differ, so this is the statistical bar; the deterministic
+modes are tested bit-exactly in test_gpu_parity.py."""
 import glob
 import os
 
@@ -16,7 +22,7 @@ import numpy as np
 import pytest
 
 from chllmodel_amd.lattice import Lattice
-from chllmodel_amd.observables import blocking_error
+from chllmodel_amd.observables import blocking_error, chi_per_site, cv_per_site, p2_from_q, p2_series
 from chllmodel_amd.states import isotropic_state
 
 pytestmark = pytest.mark.gpu
@@ -50,3 +56,16 @@ def test_equilibrium_means_match_reference(path, arith):
         sig = np.hypot(eg, er)
         print(f"  {name}: gpu {mg:.5f} +- {eg:.5f}  reference {mr:.5f} +- {er:.5f}  ({(mg - mr) / sig:+.2f} sigma)")
         assert abs(mg - mr) <= 3.0 * sig, (name, mg, mr, sig)
+    N = int(np.prod(dims))
+    for name, fn, c in (("C_v", cv_per_site, 0), ("chi_c", chi_per_site, 1)):
+        (vg, eg), (vr, er) = fn(gpu[:, c], kbt, N), fn(ref[:, c], kbt, N)
+        sig = np.hypot(eg, er)
+        print(f"  {name}: gpu {vg:.4f} +- {eg:.4f}  reference {vr:.4f} +- {er:.4f}  ({(vg - vr) / sig:+.2f} sigma)")
+        assert abs(vg - vr) <= 3.0 * sig, (name, vg, vr, sig)
+    p2g = p2_series(rows, 8)
+    p2r = np.array([p2_from_q(q) for q in z["q_series"]])
+    assert p2r.shape == p2g.shape
+    mg, mr = p2g.mean(), p2r.mean()
+    sig = np.hypot(blocking_error(p2g), blocking_error(p2r))
+    print(f"  P2: gpu {mg:.5f}  reference (stream build) {mr:.5f}  ({(mg - mr) / sig:+.2f} sigma)")
+    assert abs(mg - mr) <= 3.0 * sig, ("P2", mg, mr, sig)
