@@ -29,6 +29,7 @@
 // spins int8.
 #include <cstdarg>
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -330,6 +331,45 @@ int afail(const char *fmt, ...)
     return -1;
 }
 
+// Host-to-device copies of small host tables without waiting for the stream:
+// the tables go through one pinned staging buffer per host thread, which is
+// reused only after the event recorded behind its previous copies has
+// completed.  (The event belongs to the device current at the first call.)
+struct HostStage {
+    char *buf = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+};
+thread_local HostStage g_stage;
+
+// Copies bytes[i] bytes from host src[i] to device dst[i], i < k, in the
+// order of stream st.
+int stage_copy(int k, const void *const *src, void *const *dst, const size_t *bytes, hipStream_t st)
+{
+    HostStage &H = g_stage;
+    size_t total = 0;
+    for (int i = 0; i < k; ++i) total += (bytes[i] + 15) & ~(size_t)15;
+    if (H.done && hipEventSynchronize(H.done) != hipSuccess) return afail("staging: event wait failed");
+    if (!H.done && hipEventCreateWithFlags(&H.done, hipEventDisableTiming) != hipSuccess)
+        return afail("staging: hipEventCreate failed");
+    if (H.cap < total) {
+        if (H.buf) (void)hipHostFree(H.buf);
+        H.buf = nullptr;
+        H.cap = 0;
+        if (hipHostMalloc((void **)&H.buf, total) != hipSuccess) return afail("staging: hipHostMalloc(%zu) failed", total);
+        H.cap = total;
+    }
+    size_t off = 0;
+    for (int i = 0; i < k; ++i) {
+        memcpy(H.buf + off, src[i], bytes[i]);
+        if (hipMemcpyAsync(dst[i], H.buf + off, bytes[i], hipMemcpyHostToDevice, st) != hipSuccess)
+            return afail("staging: copy failed");
+        off += (bytes[i] + 15) & ~(size_t)15;
+    }
+    if (hipEventRecord(H.done, st) != hipSuccess) return afail("staging: hipEventRecord failed");
+    return 0;
+}
+
 int isqrt_floor(int r2)
 {
     int r = 0;
@@ -494,14 +534,13 @@ int chll_shell_corr(const double *nx, const double *ny, const double *nz, const 
         if (cnt[sh] == 0) return afail("chll_shell_corr: shell %d has no pairs on a %dx%dx%d lattice", sh, ni, nj, nk);
     char *ws = (char *)workspace;
     hipStream_t st = (hipStream_t)stream;
-    // pageable sources: wait for the copies before the vectors go out of scope
-    if (hipMemcpyAsync(ws + L.tab, tab.data(), tab.size() * sizeof(int32_t), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(ws + L.start, start.data(), start.size() * sizeof(int32_t), hipMemcpyHostToDevice, st) !=
-            hipSuccess ||
-        hipMemcpyAsync(ws + L.counts, cnt.data(), cnt.size() * sizeof(int64_t), hipMemcpyHostToDevice, st) !=
-            hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        return afail("chll_shell_corr: table copy failed");
+    {
+        const void *src[3] = {tab.data(), start.data(), cnt.data()};
+        void *dst[3] = {ws + L.tab, ws + L.start, ws + L.counts};
+        const size_t bytes[3] = {tab.size() * sizeof(int32_t), start.size() * sizeof(int32_t),
+                                 cnt.size() * sizeof(int64_t)};
+        if (stage_copy(3, src, dst, bytes, st)) return -1;
+    }
     CorrArgs A;
     A.nx = nx; A.ny = ny; A.nz = nz; A.s = s;
     A.tab = (const int32_t *)(ws + L.tab);
@@ -549,10 +588,13 @@ int chll_structure_factor(const int8_t *s, int32_t ni, int32_t nj, int32_t nk, c
     if (workspace_bytes < L.total) return afail("chll_structure_factor: workspace too small");
     char *ws = (char *)workspace;
     hipStream_t st = (hipStream_t)stream;
-    // kv is the caller's host array: copied in stream order, waited for before returning
-    if (hipMemcpyAsync(ws + L.kv, kv, (size_t)bins * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        return afail("chll_structure_factor: kv copy failed");
+    // kv is the caller's host array, staged so that the call does not wait for the stream
+    {
+        const void *src[1] = {kv};
+        void *dst[1] = {ws + L.kv};
+        const size_t bytes[1] = {(size_t)bins * sizeof(double)};
+        if (stage_copy(1, src, dst, bytes, st)) return -1;
+    }
     A.s = s;
     A.part = (int *)(ws + L.part);
     A.hist = (int *)(ws + L.hist);

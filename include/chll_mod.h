@@ -233,8 +233,10 @@ int chll_energy_grad(const double *n, const int8_t *s, int32_t batch, int32_t ni
 /* Device pointers, one lattice in plain Fortran order (i fastest), float64
  * director components, int8 spins.  Functions return 0 or -1 with
  * chll_analysis_last_error() set; the device work is enqueued on `stream`
- * (HIP stream or NULL), after a wait for the small host-to-device table
- * copies each call makes. */
+ * (HIP stream or NULL) and the call does not wait for it.  The small host
+ * tables each call copies go through a pinned staging buffer per host thread;
+ * a call waits only for that buffer's previous copies.  The workspace is used
+ * in stream order. */
 const char *chll_analysis_last_error(void);
 
 /* Replaces calc_n_corr and calc_s_corr (notebooks/paper_plots2.ipynb, cell 0;
