@@ -296,7 +296,9 @@ def main():
 def dropin_run(L, st0, args):
     """The reference's own entry point run() (chll/chll_mod.f90:40) on host
     arrays: each call uploads the Fortran-ordered fields over PCIe, sweeps on
-    the device and downloads them again, as chll/chll.py:160-173 drives it.
+    the device and downloads them again, as chll/chll.py:160-173 drives it
+    (once per simulation).  Calls of n and 10 n sweeps give the fixed cost of
+    a call (the two transfers and the layout changes) and the cost per sweep.
     Reported beside value, never as value."""
     import ctypes as ct
 
@@ -306,7 +308,8 @@ def dropin_run(L, st0, args):
 
     lib = _lib.lib()
     st = {k: np.asfortranarray(v.copy()) for k, v in st0.items()}
-    n = args.dropin_sweeps
+    n1 = args.dropin_sweeps
+    n2 = 10 * n1
     nsub = L ** 3 // 2
     msteps = np.zeros(1, dtype=np.int32)
     m = np.zeros((1, 5), dtype=np.float64, order="F")
@@ -316,27 +319,34 @@ def dropin_run(L, st0, args):
     _lib.check(lib.chll_configure(0, arith, {"open": 0, "periodic": 1}[args.bc], 0))
     _lib.check(lib.chll_set_seed(20261203, 0))
 
-    def call(dd):
+    def call(n):
         lib.run(ct.c_int(1), ct.c_int(n), ct.c_int(n), st["nx"], st["ny"], st["nz"], st["s"], st["dope"], sl1, sl2,
-                ct.c_double(args.g), ct.c_double(args.K), dd, ct.c_double(args.kT), ct.c_int(L), ct.c_int(L),
-                ct.c_int(L), ct.c_int(0), msteps, m)
+                ct.c_double(args.g), ct.c_double(args.K), ct.c_double(args.d0), ct.c_double(args.kT), ct.c_int(L),
+                ct.c_int(L), ct.c_int(L), ct.c_int(0), msteps, m)
         if _lib.last_error():
             raise RuntimeError(_lib.last_error())
 
-    try:
-        call(ct.c_double(args.d0))  # first call allocates and builds the lists
+    def best(n, calls=3):
         times = []
-        for _ in range(3):
+        for _ in range(calls):
             t0 = time.perf_counter()
-            call(ct.c_double(args.d0))
+            call(n)
             times.append(time.perf_counter() - t0)
+        return min(times)
+
+    try:
+        call(n1)  # first call allocates and builds the lists
+        t1, t2 = best(n1), best(n2)
     finally:
         _lib.check(lib.chll_configure(0, 0, 0, 1))
-    wall = min(times)
-    log(f"bench: drop-in run(): {n} sweeps per call in {wall:.4f} s")
+    per_sweep = (t2 - t1) / (n2 - n1)
+    fixed = t1 - n1 * per_sweep
+    log(f"bench: drop-in run(): {n1} sweeps per call in {t1:.4f} s, {n2} in {t2:.4f} s")
     return {"entry": "run (chll/chll_mod.f90:40), host arrays in and out per call",
-            "sweeps_per_call": n, "ms_per_call": 1e3 * wall, "site_updates_per_s": L ** 3 * n / wall,
-            "arith": args.arith, "calls_timed": len(times), "statistic": "best of 3 calls"}
+            "sweeps_per_call": n1, "ms_per_call": 1e3 * t1, "site_updates_per_s": L ** 3 * n1 / t1,
+            f"site_updates_per_s_{n2}_sweeps": L ** 3 * n2 / t2,
+            "fixed_ms_per_call": 1e3 * fixed, "ms_per_sweep_in_call": 1e3 * per_sweep,
+            "arith": args.arith, "statistic": "best of 3 calls at each length"}
 
 
 def gradient_rate(args):
