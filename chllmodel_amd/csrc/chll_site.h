@@ -372,6 +372,34 @@ __device__ inline uint32_t spin2(const int8_t *base, uint32_t q)
 __device__ inline int lo_spin(uint32_t v) { return (int)(int8_t)(v & 0xffu); }
 __device__ inline int hi_spin(uint32_t v) { return (int)(int8_t)(v >> 8); }
 
+// Lane l - 1's (DPP wave_shr:1) and lane l + 1's (wave_shl:1) copy of a
+// value, in one VALU move per 32 bits; lane 0 / lane 63 get zero.
+__device__ inline uint32_t lane_prev(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+}
+__device__ inline uint32_t lane_next(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);
+}
+__device__ inline double lane_prev(double v)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    return __longlong_as_double((long long)(((uint64_t)lane_prev((uint32_t)(u >> 32)) << 32) | lane_prev((uint32_t)u)));
+}
+__device__ inline double lane_next(double v)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    return __longlong_as_double((long long)(((uint64_t)lane_next((uint32_t)(u >> 32)) << 32) | lane_next((uint32_t)u)));
+}
+
+// The kernels that call load_pair give consecutive lanes of a wave consecutive
+// pairs (m0 = 2 (tile base + lane)).  The extra x neighbour of a pair is then,
+// inside a row, the x pair that the lane beside it loaded: site q - 1 is lane
+// l - 1's q + 1 (row parity p = 0), site q + 2 is lane l + 1's q (p = 1).  It
+// moves between lanes by DPP; only a pair at a wave's edge whose neighbour
+// is in the next wave, or a periodic wrap, loads it (an open row end reads
+// zero).  That takes four loads per pair off the memory pipeline.
 template <int BC, bool OFF32>
 __device__ inline uint32_t load_pair(const Geom &G, const Fields &F, int colour, uint32_t m0, SiteIn &S0, SiteIn &S1)
 {
@@ -392,8 +420,24 @@ __device__ inline uint32_t load_pair(const Geom &G, const Fields &F, int colour,
     const uint32_t e = p ? n1[0] : n0[1];
     const double2 xx = fld2<OFF32>(F.nx, q), xy = fld2<OFF32>(F.ny, q), xz = fld2<OFF32>(F.nz, q);
     const uint32_t xs = spin2<OFF32>(F.s, q);
-    const double ex = fld<OFF32>(F.nx, e), ey = fld<OFF32>(F.ny, e), ez = fld<OFF32>(F.nz, e);
-    const int es = fld<OFF32>(F.s, e);
+    const double px = lane_prev(xx.y), py = lane_prev(xy.y), pz = lane_prev(xz.y);
+    const uint32_t ps = lane_prev(xs);
+    const double nxv = lane_next(xx.x), nyv = lane_next(xy.x), nzv = lane_next(xz.x);
+    const uint32_t ns = lane_next(xs);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t r = (st0.i >> 1) >> 1, R = G.half >> 1;  // pair position in its row, pairs per row
+    double ex, ey, ez;
+    int es;
+    if (p == 0u && r != 0u && lane != 0u) {
+        ex = px; ey = py; ez = pz; es = hi_spin(ps);
+    } else if (p == 1u && r + 1u != R && lane != 63u) {
+        ex = nxv; ey = nyv; ez = nzv; es = lo_spin(ns);
+    } else if (BC == BC_OPEN && (p == 0u ? r == 0u : r + 1u == R)) {
+        ex = ey = ez = 0.0; es = 0;
+    } else {
+        ex = fld<OFF32>(F.nx, e); ey = fld<OFF32>(F.ny, e); ez = fld<OFF32>(F.nz, e);
+        es = fld<OFF32>(F.s, e);
+    }
     if (p == 0u) {
         S0.N.x[0] = xx.x; S0.N.y[0] = xy.x; S0.N.z[0] = xz.x; S0.N.s[0] = lo_spin(xs);
         S0.N.x[1] = ex;   S0.N.y[1] = ey;   S0.N.z[1] = ez;   S0.N.s[1] = es;
