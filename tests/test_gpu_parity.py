@@ -227,7 +227,8 @@ def test_dropin_run_philox_run_counter(orc):
 
 
 SHAPES = [((256 // 8, 24, 20), "open"), ((48, 40, 30), "periodic"), ((10, 7, 5), "open"), ((130, 6, 9), "open"),
-          ((18, 34, 4), "periodic"), ((64, 64, 8), "open"), ((128, 64, 6), "periodic")]
+          ((18, 34, 4), "periodic"), ((64, 64, 8), "open"), ((128, 64, 6), "periodic"), ((256, 8, 6), "open"),
+          ((256, 12, 4), "periodic")]
 
 
 @pytest.mark.parametrize("dims,bc", SHAPES, ids=[f"{'x'.join(map(str, d))}-{b}" for d, b in SHAPES])
