@@ -210,16 +210,26 @@ __host__ __device__ inline double stream_uniform(uint64_t seed, uint64_t n)
     return (n & 1) ? u53(w.w, w.z) : u53(w.y, w.x);
 }
 
-// Production site-keyed draws.  Counter (m, istep, (run_index << 4) |
-// (colour << 1) | kind, replica), key (seed_lo, seed_hi).  kind 0 (director
-// trial): u_phi = words (y, x), u_acc = words (w, z).  kind 1 (chirality
-// trial): u_acc = words (y, x).
+// Production site-keyed draws: one block per site, sweep and colour serves
+// both trials of the site.  Counter (m, istep, (run_index << 4) |
+// (colour << 1), replica), key (seed_lo, seed_hi).  Director trial: u_phi =
+// x 2^-32 (32 bits), u_acc = (y, z >> 16) as a 48-bit fraction; chirality
+// trial: u_acc = (w, z & 0xffff) as a 48-bit fraction.
 __host__ __device__ inline U4 site_block(uint64_t seed, uint32_t run_index, uint32_t replica, uint32_t m,
-                                uint32_t istep, int colour, int kind)
+                                uint32_t istep, int colour)
 {
-    return philox4x32_10(U4{m, istep, (run_index << 4) | ((uint32_t)colour << 1) | (uint32_t)kind, replica},
-                         (uint32_t)seed, (uint32_t)(seed >> 32));
+    return philox4x32_10(U4{m, istep, (run_index << 4) | ((uint32_t)colour << 1), replica}, (uint32_t)seed,
+                         (uint32_t)(seed >> 32));
 }
+
+// hi 2^-32 + lo16 2^-48 (exact: one conversion each and one fma).
+__host__ __device__ inline double u48(uint32_t hi, uint32_t lo16)
+{
+    return fma((double)hi, 0x1.0p-32, (double)lo16 * 0x1.0p-48);
+}
+__host__ __device__ inline double site_u_phi(const U4 &w) { return (double)w.x * 0x1.0p-32; }
+__host__ __device__ inline double site_u_dir(const U4 &w) { return u48(w.y, w.z >> 16); }
+__host__ __device__ inline double site_u_spin(const U4 &w) { return u48(w.w, w.z & 0xffffu); }
 
 // ---------------------------------------------------------------------------
 __device__ inline double fl32(double x) { return (double)(float)x; }

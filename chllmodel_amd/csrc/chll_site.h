@@ -121,9 +121,9 @@ __device__ inline bool dir_compute(const PhaseArgs &A, int colour, uint32_t m, c
 {
     double u_phi, u_acc;
     if (RNG == RNG_PHILOX) {
-        const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour, 0);
-        u_phi = u53(w.y, w.x);
-        u_acc = u53(w.w, w.z);
+        const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour);
+        u_phi = site_u_phi(w);
+        u_acc = site_u_dir(w);
     } else {
         u_phi = A.u_phi[m];
         u_acc = A.u_acc[m];
@@ -152,8 +152,7 @@ __device__ inline bool spin_compute(const PhaseArgs &A, int colour, uint32_t m, 
 {
     double u_acc;
     if (RNG == RNG_PHILOX) {
-        const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour, 1);
-        u_acc = u53(w.y, w.x);
+        u_acc = site_u_spin(site_block(A.seed, A.run_index, A.replica, m, A.istep, colour));
     } else {
         u_acc = A.u_acc[m];
     }
@@ -220,18 +219,10 @@ __device__ inline bool spin_apply(const Fields &F, int colour, const PhaseArgs &
 // other colour's chirality trials.  Run right after this site's director trial
 // it reuses the neighbour directors in registers and saves one pass over the
 // lattice.  Returns the director decision; `flip` gets the chirality decision.
-// FP64 chirality decision (spin_compute) from the site's chirality sum a.
-template <int RNG>
-__device__ inline bool spin_decide_fp64(const PhaseArgs &A, int colour, uint32_t m, double a, int sg,
-                                        const double *u_det)
+// FP64 chirality decision (spin_compute) from the site's chirality sum a and
+// its uniform u_acc.
+__device__ inline bool spin_decide_fp64(const PhaseArgs &A, double a, int sg, double u_acc)
 {
-    double u_acc;
-    if (RNG == RNG_PHILOX) {
-        const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour, 1);
-        u_acc = u53(w.y, w.x);
-    } else {
-        u_acc = u_det[m];
-    }
     const double dE = (double)sg * fma(A.KK, a, 2.0 * A.g);
     return (dE < 0.0) || accept_exp(-dE * A.inv_kbt, u_acc);
 }
@@ -243,14 +234,17 @@ template <int RNG, bool OFF32>
 __device__ inline bool dir_spin_apply(const Fields &F, int colour, const PhaseArgs &A, uint32_t m, uint32_t own,
                                       const SiteIn &S, bool &flip)
 {
-    double u_phi, u_acc;
+    // one block serves the site's director and chirality trials
+    double u_phi, u_acc, u_spin;
     if (RNG == RNG_PHILOX) {
-        const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour, 0);
-        u_phi = u53(w.y, w.x);
-        u_acc = u53(w.w, w.z);
+        const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour);
+        u_phi = site_u_phi(w);
+        u_acc = site_u_dir(w);
+        u_spin = site_u_spin(w);
     } else {
         u_phi = A.u_phi[m];
         u_acc = A.u_acc[m];
+        u_spin = A.u_spin[m];
     }
     double t[3], c2o, c2n, cqo, cqn;
     const double ao = cqf_cq_sum(S.N, S.sg, S.nx, S.ny, S.nz, c2o, cqo);
@@ -264,7 +258,7 @@ __device__ inline bool dir_spin_apply(const Fields &F, int colour, const PhaseAr
         fst<OFF32>(F.ny, own, t[1]);
         fst<OFF32>(F.nz, own, t[2]);
     }
-    flip = spin_decide_fp64<RNG>(A, colour, m, acc ? cqn : cqo, S.sg, A.u_spin);
+    flip = spin_decide_fp64(A, acc ? cqn : cqo, S.sg, u_spin);
     if (flip) fst<OFF32>(F.s, own, (int8_t)(-S.sg));
     return acc;
 }
@@ -288,13 +282,14 @@ __device__ inline bool spin_dir_apply(const Fields &F, const PhaseArgs &A, uint3
     // chirality trial leaves
     double c2o, cqb[6];
     const double cqo = cq_products(S.N, S.nx, S.ny, S.nz, c2o, cqb);
-    PhaseArgs As = A;
-    As.istep = A.istep_spin;
-    flip = spin_decide_fp64<RNG_PHILOX>(As, 0, m, cqo, S.sg, nullptr);
+    // the chirality trial draws from sweep A.istep_spin's block, the director
+    // trial from sweep A.istep's
+    const U4 ws = site_block(A.seed, A.run_index, A.replica, m, A.istep_spin, 0);
+    flip = spin_decide_fp64(A, cqo, S.sg, site_u_spin(ws));
     const int sg = flip ? -S.sg : S.sg;
     // director trial of sweep A.istep (dir_compute, FP64)
-    const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, 0, 0);
-    const double u_phi = u53(w.y, w.x), u_acc = u53(w.w, w.z);
+    const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, 0);
+    const double u_phi = site_u_phi(w), u_acc = site_u_dir(w);
     double t[3], c2n;
     const double ao = weight_cq(S.N, sg, cqb);
     trial_fp64(S.nx, S.ny, S.nz, *A.d, u_phi, t[0], t[1], t[2]);

@@ -107,19 +107,27 @@ void oracle_stream_fill(uint64_t seed, uint64_t n0, int64_t count, double *out)
     for (int64_t t = 0; t < count; ++t) out[t] = oracle_stream_uniform(seed, n0 + (uint64_t)t);
 }
 
-/* Production site-keyed draws: counter (m, istep, (run_index << 4) | (colour << 1)
- * | kind, replica), key (seed_lo, seed_hi); m = packed index of the site within
- * its colour (l >> 1).  kind 0 = director trial: u_phi = words (1,0), u_acc =
- * words (3,2).  kind 1 = chirality trial: u_acc = words (1,0). */
-static void philox_site(uint64_t seed, uint32_t run_index, uint32_t replica, uint32_t m,
-                        uint32_t istep, int colour, int kind, double *u0, double *u1)
+/* Production site-keyed draws: one block per site, sweep and colour serves both
+ * trials of the site.  Counter (m, istep, (run_index << 4) | (colour << 1),
+ * replica), key (seed_lo, seed_hi); m = packed index of the site within its
+ * colour (l >> 1).  Director trial: u_phi = word 0 / 2^32, u_acc = word 1 and
+ * the high half of word 2 as a 48-bit fraction; chirality trial: u_acc = word
+ * 3 and the low half of word 2 as a 48-bit fraction. */
+static inline double u48(uint32_t hi, uint32_t lo16)
 {
-    uint32_t ctr[4] = {m, istep, (run_index << 4) | ((uint32_t)colour << 1) | (uint32_t)kind, replica};
+    return (double)hi / 4294967296.0 + (double)lo16 / 281474976710656.0;
+}
+
+static void philox_site(uint64_t seed, uint32_t run_index, uint32_t replica, uint32_t m,
+                        uint32_t istep, int colour, double *u_phi, double *u_dir, double *u_spin)
+{
+    uint32_t ctr[4] = {m, istep, (run_index << 4) | ((uint32_t)colour << 1), replica};
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     uint32_t w[4];
     oracle_philox4x32_10(ctr, key, w);
-    *u0 = u53(w[1], w[0]);
-    *u1 = u53(w[3], w[2]);
+    *u_phi = (double)w[0] / 4294967296.0;
+    *u_dir = u48(w[1], w[2] >> 16);
+    *u_spin = u48(w[3], w[2] & 0xffffu);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -290,10 +298,11 @@ static int64_t evolve_dir(lat_t *L, int colour, usrc_t *U, uint64_t base_phi, ui
                 if (((i + j + k) & 1) != colour) continue;
                 int64_t l = lidx(L, i, j, k);
                 int64_t r = l >> 1;
-                double u_phi, u_acc;
-                if (U->mode == ORACLE_RNG_PHILOX)
-                    philox_site(U->seed, U->run_index, U->replica, (uint32_t)r + U->site_offset, istep, colour, 0, &u_phi, &u_acc);
-                else {
+                double u_phi, u_acc, unused;
+                if (U->mode == ORACLE_RNG_PHILOX) {
+                    philox_site(U->seed, U->run_index, U->replica, (uint32_t)r + U->site_offset, istep, colour,
+                                &u_phi, &u_acc, &unused);
+                } else {
                     u_phi = src_at(U, base_phi + (uint64_t)r);
                     u_acc = src_at(U, base_acc + (uint64_t)r);
                 }
@@ -322,9 +331,10 @@ static int64_t evolve_s(lat_t *L, int colour, usrc_t *U, uint64_t base_acc,
                 if (((i + j + k) & 1) != colour) continue;
                 int64_t l = lidx(L, i, j, k);
                 int64_t r = l >> 1;
-                double u_acc, unused;
+                double u_acc, unused0, unused1;
                 if (U->mode == ORACLE_RNG_PHILOX)
-                    philox_site(U->seed, U->run_index, U->replica, (uint32_t)r + U->site_offset, istep, colour, 1, &u_acc, &unused);
+                    philox_site(U->seed, U->run_index, U->replica, (uint32_t)r + U->site_offset, istep, colour,
+                                &unused0, &unused1, &u_acc);
                 else
                     u_acc = src_at(U, base_acc + (uint64_t)r);
                 double nnx = L->nx[l], nny = L->ny[l], nnz = L->nz[l];
