@@ -210,11 +210,14 @@ __host__ __device__ inline double stream_uniform(uint64_t seed, uint64_t n)
     return (n & 1) ? u53(w.w, w.z) : u53(w.y, w.x);
 }
 
-// Production site-keyed draws: one block per site, sweep and colour serves
-// both trials of the site.  Counter (m, istep, (run_index << 4) |
-// (colour << 1), replica), key (seed_lo, seed_hi).  Director trial: u_phi =
-// x 2^-32 (32 bits), u_acc = (y, z >> 16) as a 48-bit fraction; chirality
-// trial: u_acc = (w, z & 0xffff) as a 48-bit fraction.
+// Production site-keyed draws: one block per site, sweep and colour.  Counter
+// (m, istep, (run_index << 4) | (colour << 1), replica), key (seed_lo,
+// seed_hi).  Director trial of sweep t: block t, u_phi = x 2^-32 (32 bits),
+// u_acc = (y, z >> 16) as a 48-bit fraction.  Chirality trial of sweep t:
+// u_acc = (w, z & 0xffff) as a 48-bit fraction of block spin_sweep(colour, t)
+// -- block t for colour 1, block t + 1 for colour 0 -- so that each fused
+// launch (colour 1: both trials of sweep t; colour 0: the chirality trial of
+// sweep t and the director trial of sweep t + 1) evaluates one block per site.
 __host__ __device__ inline U4 site_block(uint64_t seed, uint32_t run_index, uint32_t replica, uint32_t m,
                                 uint32_t istep, int colour)
 {
@@ -227,6 +230,7 @@ __host__ __device__ inline double u48(uint32_t hi, uint32_t lo16)
 {
     return fma((double)hi, 0x1.0p-32, (double)lo16 * 0x1.0p-48);
 }
+__host__ __device__ inline uint32_t spin_sweep(int colour, uint32_t istep) { return colour ? istep : istep + 1u; }
 __host__ __device__ inline double site_u_phi(const U4 &w) { return (double)w.x * 0x1.0p-32; }
 __host__ __device__ inline double site_u_dir(const U4 &w) { return u48(w.y, w.z >> 16); }
 __host__ __device__ inline double site_u_spin(const U4 &w) { return u48(w.w, w.z & 0xffffu); }

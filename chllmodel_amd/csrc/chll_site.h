@@ -152,7 +152,7 @@ __device__ inline bool spin_compute(const PhaseArgs &A, int colour, uint32_t m, 
 {
     double u_acc;
     if (RNG == RNG_PHILOX) {
-        u_acc = site_u_spin(site_block(A.seed, A.run_index, A.replica, m, A.istep, colour));
+        u_acc = site_u_spin(site_block(A.seed, A.run_index, A.replica, m, spin_sweep(colour, A.istep), colour));
     } else {
         u_acc = A.u_acc[m];
     }
@@ -234,13 +234,13 @@ template <int RNG, bool OFF32>
 __device__ inline bool dir_spin_apply(const Fields &F, int colour, const PhaseArgs &A, uint32_t m, uint32_t own,
                                       const SiteIn &S, bool &flip)
 {
-    // one block serves the site's director and chirality trials
+    // for colour 1 (the fused launch) one block serves both trials
     double u_phi, u_acc, u_spin;
     if (RNG == RNG_PHILOX) {
         const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour);
         u_phi = site_u_phi(w);
         u_acc = site_u_dir(w);
-        u_spin = site_u_spin(w);
+        u_spin = site_u_spin(colour ? w : site_block(A.seed, A.run_index, A.replica, m, spin_sweep(0, A.istep), 0));
     } else {
         u_phi = A.u_phi[m];
         u_acc = A.u_acc[m];
@@ -282,13 +282,13 @@ __device__ inline bool spin_dir_apply(const Fields &F, const PhaseArgs &A, uint3
     // chirality trial leaves
     double c2o, cqb[6];
     const double cqo = cq_products(S.N, S.nx, S.ny, S.nz, c2o, cqb);
-    // the chirality trial draws from sweep A.istep_spin's block, the director
-    // trial from sweep A.istep's
-    const U4 ws = site_block(A.seed, A.run_index, A.replica, m, A.istep_spin, 0);
-    flip = spin_decide_fp64(A, cqo, S.sg, site_u_spin(ws));
+    // one block: the chirality trial of sweep A.istep_spin and the director
+    // trial of sweep A.istep = A.istep_spin + 1 both draw from block A.istep
+    // (spin_sweep); every caller sets A.istep that way
+    const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, 0);
+    flip = spin_decide_fp64(A, cqo, S.sg, site_u_spin(w));
     const int sg = flip ? -S.sg : S.sg;
     // director trial of sweep A.istep (dir_compute, FP64)
-    const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, 0);
     const double u_phi = site_u_phi(w), u_acc = site_u_dir(w);
     double t[3], c2n;
     const double ao = weight_cq(S.N, sg, cqb);

@@ -25,9 +25,9 @@
  *   ORACLE_RNG_BUFFER : an explicit caller-supplied array consumed in the same
  *                       schedule;
  *   ORACLE_RNG_PHILOX : the library's production, site-keyed Philox counters
- *                       (one counter block per site, sweep, colour and trial
- *                       kind), so the production GPU path can be compared with
- *                       the oracle draw for draw.
+ *                       (one counter block per site, sweep and colour; see
+ *                       philox_site), so the production GPU path can be
+ *                       compared with the oracle draw for draw.
  *
  * Indices: 0-based (i, j, k), Fortran (column-major) linear order
  * l = i + ni*(j + nj*k), as the reference's arrays nx(ni,nj,nk) etc.
@@ -107,12 +107,13 @@ void oracle_stream_fill(uint64_t seed, uint64_t n0, int64_t count, double *out)
     for (int64_t t = 0; t < count; ++t) out[t] = oracle_stream_uniform(seed, n0 + (uint64_t)t);
 }
 
-/* Production site-keyed draws: one block per site, sweep and colour serves both
- * trials of the site.  Counter (m, istep, (run_index << 4) | (colour << 1),
- * replica), key (seed_lo, seed_hi); m = packed index of the site within its
- * colour (l >> 1).  Director trial: u_phi = word 0 / 2^32, u_acc = word 1 and
- * the high half of word 2 as a 48-bit fraction; chirality trial: u_acc = word
- * 3 and the low half of word 2 as a 48-bit fraction. */
+/* Production site-keyed draws: one block per site, sweep and colour.  Counter
+ * (m, istep, (run_index << 4) | (colour << 1), replica), key (seed_lo, seed_hi);
+ * m = packed index of the site within its colour (l >> 1).  Director trial of
+ * sweep t: block t, u_phi = word 0 / 2^32, u_acc = word 1 and the high half of
+ * word 2 as a 48-bit fraction.  Chirality trial of sweep t: word 3 and the low
+ * half of word 2 as a 48-bit fraction, of block t for colour 1 and of block
+ * t + 1 for colour 0. */
 static inline double u48(uint32_t hi, uint32_t lo16)
 {
     return (double)hi / 4294967296.0 + (double)lo16 / 281474976710656.0;
@@ -333,8 +334,8 @@ static int64_t evolve_s(lat_t *L, int colour, usrc_t *U, uint64_t base_acc,
                 int64_t r = l >> 1;
                 double u_acc, unused0, unused1;
                 if (U->mode == ORACLE_RNG_PHILOX)
-                    philox_site(U->seed, U->run_index, U->replica, (uint32_t)r + U->site_offset, istep, colour,
-                                &unused0, &unused1, &u_acc);
+                    philox_site(U->seed, U->run_index, U->replica, (uint32_t)r + U->site_offset,
+                                colour ? istep : istep + 1u, colour, &unused0, &unused1, &u_acc);
                 else
                     u_acc = src_at(U, base_acc + (uint64_t)r);
                 double nnx = L->nx[l], nny = L->ny[l], nnz = L->nz[l];
