@@ -368,14 +368,15 @@ __device__ inline int lo_spin(uint32_t v) { return (int)(int8_t)(v & 0xffu); }
 __device__ inline int hi_spin(uint32_t v) { return (int)(int8_t)(v >> 8); }
 
 // Lane l - 1's (DPP wave_shr:1) and lane l + 1's (wave_shl:1) copy of a
-// value, in one VALU move per 32 bits; lane 0 / lane 63 get zero.
+// value, in one VALU move per 32 bits; lane 0 / lane 63 get zero (bound_ctrl),
+// which the callers never use, so no destination needs clearing first.
 __device__ inline uint32_t lane_prev(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x138, 0xf, 0xf, true);
 }
 __device__ inline uint32_t lane_next(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, true);
 }
 __device__ inline double lane_prev(double v)
 {
