@@ -139,13 +139,15 @@ __global__ __launch_bounds__(BS) void k_pair_ens(EnsDev E, uint32_t istep_off, i
     if (threadIdx.x < ent.y) {
         SiteIn S0, S1;
         const uint32_t own = load_pair<BC, true>(D.G, D.F, KIND == 2 ? 1 : 0, m0, S0, S1);
+        SiteOut O0, O1;
         if constexpr (KIND == 2) {
-            a0 = dir_spin_apply<RNG_PHILOX, true>(D.F, 1, A, m0, own, S0, f0);
-            a1 = dir_spin_apply<RNG_PHILOX, true>(D.F, 1, A, m0 + 1u, own + 1u, S1, f1);
+            a0 = dir_spin_apply<RNG_PHILOX, true, true>(D.F, 1, A, m0, own, S0, f0, &O0);
+            a1 = dir_spin_apply<RNG_PHILOX, true, true>(D.F, 1, A, m0 + 1u, own + 1u, S1, f1, &O1);
         } else {
-            a0 = spin_dir_apply<true>(D.F, A, m0, own, S0, f0);
-            a1 = spin_dir_apply<true>(D.F, A, m0 + 1u, own + 1u, S1, f1);
+            a0 = spin_dir_apply<true, true>(D.F, A, m0, own, S0, f0, &O0);
+            a1 = spin_dir_apply<true, true>(D.F, A, m0 + 1u, own + 1u, S1, f1, &O1);
         }
+        store_pair<true, true>(D.F, own, O0, O1);
     }
     const uint32_t wa = (uint32_t)(__popcll(__ballot(a0)) + __popcll(__ballot(a1)));
     const uint32_t wf = (uint32_t)(__popcll(__ballot(f0)) + __popcll(__ballot(f1)));

@@ -143,19 +143,22 @@ __global__ __launch_bounds__(kBlock) void k_pair(Geom G, Fields F, int colour, P
     if (m0 < G.nsub) {
         SiteIn S0, S1;
         const uint32_t own = load_pair<BC, true>(G, F, colour, m0, S0, S1);
+        // both sites' states go back as 16-byte words (store_pair)
+        SiteOut O0, O1;
         if constexpr (OP == 0) {
-            a0 = dir_apply<ARITH, RNG, true>(F, colour, A, m0, own, S0);
-            a1 = dir_apply<ARITH, RNG, true>(F, colour, A, m0 + 1u, own + 1u, S1);
+            a0 = dir_apply<ARITH, RNG, true, true>(F, colour, A, m0, own, S0, &O0);
+            a1 = dir_apply<ARITH, RNG, true, true>(F, colour, A, m0 + 1u, own + 1u, S1, &O1);
         } else if constexpr (OP == 1) {
-            a0 = spin_apply<ARITH, RNG, true>(F, colour, A, m0, own, S0);
-            a1 = spin_apply<ARITH, RNG, true>(F, colour, A, m0 + 1u, own + 1u, S1);
+            a0 = spin_apply<ARITH, RNG, true, true>(F, colour, A, m0, own, S0, &O0);
+            a1 = spin_apply<ARITH, RNG, true, true>(F, colour, A, m0 + 1u, own + 1u, S1, &O1);
         } else if constexpr (OP == 2) {
-            a0 = dir_spin_apply<RNG, true>(F, colour, A, m0, own, S0, f0);
-            a1 = dir_spin_apply<RNG, true>(F, colour, A, m0 + 1u, own + 1u, S1, f1);
+            a0 = dir_spin_apply<RNG, true, true>(F, colour, A, m0, own, S0, f0, &O0);
+            a1 = dir_spin_apply<RNG, true, true>(F, colour, A, m0 + 1u, own + 1u, S1, f1, &O1);
         } else {
-            a0 = spin_dir_apply<true>(F, A, m0, own, S0, f0);
-            a1 = spin_dir_apply<true>(F, A, m0 + 1u, own + 1u, S1, f1);
+            a0 = spin_dir_apply<true, true>(F, A, m0, own, S0, f0, &O0);
+            a1 = spin_dir_apply<true, true>(F, A, m0 + 1u, own + 1u, S1, f1, &O1);
         }
+        store_pair<OP != 1, OP != 0>(F, own, O0, O1);
     }
     const uint32_t wa = (uint32_t)(__popcll(__ballot(a0)) + __popcll(__ballot(a1)));
     const uint32_t wf = (uint32_t)(__popcll(__ballot(f0)) + __popcll(__ballot(f1)));

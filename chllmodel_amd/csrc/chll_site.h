@@ -183,30 +183,60 @@ __device__ inline uint32_t load_site(const Geom &G, const Fields &F, int colour,
 }
 
 // Trials on loaded inputs S of packed site m of `colour` whose field index is
-// `own`; accepted moves are stored.
-//
-// One director trial; returns whether it is accepted.
-template <int ARITH, int RNG, bool OFF32>
-__device__ inline bool dir_apply(const Fields &F, int colour, const PhaseArgs &A, uint32_t m, uint32_t own,
-                                 const SiteIn &S)
+// `own`.  Accepted moves are stored at once, or (DEFER) the site's director
+// and spin after its trials go to *o, for a caller that stores two sites with
+// 16-byte words (store_pair).
+struct SiteOut {
+    double nx, ny, nz;
+    int s;
+};
+
+template <bool OFF32, bool DEFER>
+__device__ inline void put_dir(const Fields &F, uint32_t own, bool acc, const double t[3], const SiteIn &S,
+                               SiteOut *o)
 {
-    double t[3];
-    const bool acc = dir_compute<ARITH, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg, *A.d, t);
-    if (acc) {
+    if constexpr (DEFER) {
+        o->nx = acc ? t[0] : S.nx;
+        o->ny = acc ? t[1] : S.ny;
+        o->nz = acc ? t[2] : S.nz;
+    } else if (acc) {
         fst<OFF32>(F.nx, own, t[0]);
         fst<OFF32>(F.ny, own, t[1]);
         fst<OFF32>(F.nz, own, t[2]);
     }
+}
+
+template <bool OFF32, bool DEFER>
+__device__ inline void put_spin(const Fields &F, uint32_t own, bool flip, int sg_new, SiteOut *o)
+{
+    if constexpr (DEFER) o->s = sg_new;
+    else if (flip) fst<OFF32>(F.s, own, (int8_t)sg_new);
+}
+
+// One director trial; returns whether it is accepted.
+template <int ARITH, int RNG, bool OFF32, bool DEFER = false>
+__device__ inline bool dir_apply(const Fields &F, int colour, const PhaseArgs &A, uint32_t m, uint32_t own,
+                                 const SiteIn &S, SiteOut *o = nullptr)
+{
+    double t[3];
+    const bool acc = dir_compute<ARITH, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg, *A.d, t);
+    put_dir<OFF32, DEFER>(F, own, acc, t, S, o);
+    if constexpr (DEFER) o->s = S.sg;
     return acc;
 }
 
 // One chirality trial; returns whether the spin flips.
-template <int ARITH, int RNG, bool OFF32>
+template <int ARITH, int RNG, bool OFF32, bool DEFER = false>
 __device__ inline bool spin_apply(const Fields &F, int colour, const PhaseArgs &A, uint32_t m, uint32_t own,
-                                  const SiteIn &S)
+                                  const SiteIn &S, SiteOut *o = nullptr)
 {
     const bool acc = spin_compute<ARITH, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg);
-    if (acc) fst<OFF32>(F.s, own, (int8_t)(-S.sg));
+    put_spin<OFF32, DEFER>(F, own, acc, acc ? -S.sg : S.sg, o);
+    if constexpr (DEFER) {
+        o->nx = S.nx;
+        o->ny = S.ny;
+        o->nz = S.nz;
+    }
     return acc;
 }
 
@@ -230,9 +260,9 @@ __device__ inline bool spin_decide_fp64(const PhaseArgs &A, double a, int sg, do
 // The director trial (dir_compute, FP64) then the chirality trial on the
 // accepted director (spin_compute, FP64); the bond passes of the director
 // trial also give the chirality sums of the old and the trial director.
-template <int RNG, bool OFF32>
+template <int RNG, bool OFF32, bool DEFER = false>
 __device__ inline bool dir_spin_apply(const Fields &F, int colour, const PhaseArgs &A, uint32_t m, uint32_t own,
-                                      const SiteIn &S, bool &flip)
+                                      const SiteIn &S, bool &flip, SiteOut *o = nullptr)
 {
     // for colour 1 (the fused launch) one block serves both trials
     double u_phi, u_acc, u_spin;
@@ -253,13 +283,9 @@ __device__ inline bool dir_spin_apply(const Fields &F, int colour, const PhaseAr
     const double an = cqf_cq_sum(S.N, S.sg, t[0], t[1], t[2], c2n, cqn);
     const double dE = (c2o - c2n) - A.KK * (an - ao);
     const bool acc = (dE < 0.0) || accept_exp(-dE * A.inv_kbt, u_acc);
-    if (acc) {
-        fst<OFF32>(F.nx, own, t[0]);
-        fst<OFF32>(F.ny, own, t[1]);
-        fst<OFF32>(F.nz, own, t[2]);
-    }
+    put_dir<OFF32, DEFER>(F, own, acc, t, S, o);
     flip = spin_decide_fp64(A, acc ? cqn : cqo, S.sg, u_spin);
-    if (flip) fst<OFF32>(F.s, own, (int8_t)(-S.sg));
+    put_spin<OFF32, DEFER>(F, own, flip, flip ? -S.sg : S.sg, o);
     return acc;
 }
 
@@ -273,9 +299,9 @@ __device__ inline bool dir_spin_apply(const Fields &F, int colour, const PhaseAr
 // pair therefore gives the sequential outcome; the cone step of sweep t + 1 is
 // fixed before this launch, since it depends on sweep t's director accepts
 // only (chll/chll_mod.f90:83-90).  Returns the director decision.
-template <bool OFF32>
+template <bool OFF32, bool DEFER = false>
 __device__ inline bool spin_dir_apply(const Fields &F, const PhaseArgs &A, uint32_t m, uint32_t own,
-                                      const SiteIn &S, bool &flip)
+                                      const SiteIn &S, bool &flip, SiteOut *o = nullptr)
 {
     // one pass over the old director's bonds: the chirality sum, then the
     // director's old sum from the same bond products with the spin the
@@ -297,12 +323,8 @@ __device__ inline bool spin_dir_apply(const Fields &F, const PhaseArgs &A, uint3
     const double an = cqf_sum(S.N, sg, t[0], t[1], t[2], c2n);
     const double dE = (c2o - c2n) - A.KK * (an - ao);
     const bool acc = (dE < 0.0) || accept_exp(-dE * A.inv_kbt, u_acc);
-    if (flip) fst<OFF32>(F.s, own, (int8_t)sg);
-    if (acc) {
-        fst<OFF32>(F.nx, own, t[0]);
-        fst<OFF32>(F.ny, own, t[1]);
-        fst<OFF32>(F.nz, own, t[2]);
-    }
+    put_spin<OFF32, DEFER>(F, own, flip, sg, o);
+    put_dir<OFF32, DEFER>(F, own, acc, t, S, o);
     return acc;
 }
 
@@ -456,6 +478,26 @@ __device__ inline uint32_t load_pair(const Geom &G, const Fields &F, int colour,
         S1.N.x[b] = vx.y; S1.N.y[b] = vy.y; S1.N.z[b] = vz.y; S1.N.s[b] = hi_spin(vs);
     }
     return own;
+}
+
+// Two sites' states after their trials, stored as 16-byte words (directors,
+// DIRS) and one 2-byte word (spins, SPINS) at the pair's field index own.  A
+// site of the active colour is read by no other thread of the launch, so
+// writing back an unchanged value is harmless; every cache line of the active
+// colour is written back anyway.
+template <bool DIRS, bool SPINS>
+__device__ inline void store_pair(const Fields &F, uint32_t own, const SiteOut &O0, const SiteOut &O1)
+{
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    typedef __attribute__((address_space(1))) d2v gd2v;
+    typedef __attribute__((address_space(1))) uint16_t gu16;
+    if constexpr (DIRS) {
+        *(gd2v *)((char *)F.nx + own * 8u) = d2v{O0.nx, O1.nx};
+        *(gd2v *)((char *)F.ny + own * 8u) = d2v{O0.ny, O1.ny};
+        *(gd2v *)((char *)F.nz + own * 8u) = d2v{O0.nz, O1.nz};
+    }
+    if constexpr (SPINS)
+        *(gu16 *)((char *)F.s + own) = (uint16_t)(((uint32_t)O0.s & 0xffu) | (((uint32_t)O1.s & 0xffu) << 8));
 }
 
 // ---------------------------------------------------------------------------
