@@ -151,6 +151,28 @@ __device__ inline uint32_t sweep_block(uint32_t band)
     return band_block(b, band & 0x7fffffffu);
 }
 
+// Lane l - 1's (DPP wave_shr:1) and lane l + 1's (wave_shl:1) copy of a
+// value, in one VALU move per 32 bits; lane 0 / lane 63 get zero (bound_ctrl),
+// which the callers never use, so no destination needs clearing first.
+__device__ inline uint32_t lane_prev(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x138, 0xf, 0xf, true);
+}
+__device__ inline uint32_t lane_next(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, true);
+}
+__device__ inline double lane_prev(double v)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    return __longlong_as_double((long long)(((uint64_t)lane_prev((uint32_t)(u >> 32)) << 32) | lane_prev((uint32_t)u)));
+}
+__device__ inline double lane_next(double v)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    return __longlong_as_double((long long)(((uint64_t)lane_next((uint32_t)(u >> 32)) << 32) | lane_next((uint32_t)u)));
+}
+
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
 struct U4 {

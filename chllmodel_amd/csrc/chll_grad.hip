@@ -200,28 +200,50 @@ __global__ __launch_bounds__(kGradBlock) void k_energy_grad2(GradArgs A)
         const uint32_t j = row - k * A.nj;
         const uint32_t plane = A.ni * A.nj;
         const double2 vx = ld2(nx + l), vy = ld2(ny + l), vz = ld2(nz + l);
-        int sa, sb;
-        spin_pair(s, l, sa, sb);
+        const uint32_t sw = *reinterpret_cast<const uint16_t *>(s + l);
+        const int sa = (int)(int8_t)(sw & 0xffu), sb = (int)(int8_t)(sw >> 8);
+        // Consecutive lanes hold consecutive pairs, so inside a row the site
+        // after the pair (l + 2) is the next lane's first site and the site
+        // before it (l - 1) the previous lane's second: they move between
+        // lanes by DPP, and only the edge lanes of a wave and periodic wraps
+        // load them.
+        const uint32_t lane = threadIdx.x & 63u;
+        const double nxx = lane_next(vx.x), nxy = lane_next(vy.x), nxz = lane_next(vz.x);
+        const uint32_t nsw = lane_next(sw);
+        const double pxx = lane_prev(vx.y), pxy = lane_prev(vy.y), pxz = lane_prev(vz.y);
+        const uint32_t psw = lane_prev(sw);
         double ga[3] = {0.0, 0.0, 0.0}, gb[3] = {0.0, 0.0, 0.0};
         // x: a -> b inside the pair, b -> the site after it, the site before a -> a
         bond<true, true, true>(0, vx.x, vy.x, vz.x, vx.y, vy.y, vz.y, 0.5 * (double)(sa + sb), m2eps, K, sum_e,
                                sum_cqf, ga, gb);
         {
-            const bool ok = BC == BC_PERIODIC || i + 2u < A.ni;
-            if (ok) {
-                const uint32_t q = i + 2u < A.ni ? l + 2u : l - i;
-                const double f = 0.5 * (double)(sb + s[q]);
-                bond<true, true, false>(0, vx.y, vy.y, vz.y, nx[q], ny[q], nz[q], f, m2eps, K, sum_e, sum_cqf, gb,
-                                        gb);
+            const bool in = i + 2u < A.ni;
+            if (in || BC == BC_PERIODIC) {
+                double bx, by, bz;
+                int sq;
+                if (in && lane != 63u) {
+                    bx = nxx; by = nxy; bz = nxz; sq = (int)(int8_t)(nsw & 0xffu);
+                } else {
+                    const uint32_t q = in ? l + 2u : l - i;
+                    bx = nx[q]; by = ny[q]; bz = nz[q]; sq = s[q];
+                }
+                bond<true, true, false>(0, vx.y, vy.y, vz.y, bx, by, bz, 0.5 * (double)(sb + sq), m2eps, K, sum_e,
+                                        sum_cqf, gb, gb);
             }
         }
         {
-            const bool ok = BC == BC_PERIODIC || i >= 1u;
-            if (ok) {
-                const uint32_t q = i >= 1u ? l - 1u : l + A.ni - 1u;
-                const double f = 0.5 * (double)(s[q] + sa);
-                bond<false, false, true>(0, nx[q], ny[q], nz[q], vx.x, vy.x, vz.x, f, m2eps, K, sum_e, sum_cqf, ga,
-                                         ga);
+            const bool in = i >= 1u;
+            if (in || BC == BC_PERIODIC) {
+                double bx, by, bz;
+                int sq;
+                if (in && lane != 0u) {
+                    bx = pxx; by = pxy; bz = pxz; sq = (int)(int8_t)(psw >> 8);
+                } else {
+                    const uint32_t q = in ? l - 1u : l + A.ni - 1u;
+                    bx = nx[q]; by = ny[q]; bz = nz[q]; sq = s[q];
+                }
+                bond<false, false, true>(0, bx, by, bz, vx.x, vy.x, vz.x, 0.5 * (double)(sq + sa), m2eps, K, sum_e,
+                                         sum_cqf, ga, ga);
             }
         }
         // y and z: the pair above (a, b lower ends) and the pair below (a, b upper ends)

@@ -389,28 +389,6 @@ __device__ inline uint32_t spin2(const int8_t *base, uint32_t q)
 __device__ inline int lo_spin(uint32_t v) { return (int)(int8_t)(v & 0xffu); }
 __device__ inline int hi_spin(uint32_t v) { return (int)(int8_t)(v >> 8); }
 
-// Lane l - 1's (DPP wave_shr:1) and lane l + 1's (wave_shl:1) copy of a
-// value, in one VALU move per 32 bits; lane 0 / lane 63 get zero (bound_ctrl),
-// which the callers never use, so no destination needs clearing first.
-__device__ inline uint32_t lane_prev(uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x138, 0xf, 0xf, true);
-}
-__device__ inline uint32_t lane_next(uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, true);
-}
-__device__ inline double lane_prev(double v)
-{
-    const uint64_t u = (uint64_t)__double_as_longlong(v);
-    return __longlong_as_double((long long)(((uint64_t)lane_prev((uint32_t)(u >> 32)) << 32) | lane_prev((uint32_t)u)));
-}
-__device__ inline double lane_next(double v)
-{
-    const uint64_t u = (uint64_t)__double_as_longlong(v);
-    return __longlong_as_double((long long)(((uint64_t)lane_next((uint32_t)(u >> 32)) << 32) | lane_next((uint32_t)u)));
-}
-
 // The kernels that call load_pair give consecutive lanes of a wave consecutive
 // pairs (m0 = 2 (tile base + lane)).  The extra x neighbour of a pair is then,
 // inside a row, the x pair that the lane beside it loaded: site q - 1 is lane
