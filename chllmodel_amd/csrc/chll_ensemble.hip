@@ -115,9 +115,10 @@ __global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int colour, uint32_t
     block_count<BS>(acc, D.counts + (kind == 0 ? colour : flip_group(A.istep, colour)) * gs, lb);
 }
 
-// FP64 fused kinds 2 and 3 (as k_phase_ens) with two sites per thread and
-// 16-byte pair loads (the single lattice's k_pair), blocks from blk_rep2.
-// Needs every replica to satisfy pairs_ok.
+// FP64 kinds of k_phase_ens with two sites per thread and 16-byte pair loads
+// and stores (the single lattice's k_pair), blocks from blk_rep2: the fused
+// kinds 2 and 3, and kinds 0 and 1 for colour 0 (a block's first director and
+// last chirality launch).  Needs every replica to satisfy pairs_ok.
 template <int KIND, int BC, int BS>
 __global__ __launch_bounds__(BS) void k_pair_ens(EnsDev E, uint32_t istep_off, int rev, uint32_t base)
 {
@@ -140,18 +141,28 @@ __global__ __launch_bounds__(BS) void k_pair_ens(EnsDev E, uint32_t istep_off, i
         SiteIn S0, S1;
         const uint32_t own = load_pair<BC, true>(D.G, D.F, KIND == 2 ? 1 : 0, m0, S0, S1);
         SiteOut O0, O1;
-        if constexpr (KIND == 2) {
+        if constexpr (KIND == 0) {
+            a0 = dir_apply<ARITH_FP64, RNG_PHILOX, true, true>(D.F, 0, A, m0, own, S0, &O0);
+            a1 = dir_apply<ARITH_FP64, RNG_PHILOX, true, true>(D.F, 0, A, m0 + 1u, own + 1u, S1, &O1);
+        } else if constexpr (KIND == 1) {
+            f0 = spin_apply<ARITH_FP64, RNG_PHILOX, true, true>(D.F, 0, A, m0, own, S0, &O0);
+            f1 = spin_apply<ARITH_FP64, RNG_PHILOX, true, true>(D.F, 0, A, m0 + 1u, own + 1u, S1, &O1);
+        } else if constexpr (KIND == 2) {
             a0 = dir_spin_apply<RNG_PHILOX, true, true>(D.F, 1, A, m0, own, S0, f0, &O0);
             a1 = dir_spin_apply<RNG_PHILOX, true, true>(D.F, 1, A, m0 + 1u, own + 1u, S1, f1, &O1);
         } else {
             a0 = spin_dir_apply<true, true>(D.F, A, m0, own, S0, f0, &O0);
             a1 = spin_dir_apply<true, true>(D.F, A, m0 + 1u, own + 1u, S1, f1, &O1);
         }
-        store_pair<true, true>(D.F, own, O0, O1);
+        store_pair<KIND != 1, KIND != 0>(D.F, own, O0, O1);
     }
     const uint32_t wa = (uint32_t)(__popcll(__ballot(a0)) + __popcll(__ballot(a1)));
     const uint32_t wf = (uint32_t)(__popcll(__ballot(f0)) + __popcll(__ballot(f1)));
-    if constexpr (KIND == 2)
+    if constexpr (KIND == 0)
+        block_sum2<BS>(wa, 0u, D.counts + kGroupAcc0 * gs, D.counts + kGroupAcc0 * gs, lb);
+    else if constexpr (KIND == 1)
+        block_sum2<BS>(0u, wf, D.counts + flip_group(A.istep, 0) * gs, D.counts + flip_group(A.istep, 0) * gs, lb);
+    else if constexpr (KIND == 2)
         block_sum2<BS>(wa, wf, D.counts + kGroupAcc1 * gs, D.counts + flip_group(A.istep, 1) * gs, lb);
     else
         block_sum2<BS>(wa, wf, D.counts + kGroupAcc0 * gs, D.counts + flip_group(A.istep_spin, 0) * gs, lb);
@@ -519,7 +530,10 @@ void launch_block_t(chll_ensemble *X, int K, hipStream_t st)
                 hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, last ? 1 : 0, 0, 1, c.f);
                 continue;
             }
-            if (t == 0) hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, r1);
+            if (t == 0) {
+                if (pairs) hipLaunchKernelGGL((k_pair_ens<0, BC, BS>), g2, b, 0, st, E, off, 1, r2);
+                else hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, r1);
+            }
             if (pairs) hipLaunchKernelGGL((k_pair_ens<2, BC, BS>), g2, b, 0, st, E, off, 0, c.t2);
             else hipLaunchKernelGGL((k_phase_ens<2, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 0, c.t1);
             if (!last) {
@@ -527,7 +541,8 @@ void launch_block_t(chll_ensemble *X, int K, hipStream_t st)
                 if (pairs) hipLaunchKernelGGL((k_pair_ens<3, BC, BS>), g2, b, 0, st, E, off, 1, r2);
                 else hipLaunchKernelGGL((k_phase_ens<3, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, r1);
             } else {
-                hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, r1);
+                if (pairs) hipLaunchKernelGGL((k_pair_ens<1, BC, BS>), g2, b, 0, st, E, off, 1, r2);
+                else hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, r1);
                 hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E, c.t1);
                 hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 1, t > 0 ? 1 : 0, 1, c.f);
             }
