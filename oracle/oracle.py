@@ -144,7 +144,7 @@ class Oracle:
 
         Returns (msteps, m, d_out, stream_pos_out[, counts])."""
         ni, nj, nk = st["nx"].shape
-        rows = (1 + nstop - nstart) // nout
+        rows = max((1 + nstop - nstart) // nout, 0)
         msteps = np.zeros(max(rows, 1), dtype=np.int32)
         m = np.zeros((max(rows, 1), 5), dtype=np.float64, order="F")
         dd = ct.c_double(d)
@@ -208,7 +208,7 @@ class Reference:
         ni, nj, nk = st["nx"].shape
         nsub = (ni * nj * nk) // 2
         sl1, sl2 = self.init(st)
-        rows = (1 + nstop - nstart) // nout
+        rows = max((1 + nstop - nstart) // nout, 0)
         msteps = np.zeros(max(rows, 1), dtype=np.int32)
         m = np.zeros((max(rows, 1), 5), dtype=np.float64, order="F")
         if self.stream:

@@ -167,9 +167,9 @@ def test_upload_rejects_wide_spins():
 def _dropin_run(st, nstart, nstop, nout, g, KK, d, kbt):
     ni, nj, nk = st["nx"].shape
     nsub = ni * nj * nk // 2
-    rows = (1 + nstop - nstart) // nout
-    msteps = np.zeros(rows, dtype=np.int32)
-    m = np.zeros((rows, 5), dtype=np.float64, order="F")
+    rows = max(int((1 + nstop - nstart) / nout), 0)  # Fortran integer division (chll/chll_mod.f90:47)
+    msteps = np.zeros(max(rows, 1), dtype=np.int32)
+    m = np.zeros((max(rows, 1), 5), dtype=np.float64, order="F")
     sl1 = np.zeros(nsub, np.int32)
     sl2 = np.zeros(nsub, np.int32)
     dd = ct.c_double(d)
@@ -178,7 +178,38 @@ def _dropin_run(st, nstart, nstop, nout, g, KK, d, kbt):
                    st["dope"], sl1, sl2, ct.c_double(g), ct.c_double(KK), dd, ct.c_double(kbt),
                    ct.c_int(ni), ct.c_int(nj), ct.c_int(nk), ns, msteps, m)
     assert _lib.last_error() == ""
-    return msteps, m, dd.value, ns.value
+    return msteps[:rows], m[:rows], dd.value, ns.value
+
+
+@pytest.mark.parametrize("dims,nstart,nstop,nout", [((6, 4, 5), 1, 7, 20), ((6, 4, 5), 5, 2, 4), ((2, 1, 1), 1, 6, 2),
+                                                    ((2, 1, 3), 1, 9, 3), ((4, 3, 1), 3, 11, 4)])
+def test_dropin_run_edge_cases(orc, dims, nstart, nstop, nout):
+    """Calls the reference's interface allows: nout longer than the run (the
+    sweeps run, no row is written), an empty range nstop < nstart (nothing
+    changes), and the smallest lattices (2x1x1: one site of each colour; flat
+    4x3x1), in the deterministic stream mode against the oracle."""
+    L = _lib.lib()
+    st0 = isotropic_state(*dims, 77 + sum(dims))
+    a, b = copy_state(st0), copy_state(st0)
+    _lib.check(L.chll_configure(1, 0, 0, 0))
+    _lib.check(L.chll_set_stream(91, 0))
+    try:
+        ms, m, d_out, ns = _dropin_run(a, nstart, nstop, nout, 0.5, 1.0, 0.3, 0.7)
+        oms, om, od, _ = orc.run(b, nstart, nstop, nout, 0.5, 1.0, 0.3, 0.7, rng=RNG_STREAM, seed=91, stream_pos=0)
+    finally:
+        _lib.check(L.chll_configure(0, 0, 0, 1))
+    assert ns == np.prod(dims) // 2
+    assert np.array_equal(ms, oms) and d_out == od
+    assert np.array_equal(m[:, 1:], om[:, 1:])
+    for r in range(m.shape[0]):
+        assert abs(m[r, 0] - om[r, 0]) <= np.spacing(np.float32(abs(om[r, 0])))
+    assert np.array_equal(a["s"], b["s"])
+    for f in ("nx", "ny", "nz"):
+        assert np.abs(a[f] - b[f]).max() <= DIR_TOL, f
+    if nstop < nstart:
+        for f in ("nx", "ny", "nz", "s"):
+            assert np.array_equal(a[f], st0[f]), f
+        assert d_out == 0.3
 
 
 def test_dropin_run_stream_mode(orc):

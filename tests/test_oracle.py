@@ -226,6 +226,26 @@ def test_oracle_matches_reference_live(orc):
             assert np.array_equal(a[f], b[f])
 
 
+@pytest.mark.skipif(not os.path.exists(REF_STREAM_SO), reason="reference build absent (needs the reference source)")
+@pytest.mark.parametrize("dims,nstart,nstop,nout", [((6, 4, 5), 1, 7, 20), ((6, 4, 5), 5, 2, 4), ((2, 1, 1), 1, 6, 2),
+                                                    ((2, 1, 3), 1, 9, 3), ((4, 3, 1), 3, 11, 4)])
+def test_oracle_matches_reference_live_edge_cases(orc, dims, nstart, nstop, nout):
+    """Calls at the edges of the reference's interface: nout longer than the
+    run (sweeps, no rows), nstop < nstart (no sweep), one site per colour
+    (2x1x1) and flat lattices, bit for bit against the reference build."""
+    ref = Reference(stream=True)
+    st0 = isotropic_state(*dims, 77 + sum(dims))
+    a, b = copy_state(st0), copy_state(st0)
+    r1 = orc.run(a, nstart, nstop, nout, 0.5, 1.0, 0.3, 0.7, seed=91)
+    r2 = ref.run(b, nstart, nstop, nout, 0.5, 1.0, 0.3, 0.7, seed=91)
+    assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
+    assert r1[2] == r2[2] and r1[3] == r2[3]
+    for f in ("nx", "ny", "nz", "s"):
+        assert np.array_equal(a[f], b[f])
+    if nstop < nstart:
+        assert r1[3] == 0 and r1[2] == 0.3
+
+
 # --------------------------------------------------------------------------
 def test_zero_step_accepts_everything(orc):
     st = isotropic_state(4, 4, 4, 8)
