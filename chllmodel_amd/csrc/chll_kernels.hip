@@ -197,18 +197,31 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs A)
     const uint32_t t = threadIdx.x;
     unsigned long long ca = 0, cf = 0, cp = 0;
     const int gt = flip_group(A.istep, 0), gp = flip_group(A.istep - 1, 0);
-    for (uint32_t q = t; q < kGroups * kSlots; q += kFinBlock) {
+    // Every slot a thread owns is read before any is cleared, so its loads
+    // are in flight together (a clear between them would order them).
+    constexpr int kPer = (kGroups * kSlots + kFinBlock - 1) / kFinBlock;
+    uint32_t *slot[kPer];
+    uint32_t cv[kPer];
+    int use[kPer];  // 0 none, 1 accepts, 2 this sweep's flips, 3 the previous sweep's
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const uint32_t q = t + (uint32_t)r * kFinBlock;
         const int g = (int)(q / kSlots);
-        const bool ia = g < 2;
-        const bool it = A.close_this && (g == gt || g == gt + 1);
-        const bool ip = A.close_prev && (g == gp || g == gp + 1);
-        if (!(ia || it || ip)) continue;
-        uint32_t *slot = A.counts + g * (kSlots * kSlotStride) + (q % kSlots) * kSlotStride;
-        const uint32_t c = *slot;
-        *slot = 0u;
-        if (ia) ca += c;
-        else if (it) cf += c;
-        else cp += c;
+        use[r] = q >= (uint32_t)(kGroups * kSlots) ? 0
+                 : g < 2                                                  ? 1
+                 : A.close_this && (g == gt || g == gt + 1)               ? 2
+                 : A.close_prev && (g == gp || g == gp + 1)               ? 3
+                                                                          : 0;
+        slot[r] = use[r] ? A.counts + g * (kSlots * kSlotStride) + (q % kSlots) * kSlotStride : A.counts;
+        cv[r] = use[r] ? *slot[r] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        if (!use[r]) continue;
+        *slot[r] = 0u;
+        if (use[r] == 1) ca += cv[r];
+        else if (use[r] == 2) cf += cv[r];
+        else cp += cv[r];
     }
     double v[8];
 #pragma unroll
@@ -224,12 +237,17 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs A)
     double cad = (double)ca, cfd = (double)cf, cpd = (double)cp, ssd = (double)ss;  // exact: < 2^53
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] += __shfl_xor(v[q], off, 64);
         cad += __shfl_xor(cad, off, 64);
         cfd += __shfl_xor(cfd, off, 64);
         cpd += __shfl_xor(cpd, off, 64);
-        ssd += __shfl_xor(ssd, off, 64);
+    }
+    if (A.measure) {  // uniform: sweeps without a row skip the observable shuffles
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] += __shfl_xor(v[q], off, 64);
+            ssd += __shfl_xor(ssd, off, 64);
+        }
     }
     const uint32_t lane = t & 63u, wid = t >> 6;
     if (lane == 0) {
