@@ -184,6 +184,23 @@ __global__ __launch_bounds__(kBlock) void k_measure(Geom G, Fields F, double g, 
     block_reduce_measure<kBlock>(acc, part + (size_t)blockIdx.x * 8, spart + blockIdx.x);
 }
 
+// The same observables with four sites of one row per thread (measure_pair);
+// chunk is a multiple of 2 kBlock, so lane l + 1 holds the pair after lane
+// l's in every pass.  Needs an even half and fields below 4 GiB.
+template <int BC>
+__global__ __launch_bounds__(kBlock) void k_measure_pair(Geom G, Fields F, double g, double KK,
+                                                         double *__restrict__ part, long long *__restrict__ spart,
+                                                         uint32_t chunk)
+{
+    MeasureAcc acc;
+    const uint64_t beg = (uint64_t)blockIdx.x * chunk;
+    const uint32_t end = (uint32_t)(beg + chunk < G.nsub ? beg + chunk : G.nsub);
+    const bool last_lane = (threadIdx.x & 63u) == 63u;
+    for (uint64_t m0 = beg + 2u * threadIdx.x; m0 < end; m0 += 2u * kBlock)
+        measure_pair<BC>(G, F, g, KK, (uint32_t)m0, !last_lane && m0 + 2u < end, acc);
+    block_reduce_measure<kBlock>(acc, part + (size_t)blockIdx.x * 8, spart + blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Sweep epilogue (one block): accept/flip totals, step adaptation, measurement
 // row.  Row layout (kRowCols doubles): istep, E (reference etot), eta,
@@ -499,6 +516,15 @@ void launch_measure(int bc, const Geom &G, const Fields &F, double g, double KK,
     // block a contiguous range; trailing blocks may get an empty one
     const uint32_t nb = measure_blocks(G);
     uint32_t chunk = (G.nsub + nb - 1) / nb;
+    if (pairs_ok(G)) {
+        chunk = (chunk + 2 * kBlock - 1) / (2 * kBlock) * (2 * kBlock);
+        if (bc == BC_OPEN)
+            hipLaunchKernelGGL(k_measure_pair<BC_OPEN>, dim3(nb), dim3(kBlock), 0, s, G, F, g, KK, part, spart, chunk);
+        else
+            hipLaunchKernelGGL(k_measure_pair<BC_PERIODIC>, dim3(nb), dim3(kBlock), 0, s, G, F, g, KK, part, spart,
+                               chunk);
+        return;
+    }
     chunk = (chunk + kBlock - 1) / kBlock * kBlock;
     if (bc == BC_OPEN)
         hipLaunchKernelGGL(k_measure<BC_OPEN>, dim3(nb), dim3(kBlock), 0, s, G, F, g, KK, part, spart, chunk);

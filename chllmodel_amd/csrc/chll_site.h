@@ -568,6 +568,104 @@ __device__ inline void measure_site(const Geom &G, const Fields &F, double g, do
     }
 }
 
+// The four sites i = 2 h .. 2 h + 3 of packed indices m, m + 1 of one row, in
+// x order, for fields whose byte offsets fit 32 bits: site u has colour
+// cb ^ (u & 1) (cb = (j + k) & 1 of the row) and packed index m + (u >> 1).
+// Three 16-byte loads and one 2-byte load per colour; the colour is chosen by
+// address, so every register index is fixed at compile time.
+struct Row4 {
+    double x[4], y[4], z[4];
+    int s[4];
+};
+
+__device__ inline void load_row4(const Fields &F, uint32_t nsub, uint32_t m, uint32_t cb, Row4 &R)
+{
+    const uint32_t qa = cb * nsub + m, qb = (cb ^ 1u) * nsub + m;
+    const double2 ax = fld2<true>(F.nx, qa), ay = fld2<true>(F.ny, qa), az = fld2<true>(F.nz, qa);
+    const double2 bx = fld2<true>(F.nx, qb), by = fld2<true>(F.ny, qb), bz = fld2<true>(F.nz, qb);
+    const uint32_t sa = spin2<true>(F.s, qa), sb = spin2<true>(F.s, qb);
+    R.x[0] = ax.x; R.x[1] = bx.x; R.x[2] = ax.y; R.x[3] = bx.y;
+    R.y[0] = ay.x; R.y[1] = by.x; R.y[2] = ay.y; R.y[3] = by.y;
+    R.z[0] = az.x; R.z[1] = bz.x; R.z[2] = az.y; R.z[3] = bz.y;
+    R.s[0] = lo_spin(sa); R.s[1] = lo_spin(sb); R.s[2] = hi_spin(sa); R.s[3] = hi_spin(sb);
+}
+
+// The observables of the four sites of packed indices m0, m0 + 1 (m0 even;
+// with half even both lie in one row).  The thread reads its own row, the +y
+// row and the +z plane's row (Row4 each).  The +x neighbour of site 3 and the
+// nx of its diagonal site (i + 1, j + 1) are site 0 of the next pair's own and
+// +y rows: taken from lane + 1 by DPP when nxt (that lane holds the next pair
+// of this row in this pass), else read; past the row end i wraps to 0.  Same
+// bond terms as measure_site, summed per site in x order.
+template <int BC>
+__device__ inline void measure_pair(const Geom &G, const Fields &F, double g, double KK, uint32_t m0, bool nxt,
+                                    MeasureAcc &A)
+{
+    const uint32_t row = fdiv(m0, G.fd_half);
+    const uint32_t h0 = m0 - row * G.half;
+    const uint32_t k = fdiv(row, G.fd_nj);
+    const uint32_t j = row - k * G.nj;
+    const uint32_t jp = (j + 1 == G.nj) ? 0u : j + 1;
+    const bool zin = k + 1 < G.nk;
+    const bool hz = zin || BC == BC_PERIODIC;
+    const uint32_t kp = zin ? k + 1 : 0u;
+    const uint32_t my = m0 - j * G.half + jp * G.half;
+    const uint32_t cb = (j + k) & 1u, cy = (jp + k) & 1u, cz = (j + kp) & 1u;
+    Row4 O, Y, Z;
+    load_row4(F, G.nsub, m0, cb, O);
+    load_row4(F, G.nsub, my, cy, Y);
+    if (hz) load_row4(F, G.nsub, m0 - k * G.plane + kp * G.plane, cz, Z);
+    double ex = lane_next(O.x[0]), ey = lane_next(O.y[0]), ez = lane_next(O.z[0]);
+    int es = (int)lane_next((uint32_t)O.s[0]);
+    double edx = lane_next(Y.x[0]);
+    const bool wrap = h0 + 2u == G.half;
+    if (!nxt || wrap) {
+        const uint32_t mx = wrap ? m0 - h0 : m0 + 2u, mxy = wrap ? my - h0 : my + 2u;
+        const uint32_t q = cb * G.nsub + mx;
+        ex = F.nx[q]; ey = F.ny[q]; ez = F.nz[q]; es = F.s[q];
+        edx = F.nx[cy * G.nsub + mxy];
+    }
+    const bool hy = (BC == BC_PERIODIC) || (j + 1 < G.nj);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const double ax = O.x[u], ay = O.y[u], az = O.z[u];
+        const int sa = O.s[u];
+        const double bx0 = u < 3 ? O.x[u + 1 < 4 ? u + 1 : 3] : ex;
+        const double by0 = u < 3 ? O.y[u + 1 < 4 ? u + 1 : 3] : ey;
+        const double bz0 = u < 3 ? O.z[u + 1 < 4 ? u + 1 : 3] : ez;
+        const int sx = u < 3 ? O.s[u + 1 < 4 ? u + 1 : 3] : es;
+        const double dgx = u < 3 ? Y.x[u + 1 < 4 ? u + 1 : 3] : edx;
+        const double bx1 = Y.x[u], by1 = Y.y[u], bz1 = Y.z[u];
+        const int sy = Y.s[u];
+        const double tx = bond_term(ax, ay, az, bx0, by0, bz0, 0, sa, sx, KK);
+        // the reference's +y bond: its dot product takes nx of the diagonal site
+        double ety;
+        {
+            const double dd = ax * dgx + ay * by1 + az * bz1;
+            const double cr = az * bx1 - ax * bz1;
+            const double sf = 0.5 * (double)(sa + sy);
+            ety = (1.0 - dd * dd) - KK * dd * cr * sf;
+        }
+        const double ty = bond_term(ax, ay, az, bx1, by1, bz1, 1, sa, sy, KK);
+        double tz = 0.0, tzp = 0.0;
+        if (hz) {
+            tzp = bond_term(ax, ay, az, Z.x[u], Z.y[u], Z.z[u], 2, sa, Z.s[u], KK);
+            if (zin) tz = tzp;
+        }
+        const double gs = g * (double)sa;
+        A.v[0] += ((tx + ety) + tz) - gs;
+        const bool hx = (BC == BC_PERIODIC) || u < 3 || !wrap;
+        A.v[1] += ((hx ? tx : 0.0) + (hy ? ty : 0.0) + tzp) - gs;
+        A.v[2] += ax * ax;
+        A.v[3] += ay * ay;
+        A.v[4] += az * az;
+        A.v[5] += ax * ay;
+        A.v[6] += ax * az;
+        A.v[7] += ay * az;
+        A.s += sa;
+    }
+}
+
 // Deterministic block reduction of MeasureAcc: wave butterflies, then a
 // fixed-order pass over the waves; writes 8 doubles to part and one int to spart.
 template <int BS>
