@@ -181,6 +181,25 @@ __global__ __launch_bounds__(BS) void k_measure_ens(EnsDev E, uint32_t base)
     block_reduce_measure<BS>(acc, E.part + (size_t)gb * 8, E.spart + gb);
 }
 
+// The same blocks and partials with four sites of one row per thread
+// (measure_pair): the first BS / 2 threads of a block take its BS packed
+// indices as pairs.  Every replica has an even half and fields below 4 GiB.
+template <int BC, int BS>
+__global__ __launch_bounds__(BS) void k_measure_ens_pair(EnsDev E, uint32_t base)
+{
+    const uint32_t gb = base + blockIdx.x;
+    const uint32_t code = E.blk_rep[gb];
+    const uint32_t r = code >> 24, lb = code & 0xFFFFFFu;
+    const RepDesc &D = E.R[r];
+    MeasureAcc acc;
+    const uint32_t beg = lb * BS;
+    const uint32_t end = beg + BS < D.G.nsub ? beg + BS : D.G.nsub;
+    const uint32_t m0 = beg + 2u * threadIdx.x;
+    if (threadIdx.x < BS / 2 && m0 < end)
+        measure_pair<BC>(D.G, D.F, D.g, D.KK, m0, (threadIdx.x & 63u) != 63u && m0 + 2u < end, acc);
+    block_reduce_measure<BS>(acc, E.part + (size_t)gb * 8, E.spart + gb);
+}
+
 // Per-replica epilogue (one block per replica): accept/flip totals, step
 // adaptation of the replica's temperature slot (chll/chll_mod.f90:83-90), and
 // on measuring sweeps the observable row and the replica's energy for the swap.
@@ -514,6 +533,12 @@ void launch_block_t(chll_ensemble *X, int K, hipStream_t st)
     const dim3 b(BS), bf(kEnsFin);
     const bool fuse = ARITH == ARITH_FP64 && X->fuse != 0;
     const bool pairs = fuse && X->fuse == 1 && X->pairs;
+    // observables: four sites per thread wherever every replica allows site
+    // pairs, whatever the fusion setting, so rows do not depend on it
+    auto measure = [&](const Chunk &c) {
+        if (X->pairs) hipLaunchKernelGGL((k_measure_ens_pair<BC, BS>), dim3(c.n1), dim3(BS), 0, st, E, c.t1);
+        else hipLaunchKernelGGL((k_measure_ens<BC, BS>), dim3(c.n1), dim3(BS), 0, st, E, c.t1);
+    };
     for (const Chunk &c : X->chunks) {
         const dim3 g(c.n1), g2(c.n2), gf(c.nf);
         const uint32_t r1 = c.t1 + c.n1 - 1u, r2 = c.t2 + c.n2 - 1u;  // last entries, for reversed walks
@@ -526,7 +551,7 @@ void launch_block_t(chll_ensemble *X, int K, hipStream_t st)
                 hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 1, r1);
                 hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 0, c.t1);
                 hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 1, r1);
-                if (last) hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E, c.t1);
+                if (last) measure(c);
                 hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, last ? 1 : 0, 0, 1, c.f);
                 continue;
             }
@@ -543,7 +568,7 @@ void launch_block_t(chll_ensemble *X, int K, hipStream_t st)
             } else {
                 if (pairs) hipLaunchKernelGGL((k_pair_ens<1, BC, BS>), g2, b, 0, st, E, off, 1, r2);
                 else hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, r1);
-                hipLaunchKernelGGL((k_measure_ens<BC, BS>), g, b, 0, st, E, c.t1);
+                measure(c);
                 hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 1, t > 0 ? 1 : 0, 1, c.f);
             }
         }
