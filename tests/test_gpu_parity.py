@@ -125,11 +125,21 @@ def test_matches_oracle(orc, dims, bc, rng, arith, nsw, nout):
         assert out["stream_pos"] == pos
 
 
-def test_measure_row(orc):
-    dims = (12, 10, 8)
+# Observable shapes: half even (four sites per thread) with pairs wrapping in
+# x at every row end (ni = 4), odd nj and nk, rows longer than a wave (ni =
+# 260: 65 pairs per row), single rows and planes, and measure blocks capped so
+# that block ranges split rows (132 x 100 x 90); half odd (one site pair per
+# thread): 10 x 6 x 4.
+MEASURE_SHAPES = [(12, 10, 8), (4, 3, 5), (260, 3, 2), (8, 1, 1), (10, 6, 4), (132, 100, 90)]
+
+
+@pytest.mark.parametrize("dims", MEASURE_SHAPES, ids=["x".join(map(str, d)) for d in MEASURE_SHAPES])
+def test_measure_row(orc, dims):
     st = isotropic_state(*dims, 77)
     g, K = 0.35, 1.7
     for bc, obc in (("open", BC_OPEN), ("periodic", BC_PERIODIC)):
+        if bc == "periodic" and (dims[1] % 2 or dims[2] % 2):
+            continue  # periodic boundaries need even nj and nk
         with Lattice(*dims, bc=bc) as lat:
             lat.upload_state(st)
             row = lat.measure(g, K)
