@@ -168,6 +168,36 @@ def test_pair_kernels_equal_one_site(K, layout):
             assert np.array_equal(ta[k], tb[k]), k
 
 
+def test_pair_measure_rows_match_oracle():
+    """Replicas whose half (ni / 2) is even all measure with four sites of one
+    row per thread (k_measure_ens_pair): x wraps at every pair (ni = 4), odd
+    nj and nk, rows longer than a wave (ni = 260).  After a block of sweeps,
+    every replica's row against the oracle's observables of its final state:
+    E_true / N within 1e-12, eta exact, the Q means within 1e-14, the
+    reference's binary32 E within one binary32 ulp."""
+    orc = Oracle()
+    reps = [{"ni": 4, "nj": 3, "nk": 5, "ladder": 0, "g": 0.5, "KK": 1.0, "kbt": 0.7, "d0": 0.3},
+            {"ni": 8, "nj": 6, "nk": 4, "ladder": 1, "g": 0.5, "KK": 1.0, "kbt": 0.8, "d0": 0.3},
+            {"ni": 12, "nj": 10, "nk": 8, "ladder": 2, "g": 1.0, "KK": 0.7, "kbt": 0.9, "d0": 0.2},
+            {"ni": 260, "nj": 3, "nk": 2, "ladder": 3, "g": 0.25, "KK": 1.5, "kbt": 0.6, "d0": 0.25}]
+    with Ensemble(reps, bc="open", arith="fp64") as ens:
+        for r, rep in enumerate(reps):
+            ens.upload(r, isotropic_state(rep["ni"], rep["nj"], rep["nk"], 900 + r))
+        rows = ens.run(1, 3, seed=31)
+        for r, rep in enumerate(reps):
+            st = ens.download(r)
+            N = rep["ni"] * rep["nj"] * rep["nk"]
+            row = rows[0, r]
+            assert abs(row[2] - orc.energy_true(st, rep["g"], rep["KK"]) / N) < 1e-12, r
+            assert row[3] == st["s"].sum() / N, r
+            Q = [st["nx"] ** 2, st["ny"] ** 2, st["nz"] ** 2, st["nx"] * st["ny"], st["nx"] * st["nz"],
+                 st["ny"] * st["nz"]]
+            for c, q in enumerate(Q):
+                assert abs(row[4 + c] - q.mean()) < 1e-14, (r, c)
+            e = orc.etot(st, rep["g"], rep["KK"])
+            assert abs(row[14] - e) <= np.spacing(np.float32(abs(e))), r
+
+
 @pytest.mark.parametrize("arith,fuse", [("fp64", True), ("fp64", False), ("compat", False)])
 def test_chunks_equal_one_launch(arith, fuse):
     """A block of sweeps run chunk by chunk (a few replicas at a time, all K
