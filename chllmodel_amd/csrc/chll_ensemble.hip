@@ -217,17 +217,30 @@ __global__ __launch_bounds__(kEnsFin) void k_finalize_ens(EnsDev E, uint32_t ist
     const uint32_t istep = *E.istep + istep_off;
     const int gt = flip_group((int32_t)istep, 0), gp = flip_group((int32_t)istep - 1, 0);
     unsigned long long ca = 0, cf = 0;
-    for (uint32_t q = t; q < kGroups * kSlots; q += kEnsFin) {
+    // every slot a thread owns is read before any is cleared, so its loads
+    // are in flight together
+    constexpr int kPer = (kGroups * kSlots + kEnsFin - 1) / kEnsFin;
+    uint32_t *slot[kPer];
+    uint32_t cv[kPer];
+    int use[kPer];  // 0 none, 1 accepts, 2 this sweep's flips, 3 the previous sweep's (cleared only)
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const uint32_t q = t + (uint32_t)i * kEnsFin;
         const int g = (int)(q / kSlots);
-        const bool ia = g < 2;
-        const bool it = close_this && (g == gt || g == gt + 1);
-        const bool ip = close_prev && (g == gp || g == gp + 1);
-        if (!(ia || it || ip)) continue;
-        uint32_t *slot = D.counts + g * (kSlots * kSlotStride) + (q % kSlots) * kSlotStride;
-        const uint32_t c = *slot;
-        *slot = 0u;
-        if (ia) ca += c;
-        else if (it) cf += c;
+        use[i] = q >= (uint32_t)(kGroups * kSlots) ? 0
+                 : g < 2                                                ? 1
+                 : close_this && (g == gt || g == gt + 1)               ? 2
+                 : close_prev && (g == gp || g == gp + 1)               ? 3
+                                                                        : 0;
+        slot[i] = use[i] ? D.counts + g * (kSlots * kSlotStride) + (q % kSlots) * kSlotStride : D.counts;
+        cv[i] = use[i] ? *slot[i] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        if (!use[i]) continue;
+        *slot[i] = 0u;
+        if (use[i] == 1) ca += cv[i];
+        else if (use[i] == 2) cf += cv[i];
     }
     double v[8];
 #pragma unroll
@@ -244,11 +257,16 @@ __global__ __launch_bounds__(kEnsFin) void k_finalize_ens(EnsDev E, uint32_t ist
     double cad = (double)ca, cfd = (double)cf, ssd = (double)ss;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] += __shfl_xor(v[q], off, 64);
         cad += __shfl_xor(cad, off, 64);
         cfd += __shfl_xor(cfd, off, 64);
-        ssd += __shfl_xor(ssd, off, 64);
+    }
+    if (measure) {  // uniform: sweeps without a row skip the observable shuffles
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] += __shfl_xor(v[q], off, 64);
+            ssd += __shfl_xor(ssd, off, 64);
+        }
     }
     __shared__ double red[kEnsFin / 64][11];
     const uint32_t lane = t & 63u, wid = t >> 6;
