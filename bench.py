@@ -19,8 +19,9 @@ one with the largest total time: the director half-sweep, or with FP64 the
 fused colour-1 director + chirality launch), timed with HIP events on the
 stream it runs on, priced at its algorithmic traffic per site (BYTES below).
 The CPU baseline is
-the reference's own run() compiled from source (serial), timed on a bounded
-sample of the same workload.
+the C restatement of the reference's sweep in its own arithmetic and draw
+schedule, timed on bounded samples of the same workload on one core and on
+every host core.
 """
 from __future__ import annotations
 
@@ -54,17 +55,31 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_threads() -> int:
+    """Host threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS where the host sets it (the GPU host's CPU share)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
 def cpu_baseline(L, st0, args):
-    """Time the reference's run() (oracle/_ref, built from the reference
-    source) on a bounded sample: args.cpu_sweeps sweeps of the same L^3
-    workload, serial, host random_number included.  Falls back to the C
-    oracle when the reference build is absent."""
-    import numpy as np
+    """The C restatement of the reference's sweep (oracle/chll_oracle.c,
+    oracle_run_mt), timed on bounded samples of the same L^3 workload in the
+    reference's own arithmetic (binary32-rounded site energies) and draw
+    schedule: each sweep first fills its 10 nsub host uniforms, as the
+    reference's random_number calls do (chll/chll_mod.f90:66-80), then runs
+    its four half-sweeps.  Once on one core, once on every host core this
+    process may use (OpenMP over planes, the analogue of the reference's
+    -acc=multicore build, chll/Makefile:8,10); the headline is the all-cores
+    figure.  oracle/calibration.json (written by oracle/calibrate.py where
+    the reference builds) gives reference time / restatement time on one
+    core, measured on one host against the reference compiled from source."""
+    from oracle.oracle import RNG_STREAM, Oracle, copy_state
 
-    from oracle.oracle import REF_SO, RNG_PHILOX, Oracle, Reference, copy_state
-
-    st = copy_state(st0)
-    n = args.cpu_sweeps
+    orc = Oracle()
     cpu_model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -74,21 +89,40 @@ def cpu_baseline(L, st0, args):
                     break
     except OSError:
         pass
-    if os.path.exists(REF_SO):
-        ref = Reference(stream=False)
-        t0 = time.perf_counter()
-        # nstart = 2 skips the header line; nout beyond the range prints no rows
-        ref.run(st, 2, 1 + n, 10 ** 9, args.g, args.K, args.d0, args.kT)
-        dt = time.perf_counter() - t0
-        kind, what = "reference", "reference chll_mod.f90 run() built from source with flang -O2 (serial)"
-    else:
-        t0 = time.perf_counter()
-        Oracle().run(st, 2, 1 + n, 10 ** 9, args.g, args.K, args.d0, args.kT, rng=RNG_PHILOX, seed=1)
-        dt = time.perf_counter() - t0
-        kind, what = "port", "C restatement of the reference (oracle), serial"
     N = L ** 3
-    return {"value": N * n / dt, "unit": "site-updates/s", "cores": 1, "kind": kind,
-            "sample": f"{n} sweeps of the L={L} workload ({what}); {dt:.2f} s wall on {cpu_model}"}
+
+    def timed(nsweeps, threads):
+        st = copy_state(st0)
+        t0 = time.perf_counter()
+        # nstart = 2: no header; nout past the range: no rows
+        orc.run(st, 2, 1 + nsweeps, 10 ** 9, args.g, args.K, args.d0, args.kT, rng=RNG_STREAM, seed=1,
+                threads=threads, hostfill=True)
+        return time.perf_counter() - t0
+
+    nth = host_threads()
+    n1, nm = args.cpu_sweeps, args.cpu_sweeps_mt
+    log(f"bench: CPU baseline: {n1} sweeps on 1 core, {nm} sweeps on {nth} threads, L={L}")
+    t1 = timed(n1, 1)
+    tm = timed(nm, nth)
+    calib = None
+    cf = os.path.join(ROOT, "oracle", "calibration.json")
+    if os.path.exists(cf):
+        with open(cf) as f:
+            calib = json.load(f)
+    one = {"value": N * n1 / t1, "unit": "site-updates/s", "cores": 1,
+           "sample": f"{n1} sweeps of the L={L} workload, {t1:.2f} s wall"}
+    out = {"value": N * nm / tm, "unit": "site-updates/s", "cores": nth, "kind": "port",
+           "sample": f"{nm} sweeps of the L={L} workload (K={args.K} g={args.g} kT={args.kT} d0={args.d0}, open "
+                     f"boundaries, reference arithmetic, host-filled uniforms), {tm:.2f} s wall on {nth} threads "
+                     f"of {cpu_model}",
+           "single_core": one, "cpu_model": cpu_model,
+           "threads_scaling": (N * nm / tm) / one["value"]}
+    if calib:
+        r = calib["reference_over_restatement"]
+        out["calibration"] = {"reference_over_restatement_1core": r, "measured_on": calib["cpu"],
+                              "workload": calib["workload"],
+                              "reference_estimate_1core_site_updates_per_s": one["value"] / r}
+    return out
 
 
 def main():
@@ -105,7 +139,8 @@ def main():
     ap.add_argument("--K", type=float, default=1.0)
     ap.add_argument("--kT", type=float, default=0.7)
     ap.add_argument("--d0", type=float, default=0.3)
-    ap.add_argument("--cpu-sweeps", type=int, default=2)
+    ap.add_argument("--cpu-sweeps", type=int, default=2, help="CPU baseline sweeps on one core")
+    ap.add_argument("--cpu-sweeps-mt", type=int, default=10, help="CPU baseline sweeps on all host cores")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events")
     ap.add_argument("--no-variants", action="store_true", help="skip the other arithmetic mode")
@@ -116,9 +151,13 @@ def main():
     ap.add_argument("--analysis-only", action="store_true",
                     help="run only the analysis section (and its CPU figures unless --no-cpu)")
     ap.add_argument("--dropin-sweeps", type=int, default=20, help="sweeps per drop-in run() call")
+    ap.add_argument("--spinup-s", type=float, default=0.3,
+                    help="untimed sweeps for at least this long before the warm-up (GPU clocks leave idle)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default kernels instead of autotuning (clean profiles)")
     ap.add_argument("--batched-blocks", type=int, default=20, help="timed blocks of 10 sweeps")
+    ap.add_argument("--batched-obs-blocks", type=int, default=200,
+                    help="blocks of 10 sweeps for the scan's per-slot observables (0: skip)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run one plain elementwise copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -163,8 +202,11 @@ def main():
     def time_single(arith, timing, bc=None, nout=None):
         """Autotune the half-sweep kernels for this shape (once per arithmetic;
         runs with another boundary condition or measurement interval reuse the
-        choice), then W warm-up sweeps and K timed sweeps bracketed by
-        synchronisation."""
+        choice), then untimed sweeps -- at least W, and at least
+        args.spinup_s seconds of them, so that the GPU has left its idle
+        clocks -- and K timed sweeps bracketed by synchronisation.  The timed
+        window ends on a measurement sweep (its last sweep is a multiple of
+        nout), so even a short window contains one."""
         from chllmodel_amd.autotune import apply, tune_lattice
 
         bc = args.bc if bc is None else bc
@@ -186,26 +228,37 @@ def main():
         lat.set_step(args.d0)
         p = lat.params(g=args.g, KK=args.K, kbt=args.kT, rng=args.rng, arith=arith, seed=20261203, adapt=True)
         W, K = args.warmup, args.steps
-        lat.sweeps(1, W, nout, params=p, want_rows=False)
+        # untimed sweeps 1 .. t0 - 1: chunks of 50 until spinup_s has passed, then W more
+        done, t_spin = 0, time.perf_counter()
+        while time.perf_counter() - t_spin < args.spinup_s:
+            lat.sweeps(done + 1, done + 50, nout, params=p, want_rows=False)
+            lat.sync()
+            done += 50
+        first = done + W + 1
+        if nout > 0:
+            first += (-(first + K - 1)) % nout  # last timed sweep is a multiple of nout
+        lat.sweeps(done + 1, first - 1, nout, params=p, want_rows=False)
         lat.sync()
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ev0.record(stream)
-        lat.sweeps(W + 1, W + K, nout, params=p, want_rows=False)
-        ev1.record(stream)
+        lat.sweeps(first, first + K - 1, nout, params=p, want_rows=False)
         lat.sync()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        log(f"bench: {arith} bc={bc} nout={nout}: {K} sweeps in {wall:.4f} s (events {ev0.elapsed_time(ev1):.3f} ms)")
+        log(f"bench: {arith} bc={bc} nout={nout}: {K} sweeps {first}..{first + K - 1} in {wall:.4f} s "
+            f"after {first - 1} untimed")
         kernels = {}
         if timing:
             # per-kernel HIP events in a pass of their own, after the timed
-            # region: the events around every launch add gaps between kernels
+            # region (events around every launch add gaps between kernels):
+            # the same number of sweeps, the same alignment on nout
             KT = min(K, 100)
+            nxt = first + K
+            if nout > 0:
+                nxt += (-(nxt + KT - 1)) % nout
+                lat.sweeps(first + K, nxt - 1, nout, params=p, want_rows=False)
             lat.set_timing(True)
-            lat.sweeps(W + K + 1, W + K + KT, nout, params=p, want_rows=False)
+            lat.sweeps(nxt, nxt + KT - 1, nout, params=p, want_rows=False)
             kt = lat.timing()
             lat.set_timing(False)
             for name, (ms, cnt) in kt.items():
@@ -222,12 +275,19 @@ def main():
     if args.arith == "compat":
         BYTES["chirality"] = BYTES_CHIRALITY_COMPAT
     sweep_bytes = None
+    gap_us = None
     sweep_kinds = [k for k in BYTES if k in kernels]
     if sweep_kinds:
         for k in sweep_kinds:
             kernels[k]["GBps"] = BYTES[k] * N / (kernels[k]["avg_us"] * 1e-6) / 1e9
+        nsw_t = kernels[sweep_kinds[0]]["sweeps"]
         # bytes per site-update from the launches actually made (timing pass)
-        sweep_bytes = sum(BYTES[k] * kernels[k]["launches"] for k in sweep_kinds) / kernels[sweep_kinds[0]]["sweeps"]
+        sweep_bytes = sum(BYTES[k] * kernels[k]["launches"] for k in sweep_kinds) / nsw_t
+        # time per sweep outside kernels: the timed sweep less the sum of the
+        # kernels a sweep runs (event-timed pass; the finalize kernel's events
+        # overstate a 4-5 us kernel by ~2 us)
+        ker_us = sum(v["total_ms"] for v in kernels.values()) * 1e3 / nsw_t
+        gap_us = 1e6 * wall / K - ker_us
         dom = max(sweep_kinds, key=lambda k: kernels[k]["total_ms"])
         ach = kernels[dom]["GBps"]
         traffic = None
@@ -235,9 +295,21 @@ def main():
         if os.path.exists(tf):
             with open(tf) as f:
                 traffic = json.load(f).get("hbm_bytes_per_launch")
+        # sweep level: the reference algorithm's 125 B per site-update (four
+        # dependent full-lattice half-sweeps: 2 x 37 B + 2 x 25.5 B) at the
+        # measured site-update rate; 70 % of 8 TB/s is 44.8 G site-updates/s
+        ref_bytes = 125.0
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                 "traffic": traffic, "kernel": KERNEL_NAMES[dom],
-                "algorithmic_bytes_per_launch": BYTES[dom] * N}
+                "algorithmic_bytes_per_launch": BYTES[dom] * N,
+                "algorithmic_bytes_per_site": BYTES[dom],
+                "kernel_avg_us": kernels[dom]["avg_us"],
+                "sweep_level": {"bytes_per_site_update": ref_bytes,
+                                "accounting": "reference algorithm: four full-lattice half-sweeps per sweep",
+                                "achieved": sups * ref_bytes / 1e9, "unit": "GB/s",
+                                "frac": sups * ref_bytes / (HBM_PEAK_GBS * 1e9),
+                                "threshold_site_updates_per_s": 0.7 * HBM_PEAK_GBS * 1e9 / ref_bytes,
+                                "meets_70pct": sups * ref_bytes >= 0.7 * HBM_PEAK_GBS * 1e9}}
     variants = {}
     if not args.no_variants:
         other = "compat" if args.arith == "fp64" else "fp64"
@@ -270,6 +342,7 @@ def main():
         "sweeps_per_s": K / wall,
         "sweep_bytes_per_site_update": sweep_bytes,
         "sweep_hbm_frac": sups * sweep_bytes / (HBM_PEAK_GBS * 1e9) if sweep_bytes else None,
+        "gap_us_per_sweep": gap_us,
         "roofline": roof,
         "kernels": kernels,
         "variants": variants,
@@ -285,9 +358,9 @@ def main():
     if not args.no_batched:
         out["batched"] = batched_scan(args, stream)
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
-        log(f"bench: CPU baseline, {args.cpu_sweeps} sweeps at L={L}")
         out["cpu_baseline"] = cpu_baseline(L, st0, args)
         out["speedup_vs_cpu"] = sups / out["cpu_baseline"]["value"]
+        out["speedup_vs_cpu_1core"] = sups / out["cpu_baseline"]["single_core"]["value"]
         if "analysis" in out:
             analysis_cpu_baseline(out["analysis"])
     print(json.dumps(out), flush=True)
@@ -536,6 +609,7 @@ def batched_scan(args, stream):
         lat.close()
     log(f"bench: sequential {nsw} sweeps x {len(reps)} replicas in {t_seq:.4f} s")
     att, acc = stt["swap_att"], stt["swap_acc"]
+    obs = batched_observables(args, stream, reps, Kb, tune)
     return {"workload": "64 replicas, 5 PT ladders g={0,0.5,1,2,3} x L={32,48,64,96,128}, kT in [0.2,2] "
                         "geometric, K=1, swaps every 10 sweeps, one hipGraph replay per 10 sweeps",
             "replicas": len(reps), "sites": sites, "sweeps": nsw, "arith": args.arith,
@@ -547,7 +621,55 @@ def batched_scan(args, stream):
             "swap_attempts": int(att.sum()), "swap_acceptance": float(acc.sum() / max(att.sum(), 1)),
             "swap_note": "rungs a factor 1.21 apart in kT on lattices of 32^3-128^3 sites: beta-gaps times "
                          "energy gaps are far above 1, so swaps are attempted (and timed) but almost never "
-                         "accepted; acceptance on small lattices is covered by the ensemble tests"}
+                         "accepted; acceptance on small lattices is covered by the ensemble tests",
+            "observables": obs}
+
+
+def batched_observables(args, stream, reps, Kb, tune):
+    """Per-temperature-slot observables of the scan after the timed part: a
+    fresh ensemble of the same replicas, args.batched_obs_blocks blocks of Kb
+    sweeps, one measurement row per replica and block; the first quarter is
+    dropped as equilibration.  Canonical C_v = N var(e)/kT^2 and
+    chi_c = N var(eta)/kT with jackknife errors, means with blocking errors
+    (observables.summarize), and the notebooks' own forms: C_v =
+    var(E[-100:])/kT of the reported energy per site
+    (notebooks/paper_plots2.ipynb, calc_specific_heat) and chi_c =
+    N var(eta[-500:])/kT (notebooks/.ipynb_checkpoints/
+    paper_plots-checkpoint.ipynb:440-452)."""
+    import numpy as np
+
+    from chllmodel_amd.ensemble import Ensemble, series_by_temperature
+    from chllmodel_amd.observables import chi_notebook, cv_notebook, summarize
+
+    nb = args.batched_obs_blocks
+    if nb <= 0:
+        return None
+    ens = Ensemble(reps, bc="open", arith=args.arith)
+    ens.set_stream(stream.cuda_stream)
+    ens.set_block(tune["best"])
+    ens.set_chunk(tune["chunk"])
+    ens.init_isotropic(20261203)
+    t0 = time.perf_counter()
+    rows = ens.run(nb, Kb, seed=20261203, run_index=1, want_rows=True, use_graph=True)
+    dt = time.perf_counter() - t0
+    ens.close()
+    ser = series_by_temperature(rows, reps)
+    slots = []
+    for sl in range(len(reps)):
+        r0 = reps[sl]  # slot sl holds temperature reps[sl]['kbt'] of its ladder
+        N = r0["ni"] * r0["nj"] * r0["nk"]
+        sm = summarize(ser[sl], r0["kbt"], N)
+        eref = np.array([rows[t, int(ser[sl]["replica"][t]), 14] for t in range(rows.shape[0])])
+        ent = {"slot": sl, "ladder": r0["ladder"], "L": r0["ni"], "g": r0["g"], "kT": r0["kbt"]}
+        ent.update({k: float(f"{v:.6g}") for k, v in sm.items()})
+        ent["Cv_notebook"] = float(f"{cv_notebook(eref, r0['kbt']):.6g}")
+        ent["chi_notebook"] = float(f"{chi_notebook(ser[sl]['eta'], r0['kbt'], N):.6g}")
+        slots.append(ent)
+    log(f"bench: batched observables, {nb} blocks of {Kb} sweeps in {dt:.2f} s")
+    return {"blocks": nb, "sweeps_per_block": Kb, "samples_per_slot": int(rows.shape[0]),
+            "discarded_fraction": 0.25, "seconds": dt, "slots": slots,
+            "definitions": "E, eta, P2: means with blocking errors; Cv = N var(E/N)/kT^2, chi = N var(eta)/kT with "
+                           "jackknife errors; Cv_notebook = var(E_ref[-100:])/kT; chi_notebook = N var(eta[-500:])/kT"}
 
 
 if __name__ == "__main__":

@@ -50,15 +50,54 @@ class ChllError(RuntimeError):
 _lib = None
 
 
-def _load_torch_first() -> None:
-    """Import torch, when installed, before this library is loaded.  Both link
-    the HIP runtime under one soname, so whichever loads first serves the whole
-    process; torch's ROCm build finds no GPU ("No HIP GPUs are available") on
-    a runtime of another ROCm release, while libchll_mod.so runs on torch's."""
+def _mapped(fragment: str) -> list[str]:
+    """Paths of the shared objects mapped into this process whose path
+    contains `fragment` (Linux /proc/self/maps)."""
     try:
-        import torch  # noqa: F401
-    except ImportError:
-        pass
+        with open("/proc/self/maps") as f:
+            return sorted({ln.split()[-1] for ln in f if fragment in ln and "/" in ln})
+    except OSError:
+        return []
+
+
+def _torch_hip_runtime() -> str | None:
+    """Path of the HIP runtime bundled with an installed torch, found without
+    importing torch."""
+    import importlib.util
+
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return None
+    if spec is None or not spec.submodule_search_locations:
+        return None
+    path = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    return path if os.path.exists(path) else None
+
+
+def _one_hip_runtime() -> None:
+    """Make the process hold one HIP runtime whatever is imported first.
+
+    libchll_mod.so needs libamdhip64.so.7.  An installed torch bundles its own
+    HIP runtime under that soname; when the system's ROCm runtime is loaded
+    first, torch imported afterwards finds no GPU ("No HIP GPUs are
+    available").  So when torch is installed its runtime is loaded here, by
+    path and globally, before libchll_mod.so, without importing torch: the
+    library's dependency then resolves to it, and a later `import torch` maps
+    the same file again (same inode), so both share one runtime.  When another
+    HIP runtime is already mapped and torch's is not, this raises with the
+    reason instead of leaving a process whose torch would see no GPU."""
+    path = _torch_hip_runtime()
+    if path is None:
+        return  # no torch: the system ROCm runtime serves the library alone
+    held = _mapped("libamdhip64")
+    real = os.path.realpath(path)
+    if held and all(os.path.realpath(h) != real for h in held):
+        raise ChllError(
+            f"this process already holds the HIP runtime {held[0]}, not torch's {path}: torch imported "
+            "afterwards would find no GPU.  Load chllmodel_amd (or import torch) before any other HIP library.")
+    if not held:
+        ct.CDLL(path, mode=ct.RTLD_GLOBAL)
 
 
 def lib() -> ct.CDLL:
@@ -68,7 +107,7 @@ def lib() -> ct.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ChllError(f"{LIB_PATH} not built: run `python -m chllmodel_amd.build` (hipcc, gfx950)")
-    _load_torch_first()
+    _one_hip_runtime()
     L = ct.CDLL(LIB_PATH)
     # drop-in symbols, argtypes as chll/chll.py:9-36
     L.init.argtypes = [ctypeslib.ndpointer(dtype=np.float64, ndim=3, flags="F_CONTIGUOUS"),

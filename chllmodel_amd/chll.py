@@ -97,26 +97,26 @@ class ChLLSim:
 
     # -- output folders and files (chll/chll.py:117-262) ----------------------
     def setup_folders(self):
-        data_folder = "./data"
+        """Claim the output folder ./data/<name> (contract of chll/chll.py:117-138):
+        scripts run from the project root, ./data must already exist, an
+        existing folder is reused only with overwrite=True, and either failure
+        ends the program with exit(); name None saves nothing."""
+        self.simpath = None
         if self.name is None:
             print("> Not saving sim")
-        else:
-            if os.path.exists(data_folder):
-                simname_folder_path = os.path.join(data_folder, self.name)
-                if os.path.exists(simname_folder_path):
-                    print(f"> ERROR: '{self.name}' in '{data_folder}'!")
-                    if self.overwrite:
-                        print(f"> Overwriting '{self.name}' in '{data_folder}' ... ")
-                        os.makedirs(simname_folder_path, exist_ok=True)
-                    else:
-                        exit()
-                else:
-                    os.makedirs(simname_folder_path, exist_ok=True)
-                    print(f"> {simname_folder_path}")
-                self.simpath = simname_folder_path
-            else:
-                print(f"ERROR: '{data_folder}' does not exist.")
-                exit()
+            return
+        root = "./data"
+        if not os.path.isdir(root):
+            print(f"ERROR: '{root}' does not exist.")
+            exit()
+        target = os.path.join(root, self.name)
+        existed = os.path.exists(target)
+        if existed and not self.overwrite:
+            print(f"> ERROR: '{target}' exists and overwrite=False")
+            exit()
+        os.makedirs(target, exist_ok=True)
+        print(f"> {'reusing' if existed else 'created'} {target}")
+        self.simpath = target
 
     def save_params(self, nsteps, nout):
         if self.name is None:
@@ -204,21 +204,36 @@ class ChLLSim:
                 print(f"> ERROR: '{self.simpath}' does not exist")
                 exit()
 
-    def plot_config(self):
-        """Quiver plot of the y = nj/2 slice (chll/chll.py:264-282)."""
-        import matplotlib.pyplot as plt
+    def plot_config(self, path=None):
+        """Picture of the mid plane j = nj // 2 (the slice chll/chll.py:264-282
+        draws), saved as <simpath>/<name>.png unless `path` is given: every
+        site is a headless stick along its director's (n_x, n_z) projection,
+        centred on (i, k) and coloured by its chirality.  Returns the file."""
+        from matplotlib.backends.backend_agg import FigureCanvasAgg
+        from matplotlib.collections import LineCollection
+        from matplotlib.figure import Figure
 
-        X, Z = np.meshgrid(np.arange(0, self.ni), np.arange(0, self.nk))
-        mp = int(np.floor(self.nj / 2))
-        u = self.nx[:, mp, :]
-        w = self.nz[:, mp, :]
-        s_color = self.s[:, mp, :]
-        fig = plt.figure(figsize=(10, 10))
-        ax = fig.add_subplot(111)
-        q = ax.quiver(X, Z, u, w, s_color, pivot="mid", headlength=0, headwidth=0, headaxislength=0,
-                      scale_units="xy", scale=0.75, cmap="coolwarm")
-        ax.set_xlabel("X")
-        ax.set_ylabel("Z")
-        ax.set_title(self.name)
-        fig.colorbar(q)
-        plt.savefig(os.path.join(self.simpath, self.name + ".png"))
+        j = self.nj // 2
+        ii, kk = np.meshgrid(np.arange(self.ni, dtype=float), np.arange(self.nk, dtype=float), indexing="ij")
+        centre = np.stack([ii, kk], axis=-1)
+        arm = 0.4 * np.stack([self.nx[:, j, :], self.nz[:, j, :]], axis=-1)
+        sticks = np.stack([centre - arm, centre + arm], axis=-2).reshape(-1, 2, 2)
+        fig = Figure(figsize=(10, 10))
+        FigureCanvasAgg(fig)
+        ax = fig.add_subplot()
+        lines = LineCollection(sticks, array=self.s[:, j, :].astype(float).ravel(), cmap="coolwarm", linewidths=1.5)
+        lines.set_clim(-1.0, 1.0)
+        ax.add_collection(lines)
+        ax.set_xlim(-1, self.ni)
+        ax.set_ylim(-1, self.nk)
+        ax.set_aspect("equal")
+        ax.set_xlabel("i (x)")
+        ax.set_ylabel("k (z)")
+        ax.set_title(f"{self.name or ''}  y = {j}")
+        fig.colorbar(lines, ax=ax, label="chirality s")
+        if path is None:
+            if self.simpath is None:
+                raise ValueError("plot_config: no output folder (name is None) and no path given")
+            path = os.path.join(self.simpath, f"{self.name}.png")
+        fig.savefig(path)
+        return path

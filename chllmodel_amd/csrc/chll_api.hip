@@ -76,6 +76,7 @@ enum {
 struct chll_lattice {
     Geom G{};
     int bc = BC_OPEN;
+    int device = 0;  // HIP device current at creation
     hipStream_t own = nullptr, stream = nullptr;
     Fields F{};
     double *stage = nullptr;
@@ -163,6 +164,7 @@ int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lat
     if (check_geometry(ni, nj, nk, bc)) return -1;
     if ((uint64_t)ni * nj * nk + kFieldPad >= (1ull << 32)) return fail("lattice too large for 32-bit site indices");
     chll_lattice *L = new chll_lattice();
+    (void)hipGetDevice(&L->device);
     Geom &G = L->G;
     G.ni = (uint32_t)ni; G.nj = (uint32_t)nj; G.nk = (uint32_t)nk;
     G.half = G.ni / 2;
@@ -675,7 +677,49 @@ void init(double *nx, double *ny, double *nz, int32_t *s, int32_t *dope, int32_t
         }
 }
 
-static chll_lattice *g_run_lat = nullptr;
+// Device lattices for run(): a call takes an idle one of its shape on the
+// current device, or creates one, and gives it back on return, so calls from
+// several host threads each sweep their own lattice.  One idle lattice per
+// device is kept for the next call (the reference allocates on every call).
+namespace {
+std::mutex g_pool_mu;
+std::vector<chll_lattice *> g_pool;
+std::mutex g_print_mu;
+
+chll_lattice *pool_take(int32_t ni, int32_t nj, int32_t nk, int bc)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        for (size_t i = 0; i < g_pool.size(); ++i) {
+            chll_lattice *L = g_pool[i];
+            if (L->device == dev && L->G.ni == (uint32_t)ni && L->G.nj == (uint32_t)nj &&
+                L->G.nk == (uint32_t)nk && L->bc == bc) {
+                g_pool.erase(g_pool.begin() + (long)i);
+                return L;
+            }
+        }
+    }
+    chll_lattice *L = nullptr;
+    if (chll_lattice_create(ni, nj, nk, bc, &L)) return nullptr;
+    return L;
+}
+
+void pool_give(chll_lattice *L)
+{
+    std::vector<chll_lattice *> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        for (size_t i = 0; i < g_pool.size();) {
+            if (g_pool[i]->device == L->device) { drop.push_back(g_pool[i]); g_pool.erase(g_pool.begin() + (long)i); }
+            else ++i;
+        }
+        g_pool.push_back(L);
+    }
+    for (chll_lattice *D : drop) chll_lattice_destroy(D);
+}
+}  // namespace
 
 static void print_fixed(double v)
 {
@@ -692,8 +736,6 @@ void run(const int32_t *nstart, const int32_t *nstop, const int32_t *nout,
          int32_t *msteps, double *m)
 {
     (void)dope; (void)sl1; (void)sl2;
-    std::lock_guard<std::mutex> lk(g_mu);
-    read_env_once();
     g_err.clear();
     auto bail = [](void) { fprintf(stderr, "chll run: %s\n", g_err.c_str()); };
     if (!nstart || !nstop || !nout || !nx || !ny || !nz || !s || !g || !KK || !d || !kbt || !ni || !nj || !nk ||
@@ -702,39 +744,59 @@ void run(const int32_t *nstart, const int32_t *nstop, const int32_t *nout,
         return bail();
     }
     if (*nout <= 0) { fail("run: nout must be positive (got %d)", *nout); return bail(); }
-    if (check_geometry(*ni, *nj, *nk, g_cfg.bc)) return bail();
     const int32_t rows = (1 + *nstop - *nstart) / *nout;
     if (rows > 0 && (!msteps || !m)) { fail("run: null msteps/m"); return bail(); }
-    if (g_cfg.rng == RNG_BUFFER && !g_cfg.buffer) { fail("run: rng=buffer without chll_set_uniform_buffer"); return bail(); }
-    if (g_run_lat && (g_run_lat->G.ni != (uint32_t)*ni || g_run_lat->G.nj != (uint32_t)*nj ||
-                      g_run_lat->G.nk != (uint32_t)*nk || g_run_lat->bc != g_cfg.bc)) {
-        chll_lattice_destroy(g_run_lat);
-        g_run_lat = nullptr;
-    }
-    if (!g_run_lat && chll_lattice_create(*ni, *nj, *nk, g_cfg.bc, &g_run_lat)) return bail();
-    chll_lattice *L = g_run_lat;
-    *nsub = (int32_t)L->G.nsub;
-    if (chll_lattice_upload(L, nx, ny, nz, s)) return bail();
-    if (chll_lattice_set_step(L, *d)) return bail();
+    // the options of this call, and its share of the random numbers: run
+    // counters and stream positions are handed out under the lock, so
+    // concurrent calls draw disjoint numbers
+    RunConfig cfg;
     chll_params p{};
-    p.g = *g; p.KK = *KK; p.kbt = *kbt; p.rng = g_cfg.rng; p.arith = g_cfg.arith; p.adapt = 1;
-    if (g_cfg.rng == RNG_PHILOX) { p.seed = g_cfg.seed; p.run_index = g_cfg.run_index++; }
-    else { p.seed = g_cfg.stream_seed; p.stream_pos = g_cfg.stream_pos; }
-    p.buffer = g_cfg.buffer; p.buffer_len = g_cfg.buffer_len;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        read_env_once();
+        cfg.rng = g_cfg.rng; cfg.arith = g_cfg.arith; cfg.bc = g_cfg.bc; cfg.verbose = g_cfg.verbose;
+        if (check_geometry(*ni, *nj, *nk, cfg.bc)) return bail();
+        if (cfg.rng == RNG_BUFFER && !g_cfg.buffer) { fail("run: rng=buffer without chll_set_uniform_buffer"); return bail(); }
+        const int64_t nsw = (int64_t)*nstop - *nstart + 1;
+        const uint64_t draws = nsw > 0 ? (uint64_t)nsw * 10u * (((uint64_t)*ni * *nj * *nk) / 2) : 0;
+        p.rng = cfg.rng; p.arith = cfg.arith;
+        if (cfg.rng == RNG_PHILOX) { p.seed = g_cfg.seed; p.run_index = g_cfg.run_index++; }
+        else {
+            p.seed = g_cfg.stream_seed; p.stream_pos = g_cfg.stream_pos;
+            if (cfg.rng == RNG_BUFFER && (int64_t)(p.stream_pos + draws) > g_cfg.buffer_len) {
+                fail("uniform buffer too short: %lld sweeps need %llu draws from position %llu, buffer holds %lld",
+                     (long long)nsw, (unsigned long long)draws, (unsigned long long)p.stream_pos,
+                     (long long)g_cfg.buffer_len);
+                return bail();
+            }
+            g_cfg.stream_pos += draws;
+        }
+        p.buffer = g_cfg.buffer; p.buffer_len = g_cfg.buffer_len;
+    }
+    chll_lattice *L = pool_take(*ni, *nj, *nk, cfg.bc);
+    if (!L) return bail();
+    *nsub = (int32_t)(((int64_t)*ni * *nj * *nk) / 2);
+    p.g = *g; p.KK = *KK; p.kbt = *kbt; p.adapt = 1;
     std::vector<double> rbuf((size_t)(rows > 0 ? rows : 1) * kRowCols, 0.0);
-    const int got = chll_lattice_sweeps(L, &p, *nstart, *nstop, *nout, rbuf.data(), rows > 0 ? rows : 0, nullptr);
+    int got = -1;
+    if (!chll_lattice_upload(L, nx, ny, nz, s) && !chll_lattice_set_step(L, *d)) {
+        got = chll_lattice_sweeps(L, &p, *nstart, *nstop, *nout, rbuf.data(), rows > 0 ? rows : 0, nullptr);
+        if (got >= 0 && (chll_lattice_download(L, nx, ny, nz, s) || chll_lattice_get_step(L, d))) got = -1;
+    }
+    pool_give(L);
     if (got < 0) return bail();
-    if (g_cfg.rng != RNG_PHILOX) g_cfg.stream_pos = p.stream_pos;
-    if (chll_lattice_download(L, nx, ny, nz, s)) return bail();
-    if (chll_lattice_get_step(L, d)) return bail();
     for (int r = 0; r < got; ++r) {
         const double *row = &rbuf[(size_t)r * kRowCols];
         msteps[r] = (int32_t)row[0];
         for (int c = 0; c < 5; ++c) m[r + (size_t)c * rows] = row[1 + c];
     }
-    g_cfg.last_rows.assign(rbuf.begin(), rbuf.begin() + (size_t)got * kRowCols);
-    if (g_cfg.verbose) {
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_cfg.last_rows.assign(rbuf.begin(), rbuf.begin() + (size_t)got * kRowCols);
+    }
+    if (cfg.verbose) {
         // chll/chll_mod.f90:60 header and :98 rows, format (I8, 5(1X, F12.6))
+        std::lock_guard<std::mutex> lk(g_print_mu);
         if (*nstart == 1) printf("Steps       Energy      E. excess      Paccept      Faccept      D\n");
         for (int r = 0; r < got; ++r) {
             const double *row = &rbuf[(size_t)r * kRowCols];

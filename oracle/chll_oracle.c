@@ -281,24 +281,52 @@ typedef struct {
 static inline double src_at(usrc_t *U, uint64_t n)
 {
     if (U->mode == ORACLE_RNG_STREAM) return oracle_stream_uniform(U->seed, n);
-    if ((int64_t)n >= U->buf_len) { U->overflow = 1; return 2.0; }
+    if ((int64_t)n >= U->buf_len) { __atomic_store_n(&U->overflow, 1, __ATOMIC_RELAXED); return 2.0; }
     return U->buf[n];
+}
+
+/* ------------------------------------------------------------------------- */
+/* Rank of site (i, j, k) in its sublattice list sl1 (colour 0) or sl2 (colour
+ * 1), i.e. the index r of its uniforms rand1(r+1), rand2(r+1, :) in the
+ * reference's draw schedule (chll/chll_mod.f90:23-36, 131-134, 146, 188).
+ * The lists hold the sites of one colour in Fortran order.  A row (j, k) holds
+ * the sites i = p, p + 2, ... of colour c = (i + j + k) & 1, p = (c ^ j ^ k) & 1,
+ * so the sites before (i, j, k) in its own row number i >> 1.  With ni even
+ * every row holds ni / 2 sites of each colour and r = l >> 1 (l the 0-based
+ * linear index).  With ni odd a row holds (ni + 1) / 2 sites of the colour of
+ * its i = 0 site ((j + k) & 1) and (ni - 1) / 2 of the other. */
+static inline int64_t count_parity(int64_t n, int t) { return (n + 1 - t) >> 1; } /* x in [0,n), x = t mod 2 */
+
+int64_t oracle_rank(int ni, int nj, int i, int j, int k)
+{
+    const int64_t l = (int64_t)i + (int64_t)ni * ((int64_t)j + (int64_t)nj * k);
+    if ((ni & 1) == 0) return l >> 1;
+    const int c = (i + j + k) & 1;
+    /* rows before (j, k) whose i = 0 site has colour c: whole planes k' < k,
+     * then rows j' < j of plane k */
+    const int64_t kc = count_parity(k, c);
+    const int64_t rows_c = kc * ((nj + 1) >> 1) + ((int64_t)k - kc) * (nj >> 1) + count_parity(j, (c ^ k) & 1);
+    const int64_t rows = (int64_t)j + (int64_t)nj * k;
+    return rows * ((ni - 1) >> 1) + rows_c + (i >> 1);
 }
 
 /* ------------------------------------------------------------------------- */
 /* Director half-sweep over one colour (chll/chll_mod.f90:117-202).  colour 0 is
  * sl1 (1-based i+j+k odd = 0-based even), colour 1 is sl2.  The sublattice rank
- * of site l is l >> 1 (ni even), which indexes rand1 / rand2. */
+ * of the site (oracle_rank) indexes rand1 / rand2 and the Philox counter. */
 static int64_t evolve_dir(lat_t *L, int colour, usrc_t *U, uint64_t base_phi, uint64_t base_acc,
-                          uint32_t istep, double g, double KK, double d, double kbt)
+                          uint32_t istep, double g, double KK, double d, double kbt, int nthreads)
 {
     int64_t nacc = 0;
+    /* Sites of one colour share no bond, so the planes may run in any order
+     * and on any number of threads with the same result. */
+#pragma omp parallel for schedule(static) reduction(+ : nacc) num_threads(nthreads) if (nthreads > 1)
     for (int k = 0; k < L->nk; ++k)
         for (int j = 0; j < L->nj; ++j)
             for (int i = 0; i < L->ni; ++i) {
                 if (((i + j + k) & 1) != colour) continue;
                 int64_t l = lidx(L, i, j, k);
-                int64_t r = l >> 1;
+                int64_t r = oracle_rank(L->ni, L->nj, i, j, k);
                 double u_phi, u_acc, unused;
                 if (U->mode == ORACLE_RNG_PHILOX) {
                     philox_site(U->seed, U->run_index, U->replica, (uint32_t)r + U->site_offset, istep, colour,
@@ -323,15 +351,16 @@ static int64_t evolve_dir(lat_t *L, int colour, usrc_t *U, uint64_t base_phi, ui
 
 /* Chirality half-sweep over one colour (chll/chll_mod.f90:204-245). */
 static int64_t evolve_s(lat_t *L, int colour, usrc_t *U, uint64_t base_acc,
-                        uint32_t istep, double g, double KK, double kbt)
+                        uint32_t istep, double g, double KK, double kbt, int nthreads)
 {
     int64_t nflip = 0;
+#pragma omp parallel for schedule(static) reduction(+ : nflip) num_threads(nthreads) if (nthreads > 1)
     for (int k = 0; k < L->nk; ++k)
         for (int j = 0; j < L->nj; ++j)
             for (int i = 0; i < L->ni; ++i) {
                 if (((i + j + k) & 1) != colour) continue;
                 int64_t l = lidx(L, i, j, k);
-                int64_t r = l >> 1;
+                int64_t r = oracle_rank(L->ni, L->nj, i, j, k);
                 double u_acc, unused0, unused1;
                 if (U->mode == ORACLE_RNG_PHILOX)
                     philox_site(U->seed, U->run_index, U->replica, (uint32_t)r + U->site_offset,
@@ -429,13 +458,13 @@ double oracle_energy_true(const double *nx, const double *ny, const double *nz, 
  * k0.. of a larger lattice with the same ni, nj, whose packed indices start at
  * site_offset = ni nj k0 / 2 (k0 even keeps the colours).
  * Returns 0, or -1 when a buffer source ran out. */
-int oracle_run(int rng_mode, uint64_t seed, uint64_t *stream_pos, const double *buf, int64_t buf_len,
-               uint32_t run_index, uint32_t replica, uint32_t site_offset, int bc, int arith,
-               int nstart, int nstop, int nout,
-               double *nx, double *ny, double *nz, int32_t *s,
-               double g, double KK, double *d, double kbt,
-               int ni, int nj, int nk, int32_t *msteps, double *m,
-               int64_t *counts /* optional [nsweeps*2]: naccept, nflip per sweep */)
+static int run_impl(int rng_mode, uint64_t seed, uint64_t *stream_pos, const double *buf, int64_t buf_len,
+                    uint32_t run_index, uint32_t replica, uint32_t site_offset, int bc, int arith,
+                    int nstart, int nstop, int nout,
+                    double *nx, double *ny, double *nz, int32_t *s,
+                    double g, double KK, double *d, double kbt,
+                    int ni, int nj, int nk, int32_t *msteps, double *m, int64_t *counts,
+                    int nthreads, int hostfill)
 {
     lat_t L = {nx, ny, nz, s, ni, nj, nk, bc, arith};
     usrc_t U;
@@ -448,15 +477,34 @@ int oracle_run(int rng_mode, uint64_t seed, uint64_t *stream_pos, const double *
     int rows = (1 + nstop - nstart) / nout;
     int mc = 0;
     double dd = *d;
+    /* hostfill (stream draws): every sweep first fills all 10 nsub uniforms of
+     * its six random_number calls into a host array, as the reference does
+     * (chll/chll_mod.f90:66-80), and the half-sweeps read them from there */
+    double *fb = NULL;
+    usrc_t F = U;
+    if (hostfill && rng_mode == ORACLE_RNG_STREAM) {
+        fb = (double *)malloc((size_t)(10 * nsub > 0 ? 10 * nsub : 1) * sizeof(double));
+        if (!fb) return -2;
+        F.mode = ORACLE_RNG_BUFFER; F.buf = fb; F.buf_len = 10 * nsub;
+    }
     for (int istep = nstart; istep <= nstop; ++istep) {
         uint64_t b = U.pos;
+        usrc_t *src = &U;
+        if (fb) {
+            const int64_t nd = 10 * nsub;
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
+            for (int64_t t = 0; t < nd; ++t) fb[t] = oracle_stream_uniform(seed, b + (uint64_t)t);
+            src = &F;
+            b = 0;
+        }
         /* chunk offsets of the six random_number calls of one sweep */
         uint64_t c0 = b, c1 = b + nsub, c2 = b + 3 * nsub, c3 = b + 4 * nsub, c4 = b + 6 * nsub, c5 = b + 8 * nsub;
         int64_t nacc = 0, nflip = 0;
-        nacc += evolve_dir(&L, 0, &U, c0, c1, (uint32_t)istep, g, KK, dd, kbt);
-        nacc += evolve_dir(&L, 1, &U, c2, c3, (uint32_t)istep, g, KK, dd, kbt);
-        nflip += evolve_s(&L, 0, &U, c4 + nsub, (uint32_t)istep, g, KK, kbt);
-        nflip += evolve_s(&L, 1, &U, c5 + nsub, (uint32_t)istep, g, KK, kbt);
+        nacc += evolve_dir(&L, 0, src, c0, c1, (uint32_t)istep, g, KK, dd, kbt, nthreads);
+        nacc += evolve_dir(&L, 1, src, c2, c3, (uint32_t)istep, g, KK, dd, kbt, nthreads);
+        nflip += evolve_s(&L, 0, src, c4 + nsub, (uint32_t)istep, g, KK, kbt, nthreads);
+        nflip += evolve_s(&L, 1, src, c5 + nsub, (uint32_t)istep, g, KK, kbt, nthreads);
+        b = U.pos;
         if (U.mode != ORACLE_RNG_PHILOX) U.pos = b + 10 * (uint64_t)nsub;
         if (counts) { counts[2 * (istep - nstart)] = nacc; counts[2 * (istep - nstart) + 1] = nflip; }
         double paccept = (double)((float)(int32_t)nacc / (float)n3);
@@ -481,7 +529,36 @@ int oracle_run(int rng_mode, uint64_t seed, uint64_t *stream_pos, const double *
             ++mc;
         }
     }
+    free(fb);
     *d = dd;
     if (stream_pos) *stream_pos = U.pos;
     return U.overflow ? -1 : 0;
+}
+
+int oracle_run(int rng_mode, uint64_t seed, uint64_t *stream_pos, const double *buf, int64_t buf_len,
+               uint32_t run_index, uint32_t replica, uint32_t site_offset, int bc, int arith,
+               int nstart, int nstop, int nout,
+               double *nx, double *ny, double *nz, int32_t *s,
+               double g, double KK, double *d, double kbt,
+               int ni, int nj, int nk, int32_t *msteps, double *m,
+               int64_t *counts /* optional [nsweeps*2]: naccept, nflip per sweep */)
+{
+    return run_impl(rng_mode, seed, stream_pos, buf, buf_len, run_index, replica, site_offset, bc, arith, nstart,
+                    nstop, nout, nx, ny, nz, s, g, KK, d, kbt, ni, nj, nk, msteps, m, counts, 1, 0);
+}
+
+/* oracle_run on nthreads threads (OpenMP over the planes of each half-sweep,
+ * the analogue of the reference's -acc=multicore build, chll/Makefile:8,10);
+ * hostfill as in run_impl.  The lattice, rows and counts are those of
+ * oracle_run: sites of one colour are independent. */
+int oracle_run_mt(int rng_mode, uint64_t seed, uint64_t *stream_pos, const double *buf, int64_t buf_len,
+                  uint32_t run_index, uint32_t replica, uint32_t site_offset, int bc, int arith,
+                  int nstart, int nstop, int nout,
+                  double *nx, double *ny, double *nz, int32_t *s,
+                  double g, double KK, double *d, double kbt,
+                  int ni, int nj, int nk, int32_t *msteps, double *m, int64_t *counts, int nthreads, int hostfill)
+{
+    return run_impl(rng_mode, seed, stream_pos, buf, buf_len, run_index, replica, site_offset, bc, arith, nstart,
+                    nstop, nout, nx, ny, nz, s, g, KK, d, kbt, ni, nj, nk, msteps, m, counts,
+                    nthreads < 1 ? 1 : nthreads, hostfill);
 }

@@ -84,6 +84,10 @@ class Oracle:
                                  ct.c_double, ct.c_double, _f64p, ct.c_double,
                                  ct.c_int, ct.c_int, ct.c_int, _i32p, _f64p, _i64p]
         L.oracle_run.restype = ct.c_int
+        L.oracle_run_mt.argtypes = L.oracle_run.argtypes + [ct.c_int, ct.c_int]
+        L.oracle_run_mt.restype = ct.c_int
+        L.oracle_rank.argtypes = [ct.c_int] * 5
+        L.oracle_rank.restype = ct.c_int64
 
     # -- RNG --------------------------------------------------------------
     def philox(self, ctr, key):
@@ -133,14 +137,22 @@ class Oracle:
         return self.lib.oracle_energy_true(_ptr(st["nx"], _f64p), _ptr(st["ny"], _f64p), _ptr(st["nz"], _f64p),
                                            _ptr(st["s"], _i32p), ni, nj, nk, bc, float(g), float(KK))
 
+    def rank(self, ni, nj, i, j, k):
+        """Rank of site (i, j, k) (0-based) in its sublattice list."""
+        return int(self.lib.oracle_rank(ni, nj, i, j, k))
+
     def run(self, st, nstart, nstop, nout, g, KK, d, kbt, *, rng=RNG_STREAM, seed=0, stream_pos=0,
             buf=None, run_index=0, replica=0, bc=BC_OPEN, arith=ARITH_COMPAT, want_counts=False,
-            site_offset=0):
+            site_offset=0, threads=None, hostfill=False):
         """Sweep driver (chll/chll_mod.f90:40-115).  Mutates st in place.
 
-        site_offset (Philox draws only) is added to the packed site index of
+        site_offset (Philox draws only) is added to the sublattice rank of
         every draw's counter, so st can be the slab of planes k0.. of a larger
         lattice with the same ni, nj (site_offset = ni nj k0 / 2, k0 even).
+        threads: run the half-sweeps on that many OpenMP threads (oracle_run_mt;
+        same results).  hostfill (stream draws): fill each sweep's 10 nsub
+        uniforms into a host array first, as the reference's random_number
+        calls do (same results).
 
         Returns (msteps, m, d_out, stream_pos_out[, counts])."""
         ni, nj, nk = st["nx"].shape
@@ -156,11 +168,17 @@ class Oracle:
             bptr, blen = _ptr(buf, _f64p), buf.size
         else:
             bptr, blen = None, 0
-        rc = self.lib.oracle_run(rng, seed, ct.byref(pos), bptr, blen, run_index, replica, site_offset, bc, arith,
-                                 nstart, nstop, nout,
-                                 _ptr(st["nx"], _f64p), _ptr(st["ny"], _f64p), _ptr(st["nz"], _f64p),
-                                 _ptr(st["s"], _i32p), float(g), float(KK), ct.byref(dd), float(kbt),
-                                 ni, nj, nk, _ptr(msteps, _i32p), _ptr(m, _f64p), _ptr(counts, _i64p))
+        args = (rng, seed, ct.byref(pos), bptr, blen, run_index, replica, site_offset, bc, arith,
+                nstart, nstop, nout,
+                _ptr(st["nx"], _f64p), _ptr(st["ny"], _f64p), _ptr(st["nz"], _f64p),
+                _ptr(st["s"], _i32p), float(g), float(KK), ct.byref(dd), float(kbt),
+                ni, nj, nk, _ptr(msteps, _i32p), _ptr(m, _f64p), _ptr(counts, _i64p))
+        if threads is None and not hostfill:
+            rc = self.lib.oracle_run(*args)
+        else:
+            rc = self.lib.oracle_run_mt(*args, int(threads or 1), 1 if hostfill else 0)
+        if rc == -2:
+            raise MemoryError("oracle_run_mt: host uniform array")
         if rc != 0:
             raise RuntimeError("oracle_run: uniform buffer exhausted")
         out = (msteps[:rows], m[:rows], dd.value, pos.value)

@@ -28,6 +28,9 @@ def test_chllsim_run_and_files(tmp_path, monkeypatch):
     sim = ChLLSim(8, 6, 4, kbt=0.7, d=0.3, KK=1.0, g=0.5, rho=0.0, name="e2e", rng="stream", seed=77,
                   verbose=False)
     sim.init_rand_s()
+    # init (chll/chll_mod.f90:9-38) filled the sublattice lists through the drop-in symbol
+    o1, o2 = Oracle().init(8, 6, 4)
+    assert np.array_equal(sim.sl1, o1[: sim.nsub]) and np.array_equal(sim.sl2, o2[: sim.nsub])
     st0 = {k: getattr(sim, k).copy(order="F") for k in ("nx", "ny", "nz", "s", "dope")}
     sim.run(12, 4, save=True)
 

@@ -1,0 +1,215 @@
+"""Two workload configurations at their full sizes, and the drop-in run()
+called from two host threads at once.
+
+* L=32 periodic (32,768 sites), K = 0.5, g = 0, kT = 1.0, d0 = 0.3, 1000
+  sweeps with a row every 10, site-keyed Philox draws, in both arithmetic
+  modes, against the oracle run of the same sweeps: every sweep's accept and
+  flip counts and the final spins exact, directors within 1e-12, the 100 rows'
+  eta / paccept / faccept / d exact and the reported energy within one
+  binary32 ulp (it is a binary64 sum taken in another order before both round).
+* The 64-replica batched scan (phase_scan_replicas: ladders g = 0, 0.5, 1, 2, 3
+  on L = 32, 48, 64, 96, 128, kT geometric in [0.2, 2], 41.9 M sites), three
+  blocks of 10 sweeps with parallel-tempering swaps, replayed from a hipGraph:
+  - the 13 replicas of the L = 32 ladder bit-exact against oracle runs that
+    follow the slot each replica held in each block (kT and adapted d of
+    that slot), every swap decision recomputed from the oracle's energies;
+  - the swap decisions of the other ladders recomputed from the rows' energies;
+  - every replica's directors unit vectors, spins +-1, and the last row's
+    Hamiltonian per site equal to the independent energy kernel's to 1e-12.
+* run() from two host threads at once on different lattices (each call takes
+  its own device lattice and its own run counter): each call's result equals
+  the oracle run with one of the two run counters, and the two differ.
+"""
+import ctypes as ct
+import math
+import os
+import threading
+
+import numpy as np
+import pytest
+import torch
+
+from chllmodel_amd import _lib
+from chllmodel_amd.autograd import energy_and_grad
+from chllmodel_amd.ensemble import Ensemble, phase_scan_replicas
+from chllmodel_amd.lattice import Lattice
+from chllmodel_amd.states import isotropic_state
+from oracle.oracle import ARITH_COMPAT, ARITH_FP64, BC_OPEN, BC_PERIODIC, RNG_PHILOX, Oracle, copy_state
+
+pytestmark = pytest.mark.gpu
+
+
+def host_threads():
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
+
+
+def f32_ulp(x):
+    return float(np.spacing(np.float32(abs(x))))
+
+
+@pytest.mark.parametrize("arith", ["compat", "fp64"])
+def test_l32_periodic_1000_sweeps_match_oracle(arith):
+    L, K, g, kT, d0, seed = 32, 0.5, 0.0, 1.0, 0.3, 20261203
+    nsw, nout = 1000, 10
+    st0 = isotropic_state(L, L, L, seed)
+    print(f"L=32 periodic, {arith}: 1000 GPU sweeps", flush=True)
+    with Lattice(L, L, L, bc="periodic") as lat:
+        lat.upload_state(st0)
+        lat.set_step(d0)
+        out = lat.sweeps(1, nsw, nout, g=g, KK=K, kbt=kT, rng="philox", arith=arith, seed=seed,
+                         want_counts=True)
+        got = lat.download()
+        d_gpu = lat.get_step()
+    print(f"L=32 periodic, {arith}: oracle", flush=True)
+    st = copy_state(st0)
+    ms, m, d_ref, _, counts = Oracle().run(st, 1, nsw, nout, g, K, d0, kT, rng=RNG_PHILOX, seed=seed,
+                                           bc=BC_PERIODIC, arith=ARITH_COMPAT if arith == "compat" else ARITH_FP64,
+                                           want_counts=True, threads=host_threads())
+    rows = out["rows"]
+    assert rows.shape[0] == nsw // nout == len(ms)
+    assert np.array_equal(out["counts"], counts)
+    assert np.array_equal(rows[:, 0].astype(np.int32), ms)
+    for r in range(len(ms)):
+        assert abs(rows[r, 1] - m[r, 0]) <= f32_ulp(m[r, 0]), r
+    assert np.array_equal(rows[:, 2:6], m[:, 1:5])
+    assert d_gpu == d_ref
+    assert np.array_equal(got["s"], st["s"])
+    err = max(np.abs(got[f] - st[f]).max() for f in ("nx", "ny", "nz"))
+    assert err <= 1e-12, err
+    # the run is not trivial: both kinds of trial are accepted and rejected
+    pacc = counts[:, 0] / L ** 3
+    facc = counts[:, 1] / L ** 3
+    assert np.all((pacc > 0.05) & (pacc < 0.95)) and np.all((facc > 0.05) & (facc < 0.95))
+
+
+def _swap_rounds(rep_slot, slot_kbt, energies, lad_start, rnd, seed, run_index, orc):
+    """Slot permutation after one swap round (the parallel-tempering rule
+    min(1, exp((1/kT_t - 1/kT_{t+1}) (E_a - E_b))), even or odd neighbour pairs
+    of each ladder by round parity, uniforms from Philox counter
+    (slot, round, run_index, 0x50540000))."""
+    slot_rep = np.argsort(rep_slot)
+    for lo, hi in zip(lad_start[:-1], lad_start[1:]):
+        for t in range(lo + (rnd & 1), hi - 1, 2):
+            a, c = slot_rep[t], slot_rep[t + 1]
+            delta = (1 / slot_kbt[t] - 1 / slot_kbt[t + 1]) * (energies[a] - energies[c])
+            w = orc.philox([t, rnd, run_index, 0x50540000], [seed & 0xFFFFFFFF, seed >> 32])
+            u = ((int(w[1]) << 32 | int(w[0])) >> 11) * 2.0 ** -53
+            if delta >= 0 or u < math.exp(delta):
+                slot_rep[t], slot_rep[t + 1] = c, a
+    return np.argsort(slot_rep)
+
+
+@pytest.mark.timeout(900)
+def test_phase_scan_64_replicas():
+    reps = phase_scan_replicas()
+    n = len(reps)
+    assert n == 64 and sum(r["ni"] * r["nj"] * r["nk"] for r in reps) == 41938944
+    lad_start = [0]
+    for i in range(1, n):
+        if reps[i]["ladder"] != reps[i - 1]["ladder"]:
+            lad_start.append(i)
+    lad_start.append(n)
+    small = range(lad_start[0], lad_start[1])  # the L = 32 ladder
+    assert all(reps[r]["ni"] == 32 for r in small) and len(small) == 13
+    orc = Oracle()
+    seed, run_index, K, nblocks = 20261203, 0, 10, 3
+    print("phase scan: initial states", flush=True)
+    states = {r: isotropic_state(32, 32, 32, seed + r) for r in small}
+    slot_kbt = np.array([r["kbt"] for r in reps])
+    slot_d = np.array([r["d0"] for r in reps])
+    rep_slot = np.arange(n)
+    with Ensemble(reps, bc="open", arith="fp64") as ens:
+        ens.init_isotropic(seed)  # replica r from isotropic_state(..., seed + r)
+        for b in range(nblocks):
+            print(f"phase scan: block {b}", flush=True)
+            rows = ens.run(1, K, seed=seed, run_index=run_index, adapt=True, use_graph=True)
+            st = ens.state()
+            E = np.array([rows[0, r, 2] * reps[r]["ni"] * reps[r]["nj"] * reps[r]["nk"] for r in range(n)])
+            for r in small:
+                s = rep_slot[r]
+                _, _, d_out, _ = orc.run(states[r], 1 + b * K, (b + 1) * K, 10 ** 6, reps[r]["g"], reps[r]["KK"],
+                                         slot_d[s], slot_kbt[s], rng=RNG_PHILOX, seed=seed, run_index=run_index,
+                                         replica=r, bc=BC_OPEN, arith=ARITH_FP64, threads=host_threads())
+                slot_d[s] = d_out
+                got = ens.download(r)
+                assert np.array_equal(got["s"], states[r]["s"]), (b, r)
+                err = max(np.abs(got[f] - states[r][f]).max() for f in ("nx", "ny", "nz"))
+                assert err <= 1e-12, (b, r, err)
+                E[r] = orc.energy_true(states[r], reps[r]["g"], reps[r]["KK"])
+                assert abs(rows[0, r, 2] - E[r] / 32 ** 3) < 1e-12, (b, r)
+                assert rows[0, r, 13] == s and rows[0, r, 1] == slot_kbt[s]
+            assert np.array_equal(st["slot_d"][list(small)], slot_d[list(small)])
+            rep_slot = _swap_rounds(rep_slot, slot_kbt, E, lad_start, b + 1, seed, run_index, orc)
+            assert np.array_equal(st["rep_slot"], rep_slot), (b, st["rep_slot"], rep_slot)
+        assert st["istep"] == nblocks * K + 1
+        print("phase scan: invariants and energies of all 64 replicas", flush=True)
+        for r in range(n):
+            got = ens.download(r)
+            N = reps[r]["ni"] * reps[r]["nj"] * reps[r]["nk"]
+            assert np.all(np.isin(got["s"], (-1, 1))), r
+            norm = np.sqrt(got["nx"] ** 2 + got["ny"] ** 2 + got["nz"] ** 2)
+            assert np.abs(norm - 1.0).max() < 1e-12, r
+            nt = torch.from_numpy(np.stack([got[f].transpose(2, 1, 0) for f in ("nx", "ny", "nz")])[None]).cuda()
+            sg = torch.from_numpy(got["s"].transpose(2, 1, 0).astype(np.int8)[None]).cuda()
+            Ek, *_ = energy_and_grad(nt, sg, reps[r]["KK"], reps[r]["g"], 1.0, want_grad=False)
+            e_kernel = float(Ek[0]) / N
+            assert abs(rows[0, r, 2] - e_kernel) <= 1e-12 * max(1.0, abs(e_kernel)), (r, rows[0, r, 2], e_kernel)
+
+
+def _dropin(st, nstart, nstop, nout, g, KK, d, kbt):
+    ni, nj, nk = st["nx"].shape
+    nsub = ni * nj * nk // 2
+    rows = (1 + nstop - nstart) // nout
+    msteps = np.zeros(max(rows, 1), dtype=np.int32)
+    m = np.zeros((max(rows, 1), 5), dtype=np.float64, order="F")
+    sl1 = np.zeros(nsub, np.int32)
+    sl2 = np.zeros(nsub, np.int32)
+    dd = ct.c_double(d)
+    _lib.lib().run(ct.c_int(nstart), ct.c_int(nstop), ct.c_int(nout), st["nx"], st["ny"], st["nz"], st["s"],
+                   st["dope"], sl1, sl2, ct.c_double(g), ct.c_double(KK), dd, ct.c_double(kbt),
+                   ct.c_int(ni), ct.c_int(nj), ct.c_int(nk), ct.c_int(0), msteps, m)
+    return msteps[:rows], m[:rows], dd.value, _lib.last_error()
+
+
+def test_dropin_run_two_threads():
+    L = _lib.lib()
+    cases = [((16, 12, 10), 31), ((8, 8, 8), 32)]
+    st0 = [isotropic_state(*dims, sd) for dims, sd in cases]
+    work = [copy_state(s) for s in st0]
+    res = [None, None]
+    _lib.check(L.chll_configure(0, 1, 0, 0))
+    _lib.check(L.chll_set_seed(4242, 0))
+    try:
+        def call(i):
+            res[i] = _dropin(work[i], 1, 40, 10, 0.5, 1.0, 0.3, 0.7)
+
+        print("drop-in run() from two threads", flush=True)
+        th = [threading.Thread(target=call, args=(i,)) for i in range(2)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+    finally:
+        _lib.check(L.chll_configure(0, 0, 0, 1))
+    orc = Oracle()
+    used = []
+    for i in range(2):
+        ms, m, d_out, err = res[i]
+        assert err == "", err
+        match = None
+        for run_index in (0, 1):
+            b = copy_state(st0[i])
+            oms, om, od, _ = orc.run(b, 1, 40, 10, 0.5, 1.0, 0.3, 0.7, rng=RNG_PHILOX, seed=4242,
+                                     run_index=run_index, arith=ARITH_FP64)
+            if np.array_equal(b["s"], work[i]["s"]):
+                match = run_index
+                assert np.array_equal(ms, oms) and d_out == od
+                assert np.array_equal(m[:, 1:], om[:, 1:])
+                for f in ("nx", "ny", "nz"):
+                    assert np.abs(b[f] - work[i][f]).max() <= 1e-12
+                break
+        assert match is not None, i
+        used.append(match)
+    assert sorted(used) == [0, 1]

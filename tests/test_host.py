@@ -121,3 +121,30 @@ def test_chllsim_folders_and_params(tmp_path, monkeypatch):
     ChLLSim(4, 4, 4, kbt=0.7, d=0.3, KK=1.0, g=0.5, rho=0.0, name="t1", overwrite=True)
     # name=None: nothing saved, no error
     ChLLSim(4, 4, 4, kbt=0.7, d=0.3, KK=1.0, g=0.5, rho=0.0).save_params(1, 1)
+
+
+def test_chllsim_plot_config(tmp_path, monkeypatch):
+    pytest.importorskip("matplotlib")
+    from chllmodel_amd.chll import ChLLSim
+
+    monkeypatch.chdir(tmp_path)
+    os.makedirs("data")
+    sim = ChLLSim(6, 4, 5, kbt=0.7, d=0.3, KK=1.0, g=0.5, rho=0.0, name="pic")
+    sim.nx[:] = 1.0
+    sim.s[:] = np.where(np.arange(6)[:, None, None] % 2 == 0, 1, -1)
+    out = sim.plot_config()
+    assert out == os.path.join("./data", "pic", "pic.png")
+    with open(out, "rb") as f:
+        assert f.read(8) == b"\x89PNG\r\n\x1a\n"
+    # no output folder: an explicit path is required
+    with pytest.raises(ValueError):
+        ChLLSim(4, 4, 4, kbt=0.7, d=0.3, KK=1.0, g=0.5, rho=0.0).plot_config()
+
+
+def test_chllsim_missing_data_folder_exits(tmp_path, monkeypatch):
+    from chllmodel_amd.chll import ChLLSim
+
+    monkeypatch.chdir(tmp_path)
+    with pytest.raises(SystemExit):
+        ChLLSim(4, 4, 4, kbt=0.7, d=0.3, KK=1.0, g=0.5, rho=0.0, name="x")
+
