@@ -22,6 +22,7 @@
 // n + (b*3 + c)*N (i fastest), spins int8 at s + b*N.
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "../../include/chll_mod.h"
@@ -338,6 +339,14 @@ __global__ __launch_bounds__(256) void k_energy_final(GradArgs A, double *E, dou
 
 using namespace chll;
 
+// Workspace of chll_energy_grad_noalloc: CHLL_ENERGY_SCRATCH_BYTES per GPU in
+// the code object's own device memory, held from the library's first use on
+// that GPU until the process ends.  Host code must not pass `scratch` itself
+// to a kernel (in host code it names a host-side stand-in): it uses the
+// device address that hipGetSymbolAddress returns, looked up once per device
+// with that device current (scratch_address).
+static __device__ __align__(256) double scratch[CHLL_ENERGY_SCRATCH_BYTES / sizeof(double)];
+
 namespace {
 thread_local std::string g_gerr;
 int gfail(const char *fmt, ...)
@@ -363,19 +372,31 @@ size_t chll_energy_workspace_bytes(int32_t batch, int32_t ni, int32_t nj, int32_
     return (size_t)batch * nblk * 4 * sizeof(double);
 }
 
-int chll_energy_grad(const double *n, const int8_t *s, int32_t batch, int32_t ni, int32_t nj, int32_t nk, int32_t bc,
-                     const double *K, const double *g, const double *eps, double *E, double *dEdn, double *dEdK,
-                     double *dEdg, double *dEdeps, void *workspace, size_t workspace_bytes, void *stream)
+}  // extern "C"
+
+namespace {
+
+int check_grad_args(const char *who, const double *n, const int8_t *s, int32_t batch, int32_t ni, int32_t nj,
+                    int32_t nk, int32_t bc, const double *K, const double *g, const double *eps)
 {
-    g_gerr.clear();
-    if (!n || !s || !K || !g || !eps || !workspace) return gfail("chll_energy_grad: null argument");
-    if (batch < 1 || batch > 65535 || ni < 1 || nj < 1 || nk < 1) return gfail("chll_energy_grad: bad shape");
-    if (bc != BC_OPEN && bc != BC_PERIODIC) return gfail("chll_energy_grad: unknown boundary condition %d", bc);
+    if (!n || !s || !K || !g || !eps) return gfail("%s: null argument", who);
+    if (batch < 1 || batch > 65535 || ni < 1 || nj < 1 || nk < 1) return gfail("%s: bad shape", who);
+    if (bc != BC_OPEN && bc != BC_PERIODIC) return gfail("%s: unknown boundary condition %d", who, bc);
     const uint64_t N = (uint64_t)ni * nj * nk;
-    if (N >= (1ull << 32) / 3) return gfail("chll_energy_grad: lattice too large");
-    if (workspace_bytes < chll_energy_workspace_bytes(batch, ni, nj, nk)) return gfail("chll_energy_grad: workspace too small");
+    if (N >= (1ull << 32) / 3) return gfail("%s: lattice too large", who);
+    return 0;
+}
+
+// Both entry points: the gather kernel writes every block's partial sums into
+// part (batch x nblk x 4 doubles, each written once before it is read), the
+// final kernel reduces them in a fixed order.
+int grad_launch(const double *n, const int8_t *s, int32_t batch, int32_t ni, int32_t nj, int32_t nk, int32_t bc,
+                const double *K, const double *g, const double *eps, double *E, double *dEdn, double *dEdK,
+                double *dEdg, double *dEdeps, double *part, void *stream)
+{
+    const uint64_t N = (uint64_t)ni * nj * nk;
     GradArgs A;
-    A.n = n; A.s = s; A.K = K; A.g = g; A.eps = eps; A.dEdn = dEdn; A.part = (double *)workspace;
+    A.n = n; A.s = s; A.K = K; A.g = g; A.eps = eps; A.dEdn = dEdn; A.part = part;
     A.ni = ni; A.nj = nj; A.nk = nk; A.N = (uint32_t)N;
     A.fd_ni = make_fastdiv(ni); A.fd_nj = make_fastdiv(nj); A.bc = bc;
     // two sites per thread when every row splits into aligned pairs
@@ -399,6 +420,59 @@ int chll_energy_grad(const double *n, const int8_t *s, int32_t batch, int32_t ni
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return gfail("chll_energy_grad launch: %s", hipGetErrorString(e));
     return 0;
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+int scratch_address(double **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return gfail("chll_energy_grad_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return gfail("chll_energy_grad_noalloc: device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return gfail("chll_energy_grad_noalloc: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_dev[dev] = (double *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chll_energy_grad(const double *n, const int8_t *s, int32_t batch, int32_t ni, int32_t nj, int32_t nk, int32_t bc,
+                     const double *K, const double *g, const double *eps, double *E, double *dEdn, double *dEdK,
+                     double *dEdg, double *dEdeps, void *workspace, size_t workspace_bytes, void *stream)
+{
+    g_gerr.clear();
+    if (check_grad_args("chll_energy_grad", n, s, batch, ni, nj, nk, bc, K, g, eps)) return -1;
+    if (!workspace) return gfail("chll_energy_grad: null argument");
+    if (workspace_bytes < chll_energy_workspace_bytes(batch, ni, nj, nk)) return gfail("chll_energy_grad: workspace too small");
+    return grad_launch(n, s, batch, ni, nj, nk, bc, K, g, eps, E, dEdn, dEdK, dEdg, dEdeps, (double *)workspace, stream);
+}
+
+int chll_energy_grad_noalloc(const double *n, const int8_t *s, int32_t batch, int32_t ni, int32_t nj, int32_t nk,
+                             int32_t bc, const double *K, const double *g, const double *eps, double *E, double *dEdn,
+                             double *dEdK, double *dEdg, double *dEdeps, void *stream)
+{
+    g_gerr.clear();
+    if (check_grad_args("chll_energy_grad_noalloc", n, s, batch, ni, nj, nk, bc, K, g, eps)) return -1;
+    const size_t need = chll_energy_workspace_bytes(batch, ni, nj, nk);
+    if (need > CHLL_ENERGY_SCRATCH_BYTES)
+        return gfail("chll_energy_grad_noalloc: batch %d of %dx%dx%d needs %zu bytes of scratch, the static buffer "
+                     "holds %zu (use chll_energy_grad with a workspace)",
+                     batch, ni, nj, nk, need, (size_t)CHLL_ENERGY_SCRATCH_BYTES);
+    double *part = nullptr;
+    if (scratch_address(&part)) return -1;
+    return grad_launch(n, s, batch, ni, nj, nk, bc, K, g, eps, E, dEdn, dEdK, dEdg, dEdeps, part, stream);
 }
 
 }  // extern "C"

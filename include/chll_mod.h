@@ -229,6 +229,21 @@ int chll_energy_grad(const double *n, const int8_t *s, int32_t batch, int32_t ni
                      const double *K, const double *g, const double *eps, double *E, double *dEdn, double *dEdK,
                      double *dEdg, double *dEdeps, void *workspace, size_t workspace_bytes, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation): chll_energy_grad with its workspace taken from a static
+ * device buffer of CHLL_ENERGY_SCRATCH_BYTES per GPU (held until the process
+ * ends) instead of a caller argument.  Same kernels and results.  Fails, with
+ * chll_energy_last_error() set and nothing launched, when
+ * chll_energy_workspace_bytes(batch, ni, nj, nk) exceeds the buffer.
+ * Allocates nothing.  All calls on one GPU share the buffer: calls that
+ * overlap on one GPU (concurrent streams or host threads) race; calls in one
+ * stream's order do not. */
+#define CHLL_ENERGY_SCRATCH_BYTES ((size_t)8 << 20)
+int chll_energy_grad_noalloc(const double *n, const int8_t *s, int32_t batch, int32_t ni, int32_t nj, int32_t nk,
+                             int32_t bc, const double *K, const double *g, const double *eps, double *E, double *dEdn,
+                             double *dEdK, double *dEdg, double *dEdeps, void *stream);
+
 /* Analysis ------------------------------------------------------------- */
 /* Device pointers, one lattice in plain Fortran order (i fastest), float64
  * director components, int8 spins.  Functions return 0 or -1 with
