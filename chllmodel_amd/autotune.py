@@ -24,6 +24,8 @@ from .states import isotropic_state
 
 
 def candidates(ni, nj, nk):
+    if ni % 2:
+        return [("site", 0, 1)]  # odd ni: the one-site kernel in lattice order only
     out = [("site", 0, 1), ("site", 1, 1)]
     if ni % 4 == 0:
         out += [("site", 2, 1), ("site", 3, 1)]  # tile | 4 is tried on whole fused sweeps

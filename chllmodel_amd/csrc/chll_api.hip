@@ -112,13 +112,14 @@ namespace {
 
 int check_geometry(int32_t ni, int32_t nj, int32_t nk, int32_t bc)
 {
-    if (ni < 2 || nj < 1 || nk < 1) return fail("lattice dims must be ni>=2, nj>=1, nk>=1 (got %d,%d,%d)", ni, nj, nk);
-    if (ni % 2) return fail("ni must be even for the colour-split layout (got %d)", ni);
+    if (ni < 1 || nj < 1 || nk < 1) return fail("lattice dims must be positive (got %d,%d,%d)", ni, nj, nk);
     const uint64_t n3 = (uint64_t)ni * nj * nk;
+    // the reference's init writes ceil(N/2) entries into lists of N/2 (chll/chll_mod.f90:23-36, chll/chll.py:54)
+    if (n3 % 2) return fail("N = ni*nj*nk must be even (got %d,%d,%d)", ni, nj, nk);
     if (n3 >= (1ull << 32)) return fail("lattice too large: N = %llu sites (limit 2^32)", (unsigned long long)n3);
     if (bc != BC_OPEN && bc != BC_PERIODIC) return fail("unknown boundary condition %d", bc);
-    if (bc == BC_PERIODIC && ((nj % 2) || (nk % 2)))
-        return fail("periodic boundaries need even nj and nk for a valid checkerboard (got %d,%d)", nj, nk);
+    if (bc == BC_PERIODIC && ((ni % 2) || (nj % 2) || (nk % 2)))
+        return fail("periodic boundaries need even ni, nj and nk for a valid checkerboard (got %d,%d,%d)", ni, nj, nk);
     return 0;
 }
 
@@ -171,8 +172,10 @@ int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lat
     G.plane = G.half * G.nj;
     G.nsub = (uint32_t)(((uint64_t)ni * nj * nk) / 2);
     G.zero = 2 * G.nsub;
-    G.fd_half = make_fastdiv(G.half);
+    G.nplane = G.ni * G.nj;
+    G.fd_half = make_fastdiv(G.half ? G.half : 1u);
     G.fd_nj = make_fastdiv(G.nj);
+    G.fd_ni = make_fastdiv(G.ni);
     L->bc = bc;
     const size_t ns = G.nsub, n3 = 2 * ns;
     L->nblk = phase_blocks(G);
@@ -239,6 +242,9 @@ int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lat
                     order = pairs ? 3 : 1;
                 }
                 L->kcfg[kind][ar] = {march ? 1 : 0, march ? tile : order, ns};
+                // odd ni: rows hold (ni +- 1) / 2 sites of a colour, so only the
+                // one-site kernels in lattice order apply
+                if (G.ni & 1u) L->kcfg[kind][ar] = {0, 0, 1};
             }
     }
     *out = L;
@@ -258,6 +264,8 @@ int chll_lattice_set_kernel(chll_lattice *L, int32_t kind, int32_t arith, int32_
     if (variant == 1 && (tile < 0 || tile > 5))
         return fail("tile must be 0..2 (k-march 64x4, 32x8, 16x16) or 3..5 (plane tiles)");
     if (nslab < 1) return fail("nslab must be >= 1");
+    if ((L->G.ni & 1u) && (variant != 0 || tile != 0))
+        return fail("odd ni (%u) takes only the one-site kernel in lattice order (variant 0, tile 0)", L->G.ni);
     L->kcfg[kind][arith] = {variant, tile, (uint32_t)nslab > L->G.nk ? L->G.nk : (uint32_t)nslab};
     return 0;
 }

@@ -56,10 +56,15 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f)
 // Colour-split structure-of-arrays layout.  Site (i,j,k) (0-based, Fortran
 // order l = i + ni*(j + nj*k)) has colour c = (i+j+k) & 1 -- colour 0 is the
 // reference's sl1 (1-based i+j+k odd), colour 1 is sl2 (chll/chll_mod.f90:23-36)
-// -- and packed index m = l >> 1 within its colour.  With ni even, m is also
-// the site's rank in its sublattice list, i.e. the index of its uniforms in
-// rand1 / rand2.  Each field stores colour 0 at [0, nsub) and colour 1 at
-// [nsub, 2 nsub).
+// -- and packed index m = l >> 1 within its colour.  For every shape with N
+// even, m is also the site's rank in its sublattice list, i.e. the index of
+// its uniforms in rand1 / rand2: the colour alternates along l inside rows,
+// across row ends when ni is odd and across plane ends when ni + nj is even;
+// where it does not, the row (ni even) or plane (ni odd, nj even) just ended
+// holds an even number of sites, half of each colour.  With ni even a row
+// holds ni / 2 sites of each colour at consecutive m; with ni odd it holds
+// (ni +- 1) / 2, and the kernels decode m through l (decode, neighbours).
+// Each field stores colour 0 at [0, nsub) and colour 1 at [nsub, 2 nsub).
 //
 // Index 2 nsub of every field is a padding "zero site": director (0,0,0),
 // spin 0.  Absent neighbours (open boundaries) point there, so a bond to it
@@ -68,19 +73,40 @@ constexpr uint32_t kFieldPad = 64;  // padding elements after the 2 nsub sites
 
 struct Geom {
     uint32_t ni, nj, nk;
-    uint32_t half;   // ni / 2: packed sites per row and colour
-    uint32_t plane;  // ni * nj / 2: packed sites per plane and colour
+    uint32_t half;   // ni / 2: packed sites per row and colour (ni even)
+    uint32_t plane;  // ni * nj / 2: packed sites per plane and colour (ni even)
     uint32_t nsub;   // N / 2
     uint32_t zero;   // 2 nsub: index of the zero site
-    FastDiv fd_half, fd_nj;
+    uint32_t nplane; // ni * nj: sites per plane
+    FastDiv fd_half, fd_nj, fd_ni;
 };
 
 struct Site {
     uint32_t i, j, k;
 };
 
+// Site of Fortran linear index l.
+__device__ inline Site decode_l(const Geom &G, uint32_t l)
+{
+    const uint32_t row = fdiv(l, G.fd_ni);
+    const uint32_t k = fdiv(row, G.fd_nj);
+    return Site{l - row * G.ni, row - k * G.nj, k};
+}
+
 __device__ inline Site decode(const Geom &G, uint32_t m, int colour)
 {
+    if (G.ni & 1u) {
+        // the site of this colour among l = 2m, 2m + 1 (adjacent in Fortran
+        // order, of different colours)
+        Site s = decode_l(G, 2u * m);
+        if (((s.i + s.j + s.k) & 1u) != (uint32_t)colour) {
+            if (++s.i == G.ni) {
+                s.i = 0;
+                if (++s.j == G.nj) { s.j = 0; ++s.k; }
+            }
+        }
+        return s;
+    }
     uint32_t row = fdiv(m, G.fd_half);
     uint32_t h = m - row * G.half;
     uint32_t k = fdiv(row, G.fd_nj);
@@ -97,6 +123,18 @@ __device__ inline Site decode(const Geom &G, uint32_t m, int colour)
 template <int BC>
 __device__ inline uint32_t neighbours(const Geom &G, uint32_t m, const Site &s, uint32_t oth, uint32_t nb[6])
 {
+    if (BC == BC_OPEN && (G.ni & 1u)) {
+        // odd ni (open boundaries only): packed index of linear index l' is l' >> 1
+        const uint32_t l = s.i + G.ni * (s.j + G.nj * s.k);
+        uint32_t mask = 0;
+        nb[0] = s.i + 1 < G.ni ? (mask |= 1u, oth + ((l + 1u) >> 1)) : G.zero;
+        nb[1] = s.i >= 1 ? (mask |= 2u, oth + ((l - 1u) >> 1)) : G.zero;
+        nb[2] = s.j + 1 < G.nj ? (mask |= 4u, oth + ((l + G.ni) >> 1)) : G.zero;
+        nb[3] = s.j >= 1 ? (mask |= 8u, oth + ((l - G.ni) >> 1)) : G.zero;
+        nb[4] = s.k + 1 < G.nk ? (mask |= 16u, oth + ((l + G.nplane) >> 1)) : G.zero;
+        nb[5] = s.k >= 1 ? (mask |= 32u, oth + ((l - G.nplane) >> 1)) : G.zero;
+        return mask;
+    }
     const uint32_t p = s.i & 1u;
     const uint32_t o = oth + m;
     uint32_t mask = 0;

@@ -674,8 +674,10 @@ int chll_ensemble_create(const chll_replica_desc *reps, int32_t nrep, int32_t bc
         G.plane = G.half * G.nj;
         G.nsub = (uint32_t)(((uint64_t)G.ni * G.nj * G.nk) / 2);
         G.zero = 2 * G.nsub;
+        G.nplane = G.ni * G.nj;
         G.fd_half = make_fastdiv(G.half);
         G.fd_nj = make_fastdiv(G.nj);
+        G.fd_ni = make_fastdiv(G.ni);
         const size_t n = 2 * (size_t)G.nsub + kFieldPad;
         rc |= ealloc(&D.F.nx, n) | ealloc(&D.F.ny, n) | ealloc(&D.F.nz, n) | ealloc(&D.F.s, n);
         rc |= ealloc(&D.counts, (size_t)kCountWords);

@@ -26,10 +26,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter_f64(Geom G, const double *__
 {
     const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
     if (m >= G.nsub) return;
-    const uint32_t row = fdiv(m, G.fd_half);
-    const uint32_t k = fdiv(row, G.fd_nj);
-    const uint32_t j = row - k * G.nj;
-    const uint32_t c0 = (j + k) & 1u;  // colour of the site with even i (l = 2m)
+    const Site s0 = decode_l(G, 2u * m);
+    const uint32_t c0 = (s0.i + s0.j + s0.k) & 1u;  // colour of site l = 2m; site 2m + 1 has the other
     const double a = src[2ull * m], b = src[2ull * m + 1];
     dst[c0 * G.nsub + m] = a;
     dst[(c0 ^ 1u) * G.nsub + m] = b;
@@ -40,10 +38,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_f64(Geom G, const double *__r
 {
     const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
     if (m >= G.nsub) return;
-    const uint32_t row = fdiv(m, G.fd_half);
-    const uint32_t k = fdiv(row, G.fd_nj);
-    const uint32_t j = row - k * G.nj;
-    const uint32_t c0 = (j + k) & 1u;
+    const Site s0 = decode_l(G, 2u * m);
+    const uint32_t c0 = (s0.i + s0.j + s0.k) & 1u;
     dst[2ull * m] = src[c0 * G.nsub + m];
     dst[2ull * m + 1] = src[(c0 ^ 1u) * G.nsub + m];
 }
@@ -53,10 +49,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter_spin(Geom G, const int32_t *
 {
     const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
     if (m >= G.nsub) return;
-    const uint32_t row = fdiv(m, G.fd_half);
-    const uint32_t k = fdiv(row, G.fd_nj);
-    const uint32_t j = row - k * G.nj;
-    const uint32_t c0 = (j + k) & 1u;
+    const Site s0 = decode_l(G, 2u * m);
+    const uint32_t c0 = (s0.i + s0.j + s0.k) & 1u;
     const int32_t a = src[2ull * m], b = src[2ull * m + 1];
     // int8 storage holds the spin exactly when |s| <= 127 (so that -s fits too).
     if (a < -127 || a > 127 || b < -127 || b > 127) atomicOr(bad, 1);
@@ -69,10 +63,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_spin(Geom G, const int8_t *__
 {
     const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
     if (m >= G.nsub) return;
-    const uint32_t row = fdiv(m, G.fd_half);
-    const uint32_t k = fdiv(row, G.fd_nj);
-    const uint32_t j = row - k * G.nj;
-    const uint32_t c0 = (j + k) & 1u;
+    const Site s0 = decode_l(G, 2u * m);
+    const uint32_t c0 = (s0.i + s0.j + s0.k) & 1u;
     dst[2ull * m] = src[c0 * G.nsub + m];
     dst[2ull * m + 1] = src[(c0 ^ 1u) * G.nsub + m];
 }
@@ -380,7 +372,9 @@ void launch_gather_spin(const Geom &G, const int8_t *src, int32_t *dst, hipStrea
 
 uint32_t xcd_band(const Geom &G)
 {
-    // 8 equal bands of whole blocks per plane; every block then lies in one plane.
+    // 8 equal bands of whole blocks per plane; every block then lies in one
+    // plane (ni even: a plane holds ni nj / 2 packed sites of each colour)
+    if (G.ni & 1u) return 0u;
     return G.plane % (8u * kBlock) == 0u ? G.plane / (8u * kBlock) : 0u;
 }
 
@@ -389,7 +383,7 @@ bool field_off32(const Geom &G)
     return (2ull * G.nsub + kFieldPad) * sizeof(double) <= 0xffffffffull;
 }
 
-bool pairs_ok(const Geom &G) { return (G.half & 1u) == 0u && field_off32(G); }
+bool pairs_ok(const Geom &G) { return (G.ni & 3u) == 0u && field_off32(G); }
 
 // Pair kernels: nsub / 2 threads; XCD bands of whole blocks of pairs.
 static inline dim3 grid_pairs(const Geom &G) { return grid1(G.nsub / 2u); }

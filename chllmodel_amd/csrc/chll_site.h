@@ -488,9 +488,10 @@ struct MeasureAcc {
     long long s = 0;
 };
 
+// Packed index of site (i, j, k): l >> 1 (any shape with N even).
 __device__ inline uint32_t pidx(const Geom &G, uint32_t i, uint32_t j, uint32_t k)
 {
-    return (i >> 1) + G.half * (j + G.nj * k);
+    return (i + G.ni * (j + G.nj * k)) >> 1;
 }
 
 __device__ inline uint32_t colour_base(const Geom &G, uint32_t i, uint32_t j, uint32_t k)
@@ -510,18 +511,15 @@ __device__ inline double bond_term(double ax, double ay, double az, double bx, d
     return (1.0 - dd * dd) - KK * dd * cr * sf;
 }
 
-// Both sites (i = 2h and 2h+1) of packed index m.
+// Both sites of packed index m: linear indices l = 2m and 2m + 1.
 template <int BC>
 __device__ inline void measure_site(const Geom &G, const Fields &F, double g, double KK, uint32_t m, MeasureAcc &A)
 {
-    const uint32_t row = fdiv(m, G.fd_half);
-    const uint32_t h = m - row * G.half;
-    const uint32_t k = fdiv(row, G.fd_nj);
-    const uint32_t j = row - k * G.nj;
-    const uint32_t jp = (j + 1 == G.nj) ? 0u : j + 1;
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-        const uint32_t i = 2u * h + (uint32_t)b;
+        const Site st = decode_l(G, 2u * m + (uint32_t)b);
+        const uint32_t i = st.i, j = st.j, k = st.k;
+        const uint32_t jp = (j + 1 == G.nj) ? 0u : j + 1;
         const uint32_t ip = (i + 1 == G.ni) ? 0u : i + 1;
         const uint32_t c = colour_base(G, i, j, k) + m;
         const double ax = F.nx[c], ay = F.ny[c], az = F.nz[c];

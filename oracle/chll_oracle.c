@@ -289,25 +289,19 @@ static inline double src_at(usrc_t *U, uint64_t n)
 /* Rank of site (i, j, k) in its sublattice list sl1 (colour 0) or sl2 (colour
  * 1), i.e. the index r of its uniforms rand1(r+1), rand2(r+1, :) in the
  * reference's draw schedule (chll/chll_mod.f90:23-36, 131-134, 146, 188).
- * The lists hold the sites of one colour in Fortran order.  A row (j, k) holds
- * the sites i = p, p + 2, ... of colour c = (i + j + k) & 1, p = (c ^ j ^ k) & 1,
- * so the sites before (i, j, k) in its own row number i >> 1.  With ni even
- * every row holds ni / 2 sites of each colour and r = l >> 1 (l the 0-based
- * linear index).  With ni odd a row holds (ni + 1) / 2 sites of the colour of
- * its i = 0 site ((j + k) & 1) and (ni - 1) / 2 of the other. */
-static inline int64_t count_parity(int64_t n, int t) { return (n + 1 - t) >> 1; } /* x in [0,n), x = t mod 2 */
-
+ * The lists hold the sites of one colour in Fortran order, so r counts the
+ * sites of the same colour before linear index l = i + ni (j + nj k).  For
+ * every shape with N even, r = l >> 1: stepping l -> l + 1 changes i + j + k
+ * by 1 inside a row, by 2 - ni at a row end and by 3 - ni - nj at a plane end,
+ * so the colour alternates along l inside rows, across row ends when ni is
+ * odd and across plane ends when ni + nj is even.  Where it does not, the
+ * row (ni even) or plane (ni odd, nj even) just ended holds an even number of
+ * sites, half of each colour, and the next one starts at an even l.  Either
+ * way the sites of one colour before l number l >> 1.  (Checked exhaustively
+ * against the lists of oracle_init in the tests.) */
 int64_t oracle_rank(int ni, int nj, int i, int j, int k)
 {
-    const int64_t l = (int64_t)i + (int64_t)ni * ((int64_t)j + (int64_t)nj * k);
-    if ((ni & 1) == 0) return l >> 1;
-    const int c = (i + j + k) & 1;
-    /* rows before (j, k) whose i = 0 site has colour c: whole planes k' < k,
-     * then rows j' < j of plane k */
-    const int64_t kc = count_parity(k, c);
-    const int64_t rows_c = kc * ((nj + 1) >> 1) + ((int64_t)k - kc) * (nj >> 1) + count_parity(j, (c ^ k) & 1);
-    const int64_t rows = (int64_t)j + (int64_t)nj * k;
-    return rows * ((ni - 1) >> 1) + rows_c + (i >> 1);
+    return ((int64_t)i + (int64_t)ni * ((int64_t)j + (int64_t)nj * k)) >> 1;
 }
 
 /* ------------------------------------------------------------------------- */

@@ -13,7 +13,7 @@ and stores inputs and outputs:
     params                     [nstart, nstop, nout, g, KK, d, kbt, seed, stream_pos]
 
 Run in the build container only (needs /root/reference):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case names...]
 """
 from __future__ import annotations
 
@@ -38,11 +38,18 @@ CASES = [
     ("ref_2x2x2", (2, 2, 2), 16, 1, 40, 8, 0.5, 1.5, 0.4, 0.5, 106, "iso"),
     ("ref_4x4x4_adapt_stop", (4, 4, 4), 17, 9991, 10010, 5, 0.5, 1.0, 0.3, 0.2, 107, "iso"),
     ("ref_10x2x3_negK", (10, 2, 3), 18, 1, 16, 4, -0.75, -1.0, 0.5, 0.9, 108, "iso"),
+    # odd ni with N even: a row holds one more site of one colour than of the
+    # other, so a site's sublattice rank is not l / 2 (chll/chll_mod.f90:23-36)
+    ("ref_3x2x2_oddni", (3, 2, 2), 19, 1, 24, 6, 0.5, 1.0, 0.3, 0.7, 109, "iso"),
+    ("ref_5x4x6_oddni", (5, 4, 6), 20, 1, 15, 5, 0.25, 1.5, 0.3, 0.6, 110, "iso"),
+    ("ref_7x6x2_oddni", (7, 6, 2), 22, 1, 12, 4, 1.0, -0.8, 0.2, 1.1, 111, "iso"),
+    ("ref_9x3x4_oddni", (9, 3, 4), 23, 1, 10, 5, 0.5, 1.0, 0.3, 0.8, 112, "iso"),
 ]
 
 BUFFER_CASES = [
     # explicit host uniforms (numpy MT19937), reference draw schedule
     ("ref_buf_4x4x6", (4, 4, 6), 21, 1, 6, 2, 0.5, 1.0, 0.3, 0.8, 2024),
+    ("ref_buf_5x3x4_oddni", (5, 3, 4), 24, 1, 6, 3, 0.5, 1.0, 0.3, 0.8, 2025),
 ]
 
 
@@ -59,9 +66,12 @@ def make_state(dims, seed, kind):
     return st
 
 
-def main():
+def main(only=None):
+    """Write every case, or only the cases named in `only`."""
     ref = Reference(stream=True)
     for name, dims, iseed, nstart, nstop, nout, g, KK, d, kbt, sseed, kind in CASES:
+        if only and name not in only:
+            continue
         st0 = make_state(dims, iseed, kind)
         st = copy_state(st0)
         msteps, m, d_out, pos = ref.run(st, nstart, nstop, nout, g, KK, d, kbt, seed=sseed, stream_pos=0)
@@ -73,6 +83,8 @@ def main():
                             mode=np.array("stream"))
         print(name, "rows", len(msteps), "d_out", d_out)
     for name, dims, iseed, nstart, nstop, nout, g, KK, d, kbt, bseed in BUFFER_CASES:
+        if only and name not in only:
+            continue
         st0 = make_state(dims, iseed, "iso")
         st = copy_state(st0)
         nsub = int(np.prod(dims)) // 2
@@ -88,4 +100,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(set(sys.argv[1:]))

@@ -80,16 +80,18 @@ def _run_args(dims, nstart=1, nstop=4, nout=2, order="F", dtype=np.float64):
 
 def test_run_rejects_bad_inputs_before_device_work():
     L = _lib.lib()
-    # odd ni: the colour-split layout needs it even
-    L.run(*_run_args((3, 4, 4)))
-    assert "ni must be even" in _lib.last_error()
+    # odd N: the reference's init overflows its lists (odd ni with N even is accepted)
+    L.run(*_run_args((3, 3, 3)))
+    assert "must be even" in _lib.last_error()
     # nout = 0 would divide by zero in the reference (mod(istep, 0))
     L.run(*_run_args((4, 4, 4), nout=0))
     assert "nout must be positive" in _lib.last_error()
-    # periodic boundaries need even nj, nk
+    # periodic boundaries need even ni, nj, nk
     _lib.check(L.chll_configure(0, 0, 1, 0))
     try:
         L.run(*_run_args((4, 3, 4)))
+        assert "periodic" in _lib.last_error()
+        L.run(*_run_args((3, 4, 4)))
         assert "periodic" in _lib.last_error()
     finally:
         _lib.check(L.chll_configure(0, 0, 0, 1))

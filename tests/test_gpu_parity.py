@@ -99,6 +99,13 @@ CASES = [
     ((16, 8, 8), "open", "stream", "fp64", 5, 5),
     ((10, 6, 8), "periodic", "stream", "fp64", 6, 3),
     ((64, 64, 64), "open", "philox", "compat", 3, 3),
+    # odd ni (open boundaries): rows hold (ni +- 1) / 2 sites of a colour
+    ((5, 4, 6), "open", "philox", "compat", 6, 3),
+    ((7, 3, 4), "open", "philox", "fp64", 6, 2),
+    ((9, 5, 2), "open", "stream", "fp64", 5, 5),
+    ((3, 2, 2), "open", "stream", "compat", 8, 4),
+    ((63, 20, 12), "open", "philox", "fp64", 4, 2),
+    ((33, 17, 10), "open", "philox", "compat", 4, 4),
 ]
 
 
@@ -130,7 +137,8 @@ def test_matches_oracle(orc, dims, bc, rng, arith, nsw, nout):
 # 260: 65 pairs per row), single rows and planes, and measure blocks capped so
 # that block ranges split rows (132 x 100 x 90); half odd (one site pair per
 # thread): 10 x 6 x 4.
-MEASURE_SHAPES = [(12, 10, 8), (4, 3, 5), (260, 3, 2), (8, 1, 1), (10, 6, 4), (132, 100, 90)]
+MEASURE_SHAPES = [(12, 10, 8), (4, 3, 5), (260, 3, 2), (8, 1, 1), (10, 6, 4), (132, 100, 90), (5, 4, 6),
+                  (7, 3, 4), (3, 1, 2)]
 
 
 @pytest.mark.parametrize("dims", MEASURE_SHAPES, ids=["x".join(map(str, d)) for d in MEASURE_SHAPES])
@@ -138,8 +146,8 @@ def test_measure_row(orc, dims):
     st = isotropic_state(*dims, 77)
     g, K = 0.35, 1.7
     for bc, obc in (("open", BC_OPEN), ("periodic", BC_PERIODIC)):
-        if bc == "periodic" and (dims[1] % 2 or dims[2] % 2):
-            continue  # periodic boundaries need even nj and nk
+        if bc == "periodic" and (dims[0] % 2 or dims[1] % 2 or dims[2] % 2):
+            continue  # periodic boundaries need even ni, nj and nk
         with Lattice(*dims, bc=bc) as lat:
             lat.upload_state(st)
             row = lat.measure(g, K)
@@ -220,6 +228,29 @@ def test_dropin_run_edge_cases(orc, dims, nstart, nstop, nout):
         for f in ("nx", "ny", "nz", "s"):
             assert np.array_equal(a[f], st0[f]), f
         assert d_out == 0.3
+
+
+@pytest.mark.parametrize("name", ["ref_5x4x6_oddni", "ref_7x6x2_oddni"])
+def test_dropin_run_odd_ni_golden(name):
+    """The drop-in run() on an odd-ni lattice (chll/chll_mod.f90:23-36 builds
+    its lists for any N even) in the deterministic stream mode reproduces the
+    reference's own outputs on that shape."""
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"), allow_pickle=False)
+    st = {k: np.asfortranarray(z[k + "0"].copy()) for k in ("nx", "ny", "nz", "s")}
+    st["dope"] = np.zeros_like(st["s"])
+    p = z["params"]
+    L = _lib.lib()
+    _lib.check(L.chll_configure(1, 0, 0, 0))
+    _lib.check(L.chll_set_stream(int(p[7]), 0))
+    print(f"drop-in run() {name}", flush=True)
+    try:
+        ms, m, d_out, ns = _dropin_run(st, int(p[0]), int(p[1]), int(p[2]), p[3], p[4], p[5], p[6])
+    finally:
+        _lib.check(L.chll_configure(0, 0, 0, 1))
+    assert ns == st["s"].size // 2
+    assert_rows_match(np.column_stack([ms, m, np.zeros((len(ms), 10))]), z["msteps"], z["m"])
+    assert d_out == float(z["d_out"])
+    assert_state_match(st, {k: z[k] for k in ("nx", "ny", "nz", "s")})
 
 
 def test_dropin_run_stream_mode(orc):

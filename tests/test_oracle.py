@@ -180,6 +180,27 @@ def test_init_lists(orc):
         assert np.array_equal(sl2[: (~odd).sum()], idx[~odd])
 
 
+def test_rank_is_half_the_linear_index(orc):
+    """The uniform index of a site is its position in its sublattice list
+    (chll/chll_mod.f90:131-134, 146): the inverse of the lists equals the
+    closed form l >> 1 for every shape with N even, odd ni and nj included,
+    exhaustively on small shapes."""
+    for ni in range(1, 8):
+        for nj in range(1, 6):
+            for nk in range(1, 5):
+                if (ni * nj * nk) % 2:
+                    continue
+                sl1, sl2 = orc.init(ni, nj, nk)
+                half = ni * nj * nk // 2
+                for lst in (sl1[:half], sl2[:half]):
+                    l = lst.astype(np.int64) - 1
+                    i, j, k = l % ni, (l // ni) % nj, l // (ni * nj)
+                    want = np.arange(half)
+                    assert np.array_equal(l >> 1, want), (ni, nj, nk)
+                    for q in range(0, half, max(1, half // 7)):
+                        assert orc.rank(ni, nj, int(i[q]), int(j[q]), int(k[q])) == want[q]
+
+
 # --------------------------------------------------------------------------
 GOLDEN_FILES = sorted(glob.glob(os.path.join(GOLDEN, "ref_*.npz")))
 
