@@ -567,7 +567,6 @@ def batched_scan(args, stream):
 
     from chllmodel_amd.ensemble import Ensemble, autotune, phase_scan_replicas
     from chllmodel_amd.lattice import Lattice
-    from chllmodel_amd.states import isotropic_state
 
     reps = phase_scan_replicas()
     sites = sum(r["ni"] * r["nj"] * r["nk"] for r in reps)
@@ -578,7 +577,7 @@ def batched_scan(args, stream):
     ens.set_stream(stream.cuda_stream)
     ens.set_block(tune["best"])
     ens.set_chunk(tune["chunk"])
-    ens.init_isotropic(20261203)
+    ens.init_state("isotropic", seed=20261203)
     ens.run(2, Kb, seed=20261203, want_rows=False, use_graph=True)
     ens.sync()
     torch.cuda.synchronize()
@@ -597,7 +596,7 @@ def batched_scan(args, stream):
     for i, r in enumerate(reps):
         lat = Lattice(r["ni"], r["nj"], r["nk"])
         lat.set_stream(stream.cuda_stream)
-        lat.upload_state(isotropic_state(r["ni"], r["nj"], r["nk"], 20261203 + i))
+        lat.init_state("isotropic", seed=20261203, replica=i)
         lat.set_step(r["d0"])
         p = lat.params(g=r["g"], KK=r["KK"], kbt=r["kbt"], arith=args.arith, seed=20261203, replica=i)
         lat.sweeps(1, 2, Kb, params=p, want_rows=False)
@@ -648,7 +647,7 @@ def batched_observables(args, stream, reps, Kb, tune):
     ens.set_stream(stream.cuda_stream)
     ens.set_block(tune["best"])
     ens.set_chunk(tune["chunk"])
-    ens.init_isotropic(20261203)
+    ens.init_state("isotropic", seed=20261203)
     t0 = time.perf_counter()
     rows = ens.run(nb, Kb, seed=20261203, run_index=1, want_rows=True, use_graph=True)
     dt = time.perf_counter() - t0

@@ -24,6 +24,7 @@ ROW_COLS = 16
 ROW_NAMES = ("istep", "E", "eta", "paccept", "faccept", "d", "E_true", "eta_exact",
              "Qxx", "Qyy", "Qzz", "Qxy", "Qxz", "Qyz", "naccept", "nflip")
 TIMING_KINDS = ("director", "chirality", "measure", "finalize", "fill", "fused", "fused_next")
+INIT_KINDS = {"isotropic": 0, "aligned_x": 1, "aligned_xz": 2}
 
 _f64p = ct.POINTER(ct.c_double)
 _i32p = ct.POINTER(ct.c_int32)
@@ -149,6 +150,7 @@ def lib() -> ct.CDLL:
     L.chll_lattice_set_stream.argtypes = [ct.c_void_p, ct.c_void_p]
     L.chll_lattice_upload.argtypes = [ct.c_void_p, _f64p, _f64p, _f64p, _i32p]
     L.chll_lattice_download.argtypes = [ct.c_void_p, _f64p, _f64p, _f64p, _i32p]
+    L.chll_lattice_init_state.argtypes = [ct.c_void_p, ct.c_int32, ct.c_uint64, ct.c_uint32, ct.c_double]
     L.chll_lattice_set_step.argtypes = [ct.c_void_p, ct.c_double]
     L.chll_lattice_get_step.argtypes = [ct.c_void_p, _f64p]
     L.chll_lattice_sweeps.argtypes = [ct.c_void_p, ct.POINTER(ChllParams), ct.c_int32, ct.c_int32, ct.c_int32,
@@ -163,7 +165,7 @@ def lib() -> ct.CDLL:
     L.chll_lattice_get_fusion.argtypes = [ct.c_void_p, _i32p]
     for name in ("chll_configure", "chll_set_seed", "chll_set_stream", "chll_set_uniform_buffer",
                  "chll_lattice_create", "chll_lattice_destroy", "chll_lattice_set_stream",
-                 "chll_lattice_upload", "chll_lattice_download", "chll_lattice_set_step",
+                 "chll_lattice_upload", "chll_lattice_download", "chll_lattice_init_state", "chll_lattice_set_step",
                  "chll_lattice_get_step", "chll_lattice_sweeps", "chll_lattice_measure", "chll_lattice_sync",
                  "chll_lattice_set_timing", "chll_lattice_timing", "chll_lattice_set_kernel",
                  "chll_lattice_get_kernel", "chll_lattice_set_fusion", "chll_lattice_get_fusion"):

@@ -371,6 +371,17 @@ int chll_lattice_download(chll_lattice *L, double *nx, double *ny, double *nz, i
     return 0;
 }
 
+int chll_lattice_init_state(chll_lattice *L, int32_t kind, uint64_t seed, uint32_t replica, double p_flip)
+{
+    if (!L) return fail("null lattice");
+    if (kind < CHLL_INIT_ISOTROPIC || kind > CHLL_INIT_ALIGNED_XZ) return fail("unknown initial state kind %d", kind);
+    if (!(p_flip >= 0.0 && p_flip <= 1.0)) return fail("p_flip must lie in [0, 1]");
+    launch_init_state(L->G, L->F, kind, seed, replica, p_flip, L->stream);
+    if (launch_check()) return -1;
+    HIPCHECK(hipStreamSynchronize(L->stream));
+    return 0;
+}
+
 int chll_lattice_set_step(chll_lattice *L, double d)
 {
     if (!L) return fail("null lattice");

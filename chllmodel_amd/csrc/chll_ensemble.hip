@@ -844,6 +844,18 @@ int chll_ensemble_upload(chll_ensemble *X, int32_t r, const double *nx, const do
     return 0;
 }
 
+int chll_ensemble_init_state(chll_ensemble *X, int32_t kind, uint64_t seed, double p_flip)
+{
+    if (!X) return efail("null ensemble");
+    if (kind < CHLL_INIT_ISOTROPIC || kind > CHLL_INIT_ALIGNED_XZ) return efail("unknown initial state kind %d", kind);
+    if (!(p_flip >= 0.0 && p_flip <= 1.0)) return efail("p_flip must lie in [0, 1]");
+    for (int r = 0; r < X->nrep; ++r) launch_init_state(X->rep[r].G, X->rep[r].F, kind, seed, (uint32_t)r, p_flip,
+                                                          X->stream);
+    ECHECK(hipGetLastError());
+    ECHECK(hipStreamSynchronize(X->stream));
+    return 0;
+}
+
 int chll_ensemble_download(chll_ensemble *X, int32_t r, double *nx, double *ny, double *nz, int32_t *s)
 {
     if (!X || r < 0 || r >= X->nrep || !nx || !ny || !nz || !s) return efail("chll_ensemble_download: bad argument");

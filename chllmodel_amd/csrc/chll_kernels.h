@@ -96,5 +96,10 @@ void launch_measure(int bc, const Geom &G, const Fields &F, double g, double KK,
                     hipStream_t s);
 void launch_finalize(const FinArgs &A, hipStream_t s);
 void launch_stream_fill(double *out, uint64_t seed, uint64_t base, uint32_t nsub, hipStream_t s);
+// Initial state of every site (k_init_state): kind 0 isotropic directors, 1
+// aligned along x, 2 aligned along (1, 0, 1)/sqrt(2); spins -1 with
+// probability p_flip.
+void launch_init_state(const Geom &G, const Fields &F, int kind, uint64_t seed, uint32_t replica, double p_flip,
+                       hipStream_t s);
 
 }  // namespace chll

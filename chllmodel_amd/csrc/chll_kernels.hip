@@ -342,6 +342,45 @@ __global__ __launch_bounds__(kBlock) void k_stream_fill(double *__restrict__ out
 }
 
 // ---------------------------------------------------------------------------
+// Initial states on the device (the reference builds them in Python loops:
+// scripts/pdanneal.py:5-16 isotropic, scripts/hero480.py:18-21 aligned xz,
+// chll/chll.py:84-98 +-1 spins and the default n = (1, 0, 0)).  Site of
+// Fortran index l draws two Philox blocks, key (seed_lo, seed_hi), counters
+// (l, replica, kInitTag, 0) and (l, replica, kInitTag, 1):
+//   directors (INIT_ISOTROPIC): costh = 2 u1 - 1, u1 = u53(x0, y0);
+//     phi = 2 pi u2, u2 = u53(z0, w0); n = (sinth cos phi, sinth sin phi,
+//     costh), sinth = sqrt(1 - costh^2), the scripts' formula;
+//   spin: -1 when u53(x1, y1) < p_flip, else +1 (chll/chll.py:85-90).
+// INIT_ALIGNED_X gives n = (1, 0, 0), INIT_ALIGNED_XZ n = (1, 0, 1)/sqrt(2).
+constexpr uint32_t kInitTag = 0x494E4954u;  // "INIT"
+
+__global__ __launch_bounds__(kBlock) void k_init_state(Geom G, Fields F, int kind, uint64_t seed, uint32_t replica,
+                                                       double p_flip)
+{
+    const uint32_t l = blockIdx.x * kBlock + threadIdx.x;
+    if (l >= 2u * G.nsub) return;
+    const Site st = decode_l(G, l);
+    const uint32_t q = ((st.i + st.j + st.k) & 1u) * G.nsub + (l >> 1);
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    double nx, ny, nz;
+    if (kind == 0) {
+        const U4 w = philox4x32_10(U4{l, replica, kInitTag, 0u}, k0, k1);
+        const double costh = 2.0 * (u53(w.x, w.y) - 0.5);
+        const double sinth = sqrt(1.0 - costh * costh);
+        double sphi, cphi;
+        sincos(u53(w.z, w.w) * kTwoPi, &sphi, &cphi);
+        nx = sinth * cphi; ny = sinth * sphi; nz = costh;
+    } else if (kind == 1) {
+        nx = 1.0; ny = 0.0; nz = 0.0;
+    } else {
+        nx = 0.70710678118654752440; ny = 0.0; nz = 0.70710678118654752440;
+    }
+    const U4 v = philox4x32_10(U4{l, replica, kInitTag, 1u}, k0, k1);
+    F.nx[q] = nx; F.ny[q] = ny; F.nz[q] = nz;
+    F.s[q] = u53(v.x, v.y) < p_flip ? (int8_t)-1 : (int8_t)1;
+}
+
+// ---------------------------------------------------------------------------
 // Launchers.
 static inline dim3 grid1(uint32_t n) { return dim3((n + kBlock - 1) / kBlock); }
 
@@ -529,6 +568,12 @@ void launch_measure(int bc, const Geom &G, const Fields &F, double g, double KK,
 void launch_finalize(const FinArgs &A, hipStream_t s)
 {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, A);
+}
+
+void launch_init_state(const Geom &G, const Fields &F, int kind, uint64_t seed, uint32_t replica, double p_flip,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(k_init_state, grid1(2u * G.nsub), dim3(kBlock), 0, s, G, F, kind, seed, replica, p_flip);
 }
 
 void launch_stream_fill(double *out, uint64_t seed, uint64_t base, uint32_t nsub, hipStream_t s)

@@ -18,7 +18,7 @@ import time
 
 import numpy as np
 
-from ._lib import ARITH, BC, lib
+from ._lib import ARITH, BC, INIT_KINDS, lib
 from .states import isotropic_state
 
 ENS_COLS = 16
@@ -54,10 +54,11 @@ def _declare(L):
     L.chll_ensemble_run.argtypes = [ct.c_void_p, ct.c_uint64, ct.c_uint32, ct.c_int32, ct.c_int32, ct.c_int32,
                                     ct.c_int32, _f64p, ct.c_int32]
     L.chll_ensemble_sync.argtypes = [ct.c_void_p]
+    L.chll_ensemble_init_state.argtypes = [ct.c_void_p, ct.c_int32, ct.c_uint64, ct.c_double]
     L.chll_ensemble_state.argtypes = [ct.c_void_p, _i32p, _f64p, _f64p, _i64p, _i64p, ct.POINTER(ct.c_uint32)]
     for n in ("chll_ensemble_create", "chll_ensemble_destroy", "chll_ensemble_set_stream", "chll_ensemble_set_block",
               "chll_ensemble_set_fusion", "chll_ensemble_set_chunk", "chll_ensemble_blocks", "chll_ensemble_upload", "chll_ensemble_download", "chll_ensemble_run",
-              "chll_ensemble_sync", "chll_ensemble_state"):
+              "chll_ensemble_sync", "chll_ensemble_state", "chll_ensemble_init_state"):
         getattr(L, n).restype = ct.c_int
     L._ens_declared = True
     return L
@@ -109,6 +110,11 @@ class Ensemble:
     def init_isotropic(self, seed: int):
         for r, rr in enumerate(self.replicas):
             self.upload(r, isotropic_state(rr["ni"], rr["nj"], rr["nk"], seed + r))
+
+    def init_state(self, kind: str = "isotropic", seed: int = 0, p_flip: float = 0.5):
+        """Every replica's initial state generated on the device
+        (Lattice.init_state with replica index r)."""
+        _check(lib().chll_ensemble_init_state(self._h, INIT_KINDS[kind], int(seed), float(p_flip)))
 
     def set_block(self, bs: int):
         _check(lib().chll_ensemble_set_block(self._h, int(bs)))
@@ -240,7 +246,7 @@ def autotune(replicas, *, bc="open", arith="compat", K=10, blocks=3, candidates=
             return cache[key]
     times, ctimes = {}, {}
     with Ensemble(replicas, bc=bc, arith=arith) as ens:
-        ens.init_isotropic(seed)
+        ens.init_state("isotropic", seed=seed)
 
         def time_blocks():
             ens.run(1, K, seed=seed, want_rows=False, use_graph=True)  # capture + warm up

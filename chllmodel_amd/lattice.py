@@ -11,7 +11,7 @@ import ctypes as ct
 
 import numpy as np
 
-from ._lib import ARITH, BC, RNG, ROW_COLS, ROW_NAMES, TIMING_KINDS, ChllParams, check, lib
+from ._lib import ARITH, BC, INIT_KINDS, RNG, ROW_COLS, ROW_NAMES, TIMING_KINDS, ChllParams, check, lib
 
 _f64p = ct.POINTER(ct.c_double)
 _i32p = ct.POINTER(ct.c_int32)
@@ -53,6 +53,13 @@ class Lattice:
         check(lib().chll_lattice_download(self._h, out["nx"].ctypes.data_as(_f64p), out["ny"].ctypes.data_as(_f64p),
                                           out["nz"].ctypes.data_as(_f64p), out["s"].ctypes.data_as(_i32p)))
         return out
+
+    def init_state(self, kind: str = "isotropic", seed: int = 0, replica: int = 0, p_flip: float = 0.5) -> None:
+        """Generate the initial state on the device, keyed by (seed, replica,
+        site): 'isotropic' directors (the scripts' init_random formula),
+        'aligned_x' n = (1,0,0) or 'aligned_xz' n = (1,0,1)/sqrt(2); spins -1
+        with probability p_flip (init_rand_s).  No host arrays involved."""
+        check(lib().chll_lattice_init_state(self._h, INIT_KINDS[kind], int(seed), int(replica), float(p_flip)))
 
     def set_step(self, d: float) -> None:
         check(lib().chll_lattice_set_step(self._h, float(d)))

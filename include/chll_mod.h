@@ -117,6 +117,18 @@ int chll_lattice_set_stream(chll_lattice *lat, void *hip_stream);
 /* Host Fortran-ordered fields <-> device colour-split layout. */
 int chll_lattice_upload(chll_lattice *lat, const double *nx, const double *ny, const double *nz, const int32_t *s);
 int chll_lattice_download(chll_lattice *lat, double *nx, double *ny, double *nz, int32_t *s);
+/* Initial states generated on the device, replacing the reference's Python
+ * loops (scripts/pdanneal.py:5-16 init_random, scripts/hero480.py:18-21
+ * init_aligned2, chll/chll.py:84-98 init_rand_s and its default n = (1,0,0)).
+ * Site with Fortran index l (0-based) draws Philox4x32-10 blocks keyed by
+ * seed with counters (l, replica, 0x494E4954, 0) and (..., 1): isotropic
+ * directors costh = 2 u53(x0,y0) - 1, phi = 2 pi u53(z0,w0); spin -1 when
+ * u53(x1,y1) < p_flip, else +1.  The state depends only on (seed, replica,
+ * site), not on the kernel or layout. */
+#define CHLL_INIT_ISOTROPIC 0
+#define CHLL_INIT_ALIGNED_X 1  /* n = (1, 0, 0) */
+#define CHLL_INIT_ALIGNED_XZ 2 /* n = (1, 0, 1) / sqrt(2) */
+int chll_lattice_init_state(chll_lattice *lat, int32_t kind, uint64_t seed, uint32_t replica, double p_flip);
 int chll_lattice_set_step(chll_lattice *lat, double d);
 int chll_lattice_get_step(chll_lattice *lat, double *d);
 /* Sweeps istep = nstart..nstop.  A row is measured when nout > 0 and
@@ -201,6 +213,8 @@ int32_t chll_ensemble_blocks(chll_ensemble *e);
 int chll_ensemble_upload(chll_ensemble *e, int32_t rep, const double *nx, const double *ny, const double *nz,
                          const int32_t *s);
 int chll_ensemble_download(chll_ensemble *e, int32_t rep, double *nx, double *ny, double *nz, int32_t *s);
+/* chll_lattice_init_state for every replica, replica index r as its counter word. */
+int chll_ensemble_init_state(chll_ensemble *e, int32_t kind, uint64_t seed, double p_flip);
 /* nblocks blocks of K sweeps; each block ends with a measurement row per
  * replica and a parallel-tempering swap attempt in every ladder.  rows:
  * [min(nblocks, max_rows)][nrep][CHLL_ENS_COLS]; NULL rows only enqueues.

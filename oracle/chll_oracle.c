@@ -131,6 +131,41 @@ static void philox_site(uint64_t seed, uint32_t run_index, uint32_t replica, uin
     *u_spin = u48(w[3], w[2] & 0xffffu);
 }
 
+/* Initial states keyed by (seed, replica, site) (the library's device
+ * generators; the reference's own are Python loops over numpy's global
+ * stream: scripts/pdanneal.py:5-16, scripts/hero480.py:18-21, chll/chll.py:
+ * 84-98).  Site of Fortran index l: Philox blocks with counters
+ * (l, replica, 0x494E4954, 0) and (..., 1), key (seed_lo, seed_hi).
+ * kind 0: costh = 2 (u53(w0, w1) - 0.5), phi = 2 pi u53(w2, w3),
+ * n = (sqrt(1 - costh^2) cos phi, sqrt(1 - costh^2) sin phi, costh);
+ * kind 1: n = (1, 0, 0); kind 2: n = (1, 0, 1) / sqrt(2).
+ * Spin: -1 when u53(v0, v1) < p_flip, else +1.  Fortran-ordered arrays. */
+void oracle_init_state(double *nx, double *ny, double *nz, int32_t *s, int ni, int nj, int nk, int kind,
+                       uint64_t seed, uint32_t replica, double p_flip)
+{
+    const int64_t n3 = (int64_t)ni * nj * nk;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int64_t l = 0; l < n3; ++l) {
+        uint32_t c0[4] = {(uint32_t)l, replica, 0x494E4954u, 0u}, c1[4] = {(uint32_t)l, replica, 0x494E4954u, 1u};
+        uint32_t w[4], v[4];
+        if (kind == 0) {
+            oracle_philox4x32_10(c0, key, w);
+            double costh = 2.0 * (u53(w[0], w[1]) - 0.5);
+            double sinth = sqrt(1.0 - costh * costh);
+            double phi = u53(w[2], w[3]) * TWOPI;
+            nx[l] = sinth * cos(phi);
+            ny[l] = sinth * sin(phi);
+            nz[l] = costh;
+        } else if (kind == 1) {
+            nx[l] = 1.0; ny[l] = 0.0; nz[l] = 0.0;
+        } else {
+            nx[l] = 0.70710678118654752440; ny[l] = 0.0; nz[l] = 0.70710678118654752440;
+        }
+        oracle_philox4x32_10(c1, key, v);
+        s[l] = u53(v[0], v[1]) < p_flip ? -1 : 1;
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* Sublattice lists (chll/chll_mod.f90:9-38): Fortran order, 1-based (i+j+k)
  * odd -> sl1, even -> sl2; entries are 1-based linear indices. */

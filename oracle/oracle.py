@@ -88,6 +88,9 @@ class Oracle:
         L.oracle_run_mt.restype = ct.c_int
         L.oracle_rank.argtypes = [ct.c_int] * 5
         L.oracle_rank.restype = ct.c_int64
+        L.oracle_init_state.argtypes = [_f64p, _f64p, _f64p, _i32p, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+                                        ct.c_uint64, ct.c_uint32, ct.c_double]
+        L.oracle_init_state.restype = None
 
     # -- RNG --------------------------------------------------------------
     def philox(self, ctr, key):
@@ -136,6 +139,17 @@ class Oracle:
         ni, nj, nk = st["nx"].shape
         return self.lib.oracle_energy_true(_ptr(st["nx"], _f64p), _ptr(st["ny"], _f64p), _ptr(st["nz"], _f64p),
                                            _ptr(st["s"], _i32p), ni, nj, nk, bc, float(g), float(KK))
+
+    def init_state(self, ni, nj, nk, kind=0, seed=0, replica=0, p_flip=0.5):
+        """Initial state keyed by (seed, replica, site): kind 0 isotropic,
+        1 aligned x, 2 aligned xz; spins -1 with probability p_flip."""
+        st = {f: np.zeros((ni, nj, nk), dtype=np.float64, order="F") for f in ("nx", "ny", "nz")}
+        st["s"] = np.zeros((ni, nj, nk), dtype=np.int32, order="F")
+        self.lib.oracle_init_state(_ptr(st["nx"], _f64p), _ptr(st["ny"], _f64p), _ptr(st["nz"], _f64p),
+                                   _ptr(st["s"], _i32p), ni, nj, nk, int(kind), int(seed), int(replica),
+                                   float(p_flip))
+        st["dope"] = np.zeros((ni, nj, nk), dtype=np.int32, order="F")
+        return st
 
     def rank(self, ni, nj, i, j, k):
         """Rank of site (i, j, k) (0-based) in its sublattice list."""

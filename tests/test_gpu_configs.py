@@ -8,7 +8,8 @@ their full sizes:
   accept and flip counts, rows, the adapted cone step and the final lattice
   (spins exact, directors within 1e-12, E within one binary32 ulp).
 * L=480 (110.6 M sites, 2.65 GB of directors), K = 1, g = 0.25, kT = 0.5,
-  random initial state, FP64: sweeps at full size; the Hamiltonian of the
+  isotropic initial state generated on the device, FP64: sweeps at full
+  size; the Hamiltonian of the
   measurement rows agrees with the independent energy kernel
   (chll_energy_grad) on the final state, directors stay unit vectors and
   spins +-1, and the accept rate stays inside the adaptation band.
@@ -72,11 +73,13 @@ def test_l128_anneal_first_50_sweeps_match_oracle():
 
 def test_l480_full_size():
     L, K, g, kT, d0 = 480, 1.0, 0.25, 0.5, 0.3
-    print("L=480: building the initial state", flush=True)
-    st0 = isotropic_state(L, L, L, 480)
     with Lattice(L, L, L) as lat:
-        lat.upload_state(st0)
-        del st0
+        print("L=480: initial state generated on the device", flush=True)
+        lat.init_state("isotropic", seed=480, p_flip=0.5)
+        row0 = lat.measure(g, K)
+        # an isotropic start: <n_a n_a> = 1/3 to a few standard errors (sd of n_a^2 over sqrt(N) = 2.8e-5)
+        for q in ("Qxx", "Qyy", "Qzz"):
+            assert abs(row0[q] - 1 / 3) < 3e-4, (q, row0[q])
         lat.set_step(d0)
         print("L=480: sweeps", flush=True)
         out = lat.sweeps(1, 20, 10, g=g, KK=K, kbt=kT, rng="philox", arith="fp64", seed=4, want_counts=True)

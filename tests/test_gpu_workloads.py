@@ -116,12 +116,17 @@ def test_phase_scan_64_replicas():
     orc = Oracle()
     seed, run_index, K, nblocks = 20261203, 0, 10, 3
     print("phase scan: initial states", flush=True)
-    states = {r: isotropic_state(32, 32, 32, seed + r) for r in small}
+    states = {r: orc.init_state(32, 32, 32, kind=0, seed=seed, replica=r, p_flip=0.5) for r in small}
     slot_kbt = np.array([r["kbt"] for r in reps])
     slot_d = np.array([r["d0"] for r in reps])
     rep_slot = np.arange(n)
     with Ensemble(reps, bc="open", arith="fp64") as ens:
-        ens.init_isotropic(seed)  # replica r from isotropic_state(..., seed + r)
+        ens.init_state("isotropic", seed=seed, p_flip=0.5)  # on the device, keyed by (seed, replica r, site)
+        for r in small:
+            got = ens.download(r)
+            assert np.array_equal(got["s"], states[r]["s"])
+            assert max(np.abs(got[f] - states[r][f]).max() for f in ("nx", "ny", "nz")) <= 1e-15
+            states[r] = got  # the oracle continues from the device's state
         for b in range(nblocks):
             print(f"phase scan: block {b}", flush=True)
             rows = ens.run(1, K, seed=seed, run_index=run_index, adapt=True, use_graph=True)

@@ -201,6 +201,46 @@ def test_rank_is_half_the_linear_index(orc):
                         assert orc.rank(ni, nj, int(i[q]), int(j[q]), int(k[q])) == want[q]
 
 
+def test_init_state_generators(orc):
+    """Initial states keyed by (seed, replica, site): the aligned states
+    exactly, unit directors, spin fractions p_flip = 0 / 1 exactly, and for
+    isotropic directors the moments of the uniform distribution on the
+    sphere (<n> = 0, <n_a n_b> = delta_ab / 3, so P2 -> 0) and <s> = 1 - 2 p
+    within 5 standard errors; the same key gives the same state, another
+    replica another."""
+    from chllmodel_amd.observables import p2_from_q
+
+    ni, nj, nk = 64, 48, 40
+    N = ni * nj * nk
+    ax = orc.init_state(4, 3, 2, kind=1, seed=5, p_flip=0.0)
+    assert np.all(ax["nx"] == 1.0) and np.all(ax["ny"] == 0.0) and np.all(ax["nz"] == 0.0)
+    assert np.all(ax["s"] == 1)
+    axz = orc.init_state(4, 3, 2, kind=2, seed=5, p_flip=1.0)
+    assert np.all(axz["nx"] == 2 ** -0.5) and np.all(axz["ny"] == 0.0) and np.all(axz["nz"] == 2 ** -0.5)
+    assert np.all(axz["s"] == -1)
+    st = orc.init_state(ni, nj, nk, kind=0, seed=20261203, replica=3, p_flip=0.3)
+    n = np.stack([st["nx"].ravel(), st["ny"].ravel(), st["nz"].ravel()])
+    assert np.abs(np.sqrt((n * n).sum(0)) - 1.0).max() < 1e-15
+    se1, se2 = np.sqrt(1 / (3 * N)), np.sqrt(4 / (45 * N))  # sd of n_a, of n_a^2, over sqrt(N)
+    for a in range(3):
+        assert abs(n[a].mean()) < 5 * se1
+        assert abs((n[a] ** 2).mean() - 1 / 3) < 5 * se2
+        assert abs((n[a] * n[(a + 1) % 3]).mean()) < 5 * np.sqrt(1 / (15 * N))
+    q = [(n[0] ** 2).mean(), (n[1] ** 2).mean(), (n[2] ** 2).mean(), (n[0] * n[1]).mean(), (n[0] * n[2]).mean(),
+         (n[1] * n[2]).mean()]
+    assert p2_from_q(q) < 0.01
+    sbar = st["s"].mean()
+    assert abs(sbar - (1 - 2 * 0.3)) < 5 * 2 * np.sqrt(0.3 * 0.7 / N)
+    again = orc.init_state(ni, nj, nk, kind=0, seed=20261203, replica=3, p_flip=0.3)
+    other = orc.init_state(ni, nj, nk, kind=0, seed=20261203, replica=4, p_flip=0.3)
+    assert all(np.array_equal(st[f], again[f]) for f in ("nx", "ny", "nz", "s"))
+    assert not np.array_equal(st["nz"], other["nz"]) and not np.array_equal(st["s"], other["s"])
+    # the draw of site l: Philox counter (l, replica, "INIT", 0) gives costh, phi
+    w = orc.philox([7, 3, 0x494E4954, 0], [20261203 & 0xFFFFFFFF, 20261203 >> 32])
+    u1 = ((int(w[0]) << 32 | int(w[1])) >> 11) * 2.0 ** -53
+    assert st["nz"].ravel(order="F")[7] == 2.0 * (u1 - 0.5)
+
+
 # --------------------------------------------------------------------------
 GOLDEN_FILES = sorted(glob.glob(os.path.join(GOLDEN, "ref_*.npz")))
 
