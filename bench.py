@@ -197,6 +197,7 @@ def main():
         del src, dst
         log(f"bench: calibration copy of {args.copy_calib_mib} MiB done")
 
+    copy_rate = hbm_copy_rate()
     tuned = {}
 
     def time_single(arith, timing, bc=None, nout=None):
@@ -301,6 +302,7 @@ def main():
         ref_bytes = 125.0
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                 "traffic": traffic, "kernel": KERNEL_NAMES[dom],
+                "copy_GBps": copy_rate, "frac_of_copy": ach / copy_rate,
                 "algorithmic_bytes_per_launch": BYTES[dom] * N,
                 "algorithmic_bytes_per_site": BYTES[dom],
                 "kernel_avg_us": kernels[dom]["avg_us"],
@@ -364,6 +366,30 @@ def main():
         if "analysis" in out:
             analysis_cpu_baseline(out["analysis"])
     print(json.dumps(out), flush=True)
+
+
+def hbm_copy_rate(mib=2048, reps=5):
+    """HBM bandwidth a plain device-to-device copy reaches on this GPU (read
+    plus write bytes over the best of `reps` copies of `mib` MiB, HIP
+    events): the practical ceiling beside the 8 TB/s peak."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = float("inf")
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del src, dst
+    rate = 2 * n * 8 / best / 1e9
+    log(f"bench: device copy of {mib} MiB: {rate:.0f} GB/s")
+    return rate
 
 
 def dropin_run(L, st0, args):

@@ -208,12 +208,13 @@ int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lat
         return -1;
     }
     L->stream = L->own;
-    // Default kernels (MI355X measurements at L = 256 and 128): the one-site
-    // (or site-pair) kernel for the director trials, whose register budget a
-    // carried plane ring would halve; the k-march for the chirality trials.
-    // March shape: the widest tile the rows fill, up to ~2048 blocks with slabs
-    // of at least 8 planes; lattices that cannot fill 512 such blocks keep the
-    // one-site kernels.
+    // Default kernels (MI355X measurements at L = 256 and 128): site pairs in
+    // XCD row bands where the rows allow them, else one site per thread for
+    // the director trials (a carried plane ring would halve their register
+    // budget) and the k-march for the chirality trials.  March shape: the
+    // widest tile the rows fill, up to ~2048 blocks with slabs of at least 8
+    // planes; lattices that cannot fill 512 such blocks keep the one-site
+    // kernels.
     {
         const int tile = G.half >= 64 ? 0 : (G.half >= 32 ? 1 : 2);
         const uint32_t tiles = march_tiles(tile, G);
@@ -226,21 +227,16 @@ int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lat
                 // FP64: two sites per thread in XCD row bands where the rows
                 // allow it (2-5 % faster fused sweeps), the fused colour-0
                 // launch in reverse block order (it starts on the planes the
-                // colour-1 launch finished with: 5 % faster sweeps at L = 256),
-                // the k-march for the chirality launches outside the fused
-                // chain.  Compat: site pairs in bands for both kinds where the
-                // rows allow it (5 % faster sweeps at L = 256 than the k-march
-                // chirality), else one-site kernels in bands and the k-march.
+                // colour-1 launch finished with: 5 % faster sweeps at L = 256);
+                // the chirality launches outside the fused chain (a call's
+                // last sweep, measurement sweeps) on site pairs too, which
+                // read the state once where the k-march re-reads its slab
+                // halos (1.6x).  Compat: site pairs in bands for both kinds
+                // (5 % faster sweeps at L = 256 than the k-march chirality).
+                // Without pairs: one-site kernels in bands and the k-march.
                 const bool pairs = pairs_ok(G);
-                bool march;
-                int order;
-                if (ar == ARITH_FP64) {
-                    march = thick && kind == 1;
-                    order = pairs ? 7 : 1;
-                } else {
-                    march = !pairs && thick && kind == 1;
-                    order = pairs ? 3 : 1;
-                }
+                const bool march = !pairs && thick && kind == 1;
+                const int order = pairs ? (ar == ARITH_FP64 ? 7 : 3) : 1;
                 L->kcfg[kind][ar] = {march ? 1 : 0, march ? tile : order, ns};
                 // odd ni: rows hold (ni +- 1) / 2 sites of a colour, so only the
                 // one-site kernels in lattice order apply

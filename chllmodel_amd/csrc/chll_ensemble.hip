@@ -137,22 +137,23 @@ __global__ __launch_bounds__(BS) void k_pair_ens(EnsDev E, uint32_t istep_off, i
     const size_t gs = (size_t)kSlots * kSlotStride;
     bool a0 = false, a1 = false, f0 = false, f1 = false;
     if constexpr (KIND == 3) A.istep = A.istep_spin + 1;
+    const double d = KIND == 1 ? 0.0 : *A.d;  // the cone step, read once for both sites
     if (threadIdx.x < ent.y) {
         SiteIn S0, S1;
         const uint32_t own = load_pair<BC, true>(D.G, D.F, KIND == 2 ? 1 : 0, m0, S0, S1);
         SiteOut O0, O1;
         if constexpr (KIND == 0) {
-            a0 = dir_apply<ARITH_FP64, RNG_PHILOX, true, true>(D.F, 0, A, m0, own, S0, &O0);
-            a1 = dir_apply<ARITH_FP64, RNG_PHILOX, true, true>(D.F, 0, A, m0 + 1u, own + 1u, S1, &O1);
+            a0 = dir_apply<ARITH_FP64, RNG_PHILOX, true, true>(D.F, 0, A, d, m0, own, S0, &O0);
+            a1 = dir_apply<ARITH_FP64, RNG_PHILOX, true, true>(D.F, 0, A, d, m0 + 1u, own + 1u, S1, &O1);
         } else if constexpr (KIND == 1) {
             f0 = spin_apply<ARITH_FP64, RNG_PHILOX, true, true>(D.F, 0, A, m0, own, S0, &O0);
             f1 = spin_apply<ARITH_FP64, RNG_PHILOX, true, true>(D.F, 0, A, m0 + 1u, own + 1u, S1, &O1);
         } else if constexpr (KIND == 2) {
-            a0 = dir_spin_apply<RNG_PHILOX, true, true>(D.F, 1, A, m0, own, S0, f0, &O0);
-            a1 = dir_spin_apply<RNG_PHILOX, true, true>(D.F, 1, A, m0 + 1u, own + 1u, S1, f1, &O1);
+            a0 = dir_spin_apply<RNG_PHILOX, true, true>(D.F, 1, A, d, m0, own, S0, f0, &O0);
+            a1 = dir_spin_apply<RNG_PHILOX, true, true>(D.F, 1, A, d, m0 + 1u, own + 1u, S1, f1, &O1);
         } else {
-            a0 = spin_dir_apply<true, true>(D.F, A, m0, own, S0, f0, &O0);
-            a1 = spin_dir_apply<true, true>(D.F, A, m0 + 1u, own + 1u, S1, f1, &O1);
+            a0 = spin_dir_apply<true, true>(D.F, A, d, m0, own, S0, f0, &O0);
+            a1 = spin_dir_apply<true, true>(D.F, A, d, m0 + 1u, own + 1u, S1, f1, &O1);
         }
         store_pair<KIND != 1, KIND != 0>(D.F, own, O0, O1);
     }

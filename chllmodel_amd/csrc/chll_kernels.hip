@@ -127,28 +127,29 @@ __global__ __launch_bounds__(kBlock) void k_spin_dir(Geom G, Fields F, PhaseArgs
 // half (ni % 4 == 0) and fields below 4 GiB.
 template <int OP, int ARITH, int RNG, int BC>
 __global__ __launch_bounds__(kBlock) void k_pair(Geom G, Fields F, int colour, PhaseArgs A,
-                                                 uint32_t *__restrict__ c0, uint32_t *__restrict__ c1,
-                                                 uint32_t band)
+                                             uint32_t *__restrict__ c0, uint32_t *__restrict__ c1,
+                                             uint32_t band)
 {
     const uint32_t m0 = 2u * (sweep_block(band) * kBlock + threadIdx.x);
     bool a0 = false, a1 = false, f0 = false, f1 = false;
+    const double d = OP == 1 ? 0.0 : *A.d;  // the cone step, read once for both sites
     if (m0 < G.nsub) {
         SiteIn S0, S1;
         const uint32_t own = load_pair<BC, true>(G, F, colour, m0, S0, S1);
         // both sites' states go back as 16-byte words (store_pair)
         SiteOut O0, O1;
         if constexpr (OP == 0) {
-            a0 = dir_apply<ARITH, RNG, true, true>(F, colour, A, m0, own, S0, &O0);
-            a1 = dir_apply<ARITH, RNG, true, true>(F, colour, A, m0 + 1u, own + 1u, S1, &O1);
+            a0 = dir_apply<ARITH, RNG, true, true>(F, colour, A, d, m0, own, S0, &O0);
+            a1 = dir_apply<ARITH, RNG, true, true>(F, colour, A, d, m0 + 1u, own + 1u, S1, &O1);
         } else if constexpr (OP == 1) {
             a0 = spin_apply<ARITH, RNG, true, true>(F, colour, A, m0, own, S0, &O0);
             a1 = spin_apply<ARITH, RNG, true, true>(F, colour, A, m0 + 1u, own + 1u, S1, &O1);
         } else if constexpr (OP == 2) {
-            a0 = dir_spin_apply<RNG, true, true>(F, colour, A, m0, own, S0, f0, &O0);
-            a1 = dir_spin_apply<RNG, true, true>(F, colour, A, m0 + 1u, own + 1u, S1, f1, &O1);
+            a0 = dir_spin_apply<RNG, true, true>(F, colour, A, d, m0, own, S0, f0, &O0);
+            a1 = dir_spin_apply<RNG, true, true>(F, colour, A, d, m0 + 1u, own + 1u, S1, f1, &O1);
         } else {
-            a0 = spin_dir_apply<true, true>(F, A, m0, own, S0, f0, &O0);
-            a1 = spin_dir_apply<true, true>(F, A, m0 + 1u, own + 1u, S1, f1, &O1);
+            a0 = spin_dir_apply<true, true>(F, A, d, m0, own, S0, f0, &O0);
+            a1 = spin_dir_apply<true, true>(F, A, d, m0 + 1u, own + 1u, S1, f1, &O1);
         }
         store_pair<OP != 1, OP != 0>(F, own, O0, O1);
     }
@@ -158,39 +159,125 @@ __global__ __launch_bounds__(kBlock) void k_pair(Geom G, Fields F, int colour, P
 }
 
 // ---------------------------------------------------------------------------
-// Observables over the whole lattice, one partial per block.  Block b walks
-// its own contiguous range of `chunk` packed indices (a multiple of kBlock):
-// a row's +y neighbours are then read by the same block one step later, and
-// the +z plane is read at the same time by block b + 8 on the same XCD
-// (workgroups go round-robin to the 8 XCDs), so both come from L2.
+// Observables over the whole lattice, one partial per block.  Range r of
+// `chunk` packed indices (a multiple of kBlock) goes to the block that XCD
+// r / (grid / 8) runs (workgroup b runs on XCD b % 8), in order: each XCD
+// walks one contiguous slab of ranges, so the +y rows of a range and the
+// rows of the +z plane, a few ranges later in the same slab, come from that
+// XCD's L2, and only the slab's last plane reads the next slab's first.  The
+// partial of range r is part[r], so the reduction order does not depend on
+// the mapping.
+__device__ inline uint32_t measure_range()
+{
+    const uint32_t b = blockIdx.x, n = gridDim.x;
+    return (n & 7u) ? b : (b & 7u) * (n >> 3) + (b >> 3);
+}
+
 template <int BC>
 __global__ __launch_bounds__(kBlock) void k_measure(Geom G, Fields F, double g, double KK,
                                                     double *__restrict__ part, long long *__restrict__ spart,
                                                     uint32_t chunk)
 {
     MeasureAcc acc;
-    const uint64_t beg = (uint64_t)blockIdx.x * chunk;
+    const uint32_t rb = measure_range();
+    const uint64_t beg = (uint64_t)rb * chunk;
     const uint64_t end = beg + chunk < G.nsub ? beg + chunk : G.nsub;
     for (uint64_t m = beg + threadIdx.x; m < end; m += kBlock)
         measure_site<BC>(G, F, g, KK, (uint32_t)m, acc);
-    block_reduce_measure<kBlock>(acc, part + (size_t)blockIdx.x * 8, spart + blockIdx.x);
+    block_reduce_measure<kBlock>(acc, part + (size_t)rb * 8, spart + rb);
 }
 
 // The same observables with four sites of one row per thread (measure_pair);
 // chunk is a multiple of 2 kBlock, so lane l + 1 holds the pair after lane
 // l's in every pass.  Needs an even half and fields below 4 GiB.
+// The same observables marching along k, four sites of one row per thread.
+// Block (tile t of kBlock packed pairs of a plane, slab s of planes
+// [s ks, s ks + ks)): every step reads the +y row and one new plane row and
+// carries the other from the step before -- marching up, the +z row becomes
+// the next step's own row; marching down, the own row becomes the next
+// step's +z row -- so each plane row is read once per slab.  A row's +y row
+// is read at the step at which the lanes of row j + 1 read it as their own
+// row (down) or +z row (up), so it comes from L2.  Slabs alternate
+// direction, so the plane two adjacent slabs share is read by both at the
+// same time (both start or both end there), and the slabs of one XCD are
+// contiguous (workgroup b runs on XCD b % 8).
+// One partial per block, in block order.  Needs an even half and fields
+// below 4 GiB.
+template <int BC>
+__global__ __launch_bounds__(kBlock) void k_measure_march(Geom G, Fields F, double g, double KK,
+                                                          double *__restrict__ part, long long *__restrict__ spart,
+                                                          uint32_t tiles, uint32_t nslab, uint32_t ks)
+{
+    const uint32_t b = blockIdx.x;
+    uint32_t sl, t;
+    if ((nslab & 7u) == 0u) {
+        const uint32_t n = b >> 3;
+        sl = (b & 7u) * (nslab >> 3) + n / tiles;
+        t = n % tiles;
+    } else {
+        sl = b / tiles;
+        t = b % tiles;
+    }
+    const uint32_t quads = G.plane >> 1;  // packed pairs per plane and colour
+    const uint32_t qd = t * kBlock + threadIdx.x;
+    const uint32_t k0 = sl * ks;
+    const uint32_t k1 = k0 + ks < G.nk ? k0 + ks : G.nk;
+    MeasureAcc acc;
+    if (qd < quads && k0 < k1) {
+        const uint32_t mp = 2u * qd;  // packed index of the pair in plane 0
+        const uint32_t j = fdiv(mp, G.fd_half);
+        const uint32_t h0 = mp - j * G.half;
+        const uint32_t jp = (j + 1 == G.nj) ? 0u : j + 1;
+        const uint32_t yo = jp * G.half - j * G.half;  // +y row offset (mod 2^32)
+        // lane + 1 holds the next pair of the plane at every step
+        const bool nxt = (threadIdx.x & 63u) != 63u && qd + 1u < quads;
+        Row4 O, Y, Z;
+        if ((sl & 1u) == 0u) {
+            // up: the +y row of the next step is read with this step's +z row,
+            // when the lanes of row j + 1 read that same row as their +z row
+            load_row4(F, G.nsub, k0 * G.plane + mp, (j + k0) & 1u, O);
+            load_row4(F, G.nsub, k0 * G.plane + mp + yo, (jp + k0) & 1u, Y);
+            for (uint32_t k = k0; k < k1; ++k) {
+                const uint32_t m0 = k * G.plane + mp;
+                const bool hz = k + 1 < G.nk || BC == BC_PERIODIC;
+                const uint32_t kp = k + 1 < G.nk ? k + 1 : 0u;
+                Row4 Yn;
+                if (hz) load_row4(F, G.nsub, kp * G.plane + mp, (j + kp) & 1u, Z);
+                if (k + 1 < k1) load_row4(F, G.nsub, m0 + G.plane + yo, (jp + k + 1) & 1u, Yn);
+                measure_rows<BC>(G, F, g, KK, m0, h0, j, k, nxt, O, Y, Z, acc);
+                O = Z;
+                Y = Yn;
+            }
+        } else {
+            {
+                const uint32_t kp = k1 < G.nk ? k1 : 0u;
+                if (k1 < G.nk || BC == BC_PERIODIC) load_row4(F, G.nsub, kp * G.plane + mp, (j + kp) & 1u, Z);
+            }
+            for (uint32_t k = k1; k-- > k0;) {
+                const uint32_t m0 = k * G.plane + mp;
+                load_row4(F, G.nsub, m0, (j + k) & 1u, O);
+                load_row4(F, G.nsub, m0 + yo, (jp + k) & 1u, Y);
+                measure_rows<BC>(G, F, g, KK, m0, h0, j, k, nxt, O, Y, Z, acc);
+                Z = O;
+            }
+        }
+    }
+    block_reduce_measure<kBlock>(acc, part + (size_t)b * 8, spart + b);
+}
+
 template <int BC>
 __global__ __launch_bounds__(kBlock) void k_measure_pair(Geom G, Fields F, double g, double KK,
                                                          double *__restrict__ part, long long *__restrict__ spart,
                                                          uint32_t chunk)
 {
     MeasureAcc acc;
-    const uint64_t beg = (uint64_t)blockIdx.x * chunk;
+    const uint32_t rb = measure_range();
+    const uint64_t beg = (uint64_t)rb * chunk;
     const uint32_t end = (uint32_t)(beg + chunk < G.nsub ? beg + chunk : G.nsub);
     const bool last_lane = (threadIdx.x & 63u) == 63u;
     for (uint64_t m0 = beg + 2u * threadIdx.x; m0 < end; m0 += 2u * kBlock)
         measure_pair<BC>(G, F, g, KK, (uint32_t)m0, !last_lane && m0 + 2u < end, acc);
-    block_reduce_measure<kBlock>(acc, part + (size_t)blockIdx.x * 8, spart + blockIdx.x);
+    block_reduce_measure<kBlock>(acc, part + (size_t)rb * 8, spart + rb);
 }
 
 // ---------------------------------------------------------------------------
@@ -386,8 +473,26 @@ static inline dim3 grid1(uint32_t n) { return dim3((n + kBlock - 1) / kBlock); }
 
 uint32_t phase_blocks(const Geom &G) { return (G.nsub + kBlock - 1) / kBlock; }
 
+// Shape of k_measure_march: tiles of kBlock packed pairs per plane, slabs of
+// ks planes (at least 8 where nk allows, more where the blocks would pass
+// kMeasBlocksMax).
+static void measure_march_shape(const Geom &G, uint32_t &tiles, uint32_t &nslab, uint32_t &ks)
+{
+    tiles = ((G.plane >> 1) + kBlock - 1) / kBlock;
+    const uint32_t cap = kMeasBlocksMax / tiles ? kMeasBlocksMax / tiles : 1u;
+    ks = (G.nk + cap - 1) / cap;
+    if (ks < 8u) ks = 8u;
+    if (ks > G.nk) ks = G.nk;
+    nslab = (G.nk + ks - 1) / ks;
+}
+
 uint32_t measure_blocks(const Geom &G)
 {
+    if (pairs_ok(G) && (G.plane >> 1) <= kBlock * kMeasBlocksMax) {
+        uint32_t tiles, nslab, ks;
+        measure_march_shape(G, tiles, nslab, ks);
+        return tiles * nslab;
+    }
     const uint32_t b = phase_blocks(G);
     return b < kMeasBlocksMax ? b : kMeasBlocksMax;
 }
@@ -431,6 +536,13 @@ static uint32_t pair_band(const Geom &G, int order)
     const uint32_t per = 16u * kBlock;  // 8 bands x kBlock pairs
     return (order & 1) && G.plane % per == 0u ? G.plane / per : 0u;
 }
+
+template <int OP, int ARITH, int RNG, int BC>
+static void launch_pair(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t *c0, uint32_t *c1,
+                        uint32_t band, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pair<OP, ARITH, RNG, BC>), grid_pairs(G), dim3(kBlock), 0, s, G, F, colour, A, c0, c1, band);
+}
 static uint32_t site_band(const Geom &G, int order) { return (order & 1) ? xcd_band(G) : 0u; }
 
 // order bit 0: XCD row bands, bit 1: site pairs, bit 2: reverse block order.
@@ -441,12 +553,9 @@ static void launch_phase_t(int kind, const Geom &G, const Fields &F, int colour,
     const dim3 b(kBlock);
     const uint32_t rev = (order & 4) ? 0x80000000u : 0u;
     if ((order & 2) && pairs_ok(G)) {
-        const dim3 g = grid_pairs(G);
         const uint32_t band = pair_band(G, order) | rev;
-        if (kind == 0)
-            hipLaunchKernelGGL((k_pair<0, ARITH, RNG, BC>), g, b, 0, s, G, F, colour, A, counts, counts, band);
-        else
-            hipLaunchKernelGGL((k_pair<1, ARITH, RNG, BC>), g, b, 0, s, G, F, colour, A, counts, counts, band);
+        if (kind == 0) launch_pair<0, ARITH, RNG, BC>(G, F, colour, A, counts, counts, band, s);
+        else launch_pair<1, ARITH, RNG, BC>(G, F, colour, A, counts, counts, band, s);
         return;
     }
     const dim3 g = grid1(G.nsub);
@@ -466,8 +575,7 @@ static void launch_fused_t(const Geom &G, const Fields &F, int colour, const Pha
 {
     const dim3 b(kBlock);
     if ((order & 2) && pairs_ok(G)) {
-        hipLaunchKernelGGL((k_pair<2, ARITH_FP64, RNG, BC>), grid_pairs(G), b, 0, s, G, F, colour, A, cd, cs,
-                           pair_band(G, order));
+        launch_pair<2, ARITH_FP64, RNG, BC>(G, F, colour, A, cd, cs, pair_band(G, order), s);
         return;
     }
     const dim3 g = grid1(G.nsub);
@@ -493,16 +601,13 @@ void launch_phase_fused_next(int bc, const Geom &G, const Fields &F, const Phase
 {
     const dim3 b(kBlock);
     if ((order & 2) && pairs_ok(G)) {
-        const dim3 g = grid_pairs(G);
         // order bit 2: this launch walks its blocks in reverse, so it starts on
         // the planes the colour-1 launch before it finished with
         const uint32_t band = pair_band(G, order) | ((order & 4) ? 0x80000000u : 0u);
         if (bc == BC_OPEN)
-            hipLaunchKernelGGL((k_pair<3, ARITH_FP64, RNG_PHILOX, BC_OPEN>), g, b, 0, s, G, F, 0, A, counts_dir,
-                               counts_spin, band);
+            launch_pair<3, ARITH_FP64, RNG_PHILOX, BC_OPEN>(G, F, 0, A, counts_dir, counts_spin, band, s);
         else
-            hipLaunchKernelGGL((k_pair<3, ARITH_FP64, RNG_PHILOX, BC_PERIODIC>), g, b, 0, s, G, F, 0, A, counts_dir,
-                               counts_spin, band);
+            launch_pair<3, ARITH_FP64, RNG_PHILOX, BC_PERIODIC>(G, F, 0, A, counts_dir, counts_spin, band, s);
         return;
     }
     const uint32_t band = site_band(G, order) | ((order & 4) ? 0x80000000u : 0u);
@@ -548,6 +653,17 @@ void launch_measure(int bc, const Geom &G, const Fields &F, double g, double KK,
     // measure_blocks(G) partials (the finalize kernel reads that many), each
     // block a contiguous range; trailing blocks may get an empty one
     const uint32_t nb = measure_blocks(G);
+    if (pairs_ok(G) && (G.plane >> 1) <= kBlock * kMeasBlocksMax) {
+        uint32_t tiles, nslab, ks;
+        measure_march_shape(G, tiles, nslab, ks);
+        if (bc == BC_OPEN)
+            hipLaunchKernelGGL(k_measure_march<BC_OPEN>, dim3(nb), dim3(kBlock), 0, s, G, F, g, KK, part, spart, tiles,
+                               nslab, ks);
+        else
+            hipLaunchKernelGGL(k_measure_march<BC_PERIODIC>, dim3(nb), dim3(kBlock), 0, s, G, F, g, KK, part, spart,
+                               tiles, nslab, ks);
+        return;
+    }
     uint32_t chunk = (G.nsub + nb - 1) / nb;
     if (pairs_ok(G)) {
         chunk = (chunk + 2 * kBlock - 1) / (2 * kBlock) * (2 * kBlock);

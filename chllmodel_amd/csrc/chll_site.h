@@ -214,12 +214,14 @@ __device__ inline void put_spin(const Fields &F, uint32_t own, bool flip, int sg
 }
 
 // One director trial; returns whether it is accepted.
+// d: the cone step, read from *A.d once per thread by the caller (a second
+// read after a trial would be a dependent load in the middle of the work).
 template <int ARITH, int RNG, bool OFF32, bool DEFER = false>
-__device__ inline bool dir_apply(const Fields &F, int colour, const PhaseArgs &A, uint32_t m, uint32_t own,
+__device__ inline bool dir_apply(const Fields &F, int colour, const PhaseArgs &A, double d, uint32_t m, uint32_t own,
                                  const SiteIn &S, SiteOut *o = nullptr)
 {
     double t[3];
-    const bool acc = dir_compute<ARITH, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg, *A.d, t);
+    const bool acc = dir_compute<ARITH, RNG>(A, colour, m, S.N, S.nx, S.ny, S.nz, S.sg, d, t);
     put_dir<OFF32, DEFER>(F, own, acc, t, S, o);
     if constexpr (DEFER) o->s = S.sg;
     return acc;
@@ -261,8 +263,8 @@ __device__ inline bool spin_decide_fp64(const PhaseArgs &A, double a, int sg, do
 // accepted director (spin_compute, FP64); the bond passes of the director
 // trial also give the chirality sums of the old and the trial director.
 template <int RNG, bool OFF32, bool DEFER = false>
-__device__ inline bool dir_spin_apply(const Fields &F, int colour, const PhaseArgs &A, uint32_t m, uint32_t own,
-                                      const SiteIn &S, bool &flip, SiteOut *o = nullptr)
+__device__ inline bool dir_spin_apply(const Fields &F, int colour, const PhaseArgs &A, double d, uint32_t m,
+                                      uint32_t own, const SiteIn &S, bool &flip, SiteOut *o = nullptr)
 {
     // for colour 1 (the fused launch) one block serves both trials
     double u_phi, u_acc, u_spin;
@@ -278,7 +280,7 @@ __device__ inline bool dir_spin_apply(const Fields &F, int colour, const PhaseAr
     }
     double t[3], c2o, c2n, cqo, cqn;
     const double ao = cqf_cq_sum(S.N, S.sg, S.nx, S.ny, S.nz, c2o, cqo);
-    trial_fp64(S.nx, S.ny, S.nz, *A.d, u_phi, t[0], t[1], t[2]);
+    trial_fp64(S.nx, S.ny, S.nz, d, u_phi, t[0], t[1], t[2]);
     reload_barrier();
     const double an = cqf_cq_sum(S.N, S.sg, t[0], t[1], t[2], c2n, cqn);
     const double dE = (c2o - c2n) - A.KK * (an - ao);
@@ -300,7 +302,7 @@ __device__ inline bool dir_spin_apply(const Fields &F, int colour, const PhaseAr
 // fixed before this launch, since it depends on sweep t's director accepts
 // only (chll/chll_mod.f90:83-90).  Returns the director decision.
 template <bool OFF32, bool DEFER = false>
-__device__ inline bool spin_dir_apply(const Fields &F, const PhaseArgs &A, uint32_t m, uint32_t own,
+__device__ inline bool spin_dir_apply(const Fields &F, const PhaseArgs &A, double d, uint32_t m, uint32_t own,
                                       const SiteIn &S, bool &flip, SiteOut *o = nullptr)
 {
     // one pass over the old director's bonds: the chirality sum, then the
@@ -318,7 +320,7 @@ __device__ inline bool spin_dir_apply(const Fields &F, const PhaseArgs &A, uint3
     const double u_phi = site_u_phi(w), u_acc = site_u_dir(w);
     double t[3], c2n;
     const double ao = weight_cq(S.N, sg, cqb);
-    trial_fp64(S.nx, S.ny, S.nz, *A.d, u_phi, t[0], t[1], t[2]);
+    trial_fp64(S.nx, S.ny, S.nz, d, u_phi, t[0], t[1], t[2]);
     reload_barrier();
     const double an = cqf_sum(S.N, sg, t[0], t[1], t[2], c2n);
     const double dE = (c2o - c2n) - A.KK * (an - ao);
@@ -334,7 +336,7 @@ __device__ inline bool dir_site(const Geom &G, const Fields &F, int colour, cons
 {
     SiteIn S;
     const uint32_t own = load_site<BC, OFF32>(G, F, colour, m, S);
-    return dir_apply<ARITH, RNG, OFF32>(F, colour, A, m, own, S);
+    return dir_apply<ARITH, RNG, OFF32>(F, colour, A, *A.d, m, own, S);
 }
 
 template <int ARITH, int RNG, int BC, bool OFF32 = false>
@@ -351,7 +353,7 @@ __device__ inline bool dir_spin_site(const Geom &G, const Fields &F, int colour,
 {
     SiteIn S;
     const uint32_t own = load_site<BC, OFF32>(G, F, colour, m, S);
-    return dir_spin_apply<RNG, OFF32>(F, colour, A, m, own, S, flip);
+    return dir_spin_apply<RNG, OFF32>(F, colour, A, *A.d, m, own, S, flip);
 }
 
 template <int BC, bool OFF32>
@@ -359,7 +361,7 @@ __device__ inline bool spin_dir_site(const Geom &G, const Fields &F, const Phase
 {
     SiteIn S;
     const uint32_t own = load_site<BC, OFF32>(G, F, 0, m, S);
-    return spin_dir_apply<OFF32>(F, A, m, own, S, flip);
+    return spin_dir_apply<OFF32>(F, A, *A.d, m, own, S, flip);
 }
 
 // Site pairs.  With half = ni / 2 even, packed sites m0 = 2 P and m0 + 1 of one
@@ -407,32 +409,49 @@ __device__ inline uint32_t load_pair(const Geom &G, const Fields &F, int colour,
     S0.N.mask = neighbours<BC>(G, m0, st0, oth, n0);
     S1.N.mask = neighbours<BC>(G, m0 + 1u, st1, oth, n1);
     const uint32_t p = st0.i & 1u;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t r = (st0.i >> 1) >> 1, R = G.half >> 1;  // pair position in its row, pairs per row
+    // Every load is issued before any loaded value is used, so that the wave
+    // waits for memory once: its own pair, the x pair (q, q + 1), the extra x
+    // site where the lane beside it does not hold it, then the y and z pairs.
     const double2 ox = fld2<OFF32>(F.nx, own), oy = fld2<OFF32>(F.ny, own), oz = fld2<OFF32>(F.nz, own);
     const uint32_t os = spin2<OFF32>(F.s, own);
-    S0.nx = ox.x; S0.ny = oy.x; S0.nz = oz.x; S0.sg = lo_spin(os);
-    S1.nx = ox.y; S1.ny = oy.y; S1.nz = oz.y; S1.sg = hi_spin(os);
     // x: pair (q, q + 1) and the extra site e
     const uint32_t q = oth + m0;
     const uint32_t e = p ? n1[0] : n0[1];
     const double2 xx = fld2<OFF32>(F.nx, q), xy = fld2<OFF32>(F.ny, q), xz = fld2<OFF32>(F.nz, q);
     const uint32_t xs = spin2<OFF32>(F.s, q);
+    const bool from_prev = p == 0u && r != 0u && lane != 0u;
+    const bool from_next = p == 1u && r + 1u != R && lane != 63u;
+    const bool none = BC == BC_OPEN && (p == 0u ? r == 0u : r + 1u == R);
+    double ex = 0.0, ey = 0.0, ez = 0.0;
+    int es = 0;
+    if (!from_prev && !from_next && !none) {
+        ex = fld<OFF32>(F.nx, e); ey = fld<OFF32>(F.ny, e); ez = fld<OFF32>(F.nz, e);
+        es = fld<OFF32>(F.s, e);
+    }
+    // y and z: pairs at site m0's neighbour (its bond and site m1's exist
+    // together; absent ones read the zero pair)
+    double2 vx[4], vy[4], vz[4];
+    uint32_t vs[4];
+#pragma unroll
+    for (int b = 2; b < 6; ++b) {
+        const uint32_t rr = n0[b];
+        vx[b - 2] = fld2<OFF32>(F.nx, rr); vy[b - 2] = fld2<OFF32>(F.ny, rr); vz[b - 2] = fld2<OFF32>(F.nz, rr);
+        vs[b - 2] = spin2<OFF32>(F.s, rr);
+    }
+    S0.nx = ox.x; S0.ny = oy.x; S0.nz = oz.x; S0.sg = lo_spin(os);
+    S1.nx = ox.y; S1.ny = oy.y; S1.nz = oz.y; S1.sg = hi_spin(os);
+    // the extra x neighbour from the adjacent lane: site q - 1 is lane l - 1's
+    // q + 1 (row parity 0), site q + 2 is lane l + 1's q (parity 1)
     const double px = lane_prev(xx.y), py = lane_prev(xy.y), pz = lane_prev(xz.y);
     const uint32_t ps = lane_prev(xs);
     const double nxv = lane_next(xx.x), nyv = lane_next(xy.x), nzv = lane_next(xz.x);
     const uint32_t ns = lane_next(xs);
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t r = (st0.i >> 1) >> 1, R = G.half >> 1;  // pair position in its row, pairs per row
-    double ex, ey, ez;
-    int es;
-    if (p == 0u && r != 0u && lane != 0u) {
+    if (from_prev) {
         ex = px; ey = py; ez = pz; es = hi_spin(ps);
-    } else if (p == 1u && r + 1u != R && lane != 63u) {
+    } else if (from_next) {
         ex = nxv; ey = nyv; ez = nzv; es = lo_spin(ns);
-    } else if (BC == BC_OPEN && (p == 0u ? r == 0u : r + 1u == R)) {
-        ex = ey = ez = 0.0; es = 0;
-    } else {
-        ex = fld<OFF32>(F.nx, e); ey = fld<OFF32>(F.ny, e); ez = fld<OFF32>(F.nz, e);
-        es = fld<OFF32>(F.s, e);
     }
     if (p == 0u) {
         S0.N.x[0] = xx.x; S0.N.y[0] = xy.x; S0.N.z[0] = xz.x; S0.N.s[0] = lo_spin(xs);
@@ -445,15 +464,10 @@ __device__ inline uint32_t load_pair(const Geom &G, const Fields &F, int colour,
         S1.N.x[0] = ex;   S1.N.y[0] = ey;   S1.N.z[0] = ez;   S1.N.s[0] = es;
         S1.N.x[1] = xx.y; S1.N.y[1] = xy.y; S1.N.z[1] = xz.y; S1.N.s[1] = hi_spin(xs);
     }
-    // y and z: pairs at site m0's neighbour (its bond and site m1's exist
-    // together; absent ones read the zero pair)
 #pragma unroll
     for (int b = 2; b < 6; ++b) {
-        const uint32_t r = n0[b];
-        const double2 vx = fld2<OFF32>(F.nx, r), vy = fld2<OFF32>(F.ny, r), vz = fld2<OFF32>(F.nz, r);
-        const uint32_t vs = spin2<OFF32>(F.s, r);
-        S0.N.x[b] = vx.x; S0.N.y[b] = vy.x; S0.N.z[b] = vz.x; S0.N.s[b] = lo_spin(vs);
-        S1.N.x[b] = vx.y; S1.N.y[b] = vy.y; S1.N.z[b] = vz.y; S1.N.s[b] = hi_spin(vs);
+        S0.N.x[b] = vx[b - 2].x; S0.N.y[b] = vy[b - 2].x; S0.N.z[b] = vz[b - 2].x; S0.N.s[b] = lo_spin(vs[b - 2]);
+        S1.N.x[b] = vx[b - 2].y; S1.N.y[b] = vy[b - 2].y; S1.N.z[b] = vz[b - 2].y; S1.N.s[b] = hi_spin(vs[b - 2]);
     }
     return own;
 }
@@ -589,30 +603,23 @@ __device__ inline void load_row4(const Fields &F, uint32_t nsub, uint32_t m, uin
 }
 
 // The observables of the four sites of packed indices m0, m0 + 1 (m0 even;
-// with half even both lie in one row).  The thread reads its own row, the +y
-// row and the +z plane's row (Row4 each).  The +x neighbour of site 3 and the
-// nx of its diagonal site (i + 1, j + 1) are site 0 of the next pair's own and
-// +y rows: taken from lane + 1 by DPP when nxt (that lane holds the next pair
-// of this row in this pass), else read; past the row end i wraps to 0.  Same
-// bond terms as measure_site, summed per site in x order.
+// with half even both lie in one row (j, k), packed offset h0 in the row)
+// from its own row O, the +y row Y and the +z plane's row Z (Row4 each,
+// Z unused without a +z bond).  The +x neighbour of site 3 and the nx of its
+// diagonal site (i + 1, j + 1) are site 0 of the next pair's own and +y rows:
+// taken from lane + 1 by DPP when nxt (that lane holds the next pair of this
+// row in this pass), else read; past the row end i wraps to 0.  Same bond
+// terms as measure_site, summed per site in x order.
 template <int BC>
-__device__ inline void measure_pair(const Geom &G, const Fields &F, double g, double KK, uint32_t m0, bool nxt,
+__device__ inline void measure_rows(const Geom &G, const Fields &F, double g, double KK, uint32_t m0, uint32_t h0,
+                                    uint32_t j, uint32_t k, bool nxt, const Row4 &O, const Row4 &Y, const Row4 &Z,
                                     MeasureAcc &A)
 {
-    const uint32_t row = fdiv(m0, G.fd_half);
-    const uint32_t h0 = m0 - row * G.half;
-    const uint32_t k = fdiv(row, G.fd_nj);
-    const uint32_t j = row - k * G.nj;
     const uint32_t jp = (j + 1 == G.nj) ? 0u : j + 1;
     const bool zin = k + 1 < G.nk;
     const bool hz = zin || BC == BC_PERIODIC;
-    const uint32_t kp = zin ? k + 1 : 0u;
     const uint32_t my = m0 - j * G.half + jp * G.half;
-    const uint32_t cb = (j + k) & 1u, cy = (jp + k) & 1u, cz = (j + kp) & 1u;
-    Row4 O, Y, Z;
-    load_row4(F, G.nsub, m0, cb, O);
-    load_row4(F, G.nsub, my, cy, Y);
-    if (hz) load_row4(F, G.nsub, m0 - k * G.plane + kp * G.plane, cz, Z);
+    const uint32_t cb = (j + k) & 1u, cy = (jp + k) & 1u;
     double ex = lane_next(O.x[0]), ey = lane_next(O.y[0]), ez = lane_next(O.z[0]);
     int es = (int)lane_next((uint32_t)O.s[0]);
     double edx = lane_next(Y.x[0]);
@@ -662,6 +669,25 @@ __device__ inline void measure_pair(const Geom &G, const Fields &F, double g, do
         A.v[7] += ay * az;
         A.s += sa;
     }
+}
+
+// measure_rows on rows loaded here (the ensemble's form: one pass per row).
+template <int BC>
+__device__ inline void measure_pair(const Geom &G, const Fields &F, double g, double KK, uint32_t m0, bool nxt,
+                                    MeasureAcc &A)
+{
+    const uint32_t row = fdiv(m0, G.fd_half);
+    const uint32_t h0 = m0 - row * G.half;
+    const uint32_t k = fdiv(row, G.fd_nj);
+    const uint32_t j = row - k * G.nj;
+    const uint32_t jp = (j + 1 == G.nj) ? 0u : j + 1;
+    const bool hz = k + 1 < G.nk || BC == BC_PERIODIC;
+    const uint32_t kp = k + 1 < G.nk ? k + 1 : 0u;
+    Row4 O, Y, Z;
+    load_row4(F, G.nsub, m0, (j + k) & 1u, O);
+    load_row4(F, G.nsub, m0 - j * G.half + jp * G.half, (jp + k) & 1u, Y);
+    if (hz) load_row4(F, G.nsub, m0 - k * G.plane + kp * G.plane, (j + kp) & 1u, Z);
+    measure_rows<BC>(G, F, g, KK, m0, h0, j, k, nxt, O, Y, Z, A);
 }
 
 // Deterministic block reduction of MeasureAcc: wave butterflies, then a

@@ -132,13 +132,15 @@ def test_matches_oracle(orc, dims, bc, rng, arith, nsw, nout):
         assert out["stream_pos"] == pos
 
 
-# Observable shapes: half even (four sites per thread) with pairs wrapping in
-# x at every row end (ni = 4), odd nj and nk, rows longer than a wave (ni =
-# 260: 65 pairs per row), single rows and planes, and measure blocks capped so
-# that block ranges split rows (132 x 100 x 90); half odd (one site pair per
-# thread): 10 x 6 x 4.
+# Observable shapes: half even (four sites per thread, marching along k in
+# slabs of planes) with pairs wrapping in x at every row end (ni = 4), odd nj
+# and nk, rows longer than a wave (ni = 260: 65 pairs per row), single rows and
+# planes, slabs marching up and down with a partial last slab (132 x 100 x 90,
+# 8 x 6 x 17), slab counts that are multiples of 8 (16 x 8 x 64, 64 x 32 x 128:
+# the slabs of one XCD contiguous); half odd (one site pair per thread):
+# 10 x 6 x 4; odd ni: 5 x 4 x 6, 7 x 3 x 4, 3 x 1 x 2.
 MEASURE_SHAPES = [(12, 10, 8), (4, 3, 5), (260, 3, 2), (8, 1, 1), (10, 6, 4), (132, 100, 90), (5, 4, 6),
-                  (7, 3, 4), (3, 1, 2)]
+                  (7, 3, 4), (3, 1, 2), (16, 8, 64), (64, 32, 128), (8, 6, 17)]
 
 
 @pytest.mark.parametrize("dims", MEASURE_SHAPES, ids=["x".join(map(str, d)) for d in MEASURE_SHAPES])
@@ -153,7 +155,9 @@ def test_measure_row(orc, dims):
             row = lat.measure(g, K)
         N = np.prod(dims)
         assert abs(row["E"] - orc.etot(st, g, K)) <= f32_ulp(row["E"])
-        assert abs(row["E_true"] - orc.energy_true(st, g, K, bc=obc) / N) < 1e-13
+        # the oracle sums N site terms serially; its rounding error grows with N
+        # (one wrong bond moves the mean by ~1/N, far above this bound)
+        assert abs(row["E_true"] - orc.energy_true(st, g, K, bc=obc) / N) < 1e-13 * max(1.0, N / 1e5)
         assert row["eta_exact"] == st["s"].sum() / N
         Q = {"Qxx": st["nx"] ** 2, "Qyy": st["ny"] ** 2, "Qzz": st["nz"] ** 2, "Qxy": st["nx"] * st["ny"],
              "Qxz": st["nx"] * st["nz"], "Qyz": st["ny"] * st["nz"]}
