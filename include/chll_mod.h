@@ -41,10 +41,12 @@ void init(double *nx, double *ny, double *nz, int32_t *s, int32_t *dope, int32_t
  * (E/site, eta, paccept, faccept, d), rows = (1+nstop-nstart)/nout, and prints
  * the reference's table to stdout (silenced by chll_configure(..., verbose=0)
  * or CHLL_QUIET=1).  nx, ny, nz, s are updated in place, *d receives the
- * adapted step and *nsub = N/2.  Arrays are Fortran-ordered (ni,nj,nk);
- * ni must be even.  No device state survives the call.  Boundary condition,
- * arithmetic and random source are those set by chll_configure (default:
- * open boundaries, reference-compatible arithmetic, Philox). */
+ * adapted step and *nsub = N/2.  Arrays are Fortran-ordered (ni,nj,nk),
+ * any shape with N even (periodic boundaries: ni, nj, nk even).  No device
+ * state survives the call.  Boundary condition, arithmetic and random source
+ * are those set by chll_configure (default: open boundaries,
+ * reference-compatible arithmetic, Philox).  Re-entrant: calls from several
+ * host threads each sweep their own device lattice. */
 void run(const int32_t *nstart, const int32_t *nstop, const int32_t *nout,
          double *nx, double *ny, double *nz, int32_t *s, int32_t *dope, int32_t *sl1, int32_t *sl2,
          double *g, double *KK, double *d, double *kbt,
