@@ -382,40 +382,6 @@ __device__ inline double energy_compat(const NB &N, double nx, double ny, double
     return fl32(e - g * (double)snew);
 }
 
-// Cone trial move (chll/chll_mod.f90:149-183), operation for operation; the
-// componentwise divisions by a shared norm are exact quotients via div_by_recip.
-__device__ inline void trial_compat(double nnx, double nny, double nnz, double d, double u_phi, double &ox,
-                                    double &oy, double &oz)
-{
-    const double phi = u_phi * kTwoPi;
-    double sphi, cphi;
-    sincos(phi, &sphi, &cphi);
-    if (__builtin_expect(fabs(nnz) > 0.999, 0)) {
-        const double xx = nnx + d * cphi;
-        const double yy = nny + d * sphi;
-        const double zz = nnz;
-        const double rsq = sqrt(xx * xx + yy * yy + zz * zz);
-        const double ir = 1.0 / rsq;
-        ox = div_by_recip(xx, rsq, ir); oy = div_by_recip(yy, rsq, ir); oz = div_by_recip(zz, rsq, ir);
-    } else {
-        double ux = -nny, uy = nnx, uz = 0.0;
-        double rsq = sqrt(ux * ux + uy * uy);
-        double ir = 1.0 / rsq;
-        ux = div_by_recip(ux, rsq, ir); uy = div_by_recip(uy, rsq, ir); uz = div_by_recip(uz, rsq, ir);
-        double vx = -nnz * nnx, vy = -nnz * nny, vz = nnx * nnx + nny * nny;
-        rsq = sqrt(vx * vx + vy * vy + vz * vz);
-        ir = 1.0 / rsq;
-        vx = div_by_recip(vx, rsq, ir); vy = div_by_recip(vy, rsq, ir); vz = div_by_recip(vz, rsq, ir);
-        const double dc = d * cphi, ds = d * sphi;
-        const double tx = nnx + dc * ux + ds * vx;
-        const double ty = nny + dc * uy + ds * vy;
-        const double tz = nnz + dc * uz + ds * vz;
-        rsq = sqrt(tx * tx + ty * ty + tz * tz);
-        ir = 1.0 / rsq;
-        ox = div_by_recip(tx, rsq, ir); oy = div_by_recip(ty, rsq, ir); oz = div_by_recip(tz, rsq, ir);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Pure binary64 variants.  Per bond b with neighbour director m_b:
 //   c_b = n.m_b, q_b = (n x m_b)_axis(b), sgn_b = +1 for +x/+y/+z, -1 otherwise.

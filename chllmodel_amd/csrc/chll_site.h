@@ -129,8 +129,13 @@ __device__ inline bool dir_compute(const PhaseArgs &A, int colour, uint32_t m, c
         u_acc = A.u_acc[m];
     }
     if (ARITH == ARITH_COMPAT) {
+        // the reference's binary32-rounded energies around the binary64 cone
+        // trial (trial_fp64): its trial director agrees with the reference's
+        // operation order (chll/chll_mod.f90:149-183, IEEE divisions and
+        // square roots) to a few ulp, which the energies' binary32 roundings
+        // absorb; the reference's order cost a fifth of the director kernel
         const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
-        trial_compat(nx, ny, nz, d, u_phi, t[0], t[1], t[2]);
+        trial_fp64(nx, ny, nz, d, u_phi, t[0], t[1], t[2]);
         reload_barrier();
         const double enew = energy_compat(N, t[0], t[1], t[2], sg, A.g, A.KK);
         return (enew < eold) || accept_exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt), u_acc);
