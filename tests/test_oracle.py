@@ -401,3 +401,27 @@ def test_slab_with_site_offset_matches_whole_lattice(orc, arith):
     orc.run(ref, 20003, 20003, 1, 0.5, 1.0, 0.3, 0.7, rng=RNG_PHILOX, seed=9, run_index=2, arith=arith,
             site_offset=ni * nj * k0 // 2)
     assert not np.array_equal(other["nx"], ref["nx"])
+
+
+@pytest.mark.parametrize("arith", [ARITH_COMPAT, ARITH_FP64])
+@pytest.mark.parametrize("rng,bc", [(RNG_STREAM, BC_OPEN), (RNG_PHILOX, BC_OPEN), (RNG_PHILOX, BC_PERIODIC)])
+def test_threaded_and_hostfilled_runs_equal_the_serial_run(orc, arith, rng, bc):
+    """The threaded restatement (the CPU baseline and the checker of the
+    longer GPU runs) and the host-filled stream (every sweep's 10 nsub
+    uniforms drawn first, as the reference's random_number calls do,
+    chll/chll_mod.f90:66-80) leave the lattice, rows, counts and stream
+    position of the serial run bit for bit: sites of one colour are
+    independent within a half-sweep (chll/chll_mod.f90:129, 213)."""
+    st0 = isotropic_state(12, 10, 8, 77)
+    kw = dict(rng=rng, seed=99, bc=bc, arith=arith, want_counts=True)
+    ref = copy_state(st0)
+    want = orc.run(ref, 1, 30, 5, 0.5, 1.0, 0.3, 0.7, **kw)
+    variants = [dict(threads=4), dict(threads=1, hostfill=True), dict(threads=3, hostfill=True)]
+    for v in variants:
+        got_st = copy_state(st0)
+        got = orc.run(got_st, 1, 30, 5, 0.5, 1.0, 0.3, 0.7, **kw, **v)
+        for f in ("nx", "ny", "nz", "s"):
+            assert np.array_equal(got_st[f], ref[f]), (v, f)
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), v
+        assert got[2] == want[2] and got[3] == want[3], v
+        assert np.array_equal(got[4], want[4]), v
