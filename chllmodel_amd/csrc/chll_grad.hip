@@ -307,6 +307,202 @@ __global__ __launch_bounds__(kGradBlock) void k_energy_grad2(GradArgs A)
     }
 }
 
+// A pair of sites (l, l + 1) of one row: directors and spins.
+struct GPair {
+    double2 x, y, z;
+    uint32_t s;
+};
+
+__device__ inline GPair load_gpair(const double *nx, const double *ny, const double *nz, const int8_t *s, uint32_t q)
+{
+    return GPair{ld2(nx + q), ld2(ny + q), ld2(nz + q), *reinterpret_cast<const uint16_t *>(s + q)};
+}
+
+__device__ inline int gp_lo(uint32_t v) { return (int)(int8_t)(v & 0xffu); }
+__device__ inline int gp_hi(uint32_t v) { return (int)(int8_t)(v >> 8); }
+
+// The two-site energy and gradient marching along k.  Block (lattice
+// blockIdx.y, tile t of BS consecutive pairs of a plane, slab sl of ks
+// planes): each thread takes one pair column through the slab.  Every step
+// loads the pair ahead (+z marching up, -z marching down) and carries its own
+// pair and the one behind from the steps before, so the pairs of a plane are
+// loaded once per slab.  The +-y pairs of a pair are other threads' own pairs
+// of the same plane: each step the block puts its pairs in LDS (two buffers,
+// one barrier per step), and only the +-y pairs outside the tile (the rows
+// beyond its first and last) are loaded, from L2.  Slabs alternate direction,
+// so the planes that adjacent slabs both read are read by both at about the
+// same time, and the slabs of one XCD are contiguous (workgroup x runs on XCD
+// x % 8 when the grid is a multiple of 8).  The gradient goes out with
+// streaming stores: it is not read again, so it does not take L2 lines from
+// the directors.  Every site's bonds are taken in the same order as in
+// k_energy_grad2, so the gradient is the same bit for bit; the partial sums
+// are per block (tiles x nslab per lattice).
+template <int BC, int BS>
+__global__ __launch_bounds__(BS) void k_energy_grad_march(GradArgs A, uint32_t tiles, uint32_t nslab, uint32_t ks)
+{
+    const uint32_t b = blockIdx.y, x = blockIdx.x;
+    uint32_t sl, t;
+    if ((nslab & 7u) == 0u) {
+        const uint32_t n = x >> 3;
+        sl = (x & 7u) * (nslab >> 3) + n / tiles;
+        t = n % tiles;
+    } else {
+        sl = x / tiles;
+        t = x % tiles;
+    }
+    const double *nx = A.n + (size_t)b * 3 * A.N, *ny = nx + A.N, *nz = ny + A.N;
+    const int8_t *s = A.s + (size_t)b * A.N;
+    const double K = A.K[b], m2eps = -2.0 * A.eps[b];
+    const uint32_t plane = A.ni * A.nj, half = A.ni >> 1;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t pq = t * BS + tid;  // pair index in its plane
+    const uint32_t lp = 2u * pq;       // site a of the pair, in its plane
+    const uint32_t k0 = sl * ks, k1 = k0 + ks < A.nk ? k0 + ks : A.nk;
+    const bool act = lp < plane;
+    const uint32_t j = act ? fdiv(lp, A.fd_ni) : 0u;
+    const uint32_t i = lp - j * A.ni;
+    const bool up = (sl & 1u) == 0u;
+    const int32_t dk = up ? 1 : -1;
+    // the +-y pairs (plane pair index) and whether they lie in this tile
+    const bool hyp = act && (j + 1u < A.nj || BC == BC_PERIODIC), hym = act && (j >= 1u || BC == BC_PERIODIC);
+    const uint32_t qyp = j + 1u < A.nj ? pq + half : pq - (A.nj - 1u) * half;
+    const uint32_t qym = j >= 1u ? pq - half : pq + (A.nj - 1u) * half;
+    const uint32_t tb = t * BS;
+    const bool yp_in = qyp - tb < (uint32_t)BS, ym_in = qym - tb < (uint32_t)BS;
+    __shared__ double2 lx[2][BS], ly[2][BS], lz[2][BS];
+    __shared__ uint32_t lsp[2][BS];
+    // plane index of the neighbour of plane k in direction d (+1 / -1), or nk
+    // when it is absent
+    auto zn = [&](uint32_t k, int32_t d) -> uint32_t {
+        if (d > 0) return k + 1u < A.nk ? k + 1u : (BC == BC_PERIODIC ? 0u : A.nk);
+        return k >= 1u ? k - 1u : (BC == BC_PERIODIC ? A.nk - 1u : A.nk);
+    };
+    double sum_e = 0.0, sum_cqf = 0.0, sum_s = 0.0;
+    const uint32_t kb = up ? k0 : k1 - 1u;  // first plane of the walk
+    GPair C{}, P{};  // own pair, and the pair behind the walk
+    if (act && k0 < k1) {
+        C = load_gpair(nx, ny, nz, s, kb * plane + lp);
+        const uint32_t kq = zn(kb, -dk);
+        if (kq < A.nk) P = load_gpair(nx, ny, nz, s, kq * plane + lp);
+    }
+    for (uint32_t step = 0; k0 + step < k1; ++step) {  // same trip count in the whole block
+        const uint32_t k = up ? k0 + step : k1 - 1u - step;
+        const uint32_t l = k * plane + lp;
+        const uint32_t buf = step & 1u;
+        GPair N{}, Yp{}, Ym{};
+        if (act) {
+            const uint32_t ka = zn(k, dk);
+            if (ka < A.nk) N = load_gpair(nx, ny, nz, s, ka * plane + lp);
+            if (hyp && !yp_in) Yp = load_gpair(nx, ny, nz, s, k * plane + 2u * qyp);
+            if (hym && !ym_in) Ym = load_gpair(nx, ny, nz, s, k * plane + 2u * qym);
+        }
+        lx[buf][tid] = C.x; ly[buf][tid] = C.y; lz[buf][tid] = C.z; lsp[buf][tid] = C.s;
+        __syncthreads();
+        if (act) {
+            if (hyp && yp_in) {
+                const uint32_t u = qyp - tb;
+                Yp = GPair{lx[buf][u], ly[buf][u], lz[buf][u], lsp[buf][u]};
+            }
+            if (hym && ym_in) {
+                const uint32_t u = qym - tb;
+                Ym = GPair{lx[buf][u], ly[buf][u], lz[buf][u], lsp[buf][u]};
+            }
+        }
+        const double2 vx = C.x, vy = C.y, vz = C.z;
+        const int sa = gp_lo(C.s), sb = gp_hi(C.s);
+        const double nxx = lane_next(vx.x), nxy = lane_next(vy.x), nxz = lane_next(vz.x);
+        const uint32_t nsw = lane_next(C.s);
+        const double pxx = lane_prev(vx.y), pxy = lane_prev(vy.y), pxz = lane_prev(vz.y);
+        const uint32_t psw = lane_prev(C.s);
+        if (act) {
+            double ga[3] = {0.0, 0.0, 0.0}, gb[3] = {0.0, 0.0, 0.0};
+            bond<true, true, true>(0, vx.x, vy.x, vz.x, vx.y, vy.y, vz.y, 0.5 * (double)(sa + sb), m2eps, K, sum_e,
+                                   sum_cqf, ga, gb);
+            {
+                const bool in = i + 2u < A.ni;
+                if (in || BC == BC_PERIODIC) {
+                    double bx, by, bz;
+                    int sq;
+                    if (in && lane != 63u) {
+                        bx = nxx; by = nxy; bz = nxz; sq = gp_lo(nsw);
+                    } else {
+                        const uint32_t q = in ? l + 2u : l - i;
+                        bx = nx[q]; by = ny[q]; bz = nz[q]; sq = s[q];
+                    }
+                    bond<true, true, false>(0, vx.y, vy.y, vz.y, bx, by, bz, 0.5 * (double)(sb + sq), m2eps, K,
+                                            sum_e, sum_cqf, gb, gb);
+                }
+            }
+            {
+                const bool in = i >= 1u;
+                if (in || BC == BC_PERIODIC) {
+                    double bx, by, bz;
+                    int sq;
+                    if (in && lane != 0u) {
+                        bx = pxx; by = pxy; bz = pxz; sq = gp_hi(psw);
+                    } else {
+                        const uint32_t q = in ? l - 1u : l + A.ni - 1u;
+                        bx = nx[q]; by = ny[q]; bz = nz[q]; sq = s[q];
+                    }
+                    bond<false, false, true>(0, bx, by, bz, vx.x, vy.x, vz.x, 0.5 * (double)(sq + sa), m2eps, K,
+                                             sum_e, sum_cqf, ga, ga);
+                }
+            }
+            // y, then z: the pair above (a, b lower ends), then the pair below
+            // (a, b upper ends), as in k_energy_grad2
+#pragma unroll
+            for (int ax = 0; ax < 2; ++ax) {
+                const int axis = ax + 1;
+                const bool hp = ax == 0 ? hyp : zn(k, 1) < A.nk;
+                const bool hm = ax == 0 ? hym : zn(k, -1) < A.nk;
+                const GPair &U = ax == 0 ? Yp : (up ? N : P);
+                const GPair &D = ax == 0 ? Ym : (up ? P : N);
+                if (hp) {
+                    const int ta = gp_lo(U.s), tb2 = gp_hi(U.s);
+                    bond<true, true, false>(axis, vx.x, vy.x, vz.x, U.x.x, U.y.x, U.z.x, 0.5 * (double)(sa + ta),
+                                            m2eps, K, sum_e, sum_cqf, ga, ga);
+                    bond<true, true, false>(axis, vx.y, vy.y, vz.y, U.x.y, U.y.y, U.z.y, 0.5 * (double)(sb + tb2),
+                                            m2eps, K, sum_e, sum_cqf, gb, gb);
+                }
+                if (hm) {
+                    const int ta = gp_lo(D.s), tb2 = gp_hi(D.s);
+                    bond<false, false, true>(axis, D.x.x, D.y.x, D.z.x, vx.x, vy.x, vz.x, 0.5 * (double)(ta + sa),
+                                             m2eps, K, sum_e, sum_cqf, ga, ga);
+                    bond<false, false, true>(axis, D.x.y, D.y.y, D.z.y, vx.y, vy.y, vz.y, 0.5 * (double)(tb2 + sb),
+                                             m2eps, K, sum_e, sum_cqf, gb, gb);
+                }
+            }
+            sum_s += (double)(sa + sb);
+            if (A.dEdn) {
+                const double pa = fma(ga[2], vz.x, fma(ga[1], vy.x, ga[0] * vx.x));
+                const double pb = fma(gb[2], vz.y, fma(gb[1], vy.y, gb[0] * vx.y));
+                typedef double d2v __attribute__((ext_vector_type(2)));
+                d2v *o = reinterpret_cast<d2v *>(A.dEdn + (size_t)b * 3 * A.N + l);
+                __builtin_nontemporal_store(d2v{fma(-pa, vx.x, ga[0]), fma(-pb, vx.y, gb[0])}, o);
+                __builtin_nontemporal_store(d2v{fma(-pa, vy.x, ga[1]), fma(-pb, vy.y, gb[1])}, o + A.N / 2);
+                __builtin_nontemporal_store(d2v{fma(-pa, vz.x, ga[2]), fma(-pb, vz.y, gb[2])}, o + A.N);
+            }
+        }
+        P = C;
+        C = N;
+    }
+    __shared__ double red[BS / 64][3];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        sum_e += __shfl_xor(sum_e, off, 64);
+        sum_cqf += __shfl_xor(sum_cqf, off, 64);
+        sum_s += __shfl_xor(sum_s, off, 64);
+    }
+    const uint32_t wid = tid >> 6;
+    if (lane == 0) { red[wid][0] = sum_e; red[wid][1] = sum_cqf; red[wid][2] = sum_s; }
+    __syncthreads();
+    if (tid < 3) {
+        double v = 0.0;
+        for (int w = 0; w < BS / 64; ++w) v += red[w][tid];
+        A.part[((size_t)b * A.nblk + blockIdx.x) * 4 + tid] = v;
+    }
+}
+
 // One block per lattice: E = eps S_e - K S_cqf - g S_s and the parameter
 // derivatives dE/deps = S_e, dE/dK = -S_cqf, dE/dg = -S_s.
 __global__ __launch_bounds__(256) void k_energy_final(GradArgs A, double *E, double *dK, double *dg, double *deps)
@@ -408,6 +604,29 @@ int grad_launch(const double *n, const int8_t *s, int32_t batch, int32_t ni, int
     const uint64_t plane = (uint64_t)ni * nj / per_thread;
     A.band = plane % (8u * kGradBlock) == 0 ? (uint32_t)(plane / (8u * kGradBlock)) : 0u;
     hipStream_t st = (hipStream_t)stream;
+    // marching slabs of ks planes (k_energy_grad_march) when the blocks fit the
+    // workspace that chll_energy_workspace_bytes sizes (one partial per block
+    // of one-site threads); blocks of 1024 threads (taller tiles) ran slower
+    constexpr uint32_t kMarchBlock = kGradBlock;
+    const uint32_t tiles = (uint32_t)((plane + kMarchBlock - 1) / kMarchBlock);
+    // slabs of 8 planes (16 ran the same at 8 x 128^3; 32 slower, with more
+    // re-reads)
+    const uint32_t ks = nk < 8 ? (uint32_t)nk : 8u;
+    const uint32_t nslab = ((uint32_t)nk + ks - 1) / ks;
+    if (pairs && (uint64_t)tiles * nslab <= (N + kGradBlock - 1) / kGradBlock) {
+        A.nblk = tiles * nslab;
+        A.band = 0;
+        dim3 grid(A.nblk, batch);
+        if (bc == BC_OPEN)
+            hipLaunchKernelGGL((k_energy_grad_march<BC_OPEN, kMarchBlock>), grid, dim3(kMarchBlock), 0, st, A, tiles, nslab, ks);
+        else
+            hipLaunchKernelGGL((k_energy_grad_march<BC_PERIODIC, kMarchBlock>), grid, dim3(kMarchBlock), 0, st, A, tiles,
+                               nslab, ks);
+        hipLaunchKernelGGL(k_energy_final, dim3(batch), dim3(256), 0, st, A, E, dEdK, dEdg, dEdeps);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return gfail("chll_energy_grad launch: %s", hipGetErrorString(e));
+        return 0;
+    }
     dim3 grid(A.nblk, batch);
     if (pairs) {
         if (bc == BC_OPEN) hipLaunchKernelGGL(k_energy_grad2<BC_OPEN>, grid, dim3(kGradBlock), 0, st, A);
