@@ -39,7 +39,7 @@ struct RepDesc {
     Fields F;
     uint32_t blk_start, nblk;  // this replica's blocks in a launch
     uint32_t replica;          // Philox counter word
-    uint32_t pad;
+    uint32_t nmeas;            // its measurement partials (blocks of the measure launch)
     double g, KK;
     uint32_t *counts;          // kGroups x kSlots x kSlotStride (groups as the single lattice)
 };
@@ -49,6 +49,7 @@ struct EnsDev {
     const RepDesc *R;
     const uint32_t *blk_rep;  // per launch block: replica << 24 | local block
     const uint2 *blk_rep2;    // pair kernels: (replica << 24 | local block, live threads), or null
+    const uint32_t *blk_rep3; // k-marching observables: replica << 24 | local block, or null
     const uint32_t *order;    // replicas in block-table order (largest first)
     double *kbt, *inv_kbt;    // per temperature slot
     double *d;                // per temperature slot (adaptive cone step)
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(BS) void k_pair_ens(EnsDev E, uint32_t istep_off, i
             a0 = spin_dir_apply<true, true>(D.F, A, d, m0, own, S0, f0, &O0);
             a1 = spin_dir_apply<true, true>(D.F, A, d, m0 + 1u, own + 1u, S1, f1, &O1);
         }
-        store_pair<KIND != 1, KIND != 0>(D.F, own, O0, O1);
+        store_pair<KIND != 1, KIND != 0, true>(D.F, own, O0, O1);
     }
     const uint32_t wa = (uint32_t)(__popcll(__ballot(a0)) + __popcll(__ballot(a1)));
     const uint32_t wf = (uint32_t)(__popcll(__ballot(f0)) + __popcll(__ballot(f1)));
@@ -199,6 +200,32 @@ __global__ __launch_bounds__(BS) void k_measure_ens_pair(EnsDev E, uint32_t base
     if (threadIdx.x < BS / 2 && m0 < end)
         measure_pair<BC>(D.G, D.F, D.g, D.KK, m0, (threadIdx.x & 63u) != 63u && m0 + 2u < end, acc);
     block_reduce_measure<BS>(acc, E.part + (size_t)gb * 8, E.spart + gb);
+}
+
+// The same observables marching along k (measure_march_column, as the single
+// lattice's k_measure_march): block lb of replica r takes tile lb % tiles of
+// BS packed pairs of a plane through slab lb / tiles of 8 planes; its partial
+// is the replica's lb-th.  Used when every replica's march blocks fit its
+// partials (nmeas <= nblk).
+template <int BC, int BS>
+__global__ __launch_bounds__(BS) void k_measure_ens_march(EnsDev E, uint32_t base)
+{
+    const uint32_t code = E.blk_rep3[base + blockIdx.x];
+    const uint32_t r = code >> 24, lb = code & 0xFFFFFFu;
+    const RepDesc &D = E.R[r];
+    const uint32_t quads = D.G.plane >> 1;
+    const uint32_t tiles = (quads + BS - 1) / BS;
+    const uint32_t ks = D.G.nk < 8u ? D.G.nk : 8u;
+    const uint32_t sl = lb / tiles, t = lb - sl * tiles;
+    const uint32_t qd = t * BS + threadIdx.x;
+    const uint32_t k0 = sl * ks, k1 = k0 + ks < D.G.nk ? k0 + ks : D.G.nk;
+    MeasureAcc acc;
+    if (qd < quads && k0 < k1) {
+        const bool nxt = (threadIdx.x & 63u) != 63u && qd + 1u < quads;
+        measure_march_column<BC>(D.G, D.F, D.g, D.KK, qd, k0, k1, (sl & 1u) == 0u, nxt, acc);
+    }
+    const size_t gb = (size_t)D.blk_start + lb;
+    block_reduce_measure<BS>(acc, E.part + gb * 8, E.spart + gb);
 }
 
 // Per-replica epilogue (one block per replica): accept/flip totals, step
@@ -248,7 +275,7 @@ __global__ __launch_bounds__(kEnsFin) void k_finalize_ens(EnsDev E, uint32_t ist
     for (int q = 0; q < 8; ++q) v[q] = 0.0;
     long long ss = 0;
     if (measure) {
-        for (uint32_t b = t; b < D.nblk; b += kEnsFin) {
+        for (uint32_t b = t; b < D.nmeas; b += kEnsFin) {
             const size_t gb = (size_t)D.blk_start + b;
 #pragma unroll
             for (int q = 0; q < 8; ++q) v[q] += E.part[gb * 8 + q];
@@ -384,10 +411,10 @@ int efail(const char *fmt, ...)
 }  // namespace
 
 // One chunk of replicas (consecutive in the block-table order): its blocks in
-// the one-site table [t1, t1 + n1) and in the pair table [t2, t2 + n2), its
-// replicas order[f .. f + nf).
+// the one-site table [t1, t1 + n1), in the pair table [t2, t2 + n2) and in the
+// marching-observables table [t3, t3 + n3), its replicas order[f .. f + nf).
 struct Chunk {
-    uint32_t t1, n1, t2, n2, f, nf;
+    uint32_t t1, n1, t2, n2, t3, n3, f, nf;
 };
 
 struct chll_ensemble {
@@ -403,6 +430,9 @@ struct chll_ensemble {
     uint2 *blk_rep2 = nullptr;  // pair-kernel blocks (when every replica allows site pairs)
     uint32_t nblocks2 = 0;
     bool pairs = false;
+    uint32_t *blk_rep3 = nullptr;  // marching-observables blocks (pairs, and every replica's fit its partials)
+    uint32_t nblocks3 = 0;
+    bool march = false;
     uint32_t *order_dev = nullptr;
     // Replicas are independent between swaps, so a block of K sweeps runs
     // chunk by chunk: all K sweeps of one chunk of at most chunk_bytes of state
@@ -449,25 +479,28 @@ void drop_graph(chll_ensemble *X)
 void build_chunks(chll_ensemble *X)
 {
     X->chunks.clear();
-    Chunk c{0, 0, 0, 0, 0, 0};
+    Chunk c{0, 0, 0, 0, 0, 0, 0, 0};
     size_t bytes = 0;
-    uint32_t p1 = 0, p2 = 0;
+    uint32_t p1 = 0, p2 = 0, p3 = 0;
     for (uint32_t i = 0; i < (uint32_t)X->order.size(); ++i) {
         const RepDesc &D = X->rep[X->order[i]];
         const size_t rb = (2 * (size_t)D.G.nsub + kFieldPad) * (3 * sizeof(double) + 1);
         const uint32_t nb1 = D.nblk;
         const uint32_t nb2 = X->pairs ? (D.G.nsub / 2 + X->bs - 1) / X->bs : 0u;
+        const uint32_t nb3 = X->march ? D.nmeas : 0u;
         if (c.nf > 0 && X->chunk_bytes > 0 && bytes + rb > X->chunk_bytes) {
             X->chunks.push_back(c);
-            c = Chunk{p1, 0, p2, 0, i, 0};
+            c = Chunk{p1, 0, p2, 0, p3, 0, i, 0};
             bytes = 0;
         }
         c.n1 += nb1;
         c.n2 += nb2;
+        c.n3 += nb3;
         c.nf += 1;
         bytes += rb;
         p1 += nb1;
         p2 += nb2;
+        p3 += nb3;
     }
     if (c.nf) X->chunks.push_back(c);
 }
@@ -513,21 +546,50 @@ int build_tables(chll_ensemble *X)
         }
     }
     X->nblocks2 = (uint32_t)table2.size();
+    // Marching-observables blocks: per replica tiles of bs packed pairs of a
+    // plane x slabs of 8 planes, wherever every replica's count fits its
+    // one-site blocks (whose partial slots the measure writes).
+    std::vector<uint32_t> table3;
+    X->march = true;  // in either arithmetic: the observables are binary64 in both
+    for (uint32_t r : X->order) {
+        const Geom &G = X->rep[r].G;
+        if (!pairs_ok(G)) X->march = false;
+        const uint32_t tiles = ((G.plane >> 1) + X->bs - 1) / X->bs;
+        const uint32_t ks = G.nk < 8u ? G.nk : 8u;
+        const uint64_t nm = (uint64_t)tiles * ((G.nk + ks - 1) / ks);
+        if (nm > X->rep[r].nblk) X->march = false;
+    }
+    for (uint32_t r : X->order) {
+        RepDesc &D = X->rep[r];
+        D.nmeas = D.nblk;
+        if (!X->march) continue;
+        const uint32_t tiles = ((D.G.plane >> 1) + X->bs - 1) / X->bs;
+        const uint32_t ks = D.G.nk < 8u ? D.G.nk : 8u;
+        D.nmeas = tiles * ((D.G.nk + ks - 1) / ks);
+        for (uint32_t b = 0; b < D.nmeas; ++b) table3.push_back((r << 24) | b);
+    }
+    X->nblocks3 = (uint32_t)table3.size();
     if (X->blk_rep) (void)hipFree(X->blk_rep);
     if (X->blk_rep2) (void)hipFree(X->blk_rep2);
+    if (X->blk_rep3) (void)hipFree(X->blk_rep3);
     X->blk_rep = nullptr;
     X->blk_rep2 = nullptr;
+    X->blk_rep3 = nullptr;
     if (X->E.part) (void)hipFree(X->E.part);
     if (X->E.spart) (void)hipFree(X->E.spart);
     X->E.part = nullptr;
     X->E.spart = nullptr;
     if (ealloc(&X->blk_rep, table.size()) || ealloc(&X->E.part, (size_t)X->nblocks * 8) ||
-        ealloc(&X->E.spart, X->nblocks) || (X->pairs && ealloc(&X->blk_rep2, table2.size())))
+        ealloc(&X->E.spart, X->nblocks) || (X->pairs && ealloc(&X->blk_rep2, table2.size())) ||
+        (X->march && ealloc(&X->blk_rep3, table3.size())))
         return -1;
     ECHECK(hipMemcpy(X->blk_rep, table.data(), table.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     if (X->pairs)
         ECHECK(hipMemcpy(X->blk_rep2, table2.data(), table2.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    if (X->march)
+        ECHECK(hipMemcpy(X->blk_rep3, table3.data(), table3.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     X->E.blk_rep2 = X->blk_rep2;
+    X->E.blk_rep3 = X->blk_rep3;
     ECHECK(hipMemcpy(X->R, X->rep.data(), X->rep.size() * sizeof(RepDesc), hipMemcpyHostToDevice));
     if (!X->order_dev && ealloc(&X->order_dev, X->order.size())) return -1;
     ECHECK(hipMemcpy(X->order_dev, X->order.data(), X->order.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -555,7 +617,8 @@ void launch_block_t(chll_ensemble *X, int K, hipStream_t st)
     // observables: four sites per thread wherever every replica allows site
     // pairs, whatever the fusion setting, so rows do not depend on it
     auto measure = [&](const Chunk &c) {
-        if (X->pairs) hipLaunchKernelGGL((k_measure_ens_pair<BC, BS>), dim3(c.n1), dim3(BS), 0, st, E, c.t1);
+        if (X->march) hipLaunchKernelGGL((k_measure_ens_march<BC, BS>), dim3(c.n3), dim3(BS), 0, st, E, c.t3);
+        else if (X->pairs) hipLaunchKernelGGL((k_measure_ens_pair<BC, BS>), dim3(c.n1), dim3(BS), 0, st, E, c.t1);
         else hipLaunchKernelGGL((k_measure_ens<BC, BS>), dim3(c.n1), dim3(BS), 0, st, E, c.t1);
     };
     for (const Chunk &c : X->chunks) {
@@ -758,8 +821,9 @@ int chll_ensemble_destroy(chll_ensemble *X)
         (void)hipFree(D.counts);
     }
     EnsDev &E = X->E;
-    void *ptrs[] = {X->R, X->blk_rep, X->blk_rep2, X->order_dev, E.kbt, E.inv_kbt, E.d, E.rep_slot, E.slot_rep, (void *)E.lad_start, E.E,
-                    E.istep, E.row, E.swap_att, E.swap_acc, E.part, E.spart, E.rows, X->stage, X->stage_s, X->bad};
+    void *ptrs[] = {X->R, X->blk_rep, X->blk_rep2, X->blk_rep3, X->order_dev, E.kbt, E.inv_kbt, E.d, E.rep_slot,
+                    E.slot_rep, (void *)E.lad_start, E.E, E.istep, E.row, E.swap_att, E.swap_acc, E.part, E.spart,
+                    E.rows, X->stage, X->stage_s, X->bad};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (X->pinned) (void)hipHostFree(X->pinned);

@@ -187,22 +187,15 @@ __global__ __launch_bounds__(kBlock) void k_measure(Geom G, Fields F, double g, 
     block_reduce_measure<kBlock>(acc, part + (size_t)rb * 8, spart + rb);
 }
 
-// The same observables with four sites of one row per thread (measure_pair);
-// chunk is a multiple of 2 kBlock, so lane l + 1 holds the pair after lane
-// l's in every pass.  Needs an even half and fields below 4 GiB.
-// The same observables marching along k, four sites of one row per thread.
-// Block (tile t of kBlock packed pairs of a plane, slab s of planes
-// [s ks, s ks + ks)): every step reads the +y row and one new plane row and
-// carries the other from the step before -- marching up, the +z row becomes
-// the next step's own row; marching down, the own row becomes the next
-// step's +z row -- so each plane row is read once per slab.  A row's +y row
-// is read at the step at which the lanes of row j + 1 read it as their own
-// row (down) or +z row (up), so it comes from L2.  Slabs alternate
-// direction, so the plane two adjacent slabs share is read by both at the
-// same time (both start or both end there), and the slabs of one XCD are
-// contiguous (workgroup b runs on XCD b % 8).
-// One partial per block, in block order.  Needs an even half and fields
-// below 4 GiB.
+// The same observables marching along k, four sites of one row per thread
+// (measure_march_column).  Block (tile t of kBlock packed pairs of a plane,
+// slab s of planes [s ks, s ks + ks)).  A row's +y row is read at the step at
+// which the lanes of row j + 1 read it as their own row (down) or +z row
+// (up), so it comes from L2.  Slabs alternate direction, so the plane two
+// adjacent slabs share is read by both at the same time (both start or both
+// end there), and the slabs of one XCD are contiguous (workgroup b runs on
+// XCD b % 8).  One partial per block, in block order.  Needs an even half and
+// fields below 4 GiB.
 template <int BC>
 __global__ __launch_bounds__(kBlock) void k_measure_march(Geom G, Fields F, double g, double KK,
                                                           double *__restrict__ part, long long *__restrict__ spart,
@@ -224,47 +217,16 @@ __global__ __launch_bounds__(kBlock) void k_measure_march(Geom G, Fields F, doub
     const uint32_t k1 = k0 + ks < G.nk ? k0 + ks : G.nk;
     MeasureAcc acc;
     if (qd < quads && k0 < k1) {
-        const uint32_t mp = 2u * qd;  // packed index of the pair in plane 0
-        const uint32_t j = fdiv(mp, G.fd_half);
-        const uint32_t h0 = mp - j * G.half;
-        const uint32_t jp = (j + 1 == G.nj) ? 0u : j + 1;
-        const uint32_t yo = jp * G.half - j * G.half;  // +y row offset (mod 2^32)
         // lane + 1 holds the next pair of the plane at every step
         const bool nxt = (threadIdx.x & 63u) != 63u && qd + 1u < quads;
-        Row4 O, Y, Z;
-        if ((sl & 1u) == 0u) {
-            // up: the +y row of the next step is read with this step's +z row,
-            // when the lanes of row j + 1 read that same row as their +z row
-            load_row4(F, G.nsub, k0 * G.plane + mp, (j + k0) & 1u, O);
-            load_row4(F, G.nsub, k0 * G.plane + mp + yo, (jp + k0) & 1u, Y);
-            for (uint32_t k = k0; k < k1; ++k) {
-                const uint32_t m0 = k * G.plane + mp;
-                const bool hz = k + 1 < G.nk || BC == BC_PERIODIC;
-                const uint32_t kp = k + 1 < G.nk ? k + 1 : 0u;
-                Row4 Yn;
-                if (hz) load_row4(F, G.nsub, kp * G.plane + mp, (j + kp) & 1u, Z);
-                if (k + 1 < k1) load_row4(F, G.nsub, m0 + G.plane + yo, (jp + k + 1) & 1u, Yn);
-                measure_rows<BC>(G, F, g, KK, m0, h0, j, k, nxt, O, Y, Z, acc);
-                O = Z;
-                Y = Yn;
-            }
-        } else {
-            {
-                const uint32_t kp = k1 < G.nk ? k1 : 0u;
-                if (k1 < G.nk || BC == BC_PERIODIC) load_row4(F, G.nsub, kp * G.plane + mp, (j + kp) & 1u, Z);
-            }
-            for (uint32_t k = k1; k-- > k0;) {
-                const uint32_t m0 = k * G.plane + mp;
-                load_row4(F, G.nsub, m0, (j + k) & 1u, O);
-                load_row4(F, G.nsub, m0 + yo, (jp + k) & 1u, Y);
-                measure_rows<BC>(G, F, g, KK, m0, h0, j, k, nxt, O, Y, Z, acc);
-                Z = O;
-            }
-        }
+        measure_march_column<BC>(G, F, g, KK, qd, k0, k1, (sl & 1u) == 0u, nxt, acc);
     }
     block_reduce_measure<kBlock>(acc, part + (size_t)b * 8, spart + b);
 }
 
+// The same observables with four sites of one row per thread (measure_pair);
+// chunk is a multiple of 2 kBlock, so lane l + 1 holds the pair after lane
+// l's in every pass.  Needs an even half and fields below 4 GiB.
 template <int BC>
 __global__ __launch_bounds__(kBlock) void k_measure_pair(Geom G, Fields F, double g, double KK,
                                                          double *__restrict__ part, long long *__restrict__ spart,

@@ -481,20 +481,26 @@ __device__ inline uint32_t load_pair(const Geom &G, const Fields &F, int colour,
 // DIRS) and one 2-byte word (spins, SPINS) at the pair's field index own.  A
 // site of the active colour is read by no other thread of the launch, so
 // writing back an unchanged value is harmless; every cache line of the active
-// colour is written back anyway.
-template <bool DIRS, bool SPINS>
+// colour is written back anyway.  NT: streaming (nontemporal) stores, which
+// the replica ensemble takes (its chunks run faster with them; the single
+// lattice's launches do not).
+template <bool DIRS, bool SPINS, bool NT = false>
 __device__ inline void store_pair(const Fields &F, uint32_t own, const SiteOut &O0, const SiteOut &O1)
 {
     typedef double d2v __attribute__((ext_vector_type(2)));
     typedef __attribute__((address_space(1))) d2v gd2v;
     typedef __attribute__((address_space(1))) uint16_t gu16;
+    auto put = [](auto v, auto *p) {
+        if constexpr (NT) __builtin_nontemporal_store(v, p);
+        else *p = v;
+    };
     if constexpr (DIRS) {
-        *(gd2v *)((char *)F.nx + own * 8u) = d2v{O0.nx, O1.nx};
-        *(gd2v *)((char *)F.ny + own * 8u) = d2v{O0.ny, O1.ny};
-        *(gd2v *)((char *)F.nz + own * 8u) = d2v{O0.nz, O1.nz};
+        put(d2v{O0.nx, O1.nx}, (gd2v *)((char *)F.nx + own * 8u));
+        put(d2v{O0.ny, O1.ny}, (gd2v *)((char *)F.ny + own * 8u));
+        put(d2v{O0.nz, O1.nz}, (gd2v *)((char *)F.nz + own * 8u));
     }
     if constexpr (SPINS)
-        *(gu16 *)((char *)F.s + own) = (uint16_t)(((uint32_t)O0.s & 0xffu) | (((uint32_t)O1.s & 0xffu) << 8));
+        put((uint16_t)(((uint32_t)O0.s & 0xffu) | (((uint32_t)O1.s & 0xffu) << 8)), (gu16 *)((char *)F.s + own));
 }
 
 // ---------------------------------------------------------------------------
@@ -693,6 +699,54 @@ __device__ inline void measure_pair(const Geom &G, const Fields &F, double g, do
     load_row4(F, G.nsub, m0 - j * G.half + jp * G.half, (jp + k) & 1u, Y);
     if (hz) load_row4(F, G.nsub, m0 - k * G.plane + kp * G.plane, (j + kp) & 1u, Z);
     measure_rows<BC>(G, F, g, KK, m0, h0, j, k, nxt, O, Y, Z, A);
+}
+
+// The observables of one column of four-site rows through the planes
+// [k0, k1) of a slab, marching along k: pair qd of a plane (packed indices
+// 2 qd, 2 qd + 1 of each colour), up when `up`, else down.  Every step reads
+// the +y row and one new plane row and carries the other from the step before
+// -- marching up, the +z row becomes the next step's own row; marching down,
+// the own row becomes the next step's +z row -- so each plane row is read
+// once per slab.  Marching up, the +y row of the next step is read with this
+// step's +z row, when the lanes of row j + 1 read that same row as their +z
+// row.  nxt: the lane after this one holds the next pair of the plane.
+template <int BC>
+__device__ inline void measure_march_column(const Geom &G, const Fields &F, double g, double KK, uint32_t qd,
+                                            uint32_t k0, uint32_t k1, bool up, bool nxt, MeasureAcc &acc)
+{
+    const uint32_t mp = 2u * qd;  // packed index of the pair in plane 0
+    const uint32_t j = fdiv(mp, G.fd_half);
+    const uint32_t h0 = mp - j * G.half;
+    const uint32_t jp = (j + 1 == G.nj) ? 0u : j + 1;
+    const uint32_t yo = jp * G.half - j * G.half;  // +y row offset (mod 2^32)
+    Row4 O, Y, Z;
+    if (up) {
+        load_row4(F, G.nsub, k0 * G.plane + mp, (j + k0) & 1u, O);
+        load_row4(F, G.nsub, k0 * G.plane + mp + yo, (jp + k0) & 1u, Y);
+        for (uint32_t k = k0; k < k1; ++k) {
+            const uint32_t m0 = k * G.plane + mp;
+            const bool hz = k + 1 < G.nk || BC == BC_PERIODIC;
+            const uint32_t kp = k + 1 < G.nk ? k + 1 : 0u;
+            Row4 Yn;
+            if (hz) load_row4(F, G.nsub, kp * G.plane + mp, (j + kp) & 1u, Z);
+            if (k + 1 < k1) load_row4(F, G.nsub, m0 + G.plane + yo, (jp + k + 1) & 1u, Yn);
+            measure_rows<BC>(G, F, g, KK, m0, h0, j, k, nxt, O, Y, Z, acc);
+            O = Z;
+            Y = Yn;
+        }
+    } else {
+        {
+            const uint32_t kp = k1 < G.nk ? k1 : 0u;
+            if (k1 < G.nk || BC == BC_PERIODIC) load_row4(F, G.nsub, kp * G.plane + mp, (j + kp) & 1u, Z);
+        }
+        for (uint32_t k = k1; k-- > k0;) {
+            const uint32_t m0 = k * G.plane + mp;
+            load_row4(F, G.nsub, m0, (j + k) & 1u, O);
+            load_row4(F, G.nsub, m0 + yo, (jp + k) & 1u, Y);
+            measure_rows<BC>(G, F, g, KK, m0, h0, j, k, nxt, O, Y, Z, acc);
+            Z = O;
+        }
+    }
 }
 
 // Deterministic block reduction of MeasureAcc: wave butterflies, then a

@@ -228,7 +228,8 @@ def _signature(replicas, bc, arith, K):
 
 
 def autotune(replicas, *, bc="open", arith="compat", K=10, blocks=3, candidates=(64, 128, 256, 512),
-             chunks=(0, 64 << 20, 128 << 20, 192 << 20), seed=12345, use_cache=True, verbose=False) -> dict:
+             chunks=(0, 128 << 20, 192 << 20, 256 << 20, 320 << 20), seed=12345, use_cache=True,
+             verbose=False) -> dict:
     """Time each candidate block size, then each chunk budget (bytes of state
     per chunk, 0 = all replicas at once) with the best block size, on a scratch
     ensemble of the same shape (so the caller's ensemble is not advanced).
