@@ -18,7 +18,7 @@ stream, and keeps the means <n_a n_b> (xx, yy, zz, xy, xz, yz) of the state
 after each call (q_series) beside that chain's rows (stream_series).
 
 Run in the build container only (needs /root/reference):
-    python tests/golden/make_equilibrium.py
+    python tests/golden/make_equilibrium.py [case names]
 """
 from __future__ import annotations
 
@@ -38,6 +38,9 @@ CASES = [
     ("eq_10c_K1_g05_T07", (10, 10, 10), 21, 0.5, 1.0, 0.3, 0.7),
     ("eq_10c_K05_g0_T15", (10, 10, 10), 22, 0.0, 0.5, 0.4, 1.5),
     ("eq_12c_K15_g1_T10", (12, 12, 12), 23, 1.0, 1.5, 0.3, 1.0),
+    # the L=32 workload's couplings (K = 0.5, g = 0, kT = 1) at its full size,
+    # with the reference's open boundaries (its sweep has no periodic mode)
+    ("eq_32c_K05_g0_T10", (32, 32, 32), 24, 0.0, 0.5, 0.3, 1.0),
 ]
 NSTART, NSWEEPS, NOUT, DISCARD = 10001, 20000, 10, 400
 STREAM_SEED = 0x51E4
@@ -64,9 +67,11 @@ def stream_chain(dims, seed, g, KK, d, kbt):
     return np.array(qs)[DISCARD:], np.array(rows)[DISCARD:]
 
 
-def main():
+def main(only=None):
     ref = Reference(stream=False)
     for name, dims, seed, g, KK, d, kbt in CASES:
+        if only and name not in only:
+            continue
         st = isotropic_state(*dims, seed)
         msteps, m, d_out, _ = ref.run(st, NSTART, NSTART + NSWEEPS - 1, NOUT, g, KK, d, kbt)
         assert d_out == d, (name, d_out)
@@ -79,4 +84,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])
