@@ -358,7 +358,7 @@ struct Nbr {
 // bonds in the order +x, -x, +y, -y, +z, -z; each bond
 //   e = fl32( (e + (1 - c*c)) -/+ ((K*c)*q)*f ),  c = (nx*mx + ny*my) + nz*mz,
 //   f = 0.5 * (snew + s_b);
-// then e = fl32(e - g*snew).  Absent bonds are skipped.
+// then e = fl32(e - g*snew).  Absent bonds leave e unchanged.
 template <class NB>
 __device__ inline double energy_compat(const NB &N, double nx, double ny, double nz, int snew, double g, double KK)
 {
@@ -373,11 +373,15 @@ __device__ inline double energy_compat(const NB &N, double nx, double ny, double
         else if (b < 4) cr = nz * mx - nx * mz;
         else cr = nx * my - ny * mx;
         const double sfac = 0.5 * (double)(snew + N.spin(b));
+        // an absent bond's neighbour reads zero (c = cr = 0), so with 0 in
+        // place of the 1 its term is +-0 and e stays as it is: the select
+        // moves from each sum to the constant, which the site's two energies
+        // share
+        const double one = (N.mask >> b) & 1u ? 1.0 : 0.0;
         double r;
-        if ((b & 1) == 0) r = e + (1.0 - dott * dott) - KK * dott * cr * sfac;
-        else r = e + (1.0 - dott * dott) + KK * dott * cr * sfac;
-        r = fl32(r);
-        e = (N.mask >> b) & 1u ? r : e;
+        if ((b & 1) == 0) r = e + (one - dott * dott) - KK * dott * cr * sfac;
+        else r = e + (one - dott * dott) + KK * dott * cr * sfac;
+        e = fl32(r);
     }
     return fl32(e - g * (double)snew);
 }
