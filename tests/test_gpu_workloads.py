@@ -19,6 +19,18 @@ called from two host threads at once.
 * run() from two host threads at once on different lattices (each call takes
   its own device lattice and its own run counter): each call's result equals
   the oracle run with one of the two run counters, and the two differ.
+* The L=480 coarsening run at its full length (110.6 M sites, K = 1,
+  g = 0.25, kT = 0.5, 5000 sweeps from a random state generated on the
+  device, a row every 50 sweeps): no oracle finishes this, so the checks are
+  the size-independent ones -- every row finite and in range, the adapted
+  acceptance in the 0.4 - 0.6 band, the energy lowered by the coarsening,
+  unit directors and +-1 spins at the end, and the last row's Hamiltonian
+  equal to the independent energy kernel's on the final state to 1e-12.
+* The differentiable-energy workload at its full length: 16 lattices of
+  128^3 with K in [0.5, 2] and g in [0, 1], a 500-step gradient quench
+  (final energy below the initial one) and then 2000 exact Metropolis
+  sweeps of every lattice at kT = 0.2, whose last rows must match the energy
+  kernel on the final states to 1e-12.
 """
 import ctypes as ct
 import math
@@ -30,9 +42,10 @@ import pytest
 import torch
 
 from chllmodel_amd import _lib
-from chllmodel_amd.autograd import energy_and_grad
+from chllmodel_amd.autograd import energy_and_grad, quench_then_metropolis
 from chllmodel_amd.ensemble import Ensemble, phase_scan_replicas
 from chllmodel_amd.lattice import Lattice
+from chllmodel_amd.observables import p2_from_q
 from chllmodel_amd.states import isotropic_state
 from oracle.oracle import ARITH_COMPAT, ARITH_FP64, BC_OPEN, BC_PERIODIC, RNG_PHILOX, Oracle, copy_state
 
@@ -218,3 +231,70 @@ def test_dropin_run_two_threads():
         assert match is not None, i
         used.append(match)
     assert sorted(used) == [0, 1]
+
+
+def _kernel_energy_per_site(st, K, g):
+    n = torch.from_numpy(np.stack([st[f].transpose(2, 1, 0) for f in ("nx", "ny", "nz")])[None]).cuda()
+    sg = torch.from_numpy(st["s"].transpose(2, 1, 0).astype(np.int8)[None]).cuda()
+    E, *_ = energy_and_grad(n, sg, K, g, 1.0, want_grad=False)
+    return float(E[0]) / st["s"].size
+
+
+@pytest.mark.timeout(900)
+def test_l480_coarsening_5000_sweeps():
+    L, K, g, kT, d0 = 480, 1.0, 0.25, 0.5, 0.3
+    N = L ** 3
+    rows = []
+    with Lattice(L, L, L) as lat:
+        print("L=480 coarsening: initial state on the device", flush=True)
+        lat.init_state("isotropic", seed=4800, p_flip=0.5)
+        lat.set_step(d0)
+        for c in range(10):  # ten calls of 500 sweeps, a row every 50
+            out = lat.sweeps(1 + 500 * c, 500 * (c + 1), 50, g=g, KK=K, kbt=kT, rng="philox", arith="fp64", seed=48)
+            rows.append(out["rows"])
+            print(f"L=480 coarsening: sweep {500 * (c + 1)}, E/N {out['rows'][-1][6]:.6f}", flush=True)
+        st = lat.download()
+    rows = np.concatenate(rows)
+    assert rows.shape[0] == 100 and np.array_equal(rows[:, 0], 50.0 * np.arange(1, 101))
+    assert np.all(np.isfinite(rows))
+    e, eta, pacc, facc = rows[:, 6], rows[:, 7], rows[:, 3], rows[:, 4]
+    # the cone step adapts each sweep (istep < 10000): acceptance in the band after the first rows
+    assert np.all((pacc[4:] > 0.38) & (pacc[4:] < 0.62)), pacc
+    assert np.all((facc >= 0.0) & (facc <= 1.0)) and np.all(np.abs(eta) <= 1.0)
+    p2 = np.array([p2_from_q(r[8:14]) for r in rows])
+    assert np.all((p2 >= 0.0) & (p2 <= 1.0))
+    # coarsening from a random state at kT = 0.5 lowers the energy
+    assert e[-10:].mean() < e[:10].mean() - 1e-3, (e[:10].mean(), e[-10:].mean())
+    assert np.all(np.isin(st["s"], (-1, 1)))
+    norm = np.sqrt(st["nx"] ** 2 + st["ny"] ** 2 + st["nz"] ** 2)
+    assert np.abs(norm - 1.0).max() < 1e-12
+    print("L=480 coarsening: energy kernel on the final state", flush=True)
+    e_kernel = _kernel_energy_per_site(st, K, g)
+    assert abs(e[-1] - e_kernel) <= 1e-12 * max(1.0, abs(e_kernel)), (e[-1], e_kernel)
+
+
+@pytest.mark.timeout(900)
+def test_quench_16_l128_full_length():
+    B, L = 16, 128
+    gen = torch.Generator().manual_seed(1616)
+    n = torch.randn(B, 3, L, L, L, generator=gen, dtype=torch.float64)
+    n /= torch.linalg.vector_norm(n, dim=1, keepdim=True)
+    s = torch.where(torch.rand(B, L, L, L, generator=gen) < 0.5, -1, 1).to(torch.int8)
+    n, s = n.cuda(), s.cuda()
+    K = torch.linspace(0.5, 2.0, B, dtype=torch.float64, device="cuda")
+    g = torch.linspace(0.0, 1.0, B, dtype=torch.float64, device="cuda")
+    print("16 x 128^3: 500-step quench, then 2000 sweeps per lattice at kT = 0.2", flush=True)
+    hist, rows, finals = quench_then_metropolis(n, s, K, g, steps=500, lr=0.05, sweeps=2000, kT=0.2, nout=100)
+    e = hist.cpu().numpy()  # (steps, B)
+    assert e.shape == (500, B) and np.all(np.isfinite(e))
+    assert np.all(e[-1] < e[0])
+    for b in range(B):
+        r = rows[b]
+        assert r.shape[0] == 20 and np.all(np.isfinite(r)), b
+        st = finals[b]
+        assert np.all(np.isin(st["s"], (-1, 1))), b
+        norm = np.sqrt(st["nx"] ** 2 + st["ny"] ** 2 + st["nz"] ** 2)
+        assert np.abs(norm - 1.0).max() < 1e-12, b
+        e_kernel = _kernel_energy_per_site(st, float(K[b]), float(g[b]))
+        assert abs(r[-1][6] - e_kernel) <= 1e-12 * max(1.0, abs(e_kernel)), (b, r[-1][6], e_kernel)
+    print("16 x 128^3: done", flush=True)
