@@ -26,6 +26,11 @@ called from two host threads at once.
   acceptance in the 0.4 - 0.6 band, the energy lowered by the coarsening,
   unit directors and +-1 spins at the end, and the last row's Hamiltonian
   equal to the independent energy kernel's on the final state to 1e-12.
+* The L=128 anneal at its full length: kT from 1.5 to 0.3 linearly over
+  20,000 sweeps, one call per sweep with that sweep's kT (the reference's
+  first 50 sweeps are compared bit for bit in test_gpu_configs.py): a row
+  every 100 sweeps finite, the energy lowered as kT falls, unit directors and
+  +-1 spins at the end, and the last row against the energy kernel to 1e-12.
 * The differentiable-energy workload at its full length: 16 lattices of
   128^3 with K in [0.5, 2] and g in [0, 1], a 500-step gradient quench
   (final energy below the initial one) and then 2000 exact Metropolis
@@ -298,3 +303,31 @@ def test_quench_16_l128_full_length():
         e_kernel = _kernel_energy_per_site(st, float(K[b]), float(g[b]))
         assert abs(r[-1][6] - e_kernel) <= 1e-12 * max(1.0, abs(e_kernel)), (b, r[-1][6], e_kernel)
     print("16 x 128^3: done", flush=True)
+
+
+@pytest.mark.timeout(900)
+def test_l128_anneal_20000_sweeps():
+    L, K, g, d0, nsw = 128, 1.0, 0.5, 0.3, 20000
+    rows = []
+    with Lattice(L, L, L) as lat:
+        lat.upload_state(isotropic_state(L, L, L, 20261203))
+        lat.set_step(d0)
+        for t in range(1, nsw + 1):
+            kt = 1.5 - 1.2 * (t - 1) / (nsw - 1)
+            out = lat.sweeps(t, t, 1 if t % 100 == 0 else 10 ** 6, g=g, KK=K, kbt=kt, rng="philox", arith="fp64",
+                             seed=128, want_rows=t % 100 == 0)
+            if t % 100 == 0:
+                rows.append(out["rows"][0])
+            if t % 5000 == 0:
+                print(f"L=128 anneal: sweep {t}, kT {kt:.3f}, E/N {rows[-1][6]:.6f}", flush=True)
+        st = lat.download()
+    rows = np.array(rows)
+    assert rows.shape[0] == nsw // 100 and np.all(np.isfinite(rows))
+    assert np.array_equal(rows[:, 0], 100.0 * np.arange(1, nsw // 100 + 1))
+    e = rows[:, 6]
+    assert e[-10:].mean() < e[:10].mean() - 0.1, (e[:10].mean(), e[-10:].mean())
+    assert np.all(np.isin(st["s"], (-1, 1)))
+    norm = np.sqrt(st["nx"] ** 2 + st["ny"] ** 2 + st["nz"] ** 2)
+    assert np.abs(norm - 1.0).max() < 1e-12
+    e_kernel = _kernel_energy_per_site(st, K, g)
+    assert abs(e[-1] - e_kernel) <= 1e-12 * max(1.0, abs(e_kernel)), (e[-1], e_kernel)
