@@ -156,8 +156,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default kernels instead of autotuning (clean profiles)")
     ap.add_argument("--batched-blocks", type=int, default=20, help="timed blocks of 10 sweeps")
-    ap.add_argument("--batched-obs-blocks", type=int, default=200,
-                    help="blocks of 10 sweeps for the scan's per-slot observables (0: skip)")
+    ap.add_argument("--batched-obs-blocks", type=int, default=1000,
+                    help="blocks of 10 sweeps for the scan's per-slot observables (0: skip); 1000 = the scan "
+                         "workload's 10k sweeps")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run one plain elementwise copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()

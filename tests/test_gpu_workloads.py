@@ -26,6 +26,13 @@ called from two host threads at once.
   acceptance in the 0.4 - 0.6 band, the energy lowered by the coarsening,
   unit directors and +-1 spins at the end, and the last row's Hamiltonian
   equal to the independent energy kernel's on the final state to 1e-12.
+* The 64-replica scan at its full length, 1000 blocks of 10 sweeps (10k
+  sweeps of every replica, swaps every 10, one graph replay per block): every
+  row finite, every temperature slot's C_v (jackknife) finite and positive
+  and chi_c finite and non-negative (0 where a strong field saturates the
+  spins), the sweep counter at 10001, and every replica's last row against
+  the energy kernel on its final state to 1e-12, with unit directors and +-1
+  spins.
 * The L=128 anneal at its full length: kT from 1.5 to 0.3 linearly over
   20,000 sweeps, one call per sweep with that sweep's kT (the reference's
   first 50 sweeps are compared bit for bit in test_gpu_configs.py): a row
@@ -48,9 +55,9 @@ import torch
 
 from chllmodel_amd import _lib
 from chllmodel_amd.autograd import energy_and_grad, quench_then_metropolis
-from chllmodel_amd.ensemble import Ensemble, phase_scan_replicas
+from chllmodel_amd.ensemble import Ensemble, phase_scan_replicas, series_by_temperature
 from chllmodel_amd.lattice import Lattice
-from chllmodel_amd.observables import p2_from_q
+from chllmodel_amd.observables import p2_from_q, summarize
 from chllmodel_amd.states import isotropic_state
 from oracle.oracle import ARITH_COMPAT, ARITH_FP64, BC_OPEN, BC_PERIODIC, RNG_PHILOX, Oracle, copy_state
 
@@ -331,3 +338,32 @@ def test_l128_anneal_20000_sweeps():
     assert np.abs(norm - 1.0).max() < 1e-12
     e_kernel = _kernel_energy_per_site(st, K, g)
     assert abs(e[-1] - e_kernel) <= 1e-12 * max(1.0, abs(e_kernel)), (e[-1], e_kernel)
+
+
+@pytest.mark.timeout(900)
+def test_phase_scan_10000_sweeps():
+    reps = phase_scan_replicas()
+    n = len(reps)
+    print("phase scan, 10k sweeps: 1000 blocks of 10", flush=True)
+    with Ensemble(reps, bc="open", arith="fp64") as ens:
+        ens.init_state("isotropic", seed=10000, p_flip=0.5)
+        rows = ens.run(1000, 10, seed=10000, run_index=2, adapt=True, use_graph=True)
+        st = ens.state()
+        assert st["istep"] == 10001
+        assert rows.shape == (1000, n, rows.shape[2]) and np.all(np.isfinite(rows))
+        ser = series_by_temperature(rows, reps)
+        for sl in range(n):
+            N = reps[sl]["ni"] * reps[sl]["nj"] * reps[sl]["nk"]
+            sm = summarize(ser[sl], reps[sl]["kbt"], N)
+            assert all(np.isfinite(v) for v in sm.values()), (sl, sm)
+            # chi_c is 0 where a strong field has saturated every spin
+            assert sm["Cv"] > 0 and sm["chi"] >= 0, (sl, sm)
+        print("phase scan, 10k sweeps: final states against the energy kernel", flush=True)
+        for r in range(n):
+            got = ens.download(r)
+            N = reps[r]["ni"] * reps[r]["nj"] * reps[r]["nk"]
+            assert np.all(np.isin(got["s"], (-1, 1))), r
+            norm = np.sqrt(got["nx"] ** 2 + got["ny"] ** 2 + got["nz"] ** 2)
+            assert np.abs(norm - 1.0).max() < 1e-12, r
+            e_kernel = _kernel_energy_per_site(got, reps[r]["KK"], reps[r]["g"])
+            assert abs(rows[-1, r, 2] - e_kernel) <= 1e-12 * max(1.0, abs(e_kernel)), (r, rows[-1, r, 2], e_kernel)
