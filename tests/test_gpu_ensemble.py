@@ -170,8 +170,9 @@ def test_pair_kernels_equal_one_site(K, layout):
 
 def test_pair_measure_rows_match_oracle():
     """Replicas whose half (ni / 2) is even all measure with four sites of one
-    row per thread (k_measure_ens_pair): x wraps at every pair (ni = 4), odd
-    nj and nk, rows longer than a wave (ni = 260).  After a block of sweeps,
+    row per thread, marching slabs of 8 planes (k_measure_ens_march): x wraps
+    at every pair (ni = 4), odd nj and nk, a partial slab (nk = 5, 10), rows
+    longer than a wave (ni = 260).  After a block of sweeps,
     every replica's row against the oracle's observables of its final state:
     E_true / N within 1e-12, eta exact, the Q means within 1e-14, the
     reference's binary32 E within one binary32 ulp."""
@@ -179,7 +180,8 @@ def test_pair_measure_rows_match_oracle():
     reps = [{"ni": 4, "nj": 3, "nk": 5, "ladder": 0, "g": 0.5, "KK": 1.0, "kbt": 0.7, "d0": 0.3},
             {"ni": 8, "nj": 6, "nk": 4, "ladder": 1, "g": 0.5, "KK": 1.0, "kbt": 0.8, "d0": 0.3},
             {"ni": 12, "nj": 10, "nk": 8, "ladder": 2, "g": 1.0, "KK": 0.7, "kbt": 0.9, "d0": 0.2},
-            {"ni": 260, "nj": 3, "nk": 2, "ladder": 3, "g": 0.25, "KK": 1.5, "kbt": 0.6, "d0": 0.25}]
+            {"ni": 260, "nj": 3, "nk": 2, "ladder": 3, "g": 0.25, "KK": 1.5, "kbt": 0.6, "d0": 0.25},
+            {"ni": 16, "nj": 7, "nk": 10, "ladder": 4, "g": 0.5, "KK": 1.0, "kbt": 0.75, "d0": 0.3}]
     with Ensemble(reps, bc="open", arith="fp64") as ens:
         for r, rep in enumerate(reps):
             ens.upload(r, isotropic_state(rep["ni"], rep["nj"], rep["nk"], 900 + r))
