@@ -604,6 +604,7 @@ def batched_scan(args, stream):
     ens.set_stream(stream.cuda_stream)
     ens.set_block(tune["best"])
     ens.set_chunk(tune["chunk"])
+    ens.set_branches(tune["branches"])
     ens.init_state("isotropic", seed=20261203)
     ens.run(2, Kb, seed=20261203, want_rows=False, use_graph=True)
     ens.sync()
@@ -642,7 +643,7 @@ def batched_scan(args, stream):
             "site_updates_per_s": sites * nsw / t_b, "ms_per_sweep": 1e3 * t_b / nsw,
             "sequential_site_updates_per_s": sites * nsw / t_seq, "sequential_ms_per_sweep": 1e3 * t_seq / nsw,
             "speedup_vs_sequential": t_seq / t_b, "block_size": tune["best"],
-            "chunk_bytes": tune["chunk"], "chunks": nchunks,
+            "chunk_bytes": tune["chunk"], "chunks": nchunks, "branches": tune["branches"],
             "autotune_ms_per_sweep": tune["ms_per_sweep"], "autotune_ms_per_sweep_chunk": tune["ms_per_sweep_chunk"],
             "swap_attempts": int(att.sum()), "swap_acceptance": float(acc.sum() / max(att.sum(), 1)),
             "swap_note": "rungs a factor 1.21 apart in kT on lattices of 32^3-128^3 sites: beta-gaps times "
@@ -674,6 +675,7 @@ def batched_observables(args, stream, reps, Kb, tune):
     ens.set_stream(stream.cuda_stream)
     ens.set_block(tune["best"])
     ens.set_chunk(tune["chunk"])
+    ens.set_branches(tune["branches"])
     ens.init_state("isotropic", seed=20261203)
     t0 = time.perf_counter()
     rows = ens.run(nb, Kb, seed=20261203, run_index=1, want_rows=True, use_graph=True)

@@ -439,6 +439,13 @@ struct chll_ensemble {
     // (0: one chunk), which then stays in the memory-side cache.
     size_t chunk_bytes = 0;
     std::vector<Chunk> chunks;
+    // Chunks are independent, so with branches = 2 they alternate between two
+    // streams forked from and joined back to the run's stream (two branches of
+    // the captured graph): one chunk's short launches (epilogues, tails) run
+    // beside the other's sweeps.
+    int branches = 1;
+    hipStream_t br[2] = {nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
     EnsDev E{};
     double *stage = nullptr;
     int32_t *stage_s = nullptr;
@@ -608,20 +615,31 @@ int build_tables(chll_ensemble *X)
 // observable reduction and the closing epilogue.  The swap kernel then moves
 // replicas between temperature slots, so no fused launch crosses it.
 template <int ARITH, int BC, int BS>
-void launch_block_t(chll_ensemble *X, int K, hipStream_t st)
+int launch_block_t(chll_ensemble *X, int K, hipStream_t st)
 {
     const EnsDev &E = X->E;
+    const int nbr = X->branches > 1 && X->chunks.size() > 1 ? 2 : 1;
+    hipStream_t cst[2] = {st, st};
+    if (nbr == 2) {
+        ECHECK(hipEventRecord(X->ev_fork, st));
+        for (int q = 0; q < 2; ++q) {
+            ECHECK(hipStreamWaitEvent(X->br[q], X->ev_fork, 0));
+            cst[q] = X->br[q];
+        }
+    }
     const dim3 b(BS), bf(kEnsFin);
     const bool fuse = ARITH == ARITH_FP64 && X->fuse != 0;
     const bool pairs = fuse && X->fuse == 1 && X->pairs;
     // observables: four sites per thread wherever every replica allows site
     // pairs, whatever the fusion setting, so rows do not depend on it
-    auto measure = [&](const Chunk &c) {
-        if (X->march) hipLaunchKernelGGL((k_measure_ens_march<BC, BS>), dim3(c.n3), dim3(BS), 0, st, E, c.t3);
-        else if (X->pairs) hipLaunchKernelGGL((k_measure_ens_pair<BC, BS>), dim3(c.n1), dim3(BS), 0, st, E, c.t1);
-        else hipLaunchKernelGGL((k_measure_ens<BC, BS>), dim3(c.n1), dim3(BS), 0, st, E, c.t1);
+    auto measure = [&](const Chunk &c, hipStream_t ms) {
+        if (X->march) hipLaunchKernelGGL((k_measure_ens_march<BC, BS>), dim3(c.n3), dim3(BS), 0, ms, E, c.t3);
+        else if (X->pairs) hipLaunchKernelGGL((k_measure_ens_pair<BC, BS>), dim3(c.n1), dim3(BS), 0, ms, E, c.t1);
+        else hipLaunchKernelGGL((k_measure_ens<BC, BS>), dim3(c.n1), dim3(BS), 0, ms, E, c.t1);
     };
-    for (const Chunk &c : X->chunks) {
+    for (size_t ci = 0; ci < X->chunks.size(); ++ci) {
+        const Chunk &c = X->chunks[ci];
+        const hipStream_t cs = cst[ci % nbr];  // this chunk's branch
         const dim3 g(c.n1), g2(c.n2), gf(c.nf);
         const uint32_t r1 = c.t1 + c.n1 - 1u, r2 = c.t2 + c.n2 - 1u;  // last entries, for reversed walks
         for (int t = 0; t < K; ++t) {
@@ -629,46 +647,52 @@ void launch_block_t(chll_ensemble *X, int K, hipStream_t st)
             const bool last = t + 1 == K;
             // consecutive sweep launches alternate their block order
             if (!fuse) {
-                hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 0, c.t1);
-                hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 1, r1);
-                hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 0, c.t1);
-                hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 1, r1);
-                if (last) measure(c);
-                hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, last ? 1 : 0, 0, 1, c.f);
+                hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, cs, E, 0, off, 0, c.t1);
+                hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, cs, E, 1, off, 1, r1);
+                hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, cs, E, 0, off, 0, c.t1);
+                hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, cs, E, 1, off, 1, r1);
+                if (last) measure(c, cs);
+                hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, cs, E, off, last ? 1 : 0, 0, 1, c.f);
                 continue;
             }
             if (t == 0) {
-                if (pairs) hipLaunchKernelGGL((k_pair_ens<0, BC, BS>), g2, b, 0, st, E, off, 1, r2);
-                else hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, r1);
+                if (pairs) hipLaunchKernelGGL((k_pair_ens<0, BC, BS>), g2, b, 0, cs, E, off, 1, r2);
+                else hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, cs, E, 0, off, 1, r1);
             }
-            if (pairs) hipLaunchKernelGGL((k_pair_ens<2, BC, BS>), g2, b, 0, st, E, off, 0, c.t2);
-            else hipLaunchKernelGGL((k_phase_ens<2, ARITH, BC, BS>), g, b, 0, st, E, 1, off, 0, c.t1);
+            if (pairs) hipLaunchKernelGGL((k_pair_ens<2, BC, BS>), g2, b, 0, cs, E, off, 0, c.t2);
+            else hipLaunchKernelGGL((k_phase_ens<2, ARITH, BC, BS>), g, b, 0, cs, E, 1, off, 0, c.t1);
             if (!last) {
-                hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 0, t > 0 ? 1 : 0, 0, c.f);
-                if (pairs) hipLaunchKernelGGL((k_pair_ens<3, BC, BS>), g2, b, 0, st, E, off, 1, r2);
-                else hipLaunchKernelGGL((k_phase_ens<3, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, r1);
+                hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, cs, E, off, 0, t > 0 ? 1 : 0, 0, c.f);
+                if (pairs) hipLaunchKernelGGL((k_pair_ens<3, BC, BS>), g2, b, 0, cs, E, off, 1, r2);
+                else hipLaunchKernelGGL((k_phase_ens<3, ARITH, BC, BS>), g, b, 0, cs, E, 0, off, 1, r1);
             } else {
-                if (pairs) hipLaunchKernelGGL((k_pair_ens<1, BC, BS>), g2, b, 0, st, E, off, 1, r2);
-                else hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, st, E, 0, off, 1, r1);
-                measure(c);
-                hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, st, E, off, 1, t > 0 ? 1 : 0, 1, c.f);
+                if (pairs) hipLaunchKernelGGL((k_pair_ens<1, BC, BS>), g2, b, 0, cs, E, off, 1, r2);
+                else hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, cs, E, 0, off, 1, r1);
+                measure(c, cs);
+                hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, cs, E, off, 1, t > 0 ? 1 : 0, 1, c.f);
             }
         }
     }
+    if (nbr == 2) {
+        for (int q = 0; q < 2; ++q) {
+            ECHECK(hipEventRecord(X->ev_join[q], X->br[q]));
+            ECHECK(hipStreamWaitEvent(st, X->ev_join[q], 0));
+        }
+    }
     hipLaunchKernelGGL(k_swap_ens, dim3(1), dim3(256), 0, st, E, 1);
+    return 0;
 }
 
 template <int ARITH, int BC>
 int launch_block_bs(chll_ensemble *X, int K, hipStream_t st)
 {
     switch (X->bs) {
-    case 64: launch_block_t<ARITH, BC, 64>(X, K, st); break;
-    case 128: launch_block_t<ARITH, BC, 128>(X, K, st); break;
-    case 256: launch_block_t<ARITH, BC, 256>(X, K, st); break;
-    case 512: launch_block_t<ARITH, BC, 512>(X, K, st); break;
+    case 64: return launch_block_t<ARITH, BC, 64>(X, K, st);
+    case 128: return launch_block_t<ARITH, BC, 128>(X, K, st);
+    case 256: return launch_block_t<ARITH, BC, 256>(X, K, st);
+    case 512: return launch_block_t<ARITH, BC, 512>(X, K, st);
     default: return efail("unsupported block size %d", X->bs);
     }
-    return 0;
 }
 
 // Enqueue one block of K sweeps (+ measurement + swap).
@@ -827,6 +851,11 @@ int chll_ensemble_destroy(chll_ensemble *X)
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (X->pinned) (void)hipHostFree(X->pinned);
+    for (int q = 0; q < 2; ++q) {
+        if (X->br[q]) (void)hipStreamDestroy(X->br[q]);
+        if (X->ev_join[q]) (void)hipEventDestroy(X->ev_join[q]);
+    }
+    if (X->ev_fork) (void)hipEventDestroy(X->ev_fork);
     if (X->own) (void)hipStreamDestroy(X->own);
     delete X;
     return 0;
@@ -860,6 +889,21 @@ int chll_ensemble_set_chunk(chll_ensemble *X, int64_t bytes)
 }
 
 int32_t chll_ensemble_chunks(chll_ensemble *X) { return X ? (int32_t)X->chunks.size() : -1; }
+
+int chll_ensemble_set_branches(chll_ensemble *X, int32_t n)
+{
+    if (!X) return efail("null ensemble");
+    if (n != 1 && n != 2) return efail("branches must be 1 or 2");
+    ECHECK(hipStreamSynchronize(X->stream));
+    if (n == 2 && !X->br[0]) {
+        for (int q = 0; q < 2; ++q) ECHECK(hipStreamCreateWithFlags(&X->br[q], hipStreamNonBlocking));
+        ECHECK(hipEventCreateWithFlags(&X->ev_fork, hipEventDisableTiming));
+        for (int q = 0; q < 2; ++q) ECHECK(hipEventCreateWithFlags(&X->ev_join[q], hipEventDisableTiming));
+    }
+    X->branches = n;
+    drop_graph(X);
+    return 0;
+}
 
 int chll_ensemble_set_block(chll_ensemble *X, int32_t bs)
 {

@@ -48,6 +48,7 @@ def _declare(L):
     L.chll_ensemble_set_chunk.argtypes = [ct.c_void_p, ct.c_int64]
     L.chll_ensemble_chunks.argtypes = [ct.c_void_p]
     L.chll_ensemble_chunks.restype = ct.c_int32
+    L.chll_ensemble_set_branches.argtypes = [ct.c_void_p, ct.c_int32]
     L.chll_ensemble_blocks.argtypes = [ct.c_void_p]
     L.chll_ensemble_upload.argtypes = [ct.c_void_p, ct.c_int32, _f64p, _f64p, _f64p, _i32p]
     L.chll_ensemble_download.argtypes = [ct.c_void_p, ct.c_int32, _f64p, _f64p, _f64p, _i32p]
@@ -57,7 +58,8 @@ def _declare(L):
     L.chll_ensemble_init_state.argtypes = [ct.c_void_p, ct.c_int32, ct.c_uint64, ct.c_double]
     L.chll_ensemble_state.argtypes = [ct.c_void_p, _i32p, _f64p, _f64p, _i64p, _i64p, ct.POINTER(ct.c_uint32)]
     for n in ("chll_ensemble_create", "chll_ensemble_destroy", "chll_ensemble_set_stream", "chll_ensemble_set_block",
-              "chll_ensemble_set_fusion", "chll_ensemble_set_chunk", "chll_ensemble_blocks", "chll_ensemble_upload", "chll_ensemble_download", "chll_ensemble_run",
+              "chll_ensemble_set_fusion", "chll_ensemble_set_chunk", "chll_ensemble_set_branches",
+              "chll_ensemble_blocks", "chll_ensemble_upload", "chll_ensemble_download", "chll_ensemble_run",
               "chll_ensemble_sync", "chll_ensemble_state", "chll_ensemble_init_state"):
         getattr(L, n).restype = ct.c_int
     L._ens_declared = True
@@ -134,6 +136,11 @@ class Ensemble:
 
     def chunks(self) -> int:
         return int(lib().chll_ensemble_chunks(self._h))
+
+    def set_branches(self, n: int):
+        """Run the chunks on 1 or 2 concurrent streams (graph branches);
+        results are unchanged."""
+        _check(lib().chll_ensemble_set_branches(self._h, int(n)))
 
     def set_stream(self, ptr):
         _check(lib().chll_ensemble_set_stream(self._h, ct.c_void_p(ptr or 0)))
@@ -228,13 +235,16 @@ def _signature(replicas, bc, arith, K):
 
 
 def autotune(replicas, *, bc="open", arith="compat", K=10, blocks=3, candidates=(64, 128, 256, 512),
-             chunks=(0, 128 << 20, 192 << 20, 256 << 20, 320 << 20), seed=12345, use_cache=True,
+             chunks=(0, 96 << 20, 128 << 20, 160 << 20, 192 << 20, 256 << 20, 320 << 20), seed=12345, use_cache=True,
              verbose=False) -> dict:
     """Time each candidate block size, then each chunk budget (bytes of state
-    per chunk, 0 = all replicas at once) with the best block size, on a scratch
-    ensemble of the same shape (so the caller's ensemble is not advanced).
-    Returns {'best': bs, 'chunk': bytes, 'ms_per_sweep': {...}}; cached per
-    shape in CACHE.  Apply with set_block(res['best']), set_chunk(res['chunk'])."""
+    per chunk, 0 = all replicas at once) with the best block size, each on one
+    stream and, where it makes two chunks or more, on two (set_branches), on a
+    scratch ensemble of the same shape (so the caller's ensemble is not
+    advanced).  Returns {'best': bs, 'chunk': bytes, 'branches': 1 or 2,
+    'ms_per_sweep': {...}, 'ms_per_sweep_chunk': {'<bytes>' or '<bytes>x2':
+    ms}}; cached per shape in CACHE.  Apply with set_block(res['best']),
+    set_chunk(res['chunk']), set_branches(res['branches'])."""
     key = _signature(replicas, bc, arith, K)
     cache = {}
     if use_cache and os.path.exists(CACHE):
@@ -243,7 +253,7 @@ def autotune(replicas, *, bc="open", arith="compat", K=10, blocks=3, candidates=
                 cache = json.load(f)
         except (OSError, ValueError):
             cache = {}
-        if key in cache and "chunk" in cache[key]:
+        if key in cache and "branches" in cache[key]:
             return cache[key]
     times, ctimes = {}, {}
     with Ensemble(replicas, bc=bc, arith=arith) as ens:
@@ -266,13 +276,19 @@ def autotune(replicas, *, bc="open", arith="compat", K=10, blocks=3, candidates=
         ens.set_block(best)
         for cb in chunks:
             ens.set_chunk(cb)
-            ctimes[cb] = time_blocks()
-            if verbose:
-                print(f"autotune bs={best} chunk={cb >> 20} MiB ({ens.chunks()} chunks): {ctimes[cb]:.3f} ms/sweep",
-                      flush=True)
-    chunk = min(ctimes, key=ctimes.get)
-    res = {"best": int(best), "chunk": int(chunk), "ms_per_sweep": {str(k): v for k, v in times.items()},
-           "ms_per_sweep_chunk": {str(k): v for k, v in ctimes.items()}}
+            for nb in (1, 2):
+                if nb == 2 and ens.chunks() < 2:
+                    continue
+                ens.set_branches(nb)
+                ctimes[(cb, nb)] = time_blocks()
+                if verbose:
+                    print(f"autotune bs={best} chunk={cb >> 20} MiB ({ens.chunks()} chunks, {nb} branches): "
+                          f"{ctimes[(cb, nb)]:.3f} ms/sweep", flush=True)
+            ens.set_branches(1)
+    chunk, branches = min(ctimes, key=ctimes.get)
+    res = {"best": int(best), "chunk": int(chunk), "branches": int(branches),
+           "ms_per_sweep": {str(k): v for k, v in times.items()},
+           "ms_per_sweep_chunk": {(str(c) if b == 1 else f"{c}x2"): v for (c, b), v in ctimes.items()}}
     if use_cache:
         cache[key] = res
         try:

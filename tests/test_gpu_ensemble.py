@@ -203,22 +203,26 @@ def test_pair_measure_rows_match_oracle():
 @pytest.mark.parametrize("arith,fuse", [("fp64", True), ("fp64", False), ("compat", False)])
 def test_chunks_equal_one_launch(arith, fuse):
     """A block of sweeps run chunk by chunk (a few replicas at a time, all K
-    sweeps each) gives the same replicas, rows, slots and swap counts as every
-    replica in one launch per half-sweep."""
+    sweeps each), on one stream or with the chunks alternating between two
+    forked streams, eager or replayed from a graph, gives the same replicas,
+    rows, slots and swap counts as every replica in one launch per
+    half-sweep."""
     ladders = [((16, 12, 10), 0.5, 1.0, (0.3, 0.6, 1.0), 0.3), ((8, 8, 8), 1.0, 0.7, (0.4, 0.7), 0.2),
                ((12, 4, 6), 0.0, 1.5, (0.5, 0.8, 1.2), 0.25)]
     reps = [{"ni": d[0], "nj": d[1], "nk": d[2], "ladder": lad, "g": g, "KK": KK, "kbt": kT, "d0": d0}
             for lad, (d, g, KK, kTs, d0) in enumerate(ladders) for kT in kTs]
     outs = []
-    for chunk in (0, 60000, 1):
-        print(f"ensemble chunks {arith} fuse={fuse} chunk={chunk}", flush=True)
+    for chunk, branches, graph in ((0, 1, True), (60000, 1, True), (1, 1, True), (60000, 2, True), (1, 2, True),
+                                   (1, 2, False)):
+        print(f"ensemble chunks {arith} fuse={fuse} chunk={chunk} branches={branches} graph={graph}", flush=True)
         with Ensemble(reps, arith=arith) as ens:
             ens.set_fusion(fuse)
             ens.set_chunk(chunk)
+            ens.set_branches(branches)
             n = ens.chunks()
             assert n == {0: 1, 1: len(reps)}.get(chunk, n) and (chunk != 60000 or 1 < n < len(reps))
             ens.init_isotropic(9)
-            rows = ens.run(4, 3, seed=23, use_graph=True)
+            rows = ens.run(4, 3, seed=23, use_graph=graph)
             st = ens.state()
             outs.append((rows, [ens.download(r) for r in range(len(reps))], st))
     ra, sa, ta = outs[0]
