@@ -616,12 +616,11 @@ int grad_launch(const double *n, const int8_t *s, int32_t batch, int32_t ni, int
     if (pairs && (uint64_t)tiles * nslab <= (N + kGradBlock - 1) / kGradBlock) {
         A.nblk = tiles * nslab;
         A.band = 0;
-        dim3 grid(A.nblk, batch);
+        const dim3 grid(A.nblk, batch), block(kMarchBlock);
         if (bc == BC_OPEN)
-            hipLaunchKernelGGL((k_energy_grad_march<BC_OPEN, kMarchBlock>), grid, dim3(kMarchBlock), 0, st, A, tiles, nslab, ks);
+            hipLaunchKernelGGL((k_energy_grad_march<BC_OPEN, kMarchBlock>), grid, block, 0, st, A, tiles, nslab, ks);
         else
-            hipLaunchKernelGGL((k_energy_grad_march<BC_PERIODIC, kMarchBlock>), grid, dim3(kMarchBlock), 0, st, A, tiles,
-                               nslab, ks);
+            hipLaunchKernelGGL((k_energy_grad_march<BC_PERIODIC, kMarchBlock>), grid, block, 0, st, A, tiles, nslab, ks);
         hipLaunchKernelGGL(k_energy_final, dim3(batch), dim3(256), 0, st, A, E, dEdK, dEdg, dEdeps);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return gfail("chll_energy_grad launch: %s", hipGetErrorString(e));

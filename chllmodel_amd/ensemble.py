@@ -235,8 +235,8 @@ def _signature(replicas, bc, arith, K):
 
 
 def autotune(replicas, *, bc="open", arith="compat", K=10, blocks=3, candidates=(64, 128, 256, 512),
-             chunks=(0, 96 << 20, 128 << 20, 160 << 20, 192 << 20, 256 << 20, 320 << 20), seed=12345, use_cache=True,
-             verbose=False) -> dict:
+             chunks=(0, 96 << 20, 128 << 20, 160 << 20, 192 << 20, 256 << 20, 320 << 20), seed=12345,
+             use_cache=True, verbose=False) -> dict:
     """Time each candidate block size, then each chunk budget (bytes of state
     per chunk, 0 = all replicas at once) with the best block size, each on one
     stream and, where it makes two chunks or more, on two (set_branches), on a
