@@ -439,13 +439,14 @@ struct chll_ensemble {
     // (0: one chunk), which then stays in the memory-side cache.
     size_t chunk_bytes = 0;
     std::vector<Chunk> chunks;
-    // Chunks are independent, so with branches = 2 they alternate between two
-    // streams forked from and joined back to the run's stream (two branches of
+    // Chunks are independent, so with branches = n > 1 they take turns on n
+    // streams forked from and joined back to the run's stream (n branches of
     // the captured graph): one chunk's short launches (epilogues, tails) run
-    // beside the other's sweeps.
+    // beside another's sweeps.
+    static constexpr int kMaxBranches = 3;
     int branches = 1;
-    hipStream_t br[2] = {nullptr, nullptr};
-    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+    hipStream_t br[kMaxBranches] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kMaxBranches] = {};
     EnsDev E{};
     double *stage = nullptr;
     int32_t *stage_s = nullptr;
@@ -618,11 +619,11 @@ template <int ARITH, int BC, int BS>
 int launch_block_t(chll_ensemble *X, int K, hipStream_t st)
 {
     const EnsDev &E = X->E;
-    const int nbr = X->branches > 1 && X->chunks.size() > 1 ? 2 : 1;
-    hipStream_t cst[2] = {st, st};
-    if (nbr == 2) {
+    const int nbr = std::min<int>(X->branches, (int)X->chunks.size());
+    hipStream_t cst[chll_ensemble::kMaxBranches] = {st};
+    if (nbr > 1) {
         ECHECK(hipEventRecord(X->ev_fork, st));
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < nbr; ++q) {
             ECHECK(hipStreamWaitEvent(X->br[q], X->ev_fork, 0));
             cst[q] = X->br[q];
         }
@@ -673,8 +674,8 @@ int launch_block_t(chll_ensemble *X, int K, hipStream_t st)
             }
         }
     }
-    if (nbr == 2) {
-        for (int q = 0; q < 2; ++q) {
+    if (nbr > 1) {
+        for (int q = 0; q < nbr; ++q) {
             ECHECK(hipEventRecord(X->ev_join[q], X->br[q]));
             ECHECK(hipStreamWaitEvent(st, X->ev_join[q], 0));
         }
@@ -851,7 +852,7 @@ int chll_ensemble_destroy(chll_ensemble *X)
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (X->pinned) (void)hipHostFree(X->pinned);
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < chll_ensemble::kMaxBranches; ++q) {
         if (X->br[q]) (void)hipStreamDestroy(X->br[q]);
         if (X->ev_join[q]) (void)hipEventDestroy(X->ev_join[q]);
     }
@@ -893,12 +894,12 @@ int32_t chll_ensemble_chunks(chll_ensemble *X) { return X ? (int32_t)X->chunks.s
 int chll_ensemble_set_branches(chll_ensemble *X, int32_t n)
 {
     if (!X) return efail("null ensemble");
-    if (n != 1 && n != 2) return efail("branches must be 1 or 2");
+    if (n < 1 || n > chll_ensemble::kMaxBranches) return efail("branches must be 1 to %d", chll_ensemble::kMaxBranches);
     ECHECK(hipStreamSynchronize(X->stream));
-    if (n == 2 && !X->br[0]) {
-        for (int q = 0; q < 2; ++q) ECHECK(hipStreamCreateWithFlags(&X->br[q], hipStreamNonBlocking));
-        ECHECK(hipEventCreateWithFlags(&X->ev_fork, hipEventDisableTiming));
-        for (int q = 0; q < 2; ++q) ECHECK(hipEventCreateWithFlags(&X->ev_join[q], hipEventDisableTiming));
+    if (n > 1 && !X->ev_fork) ECHECK(hipEventCreateWithFlags(&X->ev_fork, hipEventDisableTiming));
+    for (int q = 0; q < n && n > 1; ++q) {
+        if (!X->br[q]) ECHECK(hipStreamCreateWithFlags(&X->br[q], hipStreamNonBlocking));
+        if (!X->ev_join[q]) ECHECK(hipEventCreateWithFlags(&X->ev_join[q], hipEventDisableTiming));
     }
     X->branches = n;
     drop_graph(X);

@@ -138,7 +138,7 @@ class Ensemble:
         return int(lib().chll_ensemble_chunks(self._h))
 
     def set_branches(self, n: int):
-        """Run the chunks on 1 or 2 concurrent streams (graph branches);
+        """Run the chunks on 1 to 3 concurrent streams (graph branches);
         results are unchanged."""
         _check(lib().chll_ensemble_set_branches(self._h, int(n)))
 
@@ -239,11 +239,11 @@ def autotune(replicas, *, bc="open", arith="compat", K=10, blocks=3, candidates=
              use_cache=True, verbose=False) -> dict:
     """Time each candidate block size, then each chunk budget (bytes of state
     per chunk, 0 = all replicas at once) with the best block size, each on one
-    stream and, where it makes two chunks or more, on two (set_branches), on a
+    stream and, where it makes enough chunks, on two and three (set_branches), on a
     scratch ensemble of the same shape (so the caller's ensemble is not
-    advanced).  Returns {'best': bs, 'chunk': bytes, 'branches': 1 or 2,
-    'ms_per_sweep': {...}, 'ms_per_sweep_chunk': {'<bytes>' or '<bytes>x2':
-    ms}}; cached per shape in CACHE.  Apply with set_block(res['best']),
+    advanced).  Returns {'best': bs, 'chunk': bytes, 'branches': 1 to 3,
+    'ms_per_sweep': {...}, 'ms_per_sweep_chunk': {'<bytes>' or
+    '<bytes>x<branches>': ms}}; cached per shape in CACHE.  Apply with set_block(res['best']),
     set_chunk(res['chunk']), set_branches(res['branches'])."""
     key = _signature(replicas, bc, arith, K)
     cache = {}
@@ -276,8 +276,8 @@ def autotune(replicas, *, bc="open", arith="compat", K=10, blocks=3, candidates=
         ens.set_block(best)
         for cb in chunks:
             ens.set_chunk(cb)
-            for nb in (1, 2):
-                if nb == 2 and ens.chunks() < 2:
+            for nb in (1, 2, 3):
+                if nb > ens.chunks():
                     continue
                 ens.set_branches(nb)
                 ctimes[(cb, nb)] = time_blocks()
@@ -288,7 +288,7 @@ def autotune(replicas, *, bc="open", arith="compat", K=10, blocks=3, candidates=
     chunk, branches = min(ctimes, key=ctimes.get)
     res = {"best": int(best), "chunk": int(chunk), "branches": int(branches),
            "ms_per_sweep": {str(k): v for k, v in times.items()},
-           "ms_per_sweep_chunk": {(str(c) if b == 1 else f"{c}x2"): v for (c, b), v in ctimes.items()}}
+           "ms_per_sweep_chunk": {(str(c) if b == 1 else f"{c}x{b}"): v for (c, b), v in ctimes.items()}}
     if use_cache:
         cache[key] = res
         try:

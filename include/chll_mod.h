@@ -211,8 +211,8 @@ int chll_ensemble_set_fusion(chll_ensemble *e, int32_t on);
  * chunk.  Results are unchanged.  chll_ensemble_chunks: the number of chunks. */
 int chll_ensemble_set_chunk(chll_ensemble *e, int64_t bytes);
 int32_t chll_ensemble_chunks(chll_ensemble *e);
-/* With n = 2 the chunks alternate between two streams forked from and joined
- * back to the ensemble's stream (two branches of the captured graph), so one
+/* With n = 2 or 3 the chunks take turns on n streams forked from and joined
+ * back to the ensemble's stream (n branches of the captured graph), so one
  * chunk's short launches run beside another's sweeps; 1 (default): one after
  * another.  Results are unchanged.  No counterpart in the reference (its scans
  * run one ChLLSim after another, e.g. scripts/phase_diagram_large1.py). */

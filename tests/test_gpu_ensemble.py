@@ -203,8 +203,8 @@ def test_pair_measure_rows_match_oracle():
 @pytest.mark.parametrize("arith,fuse", [("fp64", True), ("fp64", False), ("compat", False)])
 def test_chunks_equal_one_launch(arith, fuse):
     """A block of sweeps run chunk by chunk (a few replicas at a time, all K
-    sweeps each), on one stream or with the chunks alternating between two
-    forked streams, eager or replayed from a graph, gives the same replicas,
+    sweeps each), on one stream or with the chunks taking turns on two or
+    three forked streams, eager or replayed from a graph, gives the same replicas,
     rows, slots and swap counts as every replica in one launch per
     half-sweep."""
     ladders = [((16, 12, 10), 0.5, 1.0, (0.3, 0.6, 1.0), 0.3), ((8, 8, 8), 1.0, 0.7, (0.4, 0.7), 0.2),
@@ -213,7 +213,7 @@ def test_chunks_equal_one_launch(arith, fuse):
             for lad, (d, g, KK, kTs, d0) in enumerate(ladders) for kT in kTs]
     outs = []
     for chunk, branches, graph in ((0, 1, True), (60000, 1, True), (1, 1, True), (60000, 2, True), (1, 2, True),
-                                   (1, 2, False)):
+                                   (1, 2, False), (1, 3, True)):
         print(f"ensemble chunks {arith} fuse={fuse} chunk={chunk} branches={branches} graph={graph}", flush=True)
         with Ensemble(reps, arith=arith) as ens:
             ens.set_fusion(fuse)
