@@ -27,7 +27,8 @@ called from two host threads at once.
   unit directors and +-1 spins at the end, and the last row's Hamiltonian
   equal to the independent energy kernel's on the final state to 1e-12.
 * The 64-replica scan at its full length, 1000 blocks of 10 sweeps (10k
-  sweeps of every replica, swaps every 10, one graph replay per block): every
+  sweeps of every replica, swaps every 10, one graph replay per block, chunks
+  of 128 MiB on two graph branches as the bench runs it): every
   row finite, every temperature slot's C_v (jackknife) finite and positive
   and chi_c finite and non-negative (0 where a strong field saturates the
   spins), the sweep counter at 10001, and every replica's last row against
@@ -346,6 +347,11 @@ def test_phase_scan_10000_sweeps():
     n = len(reps)
     print("phase scan, 10k sweeps: 1000 blocks of 10", flush=True)
     with Ensemble(reps, bc="open", arith="fp64") as ens:
+        # the configuration the bench's autotuner picks on MI355X: chunks of
+        # 128 MiB taking turns on two graph branches
+        ens.set_chunk(128 << 20)
+        ens.set_branches(2)
+        assert ens.chunks() > 2
         ens.init_state("isotropic", seed=10000, p_flip=0.5)
         rows = ens.run(1000, 10, seed=10000, run_index=2, adapt=True, use_graph=True)
         st = ens.state()
