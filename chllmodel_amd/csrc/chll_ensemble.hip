@@ -203,9 +203,10 @@ __global__ __launch_bounds__(BS) void k_measure_ens_pair(EnsDev E, uint32_t base
 }
 
 // The same observables marching along k (measure_march_column, as the single
-// lattice's k_measure_march): block lb of replica r takes tile lb % tiles of
-// BS packed pairs of a plane through slab lb / tiles of 8 planes; its partial
-// is the replica's lb-th.  Used when every replica's march blocks fit its
+// lattice's k_measure_march): block lb of replica r takes tile lb / nslab of
+// BS packed pairs of a plane through slab lb % nslab of 8 planes, so the
+// blocks in flight cover all slabs of a few tiles; its partial is the
+// replica's lb-th.  Used when every replica's march blocks fit its
 // partials (nmeas <= nblk).
 template <int BC, int BS>
 __global__ __launch_bounds__(BS) void k_measure_ens_march(EnsDev E, uint32_t base)
@@ -216,7 +217,8 @@ __global__ __launch_bounds__(BS) void k_measure_ens_march(EnsDev E, uint32_t bas
     const uint32_t quads = D.G.plane >> 1;
     const uint32_t tiles = (quads + BS - 1) / BS;
     const uint32_t ks = D.G.nk < 8u ? D.G.nk : 8u;
-    const uint32_t sl = lb / tiles, t = lb - sl * tiles;
+    const uint32_t nslab = (D.G.nk + ks - 1) / ks;
+    const uint32_t t = lb / nslab, sl = lb - t * nslab;  // tile by tile through the slabs
     const uint32_t qd = t * BS + threadIdx.x;
     const uint32_t k0 = sl * ks, k1 = k0 + ks < D.G.nk ? k0 + ks : D.G.nk;
     MeasureAcc acc;

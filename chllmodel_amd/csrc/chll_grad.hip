@@ -331,8 +331,9 @@ __device__ inline int gp_hi(uint32_t v) { return (int)(int8_t)(v >> 8); }
 // one barrier per step), and only the +-y pairs outside the tile (the rows
 // beyond its first and last) are loaded, from L2.  Slabs alternate direction,
 // so the planes that adjacent slabs both read are read by both at about the
-// same time, and the slabs of one XCD are contiguous (workgroup x runs on XCD
-// x % 8 when the grid is a multiple of 8).  The gradient goes out with
+// same time: the slabs of one XCD are contiguous (workgroup x runs on XCD
+// x % 8 when the grid is a multiple of 8) and its blocks run tile by tile
+// through all of them.  The gradient goes out with
 // streaming stores: it is not read again, so it does not take L2 lines from
 // the directors.  Every site's bonds are taken in the same order as in
 // k_energy_grad2, so the gradient is the same bit for bit; the partial sums
@@ -343,12 +344,12 @@ __global__ __launch_bounds__(BS) void k_energy_grad_march(GradArgs A, uint32_t t
     const uint32_t b = blockIdx.y, x = blockIdx.x;
     uint32_t sl, t;
     if ((nslab & 7u) == 0u) {
-        const uint32_t n = x >> 3;
-        sl = (x & 7u) * (nslab >> 3) + n / tiles;
-        t = n % tiles;
+        const uint32_t n = x >> 3, spx = nslab >> 3;
+        t = n / spx;
+        sl = (x & 7u) * spx + n % spx;
     } else {
-        sl = x / tiles;
-        t = x % tiles;
+        t = x / nslab;
+        sl = x % nslab;
     }
     const double *nx = A.n + (size_t)b * 3 * A.N, *ny = nx + A.N, *nz = ny + A.N;
     const int8_t *s = A.s + (size_t)b * A.N;

@@ -193,8 +193,9 @@ __global__ __launch_bounds__(kBlock) void k_measure(Geom G, Fields F, double g, 
 // which the lanes of row j + 1 read it as their own row (down) or +z row
 // (up), so it comes from L2.  Slabs alternate direction, so the plane two
 // adjacent slabs share is read by both at the same time (both start or both
-// end there), and the slabs of one XCD are contiguous (workgroup b runs on
-// XCD b % 8).  One partial per block, in block order.  Needs an even half and
+// end there): the slabs of one XCD are contiguous (workgroup b runs on XCD
+// b % 8) and its blocks run tile by tile through all of them.  One partial
+// per block, in block order.  Needs an even half and
 // fields below 4 GiB.
 template <int BC>
 __global__ __launch_bounds__(kBlock) void k_measure_march(Geom G, Fields F, double g, double KK,
@@ -204,12 +205,15 @@ __global__ __launch_bounds__(kBlock) void k_measure_march(Geom G, Fields F, doub
     const uint32_t b = blockIdx.x;
     uint32_t sl, t;
     if ((nslab & 7u) == 0u) {
-        const uint32_t n = b >> 3;
-        sl = (b & 7u) * (nslab >> 3) + n / tiles;
-        t = n % tiles;
+        // XCD b % 8 takes a contiguous range of slabs, tile by tile: the blocks
+        // in flight cover all its slabs of a few tiles, so the two slabs that
+        // share a plane read it at about the same time
+        const uint32_t n = b >> 3, spx = nslab >> 3;
+        t = n / spx;
+        sl = (b & 7u) * spx + n % spx;
     } else {
-        sl = b / tiles;
-        t = b % tiles;
+        t = b / nslab;
+        sl = b % nslab;
     }
     const uint32_t quads = G.plane >> 1;  // packed pairs per plane and colour
     const uint32_t qd = t * kBlock + threadIdx.x;
