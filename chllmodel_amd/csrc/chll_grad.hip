@@ -339,11 +339,24 @@ __device__ inline int gp_hi(uint32_t v) { return (int)(int8_t)(v >> 8); }
 // k_energy_grad2, so the gradient is the same bit for bit; the partial sums
 // are per block (tiles x nslab per lattice).
 template <int BC, int BS>
-__global__ __launch_bounds__(BS) void k_energy_grad_march(GradArgs A, uint32_t tiles, uint32_t nslab, uint32_t ks)
+__global__ __launch_bounds__(BS) void k_energy_grad_march(GradArgs A, uint32_t tiles, uint32_t nslab, uint32_t ks,
+                                                          int whole)
 {
-    const uint32_t b = blockIdx.y, x = blockIdx.x;
+    // whole (a 1-D grid, batch a multiple of 8): XCD q = blockIdx.x % 8 takes
+    // lattices q, q + 8, ... whole, so no plane of a lattice is read by two
+    // XCDs; else (grid (tiles nslab, batch)) every XCD takes a contiguous
+    // range of slabs of every lattice
+    uint32_t b, x;
+    if (whole) {
+        const uint32_t q = blockIdx.x & 7u, n = blockIdx.x >> 3, per = tiles * nslab;
+        b = q + 8u * (n / per);
+        x = n % per;
+    } else {
+        b = blockIdx.y;
+        x = blockIdx.x;
+    }
     uint32_t sl, t;
-    if ((nslab & 7u) == 0u) {
+    if (!whole && (nslab & 7u) == 0u) {
         const uint32_t n = x >> 3, spx = nslab >> 3;
         t = n / spx;
         sl = (x & 7u) * spx + n % spx;
@@ -500,7 +513,7 @@ __global__ __launch_bounds__(BS) void k_energy_grad_march(GradArgs A, uint32_t t
     if (tid < 3) {
         double v = 0.0;
         for (int w = 0; w < BS / 64; ++w) v += red[w][tid];
-        A.part[((size_t)b * A.nblk + blockIdx.x) * 4 + tid] = v;
+        A.part[((size_t)b * A.nblk + x) * 4 + tid] = v;
     }
 }
 
@@ -617,11 +630,15 @@ int grad_launch(const double *n, const int8_t *s, int32_t batch, int32_t ni, int
     if (pairs && (uint64_t)tiles * nslab <= (N + kGradBlock - 1) / kGradBlock) {
         A.nblk = tiles * nslab;
         A.band = 0;
-        const dim3 grid(A.nblk, batch), block(kMarchBlock);
+        // a batch of whole multiples of 8 lattices: one lattice per XCD at a time
+        const int whole = batch % 8 == 0;
+        const dim3 grid = whole ? dim3(A.nblk * (uint32_t)batch) : dim3(A.nblk, batch), block(kMarchBlock);
         if (bc == BC_OPEN)
-            hipLaunchKernelGGL((k_energy_grad_march<BC_OPEN, kMarchBlock>), grid, block, 0, st, A, tiles, nslab, ks);
+            hipLaunchKernelGGL((k_energy_grad_march<BC_OPEN, kMarchBlock>), grid, block, 0, st, A, tiles, nslab, ks,
+                               whole);
         else
-            hipLaunchKernelGGL((k_energy_grad_march<BC_PERIODIC, kMarchBlock>), grid, block, 0, st, A, tiles, nslab, ks);
+            hipLaunchKernelGGL((k_energy_grad_march<BC_PERIODIC, kMarchBlock>), grid, block, 0, st, A, tiles, nslab,
+                               ks, whole);
         hipLaunchKernelGGL(k_energy_final, dim3(batch), dim3(256), 0, st, A, E, dEdK, dEdg, dEdeps);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return gfail("chll_energy_grad launch: %s", hipGetErrorString(e));

@@ -72,6 +72,30 @@ def test_against_cpu_autograd(bc, shape):
     assert torch.allclose(de.cpu(), ec.grad, rtol=1e-13, atol=1e-11)
 
 
+@pytest.mark.parametrize("bc", ["open", "periodic"])
+def test_whole_lattices_per_xcd_match_cpu_autograd(bc):
+    """A batch of 8 (a multiple of the XCD count) takes the march kernel's
+    one-lattice-per-XCD block order; 8 lattices of (nk, nj, ni) = (19, 6, 40):
+    3 slabs (the last partial) of one tile each, against CPU autograd."""
+    B = 8
+    n, s = random_batch(B, 19, 6, 40, seed=88)
+    K = torch.linspace(0.5, 2.0, B, dtype=torch.float64)
+    g = torch.linspace(0.0, 1.0, B, dtype=torch.float64)
+    eps = torch.linspace(0.8, 1.2, B, dtype=torch.float64)
+    E, dn, dK, dg, de = energy_and_grad(n, s, K.cuda(), g.cuda(), eps.cuda(), bc=bc)
+    nc = n.cpu().requires_grad_(True)
+    Kc, gc, ec = (x.clone().requires_grad_(True) for x in (K, g, eps))
+    Er = ref_energy(nc, s.cpu(), Kc, gc, ec, bc)
+    Er.sum().backward()
+    G = nc.grad
+    Gp = G - (G * nc.detach()).sum(dim=1, keepdim=True) * nc.detach()
+    assert torch.allclose(E.cpu(), Er.detach(), rtol=1e-13, atol=1e-11)
+    assert (dn.cpu() - Gp).abs().max() < 1e-12
+    assert torch.allclose(dK.cpu(), Kc.grad, rtol=1e-13, atol=1e-11)
+    assert torch.allclose(dg.cpu(), gc.grad, rtol=1e-13, atol=1e-11)
+    assert torch.allclose(de.cpu(), ec.grad, rtol=1e-13, atol=1e-11)
+
+
 def _tangent_field(n, seed):
     g = torch.Generator(device="cpu").manual_seed(seed)
     t = torch.randn(n.shape, generator=g, dtype=torch.float64).to(n.device)
