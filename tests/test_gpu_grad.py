@@ -96,6 +96,21 @@ def test_whole_lattices_per_xcd_match_cpu_autograd(bc):
     assert torch.allclose(de.cpu(), ec.grad, rtol=1e-13, atol=1e-11)
 
 
+def test_block_orders_give_the_same_gradient():
+    """The march kernel's two block orders (a batch of 8: whole lattices per
+    XCD; a batch of 3: slabs of every lattice across the XCDs) give every
+    site's gradient bit for bit (each site's bonds are summed in one fixed
+    order by one thread), and E to the rounding of the partial sums."""
+    n, s = random_batch(8, 24, 16, 32, seed=5)
+    K = torch.linspace(0.5, 2.0, 8, dtype=torch.float64, device="cuda")
+    g = torch.linspace(0.0, 1.0, 8, dtype=torch.float64, device="cuda")
+    eps = torch.ones(8, dtype=torch.float64, device="cuda")
+    E8, dn8, *_ = energy_and_grad(n, s, K, g, eps)
+    E3, dn3, *_ = energy_and_grad(n[:3].contiguous(), s[:3].contiguous(), K[:3], g[:3], eps[:3])
+    assert torch.equal(dn8[:3], dn3)
+    assert torch.allclose(E8[:3], E3, rtol=1e-14, atol=0)
+
+
 def _tangent_field(n, seed):
     g = torch.Generator(device="cpu").manual_seed(seed)
     t = torch.randn(n.shape, generator=g, dtype=torch.float64).to(n.device)
