@@ -165,24 +165,31 @@ __device__ inline uint32_t neighbours(const Geom &G, uint32_t m, const Site &s, 
     return mask;
 }
 
-// Block order for one-site-per-thread launches.  Workgroups go round-robin to
-// the 8 XCDs (workgroup b to XCD b % 8), each with its own L2.  With band > 0
-// a plane's blocks are cut into 8 bands of `band` blocks (rows) and XCD x
-// takes band x of every plane, planes in order: a row's y neighbours then sit
-// in the same L2 except at the band edges, and the z neighbours of a band stay
-// in that L2 while its next planes run.  band = 0 keeps the identity order.
+// Block order for banded launches.  Workgroups go round-robin to the 8 XCDs
+// (workgroup b to XCD b % 8), each with its own L2.  With band = P > 0 the
+// lattice's blocks are taken in groups of P (one plane's blocks), each group
+// cut into 8 bands of ceil(P / 8) blocks (rows), and XCD x takes band x of
+// every group, groups in order: a row's y neighbours then sit in the same L2
+// except at the band edges, and the z neighbours of a band (P blocks later)
+// stay in that L2 while its next planes run.  When P is not a multiple of 8
+// the last band of a group is short and the grid holds 8 ceil(P / 8) blocks
+// per group; the padding blocks get kIdleBlock and do no work.  band = 0
+// keeps the identity order.
+constexpr uint32_t kIdleBlock = 0xffffffffu;
 __device__ inline uint32_t band_block(uint32_t b, uint32_t band)
 {
     if (band == 0u) return b;
+    const uint32_t per = (band + 7u) >> 3;
     const uint32_t x = b & 7u, n = b >> 3;
-    const uint32_t p = n / band, r = n - p * band;
-    return (p * 8u + x) * band + r;
+    const uint32_t p = n / per, r = n - p * per, q = x * per + r;
+    return q < band ? p * band + q : kIdleBlock;
 }
 
 // Block of a sweep launch: bits 0..30 of `band` as band_block, bit 31 = walk
 // the grid last to first (a launch then starts on the planes the launch before
-// it finished with, which the memory-side cache still holds; the grid must be
-// a multiple of 8 blocks when band > 0, as it is whenever band_block bands).
+// it finished with, which the memory-side cache still holds; a banded grid is
+// a multiple of 8 blocks, so reversed block b runs on XCD 7 - b % 8 and keeps
+// the bands).
 __device__ inline uint32_t sweep_block(uint32_t band)
 {
     const uint32_t b = (band >> 31) ? gridDim.x - 1u - blockIdx.x : blockIdx.x;

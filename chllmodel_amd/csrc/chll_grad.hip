@@ -616,7 +616,7 @@ int grad_launch(const double *n, const int8_t *s, int32_t batch, int32_t ni, int
     // XCD row bands when a plane is 8 bands of whole blocks (then every
     // lattice's block count is a multiple of 8 and block (x, b) runs on XCD x % 8)
     const uint64_t plane = (uint64_t)ni * nj / per_thread;
-    A.band = plane % (8u * kGradBlock) == 0 ? (uint32_t)(plane / (8u * kGradBlock)) : 0u;
+    A.band = plane % (8u * kGradBlock) == 0 ? (uint32_t)(plane / kGradBlock) : 0u;
     hipStream_t st = (hipStream_t)stream;
     // marching slabs of ks planes (k_energy_grad_march) when the blocks fit the
     // workspace that chll_energy_workspace_bytes sizes (one partial per block

@@ -69,9 +69,10 @@ void launch_scatter_f64(const Geom &G, const double *src, double *dst, hipStream
 void launch_gather_f64(const Geom &G, const double *src, double *dst, hipStream_t s);
 void launch_scatter_spin(const Geom &G, const int32_t *src, int8_t *dst, int *bad, hipStream_t s);
 void launch_gather_spin(const Geom &G, const int8_t *src, int32_t *dst, hipStream_t s);
-// kind 0 = director, 1 = chirality.  order bit 0: XCD row bands (xcd_band
-// blocks per band; lattice order when 0); bit 1: two sites per thread with
-// 16-byte pair loads where pairs_ok (else one site per thread).
+// kind 0 = director, 1 = chirality.  order bit 0: XCD row bands (band_block
+// with xcd_band blocks per plane group; lattice order when 0); bit 1: two
+// sites per thread with 16-byte pair loads where pairs_ok (else one site per
+// thread).
 void launch_phase(int kind, int arith, int rng, int bc, const Geom &G, const Fields &F, int colour,
                   const PhaseArgs &A, uint32_t *counts, int order, hipStream_t s);
 uint32_t xcd_band(const Geom &G);

@@ -77,8 +77,8 @@ template <int ARITH, int RNG, int BC, bool OFF32>
 __global__ __launch_bounds__(kBlock) void k_dir(Geom G, Fields F, int colour, PhaseArgs A,
                                                 uint32_t *__restrict__ counts, uint32_t band)
 {
-    const uint32_t m = sweep_block(band) * kBlock + threadIdx.x;
-    const bool acc = m < G.nsub && dir_site<ARITH, RNG, BC, OFF32>(G, F, colour, A, m);
+    const uint32_t blk = sweep_block(band), m = blk * kBlock + threadIdx.x;
+    const bool acc = blk != kIdleBlock && m < G.nsub && dir_site<ARITH, RNG, BC, OFF32>(G, F, colour, A, m);
     block_count<kBlock>(acc, counts, blockIdx.x);
 }
 
@@ -87,8 +87,8 @@ template <int ARITH, int RNG, int BC, bool OFF32>
 __global__ __launch_bounds__(kBlock) void k_spin(Geom G, Fields F, int colour, PhaseArgs A,
                                                  uint32_t *__restrict__ counts, uint32_t band)
 {
-    const uint32_t m = sweep_block(band) * kBlock + threadIdx.x;
-    const bool acc = m < G.nsub && spin_site<ARITH, RNG, BC, OFF32>(G, F, colour, A, m);
+    const uint32_t blk = sweep_block(band), m = blk * kBlock + threadIdx.x;
+    const bool acc = blk != kIdleBlock && m < G.nsub && spin_site<ARITH, RNG, BC, OFF32>(G, F, colour, A, m);
     block_count<kBlock>(acc, counts, blockIdx.x);
 }
 
@@ -99,9 +99,9 @@ __global__ __launch_bounds__(kBlock) void k_dir_spin(Geom G, Fields F, int colou
                                                      uint32_t *__restrict__ counts_dir,
                                                      uint32_t *__restrict__ counts_spin, uint32_t band)
 {
-    const uint32_t m = sweep_block(band) * kBlock + threadIdx.x;
+    const uint32_t blk = sweep_block(band), m = blk * kBlock + threadIdx.x;
     bool flip = false;
-    const bool acc = m < G.nsub && dir_spin_site<RNG, BC, OFF32>(G, F, colour, A, m, flip);
+    const bool acc = blk != kIdleBlock && m < G.nsub && dir_spin_site<RNG, BC, OFF32>(G, F, colour, A, m, flip);
     block_count2<kBlock>(acc, flip, counts_dir, counts_spin, blockIdx.x);
 }
 
@@ -113,9 +113,9 @@ __global__ __launch_bounds__(kBlock) void k_spin_dir(Geom G, Fields F, PhaseArgs
                                                      uint32_t *__restrict__ counts_dir,
                                                      uint32_t *__restrict__ counts_spin, uint32_t band)
 {
-    const uint32_t m = sweep_block(band) * kBlock + threadIdx.x;
+    const uint32_t blk = sweep_block(band), m = blk * kBlock + threadIdx.x;
     bool flip = false;
-    const bool acc = m < G.nsub && spin_dir_site<BC, OFF32>(G, F, A, m, flip);
+    const bool acc = blk != kIdleBlock && m < G.nsub && spin_dir_site<BC, OFF32>(G, F, A, m, flip);
     block_count2<kBlock>(acc, flip, counts_dir, counts_spin, blockIdx.x);
 }
 
@@ -130,10 +130,10 @@ __global__ __launch_bounds__(kBlock) void k_pair(Geom G, Fields F, int colour, P
                                              uint32_t *__restrict__ c0, uint32_t *__restrict__ c1,
                                              uint32_t band)
 {
-    const uint32_t m0 = 2u * (sweep_block(band) * kBlock + threadIdx.x);
+    const uint32_t blk = sweep_block(band), m0 = 2u * (blk * kBlock + threadIdx.x);
     bool a0 = false, a1 = false, f0 = false, f1 = false;
     const double d = OP == 1 ? 0.0 : *A.d;  // the cone step, read once for both sites
-    if (m0 < G.nsub) {
+    if (blk != kIdleBlock && m0 < G.nsub) {
         SiteIn S0, S1;
         const uint32_t own = load_pair<BC, true>(G, F, colour, m0, S0, S1);
         // both sites' states go back as 16-byte words (store_pair)
@@ -480,12 +480,30 @@ void launch_gather_spin(const Geom &G, const int8_t *src, int32_t *dst, hipStrea
     hipLaunchKernelGGL(k_gather_spin, grid1(G.nsub), dim3(kBlock), 0, s, G, src, dst);
 }
 
+// XCD row bands (band_block) for a plane of `units` threads: P = floor(units /
+// kBlock) blocks per group (blocks straddle planes when units is no multiple
+// of kBlock; group p + 1 then starts less than a block away from plane p + 1,
+// so the bands still line up with their z neighbours).  0 (lattice order) when
+// a plane holds fewer than 8 blocks or padding its 8 bands to equal size would
+// add more than 1/16 to the grid.
+static uint32_t plane_band(uint64_t units)
+{
+    const uint64_t P = units / kBlock, per = (P + 7u) / 8u;
+    return P >= 8u && 16u * 8u * per <= 17u * P ? (uint32_t)P : 0u;
+}
+
+// Grid of a launch over nblk logical blocks in bands of `band` (bit 31 clear).
+static inline dim3 band_grid(uint32_t nblk, uint32_t band)
+{
+    if (band == 0u) return dim3(nblk);
+    const uint32_t groups = (nblk + band - 1u) / band;
+    return dim3(groups * 8u * ((band + 7u) / 8u));
+}
+
 uint32_t xcd_band(const Geom &G)
 {
-    // 8 equal bands of whole blocks per plane; every block then lies in one
-    // plane (ni even: a plane holds ni nj / 2 packed sites of each colour)
-    if (G.ni & 1u) return 0u;
-    return G.plane % (8u * kBlock) == 0u ? G.plane / (8u * kBlock) : 0u;
+    // ni even: a plane holds ni nj / 2 packed sites of each colour
+    return (G.ni & 1u) ? 0u : plane_band(G.plane);
 }
 
 bool field_off32(const Geom &G)
@@ -495,19 +513,18 @@ bool field_off32(const Geom &G)
 
 bool pairs_ok(const Geom &G) { return (G.ni & 3u) == 0u && field_off32(G); }
 
-// Pair kernels: nsub / 2 threads; XCD bands of whole blocks of pairs.
-static inline dim3 grid_pairs(const Geom &G) { return grid1(G.nsub / 2u); }
-static uint32_t pair_band(const Geom &G, int order)
+// Pair kernels: nsub / 2 threads, a plane of plane / 2 pairs.
+static uint32_t pair_band(const Geom &G, int order) { return (order & 1) ? plane_band(G.plane / 2u) : 0u; }
+static inline dim3 grid_pairs(const Geom &G, uint32_t band)
 {
-    const uint32_t per = 16u * kBlock;  // 8 bands x kBlock pairs
-    return (order & 1) && G.plane % per == 0u ? G.plane / per : 0u;
+    return band_grid(grid1(G.nsub / 2u).x, band & 0x7fffffffu);
 }
 
 template <int OP, int ARITH, int RNG, int BC>
 static void launch_pair(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t *c0, uint32_t *c1,
                         uint32_t band, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pair<OP, ARITH, RNG, BC>), grid_pairs(G), dim3(kBlock), 0, s, G, F, colour, A, c0, c1, band);
+    hipLaunchKernelGGL((k_pair<OP, ARITH, RNG, BC>), grid_pairs(G, band), dim3(kBlock), 0, s, G, F, colour, A, c0, c1, band);
 }
 static uint32_t site_band(const Geom &G, int order) { return (order & 1) ? xcd_band(G) : 0u; }
 
@@ -524,8 +541,8 @@ static void launch_phase_t(int kind, const Geom &G, const Fields &F, int colour,
         else launch_pair<1, ARITH, RNG, BC>(G, F, colour, A, counts, counts, band, s);
         return;
     }
-    const dim3 g = grid1(G.nsub);
     const uint32_t band = site_band(G, order) | rev;
+    const dim3 g = band_grid(grid1(G.nsub).x, band & 0x7fffffffu);
     if (field_off32(G)) {
         if (kind == 0) hipLaunchKernelGGL((k_dir<ARITH, RNG, BC, true>), g, b, 0, s, G, F, colour, A, counts, band);
         else hipLaunchKernelGGL((k_spin<ARITH, RNG, BC, true>), g, b, 0, s, G, F, colour, A, counts, band);
@@ -544,8 +561,8 @@ static void launch_fused_t(const Geom &G, const Fields &F, int colour, const Pha
         launch_pair<2, ARITH_FP64, RNG, BC>(G, F, colour, A, cd, cs, pair_band(G, order), s);
         return;
     }
-    const dim3 g = grid1(G.nsub);
     const uint32_t band = site_band(G, order);
+    const dim3 g = band_grid(grid1(G.nsub).x, band);
     if (field_off32(G)) hipLaunchKernelGGL((k_dir_spin<RNG, BC, true>), g, b, 0, s, G, F, colour, A, cd, cs, band);
     else hipLaunchKernelGGL((k_dir_spin<RNG, BC, false>), g, b, 0, s, G, F, colour, A, cd, cs, band);
 }
@@ -577,7 +594,7 @@ void launch_phase_fused_next(int bc, const Geom &G, const Fields &F, const Phase
         return;
     }
     const uint32_t band = site_band(G, order) | ((order & 4) ? 0x80000000u : 0u);
-    const dim3 g = grid1(G.nsub);
+    const dim3 g = band_grid(grid1(G.nsub).x, band & 0x7fffffffu);
     const bool o32 = field_off32(G);
     if (bc == BC_OPEN) {
         if (o32) hipLaunchKernelGGL((k_spin_dir<BC_OPEN, true>), g, b, 0, s, G, F, A, counts_dir, counts_spin, band);

@@ -106,6 +106,10 @@ CASES = [
     ((3, 2, 2), "open", "stream", "compat", 8, 4),
     ((63, 20, 12), "open", "philox", "fp64", 4, 2),
     ((33, 17, 10), "open", "philox", "compat", 4, 4),
+    # padded XCD bands: 480 x 480 planes hold 225 pair blocks (the last of
+    # each plane's 8 bands short)
+    ((480, 480, 2), "open", "philox", "fp64", 4, 2),
+    ((480, 480, 2), "periodic", "philox", "compat", 3, 3),
 ]
 
 
@@ -304,7 +308,7 @@ def test_dropin_run_philox_run_counter(orc):
 
 SHAPES = [((256 // 8, 24, 20), "open"), ((48, 40, 30), "periodic"), ((10, 7, 5), "open"), ((130, 6, 9), "open"),
           ((18, 34, 4), "periodic"), ((64, 64, 8), "open"), ((128, 64, 6), "periodic"), ((256, 8, 6), "open"),
-          ((256, 12, 4), "periodic")]
+          ((256, 12, 4), "periodic"), ((480, 480, 2), "open"), ((200, 200, 4), "periodic")]
 
 
 @pytest.mark.parametrize("dims,bc", SHAPES, ids=[f"{'x'.join(map(str, d))}-{b}" for d, b in SHAPES])
@@ -316,7 +320,9 @@ def test_kernel_shapes_identical(dims, bc, arith):
     each with and without the fused launches (FP64), and every k-march tile /
     slab split give the same lattice, counts and rows (partial tiles, odd open
     dimensions; 64x64 and 128x64 planes split into XCD bands of 1 and 2
-    blocks)."""
+    blocks; 480x480 planes into padded bands, 225 pair blocks or 450 site
+    blocks per plane with the last band short; 200x200 planes into padded
+    bands of blocks that straddle planes)."""
     st0 = isotropic_state(*dims, 2000 + sum(dims))
     ref = None
     for variant, tile, nslab, fuse in [("site", 0, 1, False), ("site", 0, 1, True), ("site", 1, 1, True),
