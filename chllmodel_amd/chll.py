@@ -118,19 +118,33 @@ class ChLLSim:
         print(f"> {'reusing' if existed else 'created'} {target}")
         self.simpath = target
 
+    def _output_path(self, filename):
+        """Where an output file goes, or None when the simulation is not saved.
+
+        The rules are the reference's (chll/chll.py:200-262): the simulation
+        folder must still exist and a file already there is replaced only with
+        overwrite=True; a breach of either ends the program with exit()."""
+        if self.name is None:
+            return None
+        if not os.path.exists(self.simpath):
+            print(f"> ERROR: '{self.simpath}' does not exist")
+            exit()
+        path = os.path.join(self.simpath, filename)
+        if os.path.isfile(path) and not self.overwrite:
+            print(f"> ERROR: '{path}' exists and overwrite=False")
+            exit()
+        return path
+
     def save_params(self, nsteps, nout):
+        """<name><name>_params.txt, always rewritten (chll/chll.py:139-150)."""
         if self.name is None:
             return
-        params_name = self.name + self.name + "_params.txt"
-        with open(os.path.join(self.simpath, params_name), "w") as f:
-            f.write(self.name + "\n")
-            f.write("nsteps = " + str(nsteps) + " nout = " + str(nout) + "\n")
-            f.write("ni,nj,nk = " + str(self.ni) + " " + str(self.nj) + " " + str(self.nk) + "\n")
-            f.write("kbt = " + str(self.kbt) + "\n")
-            f.write("d = " + str(self.d) + "\n")
-            f.write("KK = " + str(self.KK) + "\n")
-            f.write("g = " + str(self.g) + "\n")
-            f.write("rho = " + str(self.rho) + "\n")
+        fields = [("nsteps", nsteps), ("nout", nout)]
+        lines = [self.name, " ".join(f"{k} = {v}" for k, v in fields),
+                 f"ni,nj,nk = {self.ni} {self.nj} {self.nk}"]
+        lines += [f"{k} = {getattr(self, k)}" for k in ("kbt", "d", "KK", "g", "rho")]
+        with open(os.path.join(self.simpath, f"{self.name}{self.name}_params.txt"), "w") as f:
+            f.write("\n".join(lines) + "\n")
 
     def _configure(self):
         L = _lib.lib()
@@ -173,36 +187,23 @@ class ChLLSim:
         print("> DONE! ", et - st)
 
     def save_config(self):
-        if self.name is not None:
-            if os.path.exists(self.simpath):
-                datafile_name = self.name + "_data.npz"
-                datafile_path = os.path.join(self.simpath, datafile_name)
-                if os.path.isfile(datafile_path) and not self.overwrite:
-                    print(f"> ERROR: '{datafile_path}' exists and overwrite=False")
-                    exit()
-                np.savez_compressed(datafile_path, nx=self.nx, ny=self.ny, nz=self.nz, s=self.s, dope=self.dope)
-                print(f"> Wrote '{datafile_path}'")
-            else:
-                print(f"> ERROR: '{self.simpath}' does not exist")
-                exit()
+        """<name>_data.npz: the final state, compressed (chll/chll.py:200-225)."""
+        path = self._output_path(f"{self.name}_data.npz")
+        if path is None:
+            return
+        np.savez_compressed(path, **{k: getattr(self, k) for k in ("nx", "ny", "nz", "s", "dope")})
+        print(f"> Wrote '{path}'")
 
     def save_macros(self):
-        if self.name is not None:
-            if os.path.exists(self.simpath):
-                csv_name = self.name + ".csv"
-                csv_path = os.path.join(self.simpath, csv_name)
-                if os.path.isfile(csv_path) and not self.overwrite:
-                    print(f"> ERROR: '{csv_path}' exists and overwrite=False")
-                    exit()
-                with open(csv_path, "w") as fcsv:
-                    fcsv.write("Step,TotalEnergy,Eexcess,Paccept,Faccept,D\n")
-                    for i in range(len(self.msteps)):
-                        fcsv.write(str(int(self.msteps[i])) + "," + str(self.m[i, 0]) + "," + str(self.m[i, 1]) + ","
-                                   + str(self.m[i, 2]) + "," + str(self.m[i, 3]) + "," + str(self.m[i, 4]) + "\n")
-                print(f"> Wrote '{csv_path}'")
-            else:
-                print(f"> ERROR: '{self.simpath}' does not exist")
-                exit()
+        """<name>.csv: one line per measurement row (chll/chll.py:227-262)."""
+        path = self._output_path(f"{self.name}.csv")
+        if path is None:
+            return
+        body = "".join(",".join([str(int(step))] + [str(v) for v in self.m[r, :5]]) + "\n"
+                       for r, step in enumerate(self.msteps))
+        with open(path, "w") as f:
+            f.write("Step,TotalEnergy,Eexcess,Paccept,Faccept,D\n" + body)
+        print(f"> Wrote '{path}'")
 
     def plot_config(self, path=None):
         """Picture of the mid plane j = nj // 2 (the slice chll/chll.py:264-282

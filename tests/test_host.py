@@ -123,6 +123,42 @@ def test_chllsim_folders_and_params(tmp_path, monkeypatch):
     ChLLSim(4, 4, 4, kbt=0.7, d=0.3, KK=1.0, g=0.5, rho=0.0).save_params(1, 1)
 
 
+def test_chllsim_csv_and_npz(tmp_path, monkeypatch):
+    """The CSV and npz written after a run (chll/chll.py:200-262): header,
+    str() of each value, the npz keys, and exit() when a file exists and
+    overwrite is False."""
+    from chllmodel_amd.chll import ChLLSim
+
+    monkeypatch.chdir(tmp_path)
+    os.makedirs("data")
+    sim = ChLLSim(2, 2, 2, kbt=0.7, d=0.3, KK=1.0, g=0.5, rho=0.0, name="io")
+    sim.msteps = np.array([10, 20], dtype=np.int32)
+    sim.m = np.asfortranarray([[1.5, 0.25, 1.0, 0.5, 0.1 + 0.2], [-2.0, 0.0, 0.875, 0.125, 1e-20]])
+    sim.s[:] = -1
+    sim.nx[:] = 1.0
+    sim.save_macros()
+    sim.save_config()
+    assert open("data/io/io.csv").read() == ("Step,TotalEnergy,Eexcess,Paccept,Faccept,D\n"
+                                             "10,1.5,0.25,1.0,0.5,0.30000000000000004\n"
+                                             "20,-2.0,0.0,0.875,0.125,1e-20\n")
+    z = np.load("data/io/io_data.npz", allow_pickle=False)
+    assert sorted(z.files) == ["dope", "nx", "ny", "nz", "s"]
+    assert np.all(z["nx"] == 1.0) and np.all(z["s"] == -1) and z["s"].dtype == np.int32
+    with pytest.raises(SystemExit):
+        sim.save_macros()
+    with pytest.raises(SystemExit):
+        sim.save_config()
+    sim.overwrite = True
+    sim.m[0, 0] = 3.0
+    sim.save_macros()
+    assert open("data/io/io.csv").read().splitlines()[1] == "10,3.0,0.25,1.0,0.5,0.30000000000000004"
+    # no name: nothing is written and nothing fails
+    anon = ChLLSim(2, 2, 2, kbt=0.7, d=0.3, KK=1.0, g=0.5, rho=0.0)
+    anon.msteps, anon.m = sim.msteps, sim.m
+    anon.save_macros()
+    anon.save_config()
+
+
 def test_chllsim_plot_config(tmp_path, monkeypatch):
     pytest.importorskip("matplotlib")
     from chllmodel_amd.chll import ChLLSim
