@@ -198,7 +198,9 @@ def main():
         del src, dst
         log(f"bench: calibration copy of {args.copy_calib_mib} MiB done")
 
-    copy_rate = hbm_copy_rate()
+    ceilings = hbm_ceilings()
+    copy_rate = ceilings["copy"]["best_GBps"]
+    mix_rate = ceilings["read2_write1"]["best_GBps"]
     tuned = {}
 
     def time_single(arith, timing, bc=None, nout=None):
@@ -304,6 +306,10 @@ def main():
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                 "traffic": traffic, "kernel": KERNEL_NAMES[dom],
                 "copy_GBps": copy_rate, "frac_of_copy": ach / copy_rate,
+                "ceiling": {"pattern": "two arrays read, one rewritten in place (the sweep's mix), plain "
+                                       "streaming kernel on 2 GiB arrays", "GBps": mix_rate,
+                            "frac_of_peak": mix_rate / HBM_PEAK_GBS, "frac": ach / mix_rate},
+                "streaming_ceilings_GBps": {k: round(v["best_GBps"], 1) for k, v in ceilings.items()},
                 "algorithmic_bytes_per_launch": BYTES[dom] * N,
                 "algorithmic_bytes_per_site": BYTES[dom],
                 "kernel_avg_us": kernels[dom]["avg_us"],
@@ -356,6 +362,8 @@ def main():
         out["dropin_run"] = dropin_run(L, st0, args)
     if not args.no_grad:
         out["gradient"] = gradient_rate(args)
+        # the gradient reads one set of fields and writes another: a copy's mix
+        out["gradient"]["frac_of_copy"] = out["gradient"]["GBps"] / copy_rate
     if not args.no_analysis:
         out["analysis"] = analysis_rates(args)
     if not args.no_batched:
@@ -369,28 +377,22 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def hbm_copy_rate(mib=2048, reps=5):
-    """HBM bandwidth a plain device-to-device copy reaches on this GPU (read
-    plus write bytes over the best of `reps` copies of `mib` MiB, HIP
-    events): the practical ceiling beside the 8 TB/s peak."""
-    import torch
+def hbm_ceilings(mib=2048, reps=10):
+    """Rates plain streaming kernels reach on this GPU for the load/store
+    mixes the sweep moves (chll_bandwidth_probe, two arrays of `mib` MiB, far
+    past the memory-side cache; best of `reps` passes, HIP events): a copy,
+    the sweep's own mix (two arrays read, one rewritten in place), reads only
+    and writes only.  These are the practical ceilings beside the 8 TB/s
+    peak."""
+    from chllmodel_amd._lib import BW_PATTERNS, bandwidth_probe
 
-    n = mib * (1 << 20) // 8
-    src = torch.ones(n, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    best = float("inf")
-    for _ in range(reps):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        dst.copy_(src)
-        e1.record()
-        torch.cuda.synchronize()
-        best = min(best, e0.elapsed_time(e1) * 1e-3)
-    del src, dst
-    rate = 2 * n * 8 / best / 1e9
-    log(f"bench: device copy of {mib} MiB: {rate:.0f} GB/s")
-    return rate
+    out = {}
+    for name in BW_PATTERNS:
+        best, med = bandwidth_probe(name, mib, reps)
+        out[name] = {"best_GBps": best, "median_GBps": med}
+    log("bench: streaming ceilings (GB/s, best of %d on %d MiB arrays): %s" % (
+        reps, mib, ", ".join(f"{k} {v['best_GBps']:.0f}" for k, v in out.items())))
+    return out
 
 
 def dropin_run(L, st0, args):

@@ -1,5 +1,6 @@
 // chll_api.hip -- C ABI of libchll_mod.so: the drop-in init/run entry points
 // of chll/chll_mod.f90 and the device-resident lattice extensions.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -185,12 +186,11 @@ int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lat
     rc |= dalloc(&L->F.ny, n3 + kFieldPad);
     rc |= dalloc(&L->F.nz, n3 + kFieldPad);
     rc |= dalloc(&L->F.s, n3 + kFieldPad);
-    if (!rc) {
-        HIPCHECK(hipMemset(L->F.nx + n3, 0, kFieldPad * sizeof(double)));
-        HIPCHECK(hipMemset(L->F.ny + n3, 0, kFieldPad * sizeof(double)));
-        HIPCHECK(hipMemset(L->F.nz + n3, 0, kFieldPad * sizeof(double)));
-        HIPCHECK(hipMemset(L->F.s + n3, 0, kFieldPad * sizeof(int8_t)));
-    }
+    if (!rc && (hipMemset(L->F.nx + n3, 0, kFieldPad * sizeof(double)) != hipSuccess ||
+                hipMemset(L->F.ny + n3, 0, kFieldPad * sizeof(double)) != hipSuccess ||
+                hipMemset(L->F.nz + n3, 0, kFieldPad * sizeof(double)) != hipSuccess ||
+                hipMemset(L->F.s + n3, 0, kFieldPad * sizeof(int8_t)) != hipSuccess))
+        rc = fail("hipMemset of the zero-site padding failed");
     rc |= dalloc(&L->bad, 1);
     L->nblk_meas = measure_blocks(G);
     rc |= dalloc(&L->counts, (size_t)kCountWords);
@@ -428,6 +428,52 @@ int chll_lattice_timing(chll_lattice *L, double *ms, int64_t *launches, int32_t 
         }
     }
     L->ev_used = 0;
+    return 0;
+}
+
+int chll_bandwidth_probe(int32_t pattern, int64_t bytes, int32_t reps, double *best_gbps, double *median_gbps)
+{
+    if (pattern < CHLL_BW_COPY || pattern > CHLL_BW_WRITE) return fail("unknown bandwidth pattern %d", pattern);
+    if (bytes < (256ll << 20) || bytes > (16ll << 30)) return fail("probe arrays must hold 256 MiB to 16 GiB each");
+    if (reps < 1 || reps > 1000) return fail("reps must lie in 1..1000");
+    const uint64_t nb = (uint64_t)bytes & ~(uint64_t)15;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<float> ms;
+    int rc = 0;
+    const auto step = [&](hipError_t e, const char *what) {
+        if (!rc && e != hipSuccess) rc = fail("chll_bandwidth_probe: %s: %s", what, hipGetErrorString(e));
+    };
+    step(hipMalloc(&a, nb), "hipMalloc");
+    step(hipMalloc(&b, nb), "hipMalloc");
+    step(hipStreamCreate(&st), "hipStreamCreate");
+    step(hipEventCreate(&e0), "hipEventCreate");
+    step(hipEventCreate(&e1), "hipEventCreate");
+    if (!rc) step(hipMemsetAsync(a, 0, nb, st), "hipMemsetAsync");
+    if (!rc) step(hipMemsetAsync(b, 0, nb, st), "hipMemsetAsync");
+    for (int r = 0; !rc && r < 3 + reps; ++r) {
+        if (r >= 3) step(hipEventRecord(e0, st), "hipEventRecord");
+        if (!rc && launch_bandwidth(pattern, a, b, nb, st)) rc = fail("chll_bandwidth_probe: bad launch size");
+        if (!rc) step(hipGetLastError(), "kernel launch");
+        if (r < 3) continue;
+        step(hipEventRecord(e1, st), "hipEventRecord");
+        step(hipEventSynchronize(e1), "hipEventSynchronize");
+        float t = 0.f;
+        step(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+        ms.push_back(t);
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (rc) return -1;
+    std::sort(ms.begin(), ms.end());
+    const double moved = (double)nb * (pattern == CHLL_BW_COPY ? 2.0 : pattern == CHLL_BW_READ2_WRITE1 ? 3.0 : 1.0);
+    if (best_gbps) *best_gbps = moved / (ms.front() * 1e-3) / 1e9;
+    if (median_gbps) *median_gbps = moved / (ms[ms.size() / 2] * 1e-3) / 1e9;
     return 0;
 }
 

@@ -102,5 +102,9 @@ void launch_stream_fill(double *out, uint64_t seed, uint64_t base, uint32_t nsub
 // probability p_flip.
 void launch_init_state(const Geom &G, const Fields &F, int kind, uint64_t seed, uint32_t replica, double p_flip,
                        hipStream_t s);
+// One pass of a streaming pattern over `bytes` of each array (k_bandwidth):
+// 0 copy a -> b, 1 read a and b and rewrite a, 2 read a, 3 write b.  -1 on a
+// bad pattern or size.
+int launch_bandwidth(int pattern, void *a, void *b, uint64_t bytes, hipStream_t s);
 
 }  // namespace chll

@@ -681,4 +681,60 @@ void launch_stream_fill(double *out, uint64_t seed, uint64_t base, uint32_t nsub
     hipLaunchKernelGGL(k_stream_fill, grid, dim3(kBlock), 0, s, out, seed, base, nsub);
 }
 
+
+// ---------------------------------------------------------------------------
+// Streaming ceilings of this GPU's memory for the mixes the sweep kernels
+// move (chll_bandwidth_probe).  Every thread moves four 16-byte words at
+// i = 1024 b + 256 u + t, u = 0..3, all loads issued before the first store;
+// one launch covers the arrays once (n16 words, tail guarded).
+namespace {
+typedef double d2v __attribute__((ext_vector_type(2)));
+constexpr int kBwUnroll = 4;
+
+template <int PAT>
+__global__ __launch_bounds__(kBlock) void k_bandwidth(d2v *__restrict__ a, d2v *__restrict__ b, uint64_t n16)
+{
+    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kBwUnroll) + threadIdx.x;
+    d2v va[kBwUnroll], vb[kBwUnroll];
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < kBwUnroll; ++u) {
+        const uint64_t i = base + (uint64_t)u * kBlock;
+        const bool in = i < n16;
+        if constexpr (PAT != 3) va[u] = in ? a[i] : d2v{0.0, 0.0};
+        if constexpr (PAT == 1) vb[u] = in ? b[i] : d2v{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < kBwUnroll; ++u) {
+        const uint64_t i = base + (uint64_t)u * kBlock;
+        if (i >= n16) continue;
+        if constexpr (PAT == 0) b[i] = va[u];                  // copy a -> b
+        if constexpr (PAT == 1) a[i] = va[u] * 0.5 + vb[u];    // read a, b; rewrite a
+        if constexpr (PAT == 2) acc += va[u].x + va[u].y;      // read a
+        if constexpr (PAT == 3) b[i] = d2v{1.0, (double)u};    // write b
+    }
+    // the read-only pattern stores only if the sum hits a value zero-filled
+    // memory never produces, which keeps its loads alive
+    if constexpr (PAT == 2)
+        if (acc == 1.25) b[blockIdx.x] = d2v{acc, acc};
+}
+}  // namespace
+
+int launch_bandwidth(int pattern, void *a, void *b, uint64_t bytes, hipStream_t s)
+{
+    const uint64_t n16 = bytes / 16u;
+    const uint64_t per = (uint64_t)kBlock * kBwUnroll;
+    const uint64_t grid = (n16 + per - 1) / per;
+    if (grid == 0 || grid > 0x7fffffffull) return -1;
+    d2v *pa = (d2v *)a, *pb = (d2v *)b;
+    switch (pattern) {
+    case 0: k_bandwidth<0><<<(uint32_t)grid, kBlock, 0, s>>>(pa, pb, n16); break;
+    case 1: k_bandwidth<1><<<(uint32_t)grid, kBlock, 0, s>>>(pa, pb, n16); break;
+    case 2: k_bandwidth<2><<<(uint32_t)grid, kBlock, 0, s>>>(pa, pb, n16); break;
+    case 3: k_bandwidth<3><<<(uint32_t)grid, kBlock, 0, s>>>(pa, pb, n16); break;
+    default: return -1;
+    }
+    return 0;
+}
+
 }  // namespace chll

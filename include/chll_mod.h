@@ -175,6 +175,22 @@ int chll_lattice_get_fusion(chll_lattice *lat, int32_t *on);
 int chll_lattice_set_timing(chll_lattice *lat, int32_t on);
 int chll_lattice_timing(chll_lattice *lat, double *ms, int64_t *launches, int32_t nkinds);
 
+/* Measurement helper, no counterpart in the reference: the rate a plain
+ * streaming kernel reaches for one mix of loads and stores, the ceiling the
+ * sweep kernels' algorithmic rate is compared with.  Allocates two device
+ * arrays of `bytes` each on the current device (256 MiB to 16 GiB; 2 GiB puts
+ * them well past the 256 MB memory-side cache), zero-fills them, runs 3
+ * untimed and then `reps` (1..1000) timed passes on a stream of its own, frees
+ * the arrays and writes the best and median rates in GB/s (10^9 B/s) of bytes
+ * moved per pass.  Waits for the GPU.  Patterns: */
+#define CHLL_BW_COPY 0         /* read a, write b (2 x bytes per pass) */
+#define CHLL_BW_READ2_WRITE1 1 /* read a and b, write a in place (3 x bytes): the
+                                  sweep's mix, its own colour read and rewritten
+                                  and the other colour read */
+#define CHLL_BW_READ 2         /* read a (1 x bytes) */
+#define CHLL_BW_WRITE 3        /* write b (1 x bytes) */
+int chll_bandwidth_probe(int32_t pattern, int64_t bytes, int32_t reps, double *best_gbps, double *median_gbps);
+
 /* Batched replicas with parallel tempering ------------------------------- */
 /* No counterpart in the reference, which scans (kT, K, g) grids by running one
  * ChLLSim after another (e.g. scripts/phase_diagram_large1.py:30-37). */

@@ -111,3 +111,15 @@ def test_configure_validates():
     assert L.chll_configure(0, 9, 0, 1) == -1 and "arith" in _lib.last_error()
     assert L.chll_configure(0, 0, 5, 1) == -1 and "boundary" in _lib.last_error()
     assert L.chll_configure(0, 0, 0, 1) == 0
+
+
+def test_bandwidth_probe_validates():
+    """The streaming probe refuses bad patterns, sizes and repeat counts
+    before touching the device."""
+    from chllmodel_amd import _lib
+
+    L = _lib.lib()
+    assert L.chll_bandwidth_probe(4, 1 << 30, 3, None, None) == -1 and "pattern" in _lib.last_error()
+    assert L.chll_bandwidth_probe(0, 255 << 20, 3, None, None) == -1 and "256 MiB" in _lib.last_error()
+    assert L.chll_bandwidth_probe(0, 17 << 30, 3, None, None) == -1 and "16 GiB" in _lib.last_error()
+    assert L.chll_bandwidth_probe(0, 1 << 30, 0, None, None) == -1 and "reps" in _lib.last_error()

@@ -32,3 +32,23 @@ def test_accept_exp_bracket():
     assert int(bad.sum()) == 0
     # the assumed error bound is 5x below the margin; the hardware must stay under it
     assert ratio.max() < 0.2
+
+
+@pytest.mark.gpu
+def test_bandwidth_probe_rates():
+    """chll_bandwidth_probe, which the bench prices the sweep kernels against:
+    every pattern runs and reports a rate between 1 TB/s and twice the 8 TB/s
+    peak, the best pass at least the median, and a pass that only reads is
+    faster per byte than the copy that also writes."""
+    import torch  # noqa: F401  (one HIP runtime: torch first, as the library's users do)
+
+    from chllmodel_amd._lib import BW_PATTERNS, bandwidth_probe
+
+    rates = {}
+    for name in BW_PATTERNS:
+        print(f"bandwidth probe {name} on 2 x 512 MiB", flush=True)
+        best, med = bandwidth_probe(name, 512, 5)
+        print(f"  best {best:.0f} GB/s, median {med:.0f} GB/s", flush=True)
+        assert 1000.0 < med <= best < 16000.0
+        rates[name] = best
+    assert rates["read"] > rates["copy"]
