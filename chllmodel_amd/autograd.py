@@ -83,7 +83,9 @@ class ChllEnergy(torch.autograd.Function):
         want = any(ctx.needs_input_grad[:4])
         E, dn, dK, dg, de = energy_and_grad(n, s, K, g, eps, bc=bc, want_grad=ctx.needs_input_grad[0])
         ctx.save_for_backward(dn if dn is not None else torch.empty(0), dK, dg, de)
-        ctx.shapes = (K.shape, g.shape, eps.shape)
+        # parameters may be scalars, (1,) or (B,) tensors (broadcast over the
+        # batch) or plain numbers; their gradients are summed back to that shape
+        ctx.pshapes = tuple((x.shape, x.dtype) if torch.is_tensor(x) else None for x in (K, g, eps))
         ctx.want = want
         return E
 
@@ -93,12 +95,11 @@ class ChllEnergy(torch.autograd.Function):
         gn = gK = gg = ge = None
         if ctx.needs_input_grad[0]:
             gn = gE.view(-1, 1, 1, 1, 1) * dn
-        if ctx.needs_input_grad[1]:
-            gK = (gE * dK).reshape(ctx.shapes[0]) if len(ctx.shapes[0]) else (gE * dK).sum()
-        if ctx.needs_input_grad[2]:
-            gg = (gE * dg).reshape(ctx.shapes[1]) if len(ctx.shapes[1]) else (gE * dg).sum()
-        if ctx.needs_input_grad[3]:
-            ge = (gE * de).reshape(ctx.shapes[2]) if len(ctx.shapes[2]) else (gE * de).sum()
+        grads = []
+        for i, d in enumerate((dK, dg, de)):
+            ps = ctx.pshapes[i]
+            grads.append((gE * d).sum_to_size(ps[0]).to(ps[1]) if ctx.needs_input_grad[1 + i] and ps else None)
+        gK, gg, ge = grads
         return gn, gK, gg, ge, None, None
 
 

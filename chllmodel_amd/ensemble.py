@@ -96,11 +96,20 @@ class Ensemble:
         return [r["ni"] * r["nj"] * r["nk"] for r in self.replicas]
 
     def upload(self, r, st):
+        if not 0 <= r < self.nrep:
+            raise IndexError(f"replica {r} outside 0..{self.nrep - 1}")
+        rr = self.replicas[r]
+        shape = (rr["ni"], rr["nj"], rr["nk"])
         a = [np.asfortranarray(st[k], dtype=np.float64) for k in ("nx", "ny", "nz")]
         s = np.asfortranarray(st["s"], dtype=np.int32)
+        for x in (*a, s):
+            if x.shape != shape:  # the library reads ni nj nk elements of each
+                raise ValueError(f"field shape {x.shape} != replica {r}'s lattice {shape}")
         _check(lib().chll_ensemble_upload(self._h, r, *[x.ctypes.data_as(_f64p) for x in a], s.ctypes.data_as(_i32p)))
 
     def download(self, r) -> dict:
+        if not 0 <= r < self.nrep:
+            raise IndexError(f"replica {r} outside 0..{self.nrep - 1}")
         rr = self.replicas[r]
         shape = (rr["ni"], rr["nj"], rr["nk"])
         out = {k: np.zeros(shape, dtype=np.float64, order="F") for k in ("nx", "ny", "nz")}

@@ -280,3 +280,26 @@ def test_k0_ladder_follows_tanh():
         want = math.tanh(0.3 / o["kT"])
         sig = math.sqrt((1 - want ** 2) / N / len(o["eta"]))
         assert abs(o["eta"].mean() - want) < 5 * sig, (s, o["eta"].mean(), want)
+
+
+def test_upload_checks_shapes_and_indices():
+    """Host arrays must match the replica's lattice (the library reads ni nj nk
+    elements of each) and the replica index must exist; a good upload reads
+    back unchanged."""
+    reps = [{"ni": 4, "nj": 4, "nk": 2, "ladder": 0, "g": 0.5, "KK": 1.0, "kbt": 0.7},
+            {"ni": 6, "nj": 4, "nk": 2, "ladder": 1, "g": 0.5, "KK": 1.0, "kbt": 0.7}]
+    ens = Ensemble(reps, arith="fp64")
+    try:
+        st = isotropic_state(4, 4, 2, 5)
+        with pytest.raises(ValueError, match="field shape"):
+            ens.upload(1, st)
+        with pytest.raises(IndexError):
+            ens.upload(2, st)
+        with pytest.raises(IndexError):
+            ens.download(-1)
+        ens.upload(0, st)
+        back = ens.download(0)
+        for k in ("nx", "ny", "nz", "s"):
+            assert np.array_equal(back[k], st[k]), k
+    finally:
+        ens.close()

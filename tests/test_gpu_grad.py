@@ -159,6 +159,27 @@ def test_autograd_function_chain():
     assert torch.allclose(K.grad, w * dK) and torch.allclose(g.grad, w * dg) and torch.allclose(eps.grad, w * de)
 
 
+def test_autograd_broadcast_parameters():
+    """Parameters shared by the batch (a (1,) tensor, a 0-d tensor, a float32
+    tensor, a plain number) get gradients of their own shape and dtype, the
+    batch's per-lattice derivatives summed."""
+    n, s = random_batch(3, 4, 5, 6, seed=8)
+    K = torch.tensor([1.25], dtype=torch.float64, device="cuda", requires_grad=True)
+    g = torch.tensor(0.3, dtype=torch.float64, device="cuda", requires_grad=True)
+    eps = torch.tensor([1.0], dtype=torch.float32, device="cuda", requires_grad=True)
+    w = torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64, device="cuda")
+    (ChllEnergy.apply(n, K, g, eps, s, "periodic") * w).sum().backward()
+    _, _, dK, dg, de = energy_and_grad(n, s, K.detach(), g.detach(), eps.detach(), bc="periodic")
+    assert K.grad.shape == (1,) and g.grad.shape == () and eps.grad.shape == (1,)
+    assert eps.grad.dtype == torch.float32
+    assert torch.allclose(K.grad, (w * dK).sum().view(1))
+    assert torch.allclose(g.grad, (w * dg).sum())
+    assert torch.allclose(eps.grad, (w * de).sum().view(1).float())
+    E = ChllEnergy.apply(n, 1.25, 0.3, 1.0, s, "periodic")  # plain numbers: no gradient, same energy
+    E0, *_ = energy_and_grad(n, s, 1.25, 0.3, 1.0, bc="periodic")
+    assert torch.equal(E, E0)
+
+
 def test_quench_lowers_energy_then_metropolis():
     n, s = random_batch(2, 16, 16, 16, seed=11)
     nq, hist = gradient_quench(n, s, K=1.0, g=0.5, steps=100, lr=0.05)
