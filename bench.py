@@ -148,6 +148,8 @@ def main():
     ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in run() timing")
     ap.add_argument("--no-grad", action="store_true", help="skip the energy-gradient timing")
     ap.add_argument("--no-analysis", action="store_true", help="skip the analysis-kernel timing")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the other single-lattice workload configurations (L=32, L=128 anneal, L=480)")
     ap.add_argument("--analysis-only", action="store_true",
                     help="run only the analysis section (and its CPU figures unless --no-cpu)")
     ap.add_argument("--dropin-sweeps", type=int, default=20, help="sweeps per drop-in run() call")
@@ -364,6 +366,8 @@ def main():
         out["gradient"] = gradient_rate(args)
         # the gradient reads one set of fields and writes another: a copy's mix
         out["gradient"]["frac_of_copy"] = out["gradient"]["GBps"] / copy_rate
+    if not args.no_workloads and args.arith == "fp64":
+        out["workloads"] = workload_rates(stream)
     if not args.no_analysis:
         out["analysis"] = analysis_rates(args)
     if not args.no_batched:
@@ -585,6 +589,60 @@ def analysis_cpu_baseline(res):
     cr["cpu_oracle"] = {"pairs_per_s": rate, "cores": 1, "kind": "port",
                         "sample": f"numpy per-offset sums, {Lc}^3, {ns} shells, nematic and chiral"}
     cr["speedup_vs_cpu"] = cr["pairs_per_s"] / rate
+
+
+def workload_rates(stream):
+    """The other single-lattice workload configurations at their full sizes
+    and parameters, FP64 sweeps with Philox draws and device-generated
+    isotropic initial states, each timed on a bounded window after 0.3 s of
+    untimed sweeps (clock spin-up):
+    - L=32 periodic, K=0.5, g=0, kT=1, d0=0.3, nout=10: the whole 1000-sweep run;
+    - L=128, K=1, g=0.5, the linear anneal kT 1.5 -> 0.3 over 20,000 sweeps
+      (one call with chll_params.kbt_end, every sweep at its own kT, a row
+      every 100 sweeps): its first 2000 sweeps;
+    - L=480, K=1, g=0.25, kT=0.5, nout=50 (coarsening): 100 sweeps."""
+    import torch
+
+    from chllmodel_amd.lattice import Lattice, anneal_kt
+
+    cases = [
+        ("L=32 periodic K=0.5 g=0 kT=1 nout=10, 1000 sweeps", 32, "periodic",
+         dict(g=0.0, KK=0.5, kbt=1.0), 10, 1000),
+        ("L=128 anneal kT 1.5->0.3 over 20000 sweeps (K=1 g=0.5, nout=100): sweeps 1-2000", 128, "open",
+         dict(g=0.5, KK=1.0, kbt=1.5, kbt_end=0.3), 100, 2000),
+        ("L=480 coarsening K=1 g=0.25 kT=0.5 nout=50: 100 sweeps", 480, "open",
+         dict(g=0.25, KK=1.0, kbt=0.5), 50, 100),
+    ]
+    out = []
+    for name, L, bc, par, nout, nsw in cases:
+        with Lattice(L, L, L, bc=bc) as lat:
+            lat.set_stream(stream.cuda_stream)
+            lat.init_state("isotropic", seed=20261203)
+            lat.set_step(0.3)
+            spin = lat.params(**{**par, "kbt_end": 0.0}, rng="philox", arith="fp64", seed=7, run_index=1)
+            t0 = time.perf_counter()
+            done = 0
+            while time.perf_counter() - t0 < 0.3:
+                lat.sweeps(done + 1, done + 50, 0, params=spin, want_rows=False)
+                lat.sync()
+                done += 50
+            lat.init_state("isotropic", seed=20261203)
+            lat.set_step(0.3)
+            p = lat.params(**par, rng="philox", arith="fp64", seed=20261203)
+            if "kbt_end" in par:
+                # the anneal's own kT schedule over 20,000 sweeps, of which this
+                # window runs the first nsw: sweep nsw at its scheduled kT
+                p.kbt_end = anneal_kt(1.5, 0.3, nsw, 1, 20000)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            got = lat.sweeps(1, nsw, nout, params=p)["rows"]
+            lat.sync()
+            wall = time.perf_counter() - t0
+        out.append({"workload": name, "sites": L ** 3, "sweeps": nsw, "seconds": wall,
+                    "site_updates_per_s": L ** 3 * nsw / wall, "ms_per_sweep": 1e3 * wall / nsw,
+                    "rows": int(got.shape[0]), "E_true_last": float(got[-1, 6])})
+        log(f"bench: workload {name}: {1e3 * wall / nsw:.4f} ms per sweep, {L ** 3 * nsw / wall / 1e9:.2f} G")
+    return out
 
 
 def batched_scan(args, stream):

@@ -41,6 +41,7 @@ class ChllParams(ct.Structure):
         ("buffer", _f64p), ("buffer_len", ct.c_int64),
         ("run_index", ct.c_uint32), ("replica", ct.c_uint32),
         ("adapt", ct.c_int32), ("pad", ct.c_int32),
+        ("kbt_end", ct.c_double),
     ]
 
 

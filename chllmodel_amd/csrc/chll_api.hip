@@ -1,6 +1,7 @@
 // chll_api.hip -- C ABI of libchll_mod.so: the drop-in init/run entry points
 // of chll/chll_mod.f90 and the device-resident lattice extensions.
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -498,6 +499,8 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
     if (!L || !p) return fail("chll_lattice_sweeps: null argument");
     if (p->rng != RNG_PHILOX && p->rng != RNG_STREAM && p->rng != RNG_BUFFER) return fail("unknown rng mode %d", p->rng);
     if (p->arith != ARITH_COMPAT && p->arith != ARITH_FP64) return fail("unknown arithmetic mode %d", p->arith);
+    if (!(p->kbt_end >= 0.0 && p->kbt_end < HUGE_VAL))
+        return fail("kbt_end must be 0 (no ramp) or a finite positive end temperature (got %g)", p->kbt_end);
     if (max_rows < 0) max_rows = 0;
     const int64_t nsw = (int64_t)nstop - nstart + 1;
     if (nsw <= 0) return 0;
@@ -519,8 +522,17 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
         L->counts_cap = (size_t)nsw * 2;
     }
     hipStream_t st = L->stream;
+    // the temperature of sweep istep: kbt, or the linear ramp from kbt at
+    // nstart to kbt_end at nstop (the bracketing and operation order are part
+    // of the interface: callers reproduce the values in binary64)
+    const bool ramp = p->kbt_end > 0.0 && nstop > nstart;
+    const auto kt_of = [&](int64_t istep) {
+        if (!ramp) return p->kbt;
+        if (istep == nstop) return p->kbt_end;
+        return p->kbt + (p->kbt_end - p->kbt) * (double)(istep - nstart) / (double)(nstop - nstart);
+    };
     PhaseArgs A{};
-    A.g = p->g; A.KK = p->KK; A.kbt = p->kbt; A.inv_kbt = 1.0 / p->kbt;
+    A.g = p->g; A.KK = p->KK; A.kbt = p->kbt; A.inv_kbt = 1.0 / p->kbt; A.inv_kbt_spin = A.inv_kbt;
     A.d = L->d_dev; A.seed = p->seed; A.run_index = p->run_index; A.replica = p->replica;
     FinArgs FA{};
     FA.counts = L->counts; FA.pend = L->pend; FA.part = L->part; FA.spart = L->spart;
@@ -556,6 +568,11 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
             p->stream_pos += 10 * ns;
         }
         A.istep = (uint32_t)istep;
+        if (ramp) {
+            A.kbt = kt_of(istep);
+            A.inv_kbt = 1.0 / A.kbt;
+            A.inv_kbt_spin = A.inv_kbt;
+        }
         const auto set_u = [&](int kind, int colour) {
             if (p->rng == RNG_PHILOX) return;
             if (kind == 0) { A.u_phi = L->ustream + (2 * colour) * ns; A.u_acc = L->ustream + (2 * colour + 1) * ns; }
@@ -601,6 +618,11 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
             PhaseArgs A2 = A;
             A2.istep = (uint32_t)(istep + 1);
             A2.istep_spin = (uint32_t)istep;
+            // the director trials belong to sweep t + 1 and its temperature,
+            // the chirality trials to sweep t
+            A2.kbt = kt_of(istep + 1);
+            A2.inv_kbt = 1.0 / A2.kbt;
+            A2.inv_kbt_spin = A.inv_kbt;
             if (tbegin(L, KIND_FUSED_NEXT, &ti)) return -1;
             launch_phase_fused_next(L->bc, G, L->F, A2, grp(kGroupAcc0), grp(flip_group(istep, 0)),
                                     L->kcfg[0][p->arith].tile, st);

@@ -21,6 +21,7 @@ struct PhaseArgs {
     uint64_t seed;
     uint32_t run_index, replica, istep;
     uint32_t istep_spin;  // fused chirality colour 0 + next director colour 0: the chirality trial's sweep
+    double inv_kbt_spin;  // the same fused launch: RN(1/kbt) of the chirality trial's sweep (kT may ramp)
 };
 
 constexpr int kRowCols = 16;

@@ -258,10 +258,10 @@ __device__ inline bool spin_apply(const Fields &F, int colour, const PhaseArgs &
 // lattice.  Returns the director decision; `flip` gets the chirality decision.
 // FP64 chirality decision (spin_compute) from the site's chirality sum a and
 // its uniform u_acc.
-__device__ inline bool spin_decide_fp64(const PhaseArgs &A, double a, int sg, double u_acc)
+__device__ inline bool spin_decide_fp64(const PhaseArgs &A, double inv_kbt, double a, int sg, double u_acc)
 {
     const double dE = (double)sg * fma(A.KK, a, 2.0 * A.g);
-    return (dE < 0.0) || accept_exp(-dE * A.inv_kbt, u_acc);
+    return (dE < 0.0) || accept_exp(-dE * inv_kbt, u_acc);
 }
 
 // The director trial (dir_compute, FP64) then the chirality trial on the
@@ -291,7 +291,7 @@ __device__ inline bool dir_spin_apply(const Fields &F, int colour, const PhaseAr
     const double dE = (c2o - c2n) - A.KK * (an - ao);
     const bool acc = (dE < 0.0) || accept_exp(-dE * A.inv_kbt, u_acc);
     put_dir<OFF32, DEFER>(F, own, acc, t, S, o);
-    flip = spin_decide_fp64(A, acc ? cqn : cqo, S.sg, u_spin);
+    flip = spin_decide_fp64(A, A.inv_kbt, acc ? cqn : cqo, S.sg, u_spin);
     put_spin<OFF32, DEFER>(F, own, flip, flip ? -S.sg : S.sg, o);
     return acc;
 }
@@ -319,7 +319,7 @@ __device__ inline bool spin_dir_apply(const Fields &F, const PhaseArgs &A, doubl
     // trial of sweep A.istep = A.istep_spin + 1 both draw from block A.istep
     // (spin_sweep); every caller sets A.istep that way
     const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, 0);
-    flip = spin_decide_fp64(A, cqo, S.sg, site_u_spin(w));
+    flip = spin_decide_fp64(A, A.inv_kbt_spin, cqo, S.sg, site_u_spin(w));
     const int sg = flip ? -S.sg : S.sg;
     // director trial of sweep A.istep (dir_compute, FP64)
     const double u_phi = site_u_phi(w), u_acc = site_u_dir(w);

@@ -110,9 +110,12 @@ class Lattice:
 
     # -- sweeps --------------------------------------------------------------
     def params(self, *, g, KK, kbt, rng="philox", arith="compat", seed=0, stream_pos=0, buffer=None,
-               run_index=0, replica=0, adapt=True) -> ChllParams:
+               run_index=0, replica=0, adapt=True, kbt_end=0.0) -> ChllParams:
+        """Sweep parameters; kbt_end > 0 anneals kT linearly from kbt at the
+        call's first sweep to kbt_end at its last (anneal_kt gives each
+        sweep's value)."""
         p = ChllParams()
-        p.g, p.KK, p.kbt = float(g), float(KK), float(kbt)
+        p.g, p.KK, p.kbt, p.kbt_end = float(g), float(KK), float(kbt), float(kbt_end)
         p.rng, p.arith = RNG[rng], ARITH[arith]
         p.seed, p.stream_pos = int(seed), int(stream_pos)
         if buffer is not None:
@@ -175,6 +178,16 @@ class Lattice:
     def __exit__(self, *exc):
         self.close()
         return False
+
+
+def anneal_kt(kbt: float, kbt_end: float, istep: int, nstart: int, nstop: int) -> float:
+    """kT of sweep istep in a call with a ramp (chll_params.kbt_end > 0): the
+    library's binary64 expression, operation for operation."""
+    if not (kbt_end > 0.0 and nstop > nstart):
+        return float(kbt)
+    if istep == nstop:
+        return float(kbt_end)
+    return float(kbt) + (float(kbt_end) - float(kbt)) * float(istep - nstart) / float(nstop - nstart)
 
 
 def rows_to_dict(rows: np.ndarray) -> dict:

@@ -108,6 +108,12 @@ typedef struct chll_params {
     uint32_t run_index, replica;
     int32_t adapt;       /* 1: reference step adaptation while istep < 10000 */
     int32_t pad;
+    double kbt_end;      /* 0: every sweep at kbt.  > 0: a linear anneal, sweep istep at
+                            kbt + (kbt_end - kbt) * (double)(istep - nstart) / (double)(nstop - nstart)
+                            (binary64, in this order), sweep nstop at kbt_end exactly (kbt
+                            itself when nstop == nstart).  No
+                            counterpart in the reference, which holds kbt for a whole run
+                            (chll/chll_mod.f90:40, scripts/pdanneal.py:21-37 steps it per run) */
 } chll_params;
 
 typedef struct chll_lattice chll_lattice;

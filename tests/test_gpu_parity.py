@@ -415,3 +415,70 @@ def test_pipelined_counts_across_calls():
         assert np.array_equal(o[1], outs[0][1])
         assert np.array_equal(o[2], outs[0][2])
         assert o[3] == outs[0][3]
+
+
+ANNEAL = [((32, 32, 16), "open", "philox", "fp64"), ((32, 32, 16), "open", "philox", "compat"),
+          ((12, 10, 8), "periodic", "stream", "fp64"), ((10, 6, 8), "open", "stream", "compat"),
+          ((7, 6, 4), "open", "philox", "fp64")]
+
+
+@pytest.mark.parametrize("dims,bc,rng,arith", ANNEAL, ids=[f"{'x'.join(map(str, c[0]))}-{c[1]}-{c[2]}-{c[3]}"
+                                                           for c in ANNEAL])
+def test_anneal_ramp(orc, dims, bc, rng, arith):
+    """One call with a linear kT ramp (chll_params.kbt_end) equals one call
+    per sweep at the temperatures anneal_kt gives, bit for bit (lattice, cone
+    step, per-sweep counts, rows; with Philox draws and FP64 the ramped call
+    keeps the fused launches across sweeps of different temperature), and
+    equals the oracle run one sweep at a time at those temperatures, to the
+    bar of the other oracle cases."""
+    from chllmodel_amd.lattice import anneal_kt
+
+    st0 = isotropic_state(*dims, 4000 + sum(dims))
+    g, K, d, kt0, kt1, nsw, nout = 0.4, 1.2, 0.3, 1.5, 0.3, 24, 5
+    seed, run_index = 0x5EED, 2
+    kts = [anneal_kt(kt0, kt1, t, 1, nsw) for t in range(1, nsw + 1)]
+    assert kts[0] == kt0 and kts[-1] == kt1 and all(a > b for a, b in zip(kts, kts[1:]))
+    print(f"anneal ramp dims={dims} bc={bc} rng={rng} arith={arith}", flush=True)
+    common = dict(g=g, KK=K, rng=rng, arith=arith, seed=seed, run_index=run_index)
+    ramp = gpu_run(st0, bc, 1, nsw, nout, d, kbt=kt0, kbt_end=kt1, **common)
+    # one call per sweep; a measured sweep asks for its row with nout = 1
+    rows, counts, pos = [], [], 0
+    with Lattice(*dims, bc=bc) as lat:
+        lat.upload_state(st0)
+        lat.set_step(d)
+        for t in range(1, nsw + 1):
+            out = lat.sweeps(t, t, 1 if t % nout == 0 else 10 ** 6, kbt=kts[t - 1], stream_pos=pos, want_counts=True,
+                             **common)
+            pos = out["stream_pos"]
+            rows.extend(out["rows"])
+            counts.append(out["counts"][0])
+        per = {"state": lat.download(), "d": lat.get_step(), "rows": np.array(rows), "counts": np.array(counts)}
+    for f in ("nx", "ny", "nz", "s"):
+        assert np.array_equal(ramp["state"][f], per["state"][f]), f
+    assert ramp["d"] == per["d"]
+    assert np.array_equal(ramp["counts"], per["counts"]) and np.array_equal(ramp["rows"], per["rows"])
+    # the oracle, one sweep at a time
+    st = copy_state(st0)
+    o = dict(rng={"stream": RNG_STREAM, "philox": RNG_PHILOX}[rng], seed=seed, run_index=run_index,
+             bc={"open": BC_OPEN, "periodic": BC_PERIODIC}[bc],
+             arith={"compat": ARITH_COMPAT, "fp64": ARITH_FP64}[arith], want_counts=True)
+    dd, opos, ms_all, m_all, c_all = d, 0, [], [], []
+    for t in range(1, nsw + 1):
+        ms, m, dd, opos, c = orc.run(st, t, t, 1 if t % nout == 0 else 10 ** 6, g, K, dd, kts[t - 1],
+                                     stream_pos=opos, **o)
+        if t % nout == 0:
+            ms_all.append(ms[0])
+            m_all.append(m[0])
+        c_all.append(np.asarray(c).reshape(-1)[:2])
+    assert np.array_equal(ramp["counts"], np.array(c_all))
+    assert_state_match(ramp["state"], st)
+    assert_rows_match(ramp["rows"], np.array(ms_all, dtype=np.int32), np.array(m_all))
+    assert ramp["d"] == dd
+
+
+def test_anneal_ramp_rejects_bad_end():
+    with Lattice(8, 8, 4) as lat:
+        lat.set_step(0.3)
+        for bad in (-1.0, float("inf"), float("nan")):
+            with pytest.raises(_lib.ChllError, match="kbt_end"):
+                lat.sweeps(1, 3, 0, g=0.5, KK=1.0, kbt=1.0, kbt_end=bad, arith="fp64")

@@ -316,18 +316,14 @@ def test_quench_16_l128_full_length():
 @pytest.mark.timeout(900)
 def test_l128_anneal_20000_sweeps():
     L, K, g, d0, nsw = 128, 1.0, 0.5, 0.3, 20000
-    rows = []
     with Lattice(L, L, L) as lat:
         lat.upload_state(isotropic_state(L, L, L, 20261203))
         lat.set_step(d0)
-        for t in range(1, nsw + 1):
-            kt = 1.5 - 1.2 * (t - 1) / (nsw - 1)
-            out = lat.sweeps(t, t, 1 if t % 100 == 0 else 10 ** 6, g=g, KK=K, kbt=kt, rng="philox", arith="fp64",
-                             seed=128, want_rows=t % 100 == 0)
-            if t % 100 == 0:
-                rows.append(out["rows"][0])
-            if t % 5000 == 0:
-                print(f"L=128 anneal: sweep {t}, kT {kt:.3f}, E/N {rows[-1][6]:.6f}", flush=True)
+        # one call: kT ramps linearly from 1.5 at sweep 1 to 0.3 at sweep 20000
+        # (chll_params.kbt_end; test_anneal_ramp pins it against one call per
+        # sweep and the oracle), a row every 100 sweeps
+        print("L=128 anneal: 20000 sweeps in one call", flush=True)
+        rows = lat.sweeps(1, nsw, 100, g=g, KK=K, kbt=1.5, kbt_end=0.3, rng="philox", arith="fp64", seed=128)["rows"]
         st = lat.download()
     rows = np.array(rows)
     assert rows.shape[0] == nsw // 100 and np.all(np.isfinite(rows))
