@@ -184,3 +184,19 @@ def test_chllsim_missing_data_folder_exits(tmp_path, monkeypatch):
     with pytest.raises(SystemExit):
         ChLLSim(4, 4, 4, kbt=0.7, d=0.3, KK=1.0, g=0.5, rho=0.0, name="x")
 
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 (or a WORLD_SIZE above 1) exits with
+    a message before any GPU work."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and "one GPU" in (r.stdout + r.stderr)
+    env = dict(os.environ, WORLD_SIZE="2")
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py")], capture_output=True, text=True,
+                       timeout=120, env=env)
+    assert r.returncode != 0 and "one GPU" in (r.stdout + r.stderr)
