@@ -158,6 +158,10 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default kernels instead of autotuning (clean profiles)")
     ap.add_argument("--batched-blocks", type=int, default=20, help="timed blocks of 10 sweeps")
+    ap.add_argument("--batched-config", default=None, metavar="BLOCK,CHUNK_MIB,BRANCHES",
+                    help="run the scan with this block size, chunk budget (MiB, 0 = one chunk) and number of graph "
+                         "branches instead of autotuning (e.g. under a profiler, which changes the timings the "
+                         "autotuner compares)")
     ap.add_argument("--batched-obs-blocks", type=int, default=1000,
                     help="blocks of 10 sweeps for the scan's per-slot observables (0: skip); 1000 = the scan "
                          "workload's 10k sweeps")
@@ -658,8 +662,13 @@ def batched_scan(args, stream):
     reps = phase_scan_replicas()
     sites = sum(r["ni"] * r["nj"] * r["nk"] for r in reps)
     Kb, nb = 10, args.batched_blocks
-    log(f"bench: batched scan, {len(reps)} replicas, {sites} sites, autotuning")
-    tune = autotune(reps, bc="open", arith=args.arith, K=Kb, use_cache=False)
+    if args.batched_config:
+        bs, mib, br = (int(v) for v in args.batched_config.split(","))
+        tune = {"best": bs, "chunk": mib << 20, "branches": br, "ms_per_sweep": None, "ms_per_sweep_chunk": None}
+        log(f"bench: batched scan, {len(reps)} replicas, {sites} sites, fixed configuration {tune}")
+    else:
+        log(f"bench: batched scan, {len(reps)} replicas, {sites} sites, autotuning")
+        tune = autotune(reps, bc="open", arith=args.arith, K=Kb, use_cache=False)
     ens = Ensemble(reps, bc="open", arith=args.arith)
     ens.set_stream(stream.cuda_stream)
     ens.set_block(tune["best"])
