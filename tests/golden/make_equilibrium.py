@@ -41,6 +41,11 @@ CASES = [
     # the L=32 workload's couplings (K = 0.5, g = 0, kT = 1) at its full size,
     # with the reference's open boundaries (its sweep has no periodic mode)
     ("eq_32c_K05_g0_T10", (32, 32, 32), 24, 0.0, 0.5, 0.3, 1.0),
+    # two temperatures of the batched scan's first ladder (L = 32, g = 0, K = 1,
+    # 13 temperatures geometric in [0.2, 2]): slots 8 and 11, in its disordered
+    # phase; the GPU test reads them off the ensemble slot by slot
+    ("eq_32c_K1_g0_scan08", (32, 32, 32), 25, 0.0, 1.0, 0.3, float(np.geomspace(0.2, 2.0, 13)[8])),
+    ("eq_32c_K1_g0_scan11", (32, 32, 32), 26, 0.0, 1.0, 0.3, float(np.geomspace(0.2, 2.0, 13)[11])),
 ]
 NSTART, NSWEEPS, NOUT, DISCARD = 10001, 20000, 10, 400
 STREAM_SEED = 0x51E4

@@ -69,3 +69,59 @@ def test_equilibrium_means_match_reference(path, arith):
     sig = np.hypot(blocking_error(p2g), blocking_error(p2r))
     print(f"  P2: gpu {mg:.5f}  reference (stream build) {mr:.5f}  ({(mg - mr) / sig:+.2f} sigma)")
     assert abs(mg - mr) <= 3.0 * sig, ("P2", mg, mr, sig)
+
+
+SCAN_FIXTURES = {8: "eq_32c_K1_g0_scan08", 11: "eq_32c_K1_g0_scan11"}
+
+
+@pytest.mark.parametrize("arith", ["fp64", "compat"])
+def test_scan_ladder_slots_match_reference(arith):
+    """The batched scan's first ladder (13 replicas of 32^3, g = 0, K = 1, kT
+    geometric in [0.2, 2], swaps every 10 sweeps) run as an ensemble for
+    30,000 sweeps from device-generated isotropic states: at temperature slots
+    8 and 11 the series of whichever replica holds the slot gives the reported
+    energy, eta, C_v, chi_c and <P2> within 3 combined errors of the reference's
+    own chains at those temperatures (tests/golden/eq_32c_K1_g0_scan*.npz, the
+    reference's fixed-step chains; the ensemble adapts each slot's step over its
+    first 10,000 sweeps, which are dropped, and the acceptances, which depend
+    on the step, are not compared)."""
+    from chllmodel_amd.ensemble import Ensemble, phase_scan_replicas, series_by_temperature
+
+    reps = [r for r in phase_scan_replicas() if r["ladder"] == 0]
+    assert len(reps) == 13 and reps[0]["ni"] == 32 and reps[0]["g"] == 0.0
+    Kb, nblocks, skip = 10, 3000, 1000 + 400
+    print(f"scan ladder L=32 g=0 {arith}: {nblocks} blocks of {Kb} sweeps", flush=True)
+    ens = Ensemble(reps, bc="open", arith=arith)
+    try:
+        ens.init_state("isotropic", seed=3232)
+        rows = ens.run(nblocks, Kb, seed=0x5CA7, run_index=0, adapt=True, use_graph=True)
+    finally:
+        ens.close()
+    ser = series_by_temperature(rows, reps)
+    N = 32 ** 3
+    for slot, name in SCAN_FIXTURES.items():
+        z = np.load(os.path.join(HERE, "golden", name + ".npz"), allow_pickle=False)
+        ref = z["series"]
+        kbt = float(z["params"][7])
+        assert kbt == reps[slot]["kbt"]
+        rep_at = ser[slot]["replica"][skip:]
+        eref = np.array([rows[skip + t, r, 14] for t, r in enumerate(rep_at)])
+        eta = ser[slot]["eta"][skip:]
+        assert eref.shape[0] == ref.shape[0] == 1600
+        print(f" slot {slot} kT {kbt:.4f}", flush=True)
+        for lab, g_, r_ in (("E", eref, ref[:, 0]), ("eta", eta, ref[:, 1])):
+            mg, mr = g_.mean(), r_.mean()
+            sig = np.hypot(blocking_error(g_), blocking_error(r_))
+            print(f"  {lab}: ensemble {mg:.5f} reference {mr:.5f} ({(mg - mr) / sig:+.2f} sigma)")
+            assert abs(mg - mr) <= 3.0 * sig, (slot, lab, mg, mr, sig)
+        for lab, fn, g_, r_ in (("C_v", cv_per_site, eref, ref[:, 0]), ("chi_c", chi_per_site, eta, ref[:, 1])):
+            (vg, eg), (vr, er) = fn(g_, kbt, N), fn(r_, kbt, N)
+            sig = np.hypot(eg, er)
+            print(f"  {lab}: ensemble {vg:.4f} +- {eg:.4f} reference {vr:.4f} +- {er:.4f} ({(vg - vr) / sig:+.2f} sigma)")
+            assert abs(vg - vr) <= 3.0 * sig, (slot, lab, vg, vr, sig)
+        p2g = np.array([p2_from_q(q) for q in ser[slot]["Q"][skip:]])
+        p2r = np.array([p2_from_q(q) for q in z["q_series"]])
+        mg, mr = p2g.mean(), p2r.mean()
+        sig = np.hypot(blocking_error(p2g), blocking_error(p2r))
+        print(f"  P2: ensemble {mg:.5f} reference (stream build) {mr:.5f} ({(mg - mr) / sig:+.2f} sigma)")
+        assert abs(mg - mr) <= 3.0 * sig, (slot, "P2", mg, mr, sig)
