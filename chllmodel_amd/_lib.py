@@ -164,13 +164,14 @@ def lib() -> ct.CDLL:
     L.chll_lattice_get_kernel.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32, _i32p, _i32p, _i32p]
     L.chll_lattice_set_fusion.argtypes = [ct.c_void_p, ct.c_int32]
     L.chll_lattice_get_fusion.argtypes = [ct.c_void_p, _i32p]
+    L.chll_lattice_set_screen.argtypes = [ct.c_void_p, ct.c_double]
     L.chll_bandwidth_probe.argtypes = [ct.c_int32, ct.c_int64, ct.c_int32, _f64p, _f64p]
     for name in ("chll_configure", "chll_set_seed", "chll_set_stream", "chll_set_uniform_buffer",
                  "chll_lattice_create", "chll_lattice_destroy", "chll_lattice_set_stream",
                  "chll_lattice_upload", "chll_lattice_download", "chll_lattice_init_state", "chll_lattice_set_step",
                  "chll_lattice_get_step", "chll_lattice_sweeps", "chll_lattice_measure", "chll_lattice_sync",
                  "chll_lattice_set_timing", "chll_lattice_timing", "chll_lattice_set_kernel",
-                 "chll_lattice_get_kernel", "chll_lattice_set_fusion", "chll_lattice_get_fusion",
+                 "chll_lattice_get_kernel", "chll_lattice_set_fusion", "chll_lattice_get_fusion", "chll_lattice_set_screen",
                  "chll_bandwidth_probe"):
         getattr(L, name).restype = ct.c_int
     _lib = L

@@ -104,6 +104,7 @@ struct chll_lattice {
     // FP64 sweeps: run each colour-1 site's chirality trial inside the colour-1
     // director launch (needs the one-site director kernel)
     bool fuse = true;
+    double screen = kScreenScale;  // compat_screen's scale for reference-arithmetic sweeps (0: off)
     bool timing = false;
     struct Ev { int kind; hipEvent_t a, b; };
     std::vector<Ev> ev;
@@ -278,6 +279,15 @@ int chll_lattice_get_fusion(chll_lattice *L, int32_t *on)
 {
     if (!L || !on) return fail("null argument");
     *on = L->fuse ? 1 : 0;
+    return 0;
+}
+
+int chll_lattice_set_screen(chll_lattice *L, double scale)
+{
+    if (!L) return fail("null lattice");
+    if (!(scale == 0.0 || (scale >= kScreenScale && scale <= 1.0)))
+        return fail("screen scale must be 0 (off) or in [2^-21, 1] (got %g)", scale);
+    L->screen = scale;
     return 0;
 }
 
@@ -534,6 +544,7 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
     PhaseArgs A{};
     A.g = p->g; A.KK = p->KK; A.kbt = p->kbt; A.inv_kbt = 1.0 / p->kbt; A.inv_kbt_spin = A.inv_kbt;
     A.d = L->d_dev; A.seed = p->seed; A.run_index = p->run_index; A.replica = p->replica;
+    A.screen = L->screen;
     FinArgs FA{};
     FA.counts = L->counts; FA.pend = L->pend; FA.part = L->part; FA.spart = L->spart;
     FA.nblk_meas = L->nblk_meas; FA.n3 = 2 * ns; FA.d = L->d_dev; FA.rows = L->rows_dev;

@@ -505,6 +505,96 @@ __device__ inline double weight_cq(const NB &N, int sigma, const double cqb[6])
     return acc;
 }
 
+// ---------------------------------------------------------------------------
+// Screen for the reference-compatible decisions.  energy_compat rounds its
+// running sum to binary32 at each of its seven steps (six bonds, the field
+// term): r_k = e_{k-1} + t_k, e_k = fl32(r_k), |e_k - r_k| <= 2^-24 |r_k|.
+// Every |r_k| is at most A + |g s|, A = sum_b |t_b| <= sum_b (1 + c_b^2 +
+// |K c_b q_b f_b|), so one site energy lies within 7 2^-24 (A + |g s|) of the
+// exact one, and the binary64 operations (the terms, the difference of the
+// two rounded energies, the direct binary64 difference dE) add below 2^-40
+// of it.  Summed over a trial's two energies as M, the reference's energy
+// difference dE_c is within B = 2^-21 M + 2^-120 of dE (8 2^-24 covers
+// 7 2^-24 and the binary64 terms; 2^-120 covers binary32 underflow).
+//
+// The reference's decision, accept iff dE_c < 0 or u <= exp(RN(-dE_c / kT)),
+// is monotone in dE_c for kT > 0: it accepts every dE_c below some threshold.
+// With del = B/kT <= 1/4 and x = -dE/kT bracketed by exp_bracket as
+// lo <= exp(x) <= hi, every admissible dE_c has
+//   exp(x) (1 - del) <= exp(-dE_c / kT) <= exp(x) (1 + 2 del),
+// so u < lo (1 - del) accepts and u > hi (1 + 2 del) rejects, whatever dE_c
+// is (the brackets' 5x margin also covers the few ulp of x and of the
+// binary64 exp).  Returns 1 (accepted), 0 (rejected) or -1 (not decided:
+// the caller evaluates the two rounded energies).  scale = 0 turns the screen
+// off; the library never passes a scale below 2^-21.  The probability of -1
+// is about 2 (del + 2^-17): below 1e-4 per trial at kT >= 0.2 and K, g of
+// order one.
+__device__ inline int compat_screen(double dE, double mag, double scale, double kbt, double inv_kbt, double u)
+{
+    if (!(scale > 0.0 && kbt > 0.0 && mag < 0x1.0p100)) return -1;
+    const double B = fma(scale, mag, 0x1.0p-120);
+    if (dE + B < 0.0) return 1;
+    const double del = B * inv_kbt;
+    if (!(del <= 0.25)) return -1;
+    double lo, hi;
+    exp_bracket(-dE * inv_kbt, lo, hi);
+    if (u < lo * (1.0 - del)) return 1;
+    if (u > hi * fma(2.0, del, 1.0)) return 0;
+    return -1;
+}
+
+// cqf_sum with the screen's magnitude: also returns sum_b |c_b q_b f_b| in mag.
+template <class NB>
+__device__ inline double cqf_sum_mag(const NB &N, int sigma, double nx, double ny, double nz, double &csq, double &mag)
+{
+    double acc = 0.0, c2 = 0.0, a = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        double mx, my, mz;
+        N.get(b, mx, my, mz);
+        const double fb = (b & 1) ? -0.5 * (double)(sigma + N.spin(b)) : 0.5 * (double)(sigma + N.spin(b));
+        const double c = fma(nz, mz, fma(ny, my, nx * mx));
+        double q;
+        if (b < 2) q = fma(ny, mz, -nz * my);
+        else if (b < 4) q = fma(nz, mx, -nx * mz);
+        else q = fma(nx, my, -ny * mx);
+        const double cq = c * q;
+        acc = fma(cq, fb, acc);
+        c2 = fma(c, c, c2);
+        a = fma(fabs(cq), fabs(fb), a);
+    }
+    csq = c2;
+    mag = a;
+    return acc;
+}
+
+// cq_sum with the screen's magnitudes of a chirality flip sigma -> -sigma:
+// the c^2 sum in csq and sum_b |c_b q_b| max(|sigma|, |s_b|) in mag, which is
+// sum_b |c_b q_b| (|f_b(sigma)| + |f_b(-sigma)|).
+template <class NB>
+__device__ inline double cq_sum_mag(const NB &N, int sigma, double nx, double ny, double nz, double &csq,
+                                    double &mag)
+{
+    const double sa = fabs((double)sigma);
+    double acc = 0.0, c2 = 0.0, w = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        double mx, my, mz;
+        N.get(b, mx, my, mz);
+        const double c = fma(nz, mz, fma(ny, my, nx * mx));
+        double q;
+        if (b < 2) q = fma(ny, mz, -nz * my);
+        else if (b < 4) q = fma(nz, mx, -nx * mz);
+        else q = fma(nx, my, -ny * mx);
+        acc = (b & 1) == 0 ? fma(c, q, acc) : fma(-c, q, acc);
+        c2 = fma(c, c, c2);
+        w = fma(fabs(c * q), fmax(sa, fabs((double)N.spin(b))), w);
+    }
+    csq = c2;
+    mag = w;
+    return acc;
+}
+
 // a * b + c rounded once, with c taken from a scalar register pair: for a
 // constant c this is one v_fma_f64 fed by two scalar moves, where the compiler
 // otherwise rebuilds c in vector registers (two v_mov_b32) for a v_fmac_f64.

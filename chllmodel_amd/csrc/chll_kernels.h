@@ -22,7 +22,13 @@ struct PhaseArgs {
     uint32_t run_index, replica, istep;
     uint32_t istep_spin;  // fused chirality colour 0 + next director colour 0: the chirality trial's sweep
     double inv_kbt_spin;  // the same fused launch: RN(1/kbt) of the chirality trial's sweep (kT may ramp)
+    double screen;        // reference arithmetic: compat_screen's scale (0: every trial rounds both energies)
 };
+
+// compat_screen's scale: 2^-21 bounds the binary32 roundings of two site
+// energies (chll_device.h); larger scales only send more trials to the
+// rounded energies.
+constexpr double kScreenScale = 0x1.0p-21;
 
 constexpr int kRowCols = 16;
 

@@ -112,6 +112,15 @@ struct SiteIn {
 // through the trial move.
 __device__ inline void reload_barrier() { __asm__ volatile("" ::: "memory"); }
 
+// Makes x, y, z depend on v without changing them, so that the arithmetic
+// that follows on x, y, z starts after v is complete: a rarely taken branch
+// that evaluates two site energies then holds one energy's temporaries at a
+// time instead of interleaving both.
+__device__ inline void after(double v, double &x, double &y, double &z)
+{
+    __asm__ volatile("" : "+v"(x), "+v"(y), "+v"(z) : "v"(v));
+}
+
 // Director trial (chll/chll_mod.f90:130-200) with neighbour accessor N and the
 // site's director (nx, ny, nz) and spin sg; returns whether it is accepted and
 // the trial director in t[3].
@@ -133,10 +142,22 @@ __device__ inline bool dir_compute(const PhaseArgs &A, int colour, uint32_t m, c
         // trial (trial_fp64): its trial director agrees with the reference's
         // operation order (chll/chll_mod.f90:149-183, IEEE divisions and
         // square roots) to a few ulp, which the energies' binary32 roundings
-        // absorb; the reference's order cost a fifth of the director kernel
-        const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
+        // absorb; the reference's order cost a fifth of the director kernel.
+        // The binary64 difference decides unless the energies' roundings
+        // could change the outcome (compat_screen); those trials evaluate
+        // the rounded energies.
+        double c2o, c2n, mo, mn;
+        const double ao = cqf_sum_mag(N, sg, nx, ny, nz, c2o, mo);
         trial_fp64(nx, ny, nz, d, u_phi, t[0], t[1], t[2]);
         reload_barrier();
+        const double an = cqf_sum_mag(N, sg, t[0], t[1], t[2], c2n, mn);
+        const double dE = (c2o - c2n) - A.KK * (an - ao);
+        const double M = (12.0 + (c2o + c2n)) + fabs(A.KK) * (mo + mn) + 2.0 * fabs(A.g * (double)sg);
+        const int sc = compat_screen(dE, M, A.screen, A.kbt, A.inv_kbt, u_acc);
+        if (sc >= 0) return sc != 0;
+        reload_barrier();
+        const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
+        after(eold, t[0], t[1], t[2]);
         const double enew = energy_compat(N, t[0], t[1], t[2], sg, A.g, A.KK);
         return (enew < eold) || accept_exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt), u_acc);
     } else {
@@ -162,8 +183,17 @@ __device__ inline bool spin_compute(const PhaseArgs &A, int colour, uint32_t m, 
         u_acc = A.u_acc[m];
     }
     if (ARITH == ARITH_COMPAT) {
+        double csq, w;
+        const double a = cq_sum_mag(N, sg, nx, ny, nz, csq, w);
+        const double dE = (double)sg * fma(A.KK, a, 2.0 * A.g);
+        const double M = fma(2.0, csq, 12.0) + fabs(A.KK) * w + 2.0 * fabs(A.g * (double)sg);
+        const int sc = compat_screen(dE, M, A.screen, A.kbt, A.inv_kbt, u_acc);
+        if (sc >= 0) return sc != 0;
+        reload_barrier();
         const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
-        const double enew = energy_compat(N, nx, ny, nz, -sg, A.g, A.KK);
+        double px = nx, py = ny, pz = nz;
+        after(eold, px, py, pz);
+        const double enew = energy_compat(N, px, py, pz, -sg, A.g, A.KK);
         return (enew < eold) || accept_exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt), u_acc);
     } else {
         const double a = cq_sum(N, nx, ny, nz);

@@ -105,6 +105,14 @@ class Lattice:
         check(lib().chll_lattice_get_fusion(self._h, ct.byref(v)))
         return bool(v.value)
 
+    def set_screen(self, scale: float) -> None:
+        """Reference arithmetic: decide each trial from its binary64 energy
+        difference when the binary32 roundings of the two site energies
+        (at most scale x their magnitude, 2^-21 by default) cannot change the
+        outcome; 0 rounds both energies of every trial.  Results are the same
+        at every scale."""
+        check(lib().chll_lattice_set_screen(self._h, float(scale)))
+
     def sync(self) -> None:
         check(lib().chll_lattice_sync(self._h))
 

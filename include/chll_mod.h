@@ -175,6 +175,16 @@ int chll_lattice_get_kernel(chll_lattice *lat, int32_t kind, int32_t arith, int3
  * the four half-sweeps in turn). */
 int chll_lattice_set_fusion(chll_lattice *lat, int32_t on);
 int chll_lattice_get_fusion(chll_lattice *lat, int32_t *on);
+/* Reference arithmetic (CHLL_ARITH_COMPAT): each trial is first decided from
+ * its binary64 energy difference, which lies within scale x M of the
+ * difference of the reference's binary32-rounded site energies (M bounds the
+ * two energies' terms; scale 2^-21, the default, covers the roundings of
+ * chll/chll_mod.f90:363-412); only trials whose outcome that margin leaves
+ * open evaluate the rounded energies.  The decisions, and so the lattice,
+ * counts and rows, are the reference arithmetic's at every scale.  scale = 0
+ * rounds both energies of every trial; scales in (2^-21, 1] send more trials
+ * to them; others are rejected.  No counterpart in the reference. */
+int chll_lattice_set_screen(chll_lattice *lat, double scale);
 /* Per-kernel HIP-event timing: kinds 0 director, 1 chirality, 2 measure,
  * 3 finalize, 4 uniform fill, 5 fused colour-1 director + chirality, 6 fused
  * colour-0 chirality + next sweep's colour-0 director. */

@@ -482,3 +482,61 @@ def test_anneal_ramp_rejects_bad_end():
         for bad in (-1.0, float("inf"), float("nan")):
             with pytest.raises(_lib.ChllError, match="kbt_end"):
                 lat.sweeps(1, 3, 0, g=0.5, KK=1.0, kbt=1.0, kbt_end=bad, arith="fp64")
+
+
+# dims, bc, rng, K, g, kT, rough start (directors of lengths 0.5-2, spins in
+# [-127, 127]), sweeps
+SCREEN = [((32, 32, 16), "open", "philox", 1.0, 0.5, 0.2, False, 12),
+          ((16, 16, 16), "periodic", "philox", 4.0, -1.5, 0.05, True, 12),
+          ((12, 12, 8), "open", "stream", 1.5, 0.25, 0.7, True, 8),
+          ((64, 64, 32), "open", "philox", 1.0, 0.5, 1.1, False, 6),
+          ((16, 16, 8), "open", "philox", 1.0, 0.5, 0.0, False, 6)]
+
+
+@pytest.mark.parametrize("dims,bc,rng,K,g,kT,rough,nsw", SCREEN,
+                         ids=[f"{'x'.join(map(str, c[0]))}-{c[1]}-{c[2]}-K{c[3]}-kT{c[5]}{'-rough' if c[6] else ''}"
+                              for c in SCREEN])
+def test_compat_screen_scales_agree(dims, bc, rng, K, g, kT, rough, nsw):
+    """Reference arithmetic decides a trial from its binary64 energy
+    difference where the binary32 roundings of the two site energies cannot
+    change the outcome, and evaluates the rounded energies otherwise
+    (compat_screen).  With the screen off (every trial rounds both energies),
+    at its default scale 2^-21, and at 2^-11 and 2^-5 (a few per cent of the
+    trials, then nearly all, take the rounded energies, so waves mix both
+    paths) the lattice, cone step, counts and rows are the same bit for bit.
+    Rough starts have directors of any length and spins up to +-127, which
+    the reference's arithmetic accepts; kT = 0 is a quench, where the screen
+    decides nothing."""
+    rs = np.random.RandomState(sum(dims) + int(100 * kT))
+    st = isotropic_state(*dims, 7000 + sum(dims))
+    if rough:
+        for f in ("nx", "ny", "nz"):
+            st[f] = np.asfortranarray(st[f] * rs.uniform(0.5, 2.0, size=dims))
+        st["s"] = np.asfortranarray(rs.randint(-127, 128, size=dims).astype(np.int32))
+    print(f"compat screen dims={dims} bc={bc} rng={rng} K={K} g={g} kT={kT} rough={rough}", flush=True)
+    res = []
+    for scale in (0.0, 2.0 ** -21, 2.0 ** -11, 2.0 ** -5):
+        with Lattice(*dims, bc=bc) as lat:
+            lat.upload_state(st)
+            lat.set_step(0.3)
+            lat.set_screen(scale)
+            out = lat.sweeps(1, nsw, 3, g=g, KK=K, kbt=kT, arith="compat", rng=rng, seed=0x5C4EE, run_index=1,
+                             want_counts=True, adapt=True)
+            res.append((lat.download(), lat.get_step(), out["counts"], out["rows"]))
+    base = res[0]
+    for state, d, counts, rows in res[1:]:
+        for f in ("nx", "ny", "nz", "s"):
+            assert np.array_equal(state[f], base[0][f]), f
+        assert d == base[1]
+        assert np.array_equal(counts, base[2]) and np.array_equal(rows, base[3])
+    c = np.asarray(base[2])
+    assert c.sum() > 0 and c.min() < dims[0] * dims[1] * dims[2]  # decisions went both ways
+
+
+def test_compat_screen_scale_validation():
+    with Lattice(8, 8, 4) as lat:
+        for bad in (2.0 ** -22, -1.0, 2.0, float("nan"), float("inf")):
+            with pytest.raises(_lib.ChllError, match="screen"):
+                lat.set_screen(bad)
+        for ok in (0.0, 2.0 ** -21, 1.0):
+            lat.set_screen(ok)
