@@ -227,7 +227,7 @@ def main():
                 apply(lat, tuned[arith], arith)
         elif args.no_tune:
             tuned[arith] = {k: list(lat.get_kernel(k, arith)) for k in ("director", "chirality")}
-            tuned[arith]["fuse"] = lat.get_fusion() if arith == "fp64" else False
+            tuned[arith]["fuse"] = lat.get_fusion()
             log(f"bench: {arith}: default kernels {tuned[arith]}")
         else:
             tuned[arith] = tune_lattice(L, L, L, bc=args.bc, arith=arith, use_cache=False)

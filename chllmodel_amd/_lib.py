@@ -23,7 +23,7 @@ BC = {"open": 0, "periodic": 1}
 ROW_COLS = 16
 ROW_NAMES = ("istep", "E", "eta", "paccept", "faccept", "d", "E_true", "eta_exact",
              "Qxx", "Qyy", "Qzz", "Qxy", "Qxz", "Qyz", "naccept", "nflip")
-TIMING_KINDS = ("director", "chirality", "measure", "finalize", "fill", "fused", "fused_next")
+TIMING_KINDS = ("director", "chirality", "measure", "finalize", "fill", "fused", "fused_next", "held")
 INIT_KINDS = {"isotropic": 0, "aligned_x": 1, "aligned_xz": 2}
 
 _f64p = ct.POINTER(ct.c_double)
@@ -165,6 +165,7 @@ def lib() -> ct.CDLL:
     L.chll_lattice_set_fusion.argtypes = [ct.c_void_p, ct.c_int32]
     L.chll_lattice_get_fusion.argtypes = [ct.c_void_p, _i32p]
     L.chll_lattice_set_screen.argtypes = [ct.c_void_p, ct.c_double]
+    L.chll_lattice_held_trials.argtypes = [ct.c_void_p, _i64p]
     L.chll_bandwidth_probe.argtypes = [ct.c_int32, ct.c_int64, ct.c_int32, _f64p, _f64p]
     for name in ("chll_configure", "chll_set_seed", "chll_set_stream", "chll_set_uniform_buffer",
                  "chll_lattice_create", "chll_lattice_destroy", "chll_lattice_set_stream",
@@ -172,7 +173,7 @@ def lib() -> ct.CDLL:
                  "chll_lattice_get_step", "chll_lattice_sweeps", "chll_lattice_measure", "chll_lattice_sync",
                  "chll_lattice_set_timing", "chll_lattice_timing", "chll_lattice_set_kernel",
                  "chll_lattice_get_kernel", "chll_lattice_set_fusion", "chll_lattice_get_fusion", "chll_lattice_set_screen",
-                 "chll_bandwidth_probe"):
+                 "chll_lattice_held_trials", "chll_bandwidth_probe"):
         getattr(L, name).restype = ct.c_int
     _lib = L
     return L

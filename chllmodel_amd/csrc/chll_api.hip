@@ -72,7 +72,7 @@ bool g_cfg_env_read = false;
 
 enum {
     KIND_DIR = 0, KIND_SPIN = 1, KIND_MEASURE = 2, KIND_FINAL = 3, KIND_FILL = 4, KIND_FUSED = 5, KIND_FUSED_NEXT = 6,
-    KIND_COUNT = 7
+    KIND_HELD = 7, KIND_COUNT = 8
 };
 
 struct chll_lattice {
@@ -105,6 +105,11 @@ struct chll_lattice {
     // director launch (needs the one-site director kernel)
     bool fuse = true;
     double screen = kScreenScale;  // compat_screen's scale for reference-arithmetic sweeps (0: off)
+    // reference-arithmetic fused sweeps (held chirality trials): flip bitmap,
+    // queue of held trials, and the count of trials held since creation
+    unsigned long long *flipbits = nullptr;
+    uint32_t *held = nullptr;
+    unsigned long long *held_total = nullptr;
     bool timing = false;
     struct Ev { int kind; hipEvent_t a, b; };
     std::vector<Ev> ev;
@@ -234,11 +239,12 @@ int chll_lattice_create(int32_t ni, int32_t nj, int32_t nk, int32_t bc, chll_lat
                 // last sweep, measurement sweeps) on site pairs too, which
                 // read the state once where the k-march re-reads its slab
                 // halos (1.6x).  Compat: site pairs in bands for both kinds
-                // (5 % faster sweeps at L = 256 than the k-march chirality).
+                // (5 % faster sweeps at L = 256 than the k-march chirality),
+                // which its fused sweeps need (compat_fused).
                 // Without pairs: one-site kernels in bands and the k-march.
                 const bool pairs = pairs_ok(G);
                 const bool march = !pairs && thick && kind == 1;
-                const int order = pairs ? (ar == ARITH_FP64 ? 7 : 3) : 1;
+                const int order = pairs ? 7 : 1;
                 L->kcfg[kind][ar] = {march ? 1 : 0, march ? tile : order, ns};
                 // odd ni: rows hold (ni +- 1) / 2 sites of a colour, so only the
                 // one-site kernels in lattice order apply
@@ -311,6 +317,7 @@ int chll_lattice_destroy(chll_lattice *L)
     dfree(L->stage); dfree(L->stage_s); dfree(L->bad); dfree(L->counts); dfree(L->pend);
     dfree(L->part); dfree(L->spart); dfree(L->rows_dev); dfree(L->d_dev);
     dfree(L->ustream); dfree(L->counts_dev);
+    dfree(L->flipbits); dfree(L->held); dfree(L->held_total);
     if (L->pinned) (void)hipHostFree(L->pinned);
     for (auto &e : L->ev) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     if (L->own) (void)hipStreamDestroy(L->own);
@@ -330,6 +337,22 @@ static int ensure_stage(chll_lattice *L)
     const size_t n3 = 2 * (size_t)L->G.nsub;
     if (!L->stage && dalloc(&L->stage, n3)) return -1;
     if (!L->stage_s && dalloc(&L->stage_s, n3)) return -1;
+    return 0;
+}
+
+// The flip bitmap and the queue of held trials (one slot per colour-1 site),
+// allocated on a lattice's first fused reference-arithmetic sweep.
+static int ensure_held(chll_lattice *L)
+{
+    if (L->held) return 0;
+    if (!L->flipbits && dalloc(&L->flipbits, flipbit_words(L->G))) return -1;
+    if (!L->held_total && dalloc(&L->held_total, 1)) return -1;
+    if (dalloc(&L->held, 1 + (size_t)L->G.nsub)) return -1;
+    if (hipMemset(L->held, 0, sizeof(uint32_t)) != hipSuccess ||
+        hipMemset(L->held_total, 0, sizeof(unsigned long long)) != hipSuccess) {
+        dfree(L->held);
+        return fail("hipMemset of the held-trial queue failed");
+    }
     return 0;
 }
 
@@ -424,6 +447,18 @@ int chll_lattice_set_timing(chll_lattice *L, int32_t on)
     return 0;
 }
 
+int chll_lattice_held_trials(chll_lattice *L, int64_t *n)
+{
+    if (!L || !n) return fail("null argument");
+    *n = 0;
+    if (!L->held_total) return 0;
+    unsigned long long v = 0;
+    HIPCHECK(hipMemcpyAsync(&v, L->held_total, sizeof(v), hipMemcpyDeviceToHost, L->stream));
+    HIPCHECK(hipStreamSynchronize(L->stream));
+    *n = (int64_t)v;
+    return 0;
+}
+
 int chll_lattice_timing(chll_lattice *L, double *ms, int64_t *launches, int32_t nkinds)
 {
     if (!L) return fail("null lattice");
@@ -503,6 +538,17 @@ static bool fused_sweep(const chll_lattice *L, int arith)
     return arith == ARITH_FP64 && L->fuse && L->kcfg[0][ARITH_FP64].variant == 0;
 }
 
+// Reference arithmetic fuses colour 1's chirality trials into its director
+// launch through held trials (chll_site.h): it needs the screen, a positive
+// kT at every sweep (the screen decides nothing at kT <= 0), and site pairs
+// for the director and chirality kernels.
+static bool compat_fused(const chll_lattice *L, const chll_params *p)
+{
+    const auto &kd = L->kcfg[0][ARITH_COMPAT], &ks = L->kcfg[1][ARITH_COMPAT];
+    return p->arith == ARITH_COMPAT && L->fuse && L->screen > 0.0 && p->kbt > 0.0 && kd.variant == 0 &&
+           ks.variant == 0 && compat_fused_ok(L->G, kd.tile, ks.tile);
+}
+
 int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t nstop, int32_t nout,
                         double *rows, int32_t max_rows, int64_t *counts)
 {
@@ -543,6 +589,7 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
     };
     PhaseArgs A{};
     A.g = p->g; A.KK = p->KK; A.kbt = p->kbt; A.inv_kbt = 1.0 / p->kbt; A.inv_kbt_spin = A.inv_kbt;
+    A.kbt_spin = A.kbt;
     A.d = L->d_dev; A.seed = p->seed; A.run_index = p->run_index; A.replica = p->replica;
     A.screen = L->screen;
     FinArgs FA{};
@@ -554,14 +601,17 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
     const int rngk = p->rng == RNG_PHILOX ? RNG_PHILOX : RNG_STREAM;
     // Sweep schedule.  Reference order per sweep: director colour 0 (D0),
     // director colour 1 (D1), chirality colour 0 (C0), chirality colour 1 (C1),
-    // then the finalize kernel.  FP64 with the one-site director kernel:
-    // C1 runs inside the D1 launch; with Philox draws also C0 of sweep t and
-    // D0 of sweep t + 1 share a launch, after a finalize that closes sweep t's
-    // director accepts (and so its cone-step update).  Sweep t's flips are then
-    // closed by the next finalize.  Measurement sweeps and the last sweep of
-    // the call run their C0 alone, so the state they leave is complete.
-    const bool fuse1 = fused_sweep(L, p->arith);
+    // then the finalize kernel.  Fused: C1 runs inside the D1 launch (in
+    // reference arithmetic with its open trials held and decided before C0
+    // runs); with Philox draws also C0 of sweep t and D0 of sweep t + 1 share
+    // a launch, after a finalize that closes sweep t's director accepts (and
+    // so its cone-step update).  Sweep t's flips are then closed by the next
+    // finalize.  Measurement sweeps and the last sweep of the call run their
+    // C0 alone, so the state they leave is complete.
+    const bool cfuse = compat_fused(L, p);
+    const bool fuse1 = fused_sweep(L, p->arith) || cfuse;
     const bool fuse2 = fuse1 && p->rng == RNG_PHILOX;
+    if (cfuse && ensure_held(L)) return -1;
     const auto grp = [&](int g) { return L->counts + (size_t)g * kSlots * kSlotStride; };
     bool d0_done = false, prev_open = false;
     for (int64_t t = 0; t < nsw; ++t) {
@@ -583,6 +633,7 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
             A.kbt = kt_of(istep);
             A.inv_kbt = 1.0 / A.kbt;
             A.inv_kbt_spin = A.inv_kbt;
+            A.kbt_spin = A.kbt;
         }
         const auto set_u = [&](int kind, int colour) {
             if (p->rng == RNG_PHILOX) return;
@@ -608,10 +659,28 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
         if (!d0_done && half(0, 0, grp(kGroupAcc0))) return -1;
         if (fuse1) {
             set_u(0, 1);
+            if (cfuse) {
+                // colour 1's flips of this sweep go to the bitmap, which C0
+                // of this sweep reads; its open trials to the queue
+                A.flipbits = L->flipbits;
+                A.held = L->held;
+            }
             if (tbegin(L, KIND_FUSED, &ti)) return -1;
-            launch_phase_fused(rngk, L->bc, G, L->F, 1, A, grp(kGroupAcc1), grp(flip_group(istep, 1)),
+            launch_phase_fused(p->arith, rngk, L->bc, G, L->F, 1, A, grp(kGroupAcc1), grp(flip_group(istep, 1)),
                                L->kcfg[0][p->arith].tile, st);
             if (launch_check() || tend(L, ti)) return -1;
+            if (cfuse) {
+                // the held trials, decided on colour 0's chirality outcomes of
+                // this sweep, before any colour-0 launch
+                PhaseArgs A0 = A, A1 = A;
+                if (p->rng != RNG_PHILOX) {
+                    A0.u_acc = L->ustream + 4 * ns;
+                    A1.u_acc = L->ustream + 5 * ns;
+                }
+                if (tbegin(L, KIND_HELD, &ti)) return -1;
+                launch_held(rngk, L->bc, G, L->F, A0, A1, grp(flip_group(istep, 1)), L->held_total, st);
+                if (launch_check() || tend(L, ti)) return -1;
+            }
         } else if (half(0, 1, grp(kGroupAcc1))) {
             return -1;
         }
@@ -634,8 +703,9 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
             A2.kbt = kt_of(istep + 1);
             A2.inv_kbt = 1.0 / A2.kbt;
             A2.inv_kbt_spin = A.inv_kbt;
+            A2.kbt_spin = A.kbt;
             if (tbegin(L, KIND_FUSED_NEXT, &ti)) return -1;
-            launch_phase_fused_next(L->bc, G, L->F, A2, grp(kGroupAcc0), grp(flip_group(istep, 0)),
+            launch_phase_fused_next(p->arith, L->bc, G, L->F, A2, grp(kGroupAcc0), grp(flip_group(istep, 0)),
                                     L->kcfg[0][p->arith].tile, st);
             if (launch_check() || tend(L, ti)) return -1;
             d0_done = true;

@@ -90,6 +90,7 @@ __global__ __launch_bounds__(BS) void k_phase_ens(EnsDev E, int colour, uint32_t
     PhaseArgs A;
     A.g = D.g; A.KK = D.KK; A.kbt = E.kbt[s]; A.inv_kbt = E.inv_kbt[s]; A.inv_kbt_spin = A.inv_kbt; A.d = E.d + s;
     A.screen = kScreenScale;
+    A.kbt_spin = A.kbt; A.flipbits = nullptr; A.held = nullptr;
     A.u_phi = nullptr; A.u_acc = nullptr; A.u_spin = nullptr;
     A.seed = E.seed; A.run_index = E.run_index; A.replica = D.replica; A.istep = *E.istep + istep_off;
     A.istep_spin = A.istep;
@@ -133,6 +134,7 @@ __global__ __launch_bounds__(BS) void k_pair_ens(EnsDev E, uint32_t istep_off, i
     PhaseArgs A;
     A.g = D.g; A.KK = D.KK; A.kbt = E.kbt[s]; A.inv_kbt = E.inv_kbt[s]; A.inv_kbt_spin = A.inv_kbt; A.d = E.d + s;
     A.screen = kScreenScale;
+    A.kbt_spin = A.kbt; A.flipbits = nullptr; A.held = nullptr;
     A.u_phi = nullptr; A.u_acc = nullptr; A.u_spin = nullptr;
     A.seed = E.seed; A.run_index = E.run_index; A.replica = D.replica; A.istep = *E.istep + istep_off;
     A.istep_spin = A.istep;

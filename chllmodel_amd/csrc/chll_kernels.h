@@ -22,8 +22,23 @@ struct PhaseArgs {
     uint32_t run_index, replica, istep;
     uint32_t istep_spin;  // fused chirality colour 0 + next director colour 0: the chirality trial's sweep
     double inv_kbt_spin;  // the same fused launch: RN(1/kbt) of the chirality trial's sweep (kT may ramp)
+    double kbt_spin;      // and its kT (reference arithmetic divides by it)
     double screen;        // reference arithmetic: compat_screen's scale (0: every trial rounds both energies)
+    // Reference arithmetic with colour 1's chirality trials inside its
+    // director launch (held trials, chll_site.h): bit m % 64 of word m / 64
+    // is set when colour-1 packed site m flipped in this sweep; null when
+    // colour 1's chirality trials have not run yet in this sweep.
+    unsigned long long *flipbits;
+    // held[0]: count of colour-1 chirality trials the screen left open in
+    // this sweep; held[1 + e]: their packed indices (bit 31: decided to flip)
+    uint32_t *held;
 };
+
+// Held chirality trials: lanes per held trial in k_held_decide (six colour-0
+// neighbours and the trial itself) and its grid.
+constexpr int kHeldLanes = 8;
+constexpr uint32_t kHeldBlocks = 64;
+constexpr uint32_t kHeldFlip = 0x80000000u;
 
 // compat_screen's scale: 2^-21 bounds the binary32 roundings of two site
 // energies (chll_device.h); larger scales only send more trials to the
@@ -84,14 +99,28 @@ void launch_phase(int kind, int arith, int rng, int bc, const Geom &G, const Fie
                   const PhaseArgs &A, uint32_t *counts, int order, hipStream_t s);
 uint32_t xcd_band(const Geom &G);
 bool pairs_ok(const Geom &G);
-// FP64 only: director trials of `colour` followed at each site by its
-// chirality trial, accepts counted into counts_dir, flips into counts_spin.
-void launch_phase_fused(int rng, int bc, const Geom &G, const Fields &F, int colour, const PhaseArgs &A,
+// Director trials of `colour` followed at each site by its chirality trial,
+// accepts counted into counts_dir, flips into counts_spin.  Reference
+// arithmetic (colour 1, site pairs only: compat_fused_ok): the chirality
+// trials the screen leaves open are held (A.held) and every flip is marked
+// in A.flipbits.
+void launch_phase_fused(int arith, int rng, int bc, const Geom &G, const Fields &F, int colour, const PhaseArgs &A,
                         uint32_t *counts_dir, uint32_t *counts_spin, int order, hipStream_t s);
-// FP64, Philox: chirality trials of colour 0 for sweep A.istep_spin, each
-// followed by the same site's director trial for sweep A.istep.
-void launch_phase_fused_next(int bc, const Geom &G, const Fields &F, const PhaseArgs &A, uint32_t *counts_dir,
-                             uint32_t *counts_spin, int order, hipStream_t s);
+// Philox: chirality trials of colour 0 for sweep A.istep_spin, each followed
+// by the same site's director trial for sweep A.istep (reference arithmetic:
+// site pairs only, after the held trials of colour 1 are decided).
+void launch_phase_fused_next(int arith, int bc, const Geom &G, const Fields &F, const PhaseArgs &A,
+                             uint32_t *counts_dir, uint32_t *counts_spin, int order, hipStream_t s);
+// Whether reference-arithmetic sweeps can fuse on this geometry with these
+// kernel orders (site pairs for both the director and chirality kernels).
+bool compat_fused_ok(const Geom &G, int dir_order, int spin_order);
+// Words of the flip bitmap for a geometry (A.flipbits).
+size_t flipbit_words(const Geom &G);
+// The held chirality trials of colour 1 (A1: sweep t's colour-1 chirality
+// arguments; A0: its colour-0 ones): decide each from colour 0's outcomes,
+// count the flips into counts_spin, then apply them and clear the queue.
+void launch_held(int rng, int bc, const Geom &G, const Fields &F, const PhaseArgs &A0, const PhaseArgs &A1,
+                 uint32_t *counts_spin, unsigned long long *total, hipStream_t s);
 // Whether every field (2 nsub + kFieldPad elements of at most 8 bytes) spans
 // less than 4 GiB, so that kernels may address it with 32-bit byte offsets.
 bool field_off32(const Geom &G);

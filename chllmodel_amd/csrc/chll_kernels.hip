@@ -121,17 +121,21 @@ __global__ __launch_bounds__(kBlock) void k_spin_dir(Geom G, Fields F, PhaseArgs
 
 // Two sites per thread (load_pair: 16-byte loads of site pairs).  OP 0:
 // director trials, 1: chirality trials, 2: director then chirality trials
-// (FP64), 3: chirality trials of sweep A.istep_spin then director trials of
-// sweep A.istep (FP64, Philox, colour 0).  Counts: OP 0 accepts and OP 1 flips
-// into c0; OP 2 and 3 director accepts into c0, flips into c1.  Needs an even
-// half (ni % 4 == 0) and fields below 4 GiB.
+// (colour 1 in reference arithmetic), 3: chirality trials of sweep
+// A.istep_spin then director trials of sweep A.istep (Philox, colour 0),
+// 4: chirality trials of colour 0 in reference arithmetic after colour 1's
+// (spin_c0_compat).  Counts: OP 0 accepts and OP 1 and 4 flips into c0; OP 2
+// and 3 director accepts into c0, flips into c1.  Reference arithmetic, OP 2: each wave writes the
+// flip bits of its 128 sites (two 64-bit words, site m at bit m % 64 of word
+// m / 64) and appends its held trials to A.held.  Needs an even half
+// (ni % 4 == 0) and fields below 4 GiB.
 template <int OP, int ARITH, int RNG, int BC>
 __global__ __launch_bounds__(kBlock) void k_pair(Geom G, Fields F, int colour, PhaseArgs A,
                                              uint32_t *__restrict__ c0, uint32_t *__restrict__ c1,
                                              uint32_t band)
 {
     const uint32_t blk = sweep_block(band), m0 = 2u * (blk * kBlock + threadIdx.x);
-    bool a0 = false, a1 = false, f0 = false, f1 = false;
+    bool a0 = false, a1 = false, f0 = false, f1 = false, h0 = false, h1 = false;
     const double d = OP == 1 ? 0.0 : *A.d;  // the cone step, read once for both sites
     if (blk != kIdleBlock && m0 < G.nsub) {
         SiteIn S0, S1;
@@ -144,18 +148,138 @@ __global__ __launch_bounds__(kBlock) void k_pair(Geom G, Fields F, int colour, P
         } else if constexpr (OP == 1) {
             a0 = spin_apply<ARITH, RNG, true, true>(F, colour, A, m0, own, S0, &O0);
             a1 = spin_apply<ARITH, RNG, true, true>(F, colour, A, m0 + 1u, own + 1u, S1, &O1);
+        } else if constexpr (OP == 4) {
+            a0 = spin_c0_apply<RNG, BC>(G, A, m0, S0, &O0);
+            a1 = spin_c0_apply<RNG, BC>(G, A, m0 + 1u, S1, &O1);
         } else if constexpr (OP == 2) {
-            a0 = dir_spin_apply<RNG, true, true>(F, colour, A, d, m0, own, S0, f0, &O0);
-            a1 = dir_spin_apply<RNG, true, true>(F, colour, A, d, m0 + 1u, own + 1u, S1, f1, &O1);
+            if constexpr (ARITH == ARITH_COMPAT) {
+                a0 = dir_spin_held<RNG>(A, d, m0, S0, f0, h0, &O0);
+                a1 = dir_spin_held<RNG>(A, d, m0 + 1u, S1, f1, h1, &O1);
+            } else {
+                a0 = dir_spin_apply<RNG, true, true>(F, colour, A, d, m0, own, S0, f0, &O0);
+                a1 = dir_spin_apply<RNG, true, true>(F, colour, A, d, m0 + 1u, own + 1u, S1, f1, &O1);
+            }
         } else {
-            a0 = spin_dir_apply<true, true>(F, A, d, m0, own, S0, f0, &O0);
-            a1 = spin_dir_apply<true, true>(F, A, d, m0 + 1u, own + 1u, S1, f1, &O1);
+            if constexpr (ARITH == ARITH_COMPAT) {
+                a0 = spin_dir_held<BC>(G, A, d, m0, S0, f0, &O0);
+                a1 = spin_dir_held<BC>(G, A, d, m0 + 1u, S1, f1, &O1);
+            } else {
+                a0 = spin_dir_apply<true, true>(F, A, d, m0, own, S0, f0, &O0);
+                a1 = spin_dir_apply<true, true>(F, A, d, m0 + 1u, own + 1u, S1, f1, &O1);
+            }
         }
-        store_pair<OP != 1, OP != 0>(F, own, O0, O1);
+        store_pair<OP != 1 && OP != 4, OP != 0>(F, own, O0, O1);
+    }
+    if constexpr (OP == 2 && ARITH == ARITH_COMPAT) {
+        const uint64_t b0 = __ballot(f0), b1 = __ballot(f1), q0 = __ballot(h0), q1 = __ballot(h1);
+        const uint32_t lane = threadIdx.x & 63u;
+        if (blk != kIdleBlock) {
+            // wave w of the launch's logical blocks holds sites 128 w .. 128 w + 127
+            const uint32_t w = blk * (kBlock / 64u) + (threadIdx.x >> 6);
+            if (lane == 0u) {
+                typedef unsigned long long u2v __attribute__((ext_vector_type(2)));
+                typedef __attribute__((address_space(1))) u2v gu2v;
+                *(gu2v *)(A.flipbits + 2u * w) =
+                    u2v{spread32((uint32_t)b0) | (spread32((uint32_t)b1) << 1),
+                        spread32((uint32_t)(b0 >> 32)) | (spread32((uint32_t)(b1 >> 32)) << 1)};
+            }
+            if (q0 | q1) {
+                uint32_t base = 0;
+                if (lane == 0u) base = atomicAdd(A.held, (uint32_t)(__popcll(q0) + __popcll(q1)));
+                base = __shfl(base, 0);
+                // the queue holds nsub entries, one per colour-1 site and sweep
+                const uint64_t below = (1ull << lane) - 1ull;
+                const uint32_t e0 = base + (uint32_t)__popcll(q0 & below);
+                const uint32_t e1 = base + (uint32_t)(__popcll(q0) + __popcll(q1 & below));
+                if (h0 && e0 < G.nsub) A.held[1u + e0] = m0;
+                if (h1 && e1 < G.nsub) A.held[1u + e1] = m0 + 1u;
+            }
+        }
     }
     const uint32_t wa = (uint32_t)(__popcll(__ballot(a0)) + __popcll(__ballot(a1)));
     const uint32_t wf = (uint32_t)(__popcll(__ballot(f0)) + __popcll(__ballot(f1)));
-    block_sum2<kBlock>(wa, wf, c0, OP < 2 ? c0 : c1, blockIdx.x);
+    block_sum2<kBlock>(wa, wf, c0, (OP < 2 || OP == 4) ? c0 : c1, blockIdx.x);
+}
+
+// Held chirality trials of colour 1 (chll_site.h), kHeldLanes lanes per
+// trial.  Lane b < 6 computes the chirality outcome of the trial site's
+// colour-0 neighbour b: the trial colour 0's next launch runs, on the same
+// inputs and draw (A0, spin_c0_compat).  Lane 0 then decides the held trial
+// on the colour-0 spins those outcomes leave, as the reference's order has
+// them (A1).  Reads only the state the colour-1 launch left; its flips are
+// marked in the queue and applied by k_held_apply.  The loop over groups of
+// trials is uniform, so every lane of a group takes part in the shuffles.
+template <int RNG, int BC>
+__global__ __launch_bounds__(kBlock) void k_held_decide(Geom G, Fields F, PhaseArgs A0, PhaseArgs A1,
+                                                    uint32_t *__restrict__ held, uint32_t *__restrict__ counts_spin,
+                                                    unsigned long long *__restrict__ total)
+{
+    const uint32_t n = min(held[0], G.nsub);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && total) atomicAdd(total, (unsigned long long)n);
+    const uint32_t lane = threadIdx.x % kHeldLanes, per = gridDim.x * (kBlock / kHeldLanes);
+    uint32_t nflip = 0;
+    for (uint32_t e0 = 0; e0 < n; e0 += per) {
+        const uint32_t e = e0 + blockIdx.x * (kBlock / kHeldLanes) + threadIdx.x / kHeldLanes;
+        const bool live = e < n;
+        const uint32_t m1 = live ? held[1u + e] : 0u;
+        uint32_t nb[6] = {0u, 0u, 0u, 0u, 0u, 0u}, mask = 0u;
+        int after_c0 = 0;
+        if (live) {
+            mask = neighbours<BC>(G, m1, decode(G, m1, 1), 0u, nb);  // packed colour-0 indices
+            if (lane < 6u && ((mask >> lane) & 1u)) {
+                const uint32_t i = nb[lane];
+                SiteIn S;
+                load_site<BC, true>(G, F, 0, i, S);
+                const double u =
+                    RNG == RNG_PHILOX
+                        ? site_u_spin(site_block(A0.seed, A0.run_index, A0.replica, i, spin_sweep(0, A0.istep), 0))
+                        : A0.u_acc[i];
+                int8_t fl[6] = {0, 0, 0, 0, 0, 0};
+                const bool f = spin_c0_compat<BC>(G, A0, A0.kbt, A0.inv_kbt, i, S.N, S.nx, S.ny, S.nz, S.sg, u, fl);
+                after_c0 = f ? -S.sg : S.sg;
+            }
+        }
+        int sb[6];
+#pragma unroll
+        for (int b = 0; b < 6; ++b) sb[b] = __shfl(after_c0, b, kHeldLanes);
+        if (live && lane == 0u) {
+            SiteIn S;
+            load_site<BC, true>(G, F, 1, m1, S);
+#pragma unroll
+            for (int b = 0; b < 6; ++b)
+                if ((mask >> b) & 1u) S.N.s[b] = sb[b];
+            const double u =
+                RNG == RNG_PHILOX
+                    ? site_u_spin(site_block(A1.seed, A1.run_index, A1.replica, m1, spin_sweep(1, A1.istep), 1))
+                    : A1.u_acc[m1];
+            const int sc = spin_screen_compat(A1, A1.kbt, A1.inv_kbt, S.N, S.nx, S.ny, S.nz, S.sg, u);
+            const bool f = sc >= 0 ? sc != 0 : spin_exact_compat(A1, A1.kbt, A1.inv_kbt, S.N, S.nx, S.ny, S.nz, S.sg, u);
+            if (f) {
+                held[1u + e] = m1 | kHeldFlip;
+                ++nflip;
+            }
+        }
+    }
+    if (nflip) atomic_add_global(counts_spin + (blockIdx.x % kSlots) * kSlotStride, nflip);
+}
+
+// Applies the held flips k_held_decide marked (spin and flip bit), then
+// clears the queue.  One block: the queue holds a few per 10^5 trials.
+__global__ __launch_bounds__(kBlock) void k_held_apply(Geom G, Fields F, uint32_t *__restrict__ held,
+                                                   unsigned long long *__restrict__ flipbits)
+{
+    const uint32_t n = min(held[0], G.nsub);
+    for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
+        const uint32_t v = held[1u + e];
+        if (v & kHeldFlip) {
+            const uint32_t m1 = v & ~kHeldFlip;
+            int8_t *p = F.s + G.nsub + m1;
+            *p = (int8_t)(-*p);
+            atomicOr(flipbits + (m1 >> 6), 1ull << (m1 & 63u));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) held[0] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -538,6 +662,8 @@ static void launch_phase_t(int kind, const Geom &G, const Fields &F, int colour,
     if ((order & 2) && pairs_ok(G)) {
         const uint32_t band = pair_band(G, order) | rev;
         if (kind == 0) launch_pair<0, ARITH, RNG, BC>(G, F, colour, A, counts, counts, band, s);
+        else if (ARITH == ARITH_COMPAT && colour == 0 && A.flipbits)
+            launch_pair<4, ARITH_COMPAT, RNG, BC>(G, F, colour, A, counts, counts, band, s);
         else launch_pair<1, ARITH, RNG, BC>(G, F, colour, A, counts, counts, band, s);
         return;
     }
@@ -553,10 +679,15 @@ static void launch_phase_t(int kind, const Geom &G, const Fields &F, int colour,
 }
 
 template <int RNG, int BC>
-static void launch_fused_t(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t *cd,
+static void launch_fused_t(int arith, const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t *cd,
                            uint32_t *cs, int order, hipStream_t s)
 {
     const dim3 b(kBlock);
+    if (arith == ARITH_COMPAT) {
+        // compat_fused_ok: site pairs, colour 1
+        launch_pair<2, ARITH_COMPAT, RNG, BC>(G, F, colour, A, cd, cs, pair_band(G, order), s);
+        return;
+    }
     if ((order & 2) && pairs_ok(G)) {
         launch_pair<2, ARITH_FP64, RNG, BC>(G, F, colour, A, cd, cs, pair_band(G, order), s);
         return;
@@ -567,22 +698,33 @@ static void launch_fused_t(const Geom &G, const Fields &F, int colour, const Pha
     else hipLaunchKernelGGL((k_dir_spin<RNG, BC, false>), g, b, 0, s, G, F, colour, A, cd, cs, band);
 }
 
-void launch_phase_fused(int rng, int bc, const Geom &G, const Fields &F, int colour, const PhaseArgs &A,
+void launch_phase_fused(int arith, int rng, int bc, const Geom &G, const Fields &F, int colour, const PhaseArgs &A,
                         uint32_t *counts_dir, uint32_t *counts_spin, int order, hipStream_t s)
 {
     if (rng == RNG_PHILOX) {
-        if (bc == BC_OPEN) launch_fused_t<RNG_PHILOX, BC_OPEN>(G, F, colour, A, counts_dir, counts_spin, order, s);
-        else launch_fused_t<RNG_PHILOX, BC_PERIODIC>(G, F, colour, A, counts_dir, counts_spin, order, s);
+        if (bc == BC_OPEN)
+            launch_fused_t<RNG_PHILOX, BC_OPEN>(arith, G, F, colour, A, counts_dir, counts_spin, order, s);
+        else launch_fused_t<RNG_PHILOX, BC_PERIODIC>(arith, G, F, colour, A, counts_dir, counts_spin, order, s);
     } else {
-        if (bc == BC_OPEN) launch_fused_t<RNG_STREAM, BC_OPEN>(G, F, colour, A, counts_dir, counts_spin, order, s);
-        else launch_fused_t<RNG_STREAM, BC_PERIODIC>(G, F, colour, A, counts_dir, counts_spin, order, s);
+        if (bc == BC_OPEN)
+            launch_fused_t<RNG_STREAM, BC_OPEN>(arith, G, F, colour, A, counts_dir, counts_spin, order, s);
+        else launch_fused_t<RNG_STREAM, BC_PERIODIC>(arith, G, F, colour, A, counts_dir, counts_spin, order, s);
     }
 }
 
-void launch_phase_fused_next(int bc, const Geom &G, const Fields &F, const PhaseArgs &A, uint32_t *counts_dir,
-                             uint32_t *counts_spin, int order, hipStream_t s)
+void launch_phase_fused_next(int arith, int bc, const Geom &G, const Fields &F, const PhaseArgs &A,
+                             uint32_t *counts_dir, uint32_t *counts_spin, int order, hipStream_t s)
 {
     const dim3 b(kBlock);
+    if (arith == ARITH_COMPAT) {
+        // compat_fused_ok: site pairs; reversed like the FP64 launch
+        const uint32_t band = pair_band(G, order) | ((order & 4) ? 0x80000000u : 0u);
+        if (bc == BC_OPEN)
+            launch_pair<3, ARITH_COMPAT, RNG_PHILOX, BC_OPEN>(G, F, 0, A, counts_dir, counts_spin, band, s);
+        else
+            launch_pair<3, ARITH_COMPAT, RNG_PHILOX, BC_PERIODIC>(G, F, 0, A, counts_dir, counts_spin, band, s);
+        return;
+    }
     if ((order & 2) && pairs_ok(G)) {
         // order bit 2: this launch walks its blocks in reverse, so it starts on
         // the planes the colour-1 launch before it finished with
@@ -604,6 +746,38 @@ void launch_phase_fused_next(int bc, const Geom &G, const Fields &F, const Phase
             hipLaunchKernelGGL((k_spin_dir<BC_PERIODIC, true>), g, b, 0, s, G, F, A, counts_dir, counts_spin, band);
         else
             hipLaunchKernelGGL((k_spin_dir<BC_PERIODIC, false>), g, b, 0, s, G, F, A, counts_dir, counts_spin, band);
+    }
+}
+
+bool compat_fused_ok(const Geom &G, int dir_order, int spin_order)
+{
+    return pairs_ok(G) && (dir_order & 2) && (spin_order & 2);
+}
+
+size_t flipbit_words(const Geom &G)
+{
+    // two words per wave of the pair launch's logical blocks
+    return (size_t)grid1(G.nsub / 2u).x * (kBlock / 64u) * 2u;
+}
+
+template <int RNG, int BC>
+static void launch_held_t(const Geom &G, const Fields &F, const PhaseArgs &A0, const PhaseArgs &A1,
+                          uint32_t *counts_spin, unsigned long long *total, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_held_decide<RNG, BC>), dim3(kHeldBlocks), dim3(kBlock), 0, s, G, F, A0, A1, A1.held,
+                       counts_spin, total);
+    hipLaunchKernelGGL(k_held_apply, dim3(1), dim3(kBlock), 0, s, G, F, A1.held, A1.flipbits);
+}
+
+void launch_held(int rng, int bc, const Geom &G, const Fields &F, const PhaseArgs &A0, const PhaseArgs &A1,
+                 uint32_t *counts_spin, unsigned long long *total, hipStream_t s)
+{
+    if (rng == RNG_PHILOX) {
+        if (bc == BC_OPEN) launch_held_t<RNG_PHILOX, BC_OPEN>(G, F, A0, A1, counts_spin, total, s);
+        else launch_held_t<RNG_PHILOX, BC_PERIODIC>(G, F, A0, A1, counts_spin, total, s);
+    } else {
+        if (bc == BC_OPEN) launch_held_t<RNG_STREAM, BC_OPEN>(G, F, A0, A1, counts_spin, total, s);
+        else launch_held_t<RNG_STREAM, BC_PERIODIC>(G, F, A0, A1, counts_spin, total, s);
     }
 }
 

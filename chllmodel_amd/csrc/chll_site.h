@@ -121,22 +121,14 @@ __device__ inline void after(double v, double &x, double &y, double &z)
     __asm__ volatile("" : "+v"(x), "+v"(y), "+v"(z) : "v"(v));
 }
 
-// Director trial (chll/chll_mod.f90:130-200) with neighbour accessor N and the
-// site's director (nx, ny, nz) and spin sg; returns whether it is accepted and
-// the trial director in t[3].
-template <int ARITH, int RNG, class NB>
-__device__ inline bool dir_compute(const PhaseArgs &A, int colour, uint32_t m, const NB &N, double nx, double ny,
-                                   double nz, int sg, double d, double t[3])
+// Director trial (chll/chll_mod.f90:130-200) with neighbour accessor N, the
+// site's director (nx, ny, nz) and spin sg, and its uniforms u_phi (cone
+// angle) and u_acc (acceptance); returns whether it is accepted and the trial
+// director in t[3].
+template <int ARITH, class NB>
+__device__ inline bool dir_decide(const PhaseArgs &A, const NB &N, double nx, double ny, double nz, int sg, double d,
+                                  double u_phi, double u_acc, double t[3])
 {
-    double u_phi, u_acc;
-    if (RNG == RNG_PHILOX) {
-        const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour);
-        u_phi = site_u_phi(w);
-        u_acc = site_u_dir(w);
-    } else {
-        u_phi = A.u_phi[m];
-        u_acc = A.u_acc[m];
-    }
     if (ARITH == ARITH_COMPAT) {
         // the reference's binary32-rounded energies around the binary64 cone
         // trial (trial_fp64): its trial director agrees with the reference's
@@ -169,6 +161,24 @@ __device__ inline bool dir_compute(const PhaseArgs &A, int colour, uint32_t m, c
         const double dE = (c2o - c2n) - A.KK * (an - ao);
         return (dE < 0.0) || accept_exp(-dE * A.inv_kbt, u_acc);
     }
+}
+
+// The director trial of packed site m of `colour` in sweep A.istep with its
+// own uniforms (dir_decide).
+template <int ARITH, int RNG, class NB>
+__device__ inline bool dir_compute(const PhaseArgs &A, int colour, uint32_t m, const NB &N, double nx, double ny,
+                                   double nz, int sg, double d, double t[3])
+{
+    double u_phi, u_acc;
+    if (RNG == RNG_PHILOX) {
+        const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, colour);
+        u_phi = site_u_phi(w);
+        u_acc = site_u_dir(w);
+    } else {
+        u_phi = A.u_phi[m];
+        u_acc = A.u_acc[m];
+    }
+    return dir_decide<ARITH>(A, N, nx, ny, nz, sg, d, u_phi, u_acc, t);
 }
 
 // Chirality trial (chll/chll_mod.f90:214-243).
@@ -363,6 +373,170 @@ __device__ inline bool spin_dir_apply(const Fields &F, const PhaseArgs &A, doubl
     put_spin<OFF32, DEFER>(F, own, flip, sg, o);
     put_dir<OFF32, DEFER>(F, own, acc, t, S, o);
     return acc;
+}
+
+// ---------------------------------------------------------------------------
+// Reference arithmetic in fused launches: held chirality trials.
+//
+// The reference runs director colour 0, director colour 1, chirality colour 0,
+// chirality colour 1 (chll/chll_mod.f90:70-81), and the rounded site energies
+// of a chirality trial read the neighbours' spins.  The screen
+// (compat_screen) decides a chirality trial from the binary64 difference
+// sigma (K sum_b sgn_b c_b q_b + 2 g) and a magnitude that reads only |s_b|.
+// No flip changes either, so a screened decision of a colour-1 site is the
+// same before or after colour 0's chirality trials, and the site's chirality
+// trial runs right after its director trial, in one launch as in FP64.  A
+// trial the screen leaves open is held: the spin keeps its value, the site
+// joins A.held, and k_held_decide decides it once its six colour-0
+// neighbours' chirality outcomes are computed (each a function of the state
+// the colour-1 launch leaves and of that neighbour's own draw).  k_held_apply
+// then flips it, before colour 0's launch.  Colour 0's chirality trials thus
+// run after colour 1's; the rounded energies of one that the screen leaves
+// open read the colour-1 spins before those flips, recovered through
+// A.flipbits (a flip negates the spin).
+
+// The screen of a reference-arithmetic chirality trial (spin_compute):
+// 1 flip, 0 keep, -1 open.
+template <class NB>
+__device__ inline int spin_screen_compat(const PhaseArgs &A, double kbt, double inv_kbt, const NB &N, double nx,
+                                         double ny, double nz, int sg, double u)
+{
+    double csq, w;
+    const double a = cq_sum_mag(N, sg, nx, ny, nz, csq, w);
+    const double dE = (double)sg * fma(A.KK, a, 2.0 * A.g);
+    const double M = fma(2.0, csq, 12.0) + fabs(A.KK) * w + 2.0 * fabs(A.g * (double)sg);
+    return compat_screen(dE, M, A.screen, kbt, inv_kbt, u);
+}
+
+// The reference's chirality decision from its two rounded energies
+// (chll/chll_mod.f90:214-243), as spin_compute evaluates them.
+template <class NB>
+__device__ inline bool spin_exact_compat(const PhaseArgs &A, double kbt, double inv_kbt, const NB &N, double nx,
+                                         double ny, double nz, int sg, double u)
+{
+    const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
+    double px = nx, py = ny, pz = nz;
+    after(eold, px, py, pz);
+    const double enew = energy_compat(N, px, py, pz, -sg, A.g, A.KK);
+    return (enew < eold) || accept_exp(div_by_recip(-(enew - eold), kbt, inv_kbt), u);
+}
+
+// Chirality trial of colour-0 packed site m in reference arithmetic.  With
+// A.flipbits set, colour 1's chirality trials of this sweep have run, and an
+// open trial reads its colour-1 neighbours' spins from before their flips:
+// N's spins are then changed to those (a caller that runs a director trial
+// after this one on N passes the spins of after the flips, as they were).
+template <int BC>
+__device__ inline bool spin_c0_compat(const Geom &G, const PhaseArgs &A, double kbt, double inv_kbt, uint32_t m,
+                                      Nbr &N, double nx, double ny, double nz, int sg, double u, int8_t flipped[6])
+{
+    const int sc = spin_screen_compat(A, kbt, inv_kbt, N, nx, ny, nz, sg, u);
+    if (sc >= 0) return sc != 0;
+    reload_barrier();
+    if (A.flipbits) {
+        // the site's neighbours recomputed here, from an index the compiler
+        // cannot relate to the caller's: otherwise it keeps the caller's
+        // neighbour indices alive through the trial for this rare branch
+        uint32_t mm = m;
+        __asm__ volatile("" : "+v"(mm));
+        const Site st = decode(G, mm, 0);
+        uint32_t nb[6];
+        neighbours<BC>(G, mm, st, 0u, nb);  // packed colour-1 indices
+        const uint32_t *w32 = (const uint32_t *)A.flipbits;  // bit m % 32 of word m / 32
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            if (((N.mask >> b) & 1u) && ((w32[nb[b] >> 5] >> (nb[b] & 31u)) & 1u)) {
+                N.s[b] = -N.s[b];
+                flipped[b] = 1;
+            }
+        }
+    }
+    return spin_exact_compat(A, kbt, inv_kbt, N, nx, ny, nz, sg, u);
+}
+
+// Colour-0 chirality trial of sweep A.istep alone (reference arithmetic).
+template <int RNG, int BC>
+__device__ inline bool spin_c0_apply(const Geom &G, const PhaseArgs &A, uint32_t m, SiteIn &S, SiteOut *o)
+{
+    const double u = RNG == RNG_PHILOX
+                         ? site_u_spin(site_block(A.seed, A.run_index, A.replica, m, spin_sweep(0, A.istep), 0))
+                         : A.u_acc[m];
+    int8_t fl[6] = {0, 0, 0, 0, 0, 0};
+    const bool flip = spin_c0_compat<BC>(G, A, A.kbt, A.inv_kbt, m, S.N, S.nx, S.ny, S.nz, S.sg, u, fl);
+    o->nx = S.nx;
+    o->ny = S.ny;
+    o->nz = S.nz;
+    o->s = flip ? -S.sg : S.sg;
+    return flip;
+}
+
+// Colour-1 director trial, then its chirality trial on the director it
+// leaves (reference arithmetic): `flip` gets a screened flip, `held` an open
+// trial (the spin keeps its value).  Returns the director decision.
+template <int RNG>
+__device__ inline bool dir_spin_held(const PhaseArgs &A, double d, uint32_t m, const SiteIn &S, bool &flip,
+                                     bool &held, SiteOut *o)
+{
+    double u_phi, u_acc, u_spin;
+    if (RNG == RNG_PHILOX) {
+        const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, 1);
+        u_phi = site_u_phi(w);
+        u_acc = site_u_dir(w);
+        u_spin = site_u_spin(w);
+    } else {
+        u_phi = A.u_phi[m];
+        u_acc = A.u_acc[m];
+        u_spin = A.u_spin[m];
+    }
+    double t[3];
+    const bool acc = dir_decide<ARITH_COMPAT>(A, S.N, S.nx, S.ny, S.nz, S.sg, d, u_phi, u_acc, t);
+    o->nx = acc ? t[0] : S.nx;
+    o->ny = acc ? t[1] : S.ny;
+    o->nz = acc ? t[2] : S.nz;
+    reload_barrier();
+    const int sc = spin_screen_compat(A, A.kbt, A.inv_kbt, S.N, o->nx, o->ny, o->nz, S.sg, u_spin);
+    flip = sc == 1;
+    held = sc < 0;
+    o->s = flip ? -S.sg : S.sg;
+    return acc;
+}
+
+// Colour-0 chirality trial of sweep A.istep_spin, then the director trial of
+// sweep A.istep = A.istep_spin + 1 (reference arithmetic, Philox; both from
+// one block, spin_sweep), after colour 1's held trials are decided: the
+// director trial reads the colour-1 spins after every flip of sweep
+// A.istep_spin, as the reference's next sweep does.  Returns the director
+// decision.
+template <int BC>
+__device__ inline bool spin_dir_held(const Geom &G, const PhaseArgs &A, double d, uint32_t m, SiteIn &S, bool &flip,
+                                     SiteOut *o)
+{
+    const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, 0);
+    int8_t fl[6] = {0, 0, 0, 0, 0, 0};
+    flip = spin_c0_compat<BC>(G, A, A.kbt_spin, A.inv_kbt_spin, m, S.N, S.nx, S.ny, S.nz, S.sg, site_u_spin(w), fl);
+#pragma unroll
+    for (int b = 0; b < 6; ++b)
+        if (fl[b]) S.N.s[b] = -S.N.s[b];  // the director trial reads the spins after colour 1's flips
+    const int sg = flip ? -S.sg : S.sg;
+    double t[3];
+    const bool acc = dir_decide<ARITH_COMPAT>(A, S.N, S.nx, S.ny, S.nz, sg, d, site_u_phi(w), site_u_dir(w), t);
+    o->nx = acc ? t[0] : S.nx;
+    o->ny = acc ? t[1] : S.ny;
+    o->nz = acc ? t[2] : S.nz;
+    o->s = sg;
+    return acc;
+}
+
+// Interleaves the bits of v with zeros (bit b -> bit 2 b).
+__device__ inline unsigned long long spread32(uint32_t v)
+{
+    unsigned long long x = v;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
 }
 
 // Single sites: load, then apply.

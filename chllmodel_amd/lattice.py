@@ -113,6 +113,14 @@ class Lattice:
         at every scale."""
         check(lib().chll_lattice_set_screen(self._h, float(scale)))
 
+    def held_trials(self) -> int:
+        """Reference-arithmetic chirality trials of colour 1 held since the
+        lattice was created: those whose outcome the screen left open in a
+        fused launch, decided once colour 0's chirality outcomes were known."""
+        n = ct.c_int64(0)
+        check(lib().chll_lattice_held_trials(self._h, ct.byref(n)))
+        return int(n.value)
+
     def sync(self) -> None:
         check(lib().chll_lattice_sync(self._h))
 

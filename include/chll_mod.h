@@ -185,9 +185,17 @@ int chll_lattice_get_fusion(chll_lattice *lat, int32_t *on);
  * rounds both energies of every trial; scales in (2^-21, 1] send more trials
  * to them; others are rejected.  No counterpart in the reference. */
 int chll_lattice_set_screen(chll_lattice *lat, double scale);
+/* Reference-arithmetic sweeps run colour 1's chirality trials inside its
+ * director launch where the screen is on, kT > 0 and both kernels take site
+ * pairs (fusion on): a trial whose outcome the screen leaves open is held and
+ * decided after colour 0's chirality outcomes are known, as the reference's
+ * order (chll/chll_mod.f90:70-81) has it.  Writes the number of trials held
+ * on this lattice since its creation.  No counterpart in the reference. */
+int chll_lattice_held_trials(chll_lattice *lat, int64_t *n);
 /* Per-kernel HIP-event timing: kinds 0 director, 1 chirality, 2 measure,
  * 3 finalize, 4 uniform fill, 5 fused colour-1 director + chirality, 6 fused
- * colour-0 chirality + next sweep's colour-0 director. */
+ * colour-0 chirality + next sweep's colour-0 director, 7 held chirality
+ * trials (decision and flips). */
 int chll_lattice_set_timing(chll_lattice *lat, int32_t on);
 int chll_lattice_timing(chll_lattice *lat, double *ms, int64_t *launches, int32_t nkinds);
 

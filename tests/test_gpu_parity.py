@@ -354,7 +354,8 @@ def test_fused_sweep_counts_and_timing(rng):
     draws the colour-0 chirality trials of sweeps 1-4 share launches with the
     director trials of sweeps 2-5 (sweep 5 is measured and last, so its
     chirality colour 0 runs alone).  The lattice, per-sweep counts and rows
-    equal those of four separate half-sweeps; compat sweeps never fuse."""
+    equal those of four separate half-sweeps.  Reference-arithmetic sweeps
+    fuse the same way, each colour-1 launch followed by its held trials."""
     dims = (64, 32, 16)
     st0 = isotropic_state(*dims, 31)
     res = {}
@@ -385,7 +386,11 @@ def test_fused_sweep_counts_and_timing(rng):
         lat.set_timing(True)
         lat.sweeps(1, 2, 0, g=0.5, KK=1.0, kbt=0.7, arith="compat", rng=rng, seed=11, want_rows=False)
         t = lat.timing()
-    assert t["fused"][1] == 0 and t["fused_next"][1] == 0 and t["director"][1] == 4 and t["chirality"][1] == 4
+    launches = {k: t[k][1] for k in ("director", "chirality", "fused", "fused_next", "held")}
+    if rng == "philox":
+        assert launches == {"director": 1, "chirality": 1, "fused": 2, "fused_next": 1, "held": 2}, launches
+    else:
+        assert launches == {"director": 2, "chirality": 2, "fused": 2, "fused_next": 0, "held": 2}, launches
 
 
 def test_pipelined_counts_across_calls():
@@ -514,7 +519,7 @@ def test_compat_screen_scales_agree(dims, bc, rng, K, g, kT, rough, nsw):
             st[f] = np.asfortranarray(st[f] * rs.uniform(0.5, 2.0, size=dims))
         st["s"] = np.asfortranarray(rs.randint(-127, 128, size=dims).astype(np.int32))
     print(f"compat screen dims={dims} bc={bc} rng={rng} K={K} g={g} kT={kT} rough={rough}", flush=True)
-    res = []
+    res, held = [], []
     for scale in (0.0, 2.0 ** -21, 2.0 ** -11, 2.0 ** -5):
         with Lattice(*dims, bc=bc) as lat:
             lat.upload_state(st)
@@ -523,6 +528,15 @@ def test_compat_screen_scales_agree(dims, bc, rng, K, g, kT, rough, nsw):
             out = lat.sweeps(1, nsw, 3, g=g, KK=K, kbt=kT, arith="compat", rng=rng, seed=0x5C4EE, run_index=1,
                              want_counts=True, adapt=True)
             res.append((lat.download(), lat.get_step(), out["counts"], out["rows"]))
+            held.append(lat.held_trials())
+    # with the screen on and kT > 0 the sweeps fuse, and the trials the screen
+    # leaves open are held: at 2^-5 a large share of them
+    print(f"held chirality trials per scale: {held}", flush=True)
+    assert held[0] == 0
+    if kT > 0:
+        assert held[3] > 0
+    else:
+        assert held == [0, 0, 0, 0]
     base = res[0]
     for state, d, counts, rows in res[1:]:
         for f in ("nx", "ny", "nz", "s"):
@@ -531,6 +545,59 @@ def test_compat_screen_scales_agree(dims, bc, rng, K, g, kT, rough, nsw):
         assert np.array_equal(counts, base[2]) and np.array_equal(rows, base[3])
     c = np.asarray(base[2])
     assert c.sum() > 0 and c.min() < dims[0] * dims[1] * dims[2]  # decisions went both ways
+
+
+FUSED_COMPAT = [((32, 32, 16), "open", "philox", 2.0 ** -21, 0.0),
+                ((32, 32, 16), "open", "philox", 2.0 ** -8, 0.0),
+                ((16, 16, 24), "periodic", "philox", 2.0 ** -8, 0.0),
+                ((24, 12, 10), "open", "stream", 2.0 ** -8, 0.0),
+                ((16, 16, 16), "periodic", "stream", 2.0 ** -21, 0.0),
+                ((32, 16, 16), "open", "philox", 2.0 ** -8, 0.25)]
+
+
+@pytest.mark.parametrize("dims,bc,rng,scale,kbt_end", FUSED_COMPAT,
+                         ids=[f"{'x'.join(map(str, c[0]))}-{c[1]}-{c[2]}-s{int(-np.log2(c[3]))}"
+                              f"{'-ramp' if c[4] else ''}" for c in FUSED_COMPAT])
+def test_compat_fused_equals_reference_order(dims, bc, rng, scale, kbt_end):
+    """Reference arithmetic with colour 1's chirality trials in its director
+    launch and held trials (fusion on) against the reference's order of four
+    half-sweeps (fusion off): the same lattice, cone step, per-sweep counts
+    and rows bit for bit, over sweeps split across calls, with measured
+    sweeps inside and at the ends of calls, and with a kT ramp.  At scale
+    2^-8 thousands of trials are held; the unfused lattice holds none."""
+    st = isotropic_state(*dims, 4100 + sum(dims))
+    kw = dict(g=0.5, KK=1.3, kbt=0.6, arith="compat", rng=rng, seed=0xF05ED, run_index=2, adapt=True)
+    if kbt_end:
+        kw["kbt_end"] = kbt_end
+    splits = [(1, 5), (6, 6), (7, 16)] if not kbt_end else [(1, 16)]
+    res = []
+    for fuse in (True, False):
+        print(f"compat fused dims={dims} bc={bc} rng={rng} scale={scale} kbt_end={kbt_end} fuse={fuse}",
+              flush=True)
+        with Lattice(*dims, bc=bc) as lat:
+            lat.set_fusion(fuse)
+            lat.set_screen(scale)
+            lat.upload_state(st)
+            lat.set_step(0.3)
+            lat.set_timing(True)
+            counts, rows = [], []
+            for a, b in splits:
+                o = lat.sweeps(a, b, 4, want_counts=True, **kw)
+                counts.append(o["counts"])
+                rows.append(o["rows"])
+            t = lat.timing()
+            res.append((lat.download(), lat.get_step(), np.concatenate(counts), np.concatenate(rows),
+                        lat.held_trials(), t))
+    on, off = res
+    for f in ("nx", "ny", "nz", "s"):
+        assert np.array_equal(on[0][f], off[0][f]), f
+    assert on[1] == off[1]
+    assert np.array_equal(on[2], off[2]) and np.array_equal(on[3], off[3])
+    assert on[5]["fused"][1] == 16 and on[5]["held"][1] == 16 and off[5]["fused"][1] == 0
+    assert off[4] == 0
+    print(f"held trials: {on[4]}", flush=True)
+    if scale > 2.0 ** -21:
+        assert on[4] > 100
 
 
 def test_compat_screen_scale_validation():
