@@ -274,6 +274,9 @@ def main():
             for name, (ms, cnt) in kt.items():
                 if cnt:
                     kernels[name] = {"launches": cnt, "avg_us": 1e3 * ms / cnt, "total_ms": ms, "sweeps": KT}
+        held = lat.held_trials()
+        if held:
+            log(f"bench: {arith}: {held} chirality trials held over {first + K - 1 + (KT if timing else 0)} sweeps")
         row = lat.measure(args.g, args.K)
         lat.close()
         log(f"bench: {arith}: E_true={row['E_true']:.6f} eta={row['eta_exact']:.6f}")

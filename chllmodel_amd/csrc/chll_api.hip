@@ -347,8 +347,8 @@ static int ensure_held(chll_lattice *L)
     if (L->held) return 0;
     if (!L->flipbits && dalloc(&L->flipbits, flipbit_words(L->G))) return -1;
     if (!L->held_total && dalloc(&L->held_total, 1)) return -1;
-    if (dalloc(&L->held, 1 + (size_t)L->G.nsub)) return -1;
-    if (hipMemset(L->held, 0, sizeof(uint32_t)) != hipSuccess ||
+    if (dalloc(&L->held, kHeldHead + (size_t)L->G.nsub)) return -1;
+    if (hipMemset(L->held, 0, kHeldHead * sizeof(uint32_t)) != hipSuccess ||
         hipMemset(L->held_total, 0, sizeof(unsigned long long)) != hipSuccess) {
         dfree(L->held);
         return fail("hipMemset of the held-trial queue failed");

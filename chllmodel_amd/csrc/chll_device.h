@@ -505,6 +505,80 @@ __device__ inline double weight_cq(const NB &N, int sigma, const double cqb[6])
     return acc;
 }
 
+// cqf_sum_mag with, for a later chirality trial's screen, the chirality sum
+// sum_b sgn_b c_b q_b in cq (cq_sum's operands and order) and sum_b |c_b q_b|
+// in abscq, in the same pass over the bonds.
+template <class NB>
+__device__ inline double cqf_sum_mag_cq(const NB &N, int sigma, double nx, double ny, double nz, double &csq,
+                                        double &mag, double &cq, double &abscq)
+{
+    double acc = 0.0, c2 = 0.0, m = 0.0, a = 0.0, ab = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        double mx, my, mz;
+        N.get(b, mx, my, mz);
+        const double fb = (b & 1) ? -0.5 * (double)(sigma + N.spin(b)) : 0.5 * (double)(sigma + N.spin(b));
+        const double c = fma(nz, mz, fma(ny, my, nx * mx));
+        double q;
+        if (b < 2) q = fma(ny, mz, -nz * my);
+        else if (b < 4) q = fma(nz, mx, -nx * mz);
+        else q = fma(nx, my, -ny * mx);
+        const double p = c * q;
+        acc = fma(p, fb, acc);
+        c2 = fma(c, c, c2);
+        m = fma(fabs(p), fabs(fb), m);
+        a = (b & 1) == 0 ? fma(c, q, a) : fma(-c, q, a);
+        ab += fabs(p);
+    }
+    csq = c2;
+    mag = m;
+    cq = a;
+    abscq = ab;
+    return acc;
+}
+
+// cq_products with the chirality screen's magnitude sum_b |c_b q_b|
+// max(|sigma|, |s_b|) in w; weight_cq_mag then gives cqf_sum of any spin with
+// its magnitude sum_b |c_b q_b f_b|.
+template <class NB>
+__device__ inline double cq_products_mag(const NB &N, int sigma, double nx, double ny, double nz, double &csq,
+                                         double cqb[6], double &w)
+{
+    const double sa = fabs((double)sigma);
+    double c2 = 0.0, a = 0.0, ww = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        double mx, my, mz;
+        N.get(b, mx, my, mz);
+        const double c = fma(nz, mz, fma(ny, my, nx * mx));
+        double q;
+        if (b < 2) q = fma(ny, mz, -nz * my);
+        else if (b < 4) q = fma(nz, mx, -nx * mz);
+        else q = fma(nx, my, -ny * mx);
+        cqb[b] = c * q;
+        c2 = fma(c, c, c2);
+        a = (b & 1) == 0 ? fma(c, q, a) : fma(-c, q, a);
+        ww = fma(fabs(cqb[b]), fmax(sa, fabs((double)N.spin(b))), ww);
+    }
+    csq = c2;
+    w = ww;
+    return a;
+}
+
+template <class NB>
+__device__ inline double weight_cq_mag(const NB &N, int sigma, const double cqb[6], double &mag)
+{
+    double acc = 0.0, m = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const double fb = (b & 1) ? -0.5 * (double)(sigma + N.spin(b)) : 0.5 * (double)(sigma + N.spin(b));
+        acc = fma(cqb[b], fb, acc);
+        m = fma(fabs(cqb[b]), fabs(fb), m);
+    }
+    mag = m;
+    return acc;
+}
+
 // ---------------------------------------------------------------------------
 // Screen for the reference-compatible decisions.  energy_compat rounds its
 // running sum to binary32 at each of its seven steps (six bonds, the field

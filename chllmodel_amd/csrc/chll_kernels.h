@@ -30,14 +30,17 @@ struct PhaseArgs {
     // colour 1's chirality trials have not run yet in this sweep.
     unsigned long long *flipbits;
     // held[0]: count of colour-1 chirality trials the screen left open in
-    // this sweep; held[1 + e]: their packed indices (bit 31: decided to flip)
+    // this sweep; held[1]: blocks of k_held done; held[kHeldHead + e]: their
+    // packed indices (bit 31: decided to flip)
     uint32_t *held;
 };
 
-// Held chirality trials: lanes per held trial in k_held_decide (six colour-0
-// neighbours and the trial itself) and its grid.
+// Held chirality trials: lanes per held trial in k_held (six colour-0
+// neighbours and the trial itself), its grid, the queue's header words and
+// the flag of a decided flip.
 constexpr int kHeldLanes = 8;
 constexpr uint32_t kHeldBlocks = 64;
+constexpr uint32_t kHeldHead = 2;
 constexpr uint32_t kHeldFlip = 0x80000000u;
 
 // compat_screen's scale: 2^-21 bounds the binary32 roundings of two site

@@ -191,8 +191,8 @@ __global__ __launch_bounds__(kBlock) void k_pair(Geom G, Fields F, int colour, P
                 const uint64_t below = (1ull << lane) - 1ull;
                 const uint32_t e0 = base + (uint32_t)__popcll(q0 & below);
                 const uint32_t e1 = base + (uint32_t)(__popcll(q0) + __popcll(q1 & below));
-                if (h0 && e0 < G.nsub) A.held[1u + e0] = m0;
-                if (h1 && e1 < G.nsub) A.held[1u + e1] = m0 + 1u;
+                if (h0 && e0 < G.nsub) A.held[kHeldHead + e0] = m0;
+                if (h1 && e1 < G.nsub) A.held[kHeldHead + e1] = m0 + 1u;
             }
         }
     }
@@ -204,15 +204,17 @@ __global__ __launch_bounds__(kBlock) void k_pair(Geom G, Fields F, int colour, P
 // Held chirality trials of colour 1 (chll_site.h), kHeldLanes lanes per
 // trial.  Lane b < 6 computes the chirality outcome of the trial site's
 // colour-0 neighbour b: the trial colour 0's next launch runs, on the same
-// inputs and draw (A0, spin_c0_compat).  Lane 0 then decides the held trial
-// on the colour-0 spins those outcomes leave, as the reference's order has
-// them (A1).  Reads only the state the colour-1 launch left; its flips are
-// marked in the queue and applied by k_held_apply.  The loop over groups of
-// trials is uniform, so every lane of a group takes part in the shuffles.
+// inputs and draw (A0, spin_c0_compat).  Lane 6 loads the held site's own
+// inputs meanwhile and then decides its trial on the colour-0 spins those
+// outcomes leave, as the reference's order has them (A1).  The decisions
+// read only the state the colour-1 launch left; the last block to finish
+// applies the flips (spin and flip bit) and clears the queue.  The loop over
+// groups of trials is uniform, so every lane of a group takes part in the
+// shuffles.
 template <int RNG, int BC>
-__global__ __launch_bounds__(kBlock) void k_held_decide(Geom G, Fields F, PhaseArgs A0, PhaseArgs A1,
-                                                    uint32_t *__restrict__ held, uint32_t *__restrict__ counts_spin,
-                                                    unsigned long long *__restrict__ total)
+__global__ __launch_bounds__(kBlock) void k_held(Geom G, Fields F, PhaseArgs A0, PhaseArgs A1,
+                                             uint32_t *__restrict__ held, uint32_t *__restrict__ counts_spin,
+                                             unsigned long long *__restrict__ total)
 {
     const uint32_t n = min(held[0], G.nsub);
     if (blockIdx.x == 0 && threadIdx.x == 0 && total) atomicAdd(total, (unsigned long long)n);
@@ -221,30 +223,36 @@ __global__ __launch_bounds__(kBlock) void k_held_decide(Geom G, Fields F, PhaseA
     for (uint32_t e0 = 0; e0 < n; e0 += per) {
         const uint32_t e = e0 + blockIdx.x * (kBlock / kHeldLanes) + threadIdx.x / kHeldLanes;
         const bool live = e < n;
-        const uint32_t m1 = live ? held[1u + e] : 0u;
-        uint32_t nb[6] = {0u, 0u, 0u, 0u, 0u, 0u}, mask = 0u;
+        const uint32_t m1 = live ? held[kHeldHead + e] : 0u;
         int after_c0 = 0;
+        SiteIn S;
+        uint32_t mask = 0u;
         if (live) {
-            mask = neighbours<BC>(G, m1, decode(G, m1, 1), 0u, nb);  // packed colour-0 indices
-            if (lane < 6u && ((mask >> lane) & 1u)) {
-                const uint32_t i = nb[lane];
-                SiteIn S;
-                load_site<BC, true>(G, F, 0, i, S);
-                const double u =
-                    RNG == RNG_PHILOX
-                        ? site_u_spin(site_block(A0.seed, A0.run_index, A0.replica, i, spin_sweep(0, A0.istep), 0))
-                        : A0.u_acc[i];
-                int8_t fl[6] = {0, 0, 0, 0, 0, 0};
-                const bool f = spin_c0_compat<BC>(G, A0, A0.kbt, A0.inv_kbt, i, S.N, S.nx, S.ny, S.nz, S.sg, u, fl);
-                after_c0 = f ? -S.sg : S.sg;
+            if (lane < 6u) {
+                uint32_t nb[6];
+                mask = neighbours<BC>(G, m1, decode(G, m1, 1), 0u, nb);  // packed colour-0 indices
+                if ((mask >> lane) & 1u) {
+                    const uint32_t i = nb[lane];
+                    SiteIn Si;
+                    load_site<BC, true>(G, F, 0, i, Si);
+                    const double u =
+                        RNG == RNG_PHILOX
+                            ? site_u_spin(site_block(A0.seed, A0.run_index, A0.replica, i, spin_sweep(0, A0.istep), 0))
+                            : A0.u_acc[i];
+                    int8_t fl[6] = {0, 0, 0, 0, 0, 0};
+                    const bool f =
+                        spin_c0_compat<BC>(G, A0, A0.kbt, A0.inv_kbt, i, Si.N, Si.nx, Si.ny, Si.nz, Si.sg, u, fl);
+                    after_c0 = f ? -Si.sg : Si.sg;
+                }
+            } else if (lane == 6u) {
+                load_site<BC, true>(G, F, 1, m1, S);
+                mask = S.N.mask;
             }
         }
         int sb[6];
 #pragma unroll
         for (int b = 0; b < 6; ++b) sb[b] = __shfl(after_c0, b, kHeldLanes);
-        if (live && lane == 0u) {
-            SiteIn S;
-            load_site<BC, true>(G, F, 1, m1, S);
+        if (live && lane == 6u) {
 #pragma unroll
             for (int b = 0; b < 6; ++b)
                 if ((mask >> b) & 1u) S.N.s[b] = sb[b];
@@ -255,31 +263,38 @@ __global__ __launch_bounds__(kBlock) void k_held_decide(Geom G, Fields F, PhaseA
             const int sc = spin_screen_compat(A1, A1.kbt, A1.inv_kbt, S.N, S.nx, S.ny, S.nz, S.sg, u);
             const bool f = sc >= 0 ? sc != 0 : spin_exact_compat(A1, A1.kbt, A1.inv_kbt, S.N, S.nx, S.ny, S.nz, S.sg, u);
             if (f) {
-                held[1u + e] = m1 | kHeldFlip;
+                held[kHeldHead + e] = m1 | kHeldFlip;
                 ++nflip;
             }
         }
     }
     if (nflip) atomic_add_global(counts_spin + (blockIdx.x % kSlots) * kSlotStride, nflip);
-}
-
-// Applies the held flips k_held_decide marked (spin and flip bit), then
-// clears the queue.  One block: the queue holds a few per 10^5 trials.
-__global__ __launch_bounds__(kBlock) void k_held_apply(Geom G, Fields F, uint32_t *__restrict__ held,
-                                                   unsigned long long *__restrict__ flipbits)
-{
-    const uint32_t n = min(held[0], G.nsub);
+    // the last block applies every decision: each block's decisions are
+    // released by its fence before it counts itself done, and the last one
+    // acquires them with its own fence
+    __shared__ uint32_t is_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        is_last = atomicAdd(held + 1, 1u) == gridDim.x - 1u;
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
     for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
-        const uint32_t v = held[1u + e];
+        const uint32_t v = __hip_atomic_load(held + kHeldHead + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (v & kHeldFlip) {
-            const uint32_t m1 = v & ~kHeldFlip;
-            int8_t *p = F.s + G.nsub + m1;
+            const uint32_t m = v & ~kHeldFlip;
+            int8_t *p = F.s + G.nsub + m;
             *p = (int8_t)(-*p);
-            atomicOr(flipbits + (m1 >> 6), 1ull << (m1 & 63u));
+            atomicOr(A1.flipbits + (m >> 6), 1ull << (m & 63u));
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) held[0] = 0u;
+    if (threadIdx.x == 0) {
+        held[0] = 0u;
+        held[1] = 0u;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -764,9 +779,8 @@ template <int RNG, int BC>
 static void launch_held_t(const Geom &G, const Fields &F, const PhaseArgs &A0, const PhaseArgs &A1,
                           uint32_t *counts_spin, unsigned long long *total, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_held_decide<RNG, BC>), dim3(kHeldBlocks), dim3(kBlock), 0, s, G, F, A0, A1, A1.held,
-                       counts_spin, total);
-    hipLaunchKernelGGL(k_held_apply, dim3(1), dim3(kBlock), 0, s, G, F, A1.held, A1.flipbits);
+    hipLaunchKernelGGL((k_held<RNG, BC>), dim3(kHeldBlocks), dim3(kBlock), 0, s, G, F, A0, A1, A1.held, counts_spin,
+                       total);
 }
 
 void launch_held(int rng, int bc, const Geom &G, const Fields &F, const PhaseArgs &A0, const PhaseArgs &A1,

@@ -387,10 +387,10 @@ __device__ inline bool spin_dir_apply(const Fields &F, const PhaseArgs &A, doubl
 // same before or after colour 0's chirality trials, and the site's chirality
 // trial runs right after its director trial, in one launch as in FP64.  A
 // trial the screen leaves open is held: the spin keeps its value, the site
-// joins A.held, and k_held_decide decides it once its six colour-0
-// neighbours' chirality outcomes are computed (each a function of the state
-// the colour-1 launch leaves and of that neighbour's own draw).  k_held_apply
-// then flips it, before colour 0's launch.  Colour 0's chirality trials thus
+// joins A.held, and k_held decides it once its six colour-0 neighbours'
+// chirality outcomes are computed (each a function of the state the colour-1
+// launch leaves and of that neighbour's own draw), then flips it, before
+// colour 0's launch.  Colour 0's chirality trials thus
 // run after colour 1's; the rounded energies of one that the screen leaves
 // open read the colour-1 spins before those flips, recovered through
 // A.flipbits (a flip negates the spin).
@@ -421,18 +421,16 @@ __device__ inline bool spin_exact_compat(const PhaseArgs &A, double kbt, double 
     return (enew < eold) || accept_exp(div_by_recip(-(enew - eold), kbt, inv_kbt), u);
 }
 
-// Chirality trial of colour-0 packed site m in reference arithmetic.  With
-// A.flipbits set, colour 1's chirality trials of this sweep have run, and an
-// open trial reads its colour-1 neighbours' spins from before their flips:
-// N's spins are then changed to those (a caller that runs a director trial
-// after this one on N passes the spins of after the flips, as they were).
+// The rounded-energy decision of a chirality trial of colour-0 packed site m
+// (reference arithmetic).  With A.flipbits set, colour 1's chirality trials of
+// this sweep have run, and the trial reads its colour-1 neighbours' spins from
+// before their flips: N's spins are changed to those, and flipped[b] marks
+// the ones changed (a caller that runs a director trial on N afterwards
+// changes them back).
 template <int BC>
-__device__ inline bool spin_c0_compat(const Geom &G, const PhaseArgs &A, double kbt, double inv_kbt, uint32_t m,
-                                      Nbr &N, double nx, double ny, double nz, int sg, double u, int8_t flipped[6])
+__device__ inline bool spin_c0_exact(const Geom &G, const PhaseArgs &A, double kbt, double inv_kbt, uint32_t m,
+                                     Nbr &N, double nx, double ny, double nz, int sg, double u, int8_t flipped[6])
 {
-    const int sc = spin_screen_compat(A, kbt, inv_kbt, N, nx, ny, nz, sg, u);
-    if (sc >= 0) return sc != 0;
-    reload_barrier();
     if (A.flipbits) {
         // the site's neighbours recomputed here, from an index the compiler
         // cannot relate to the caller's: otherwise it keeps the caller's
@@ -454,6 +452,18 @@ __device__ inline bool spin_c0_compat(const Geom &G, const PhaseArgs &A, double 
     return spin_exact_compat(A, kbt, inv_kbt, N, nx, ny, nz, sg, u);
 }
 
+// Chirality trial of colour-0 packed site m in reference arithmetic: the
+// screen, else spin_c0_exact.
+template <int BC>
+__device__ inline bool spin_c0_compat(const Geom &G, const PhaseArgs &A, double kbt, double inv_kbt, uint32_t m,
+                                      Nbr &N, double nx, double ny, double nz, int sg, double u, int8_t flipped[6])
+{
+    const int sc = spin_screen_compat(A, kbt, inv_kbt, N, nx, ny, nz, sg, u);
+    if (sc >= 0) return sc != 0;
+    reload_barrier();
+    return spin_c0_exact<BC>(G, A, kbt, inv_kbt, m, N, nx, ny, nz, sg, u, flipped);
+}
+
 // Colour-0 chirality trial of sweep A.istep alone (reference arithmetic).
 template <int RNG, int BC>
 __device__ inline bool spin_c0_apply(const Geom &G, const PhaseArgs &A, uint32_t m, SiteIn &S, SiteOut *o)
@@ -470,9 +480,37 @@ __device__ inline bool spin_c0_apply(const Geom &G, const PhaseArgs &A, uint32_t
     return flip;
 }
 
+// Reference-arithmetic director trial (dir_decide) given the old director's
+// screen sums: ao (cqf_sum), c2o and mo (its magnitude).
+template <class NB>
+__device__ inline bool dir_decide_compat(const PhaseArgs &A, const NB &N, double nx, double ny, double nz, int sg,
+                                         double d, double u_phi, double u_acc, double ao, double c2o, double mo,
+                                         double t[3])
+{
+    double c2n, mn;
+    trial_fp64(nx, ny, nz, d, u_phi, t[0], t[1], t[2]);
+    reload_barrier();
+    const double an = cqf_sum_mag(N, sg, t[0], t[1], t[2], c2n, mn);
+    const double dE = (c2o - c2n) - A.KK * (an - ao);
+    const double M = (12.0 + (c2o + c2n)) + fabs(A.KK) * (mo + mn) + 2.0 * fabs(A.g * (double)sg);
+    const int sc = compat_screen(dE, M, A.screen, A.kbt, A.inv_kbt, u_acc);
+    if (sc >= 0) return sc != 0;
+    reload_barrier();
+    const double eold = energy_compat(N, nx, ny, nz, sg, A.g, A.KK);
+    after(eold, t[0], t[1], t[2]);
+    const double enew = energy_compat(N, t[0], t[1], t[2], sg, A.g, A.KK);
+    return (enew < eold) || accept_exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt), u_acc);
+}
+
 // Colour-1 director trial, then its chirality trial on the director it
 // leaves (reference arithmetic): `flip` gets a screened flip, `held` an open
-// trial (the spin keeps its value).  Returns the director decision.
+// trial (the spin keeps its value).  Returns the director decision.  The
+// director trial's two bond passes also give both directors' chirality sums
+// for the chirality screen, whose magnitude sum_b |c_b q_b| max(|s|, |s_b|)
+// is bounded by max(|s|, max_b |s_b|) sum_b |c_b q_b| (equal for spins +-1;
+// a larger bound only holds more trials).  A director trial that takes the
+// rounded energies recomputes the chirality sums afterwards instead of
+// keeping them through that branch.
 template <int RNG>
 __device__ inline bool dir_spin_held(const PhaseArgs &A, double d, uint32_t m, const SiteIn &S, bool &flip,
                                      bool &held, SiteOut *o)
@@ -488,13 +526,40 @@ __device__ inline bool dir_spin_held(const PhaseArgs &A, double d, uint32_t m, c
         u_acc = A.u_acc[m];
         u_spin = A.u_spin[m];
     }
-    double t[3];
-    const bool acc = dir_decide<ARITH_COMPAT>(A, S.N, S.nx, S.ny, S.nz, S.sg, d, u_phi, u_acc, t);
+    double c2o, mo, cqo, abo, t[3], c2n, mn, cqn, abn;
+    const double ao = cqf_sum_mag_cq(S.N, S.sg, S.nx, S.ny, S.nz, c2o, mo, cqo, abo);
+    trial_fp64(S.nx, S.ny, S.nz, d, u_phi, t[0], t[1], t[2]);
+    reload_barrier();
+    const double an = cqf_sum_mag_cq(S.N, S.sg, t[0], t[1], t[2], c2n, mn, cqn, abn);
+    const double dEd = (c2o - c2n) - A.KK * (an - ao);
+    const double Md = (12.0 + (c2o + c2n)) + fabs(A.KK) * (mo + mn) + 2.0 * fabs(A.g * (double)S.sg);
+    const int scd = compat_screen(dEd, Md, A.screen, A.kbt, A.inv_kbt, u_acc);
+    bool acc;
+    double a, csq, w;
+    if (scd >= 0) {
+        acc = scd != 0;
+        int smax = abs(S.sg);
+#pragma unroll
+        for (int b = 0; b < 6; ++b) smax = max(smax, abs(S.N.spin(b)));
+        a = acc ? cqn : cqo;
+        csq = acc ? c2n : c2o;
+        w = (acc ? abn : abo) * (double)smax;
+    } else {
+        reload_barrier();
+        const double eold = energy_compat(S.N, S.nx, S.ny, S.nz, S.sg, A.g, A.KK);
+        after(eold, t[0], t[1], t[2]);
+        const double enew = energy_compat(S.N, t[0], t[1], t[2], S.sg, A.g, A.KK);
+        acc = (enew < eold) || accept_exp(div_by_recip(-(enew - eold), A.kbt, A.inv_kbt), u_acc);
+        reload_barrier();
+        a = acc ? cq_sum_mag(S.N, S.sg, t[0], t[1], t[2], csq, w) : cq_sum_mag(S.N, S.sg, S.nx, S.ny, S.nz, csq, w);
+    }
     o->nx = acc ? t[0] : S.nx;
     o->ny = acc ? t[1] : S.ny;
     o->nz = acc ? t[2] : S.nz;
-    reload_barrier();
-    const int sc = spin_screen_compat(A, A.kbt, A.inv_kbt, S.N, o->nx, o->ny, o->nz, S.sg, u_spin);
+    // the chirality screen (spin_screen_compat) on the director the trial left
+    const double dE = (double)S.sg * fma(A.KK, a, 2.0 * A.g);
+    const double M = fma(2.0, csq, 12.0) + fabs(A.KK) * w + 2.0 * fabs(A.g * (double)S.sg);
+    const int sc = compat_screen(dE, M, A.screen, A.kbt, A.inv_kbt, u_spin);
     flip = sc == 1;
     held = sc < 0;
     o->s = flip ? -S.sg : S.sg;
@@ -505,21 +570,37 @@ __device__ inline bool dir_spin_held(const PhaseArgs &A, double d, uint32_t m, c
 // sweep A.istep = A.istep_spin + 1 (reference arithmetic, Philox; both from
 // one block, spin_sweep), after colour 1's held trials are decided: the
 // director trial reads the colour-1 spins after every flip of sweep
-// A.istep_spin, as the reference's next sweep does.  Returns the director
-// decision.
+// A.istep_spin, as the reference's next sweep does.  One pass over the old
+// director's bonds serves the chirality screen and the director trial's old
+// sums.  Returns the director decision.
 template <int BC>
 __device__ inline bool spin_dir_held(const Geom &G, const PhaseArgs &A, double d, uint32_t m, SiteIn &S, bool &flip,
                                      SiteOut *o)
 {
     const U4 w = site_block(A.seed, A.run_index, A.replica, m, A.istep, 0);
-    int8_t fl[6] = {0, 0, 0, 0, 0, 0};
-    flip = spin_c0_compat<BC>(G, A, A.kbt_spin, A.inv_kbt_spin, m, S.N, S.nx, S.ny, S.nz, S.sg, site_u_spin(w), fl);
+    const double u_spin = site_u_spin(w);
+    double c2o, cqb[6], wc;
+    const double a = cq_products_mag(S.N, S.sg, S.nx, S.ny, S.nz, c2o, cqb, wc);
+    {
+        // the chirality screen (spin_screen_compat), else the rounded energies
+        const double dE = (double)S.sg * fma(A.KK, a, 2.0 * A.g);
+        const double M = fma(2.0, c2o, 12.0) + fabs(A.KK) * wc + 2.0 * fabs(A.g * (double)S.sg);
+        const int sc = compat_screen(dE, M, A.screen, A.kbt_spin, A.inv_kbt_spin, u_spin);
+        if (sc >= 0) {
+            flip = sc != 0;
+        } else {
+            reload_barrier();
+            int8_t fl[6] = {0, 0, 0, 0, 0, 0};
+            flip = spin_c0_exact<BC>(G, A, A.kbt_spin, A.inv_kbt_spin, m, S.N, S.nx, S.ny, S.nz, S.sg, u_spin, fl);
 #pragma unroll
-    for (int b = 0; b < 6; ++b)
-        if (fl[b]) S.N.s[b] = -S.N.s[b];  // the director trial reads the spins after colour 1's flips
+            for (int b = 0; b < 6; ++b)
+                if (fl[b]) S.N.s[b] = -S.N.s[b];  // the director trial reads the spins after colour 1's flips
+        }
+    }
     const int sg = flip ? -S.sg : S.sg;
-    double t[3];
-    const bool acc = dir_decide<ARITH_COMPAT>(A, S.N, S.nx, S.ny, S.nz, sg, d, site_u_phi(w), site_u_dir(w), t);
+    double mo, t[3];
+    const double ao = weight_cq_mag(S.N, sg, cqb, mo);
+    const bool acc = dir_decide_compat(A, S.N, S.nx, S.ny, S.nz, sg, d, site_u_phi(w), site_u_dir(w), ao, c2o, mo, t);
     o->nx = acc ? t[0] : S.nx;
     o->ny = acc ? t[1] : S.ny;
     o->nz = acc ? t[2] : S.nz;
