@@ -43,12 +43,15 @@ HBM_PEAK_GBS = 8000.0
 # also read the other half's spins, 0.5 B more), write those spins 0.5 B (upper
 # bound: every flip written); fused colour-1 director + chirality, and fused
 # colour-0 chirality + next sweep's colour-0 director: the director traffic
-# plus the active half's spin writes 0.5 B.
+# plus the active half's spin writes 0.5 B; in reference arithmetic the
+# colour-1 launch also writes its flip bitmap, one bit per active site
+# (1/16 B per site).
 BYTES = {"director": 37.0, "chirality": 25.0, "fused": 37.5, "fused_next": 37.5}
 BYTES_CHIRALITY_COMPAT = 25.5
-KERNEL_NAMES = {"director": "k_dir (director half-sweep)", "chirality": "chirality half-sweep",
-                "fused": "k_dir_spin (colour-1 director + chirality, fused)",
-                "fused_next": "k_spin_dir (colour-0 chirality + next sweep's colour-0 director, fused)"}
+BYTES_FUSED_COMPAT = 37.5 + 1.0 / 16.0
+KERNEL_NAMES = {"director": "director half-sweep", "chirality": "chirality half-sweep",
+                "fused": "k_pair<2> (colour-1 director + chirality, fused)",
+                "fused_next": "k_pair<3> (colour-0 chirality + next sweep's colour-0 director, fused)"}
 
 
 def log(msg):
@@ -208,6 +211,7 @@ def main():
     copy_rate = ceilings["copy"]["best_GBps"]
     mix_rate = ceilings["read2_write1"]["best_GBps"]
     tuned = {}
+    held_rate = {}  # reference arithmetic: held chirality trials per sweep
 
     def time_single(arith, timing, bc=None, nout=None):
         """Autotune the half-sweep kernels for this shape (once per arithmetic;
@@ -275,8 +279,10 @@ def main():
                 if cnt:
                     kernels[name] = {"launches": cnt, "avg_us": 1e3 * ms / cnt, "total_ms": ms, "sweeps": KT}
         held = lat.held_trials()
+        last_sweep = nxt + KT - 1 if timing else first + K - 1
+        held_rate[arith] = held / last_sweep
         if held:
-            log(f"bench: {arith}: {held} chirality trials held over {first + K - 1 + (KT if timing else 0)} sweeps")
+            log(f"bench: {arith}: {held} chirality trials held over {last_sweep} sweeps")
         row = lat.measure(args.g, args.K)
         lat.close()
         log(f"bench: {arith}: E_true={row['E_true']:.6f} eta={row['eta_exact']:.6f}")
@@ -287,6 +293,7 @@ def main():
     roof = None
     if args.arith == "compat":
         BYTES["chirality"] = BYTES_CHIRALITY_COMPAT
+        BYTES["fused"] = BYTES_FUSED_COMPAT
     sweep_bytes = None
     gap_us = None
     sweep_kinds = [k for k in BYTES if k in kernels]
@@ -334,6 +341,8 @@ def main():
         v_sups, v_wall, v_k = time_single(other, not args.no_timing)
         variants[other] = {"site_updates_per_s": v_sups, "ms_per_step": 1e3 * v_wall / K,
                            "kernels": {k: round(v["avg_us"], 2) for k, v in v_k.items()}}
+        if other == "compat":
+            variants[other]["held_trials_per_sweep"] = round(held_rate.get(other, 0.0), 1)
         # the same workload with periodic boundaries, and without measurement sweeps
         other_bc = "periodic" if args.bc == "open" else "open"
         for name, kw in ((f"{args.arith}_{other_bc}", {"bc": other_bc}), (f"{args.arith}_no_measure", {"nout": 0})):
