@@ -611,7 +611,11 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
     const bool cfuse = compat_fused(L, p);
     const bool fuse1 = fused_sweep(L, p->arith) || cfuse;
     const bool fuse2 = fuse1 && p->rng == RNG_PHILOX;
-    if (cfuse && ensure_held(L)) return -1;
+    if (cfuse) {
+        if (ensure_held(L)) return -1;
+        launch_held_clear(L->held, st);
+        if (launch_check()) return -1;
+    }
     const auto grp = [&](int g) { return L->counts + (size_t)g * kSlots * kSlotStride; };
     bool d0_done = false, prev_open = false;
     for (int64_t t = 0; t < nsw; ++t) {

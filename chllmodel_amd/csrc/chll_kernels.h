@@ -124,6 +124,8 @@ size_t flipbit_words(const Geom &G);
 // count the flips into counts_spin, then apply them and clear the queue.
 void launch_held(int rng, int bc, const Geom &G, const Fields &F, const PhaseArgs &A0, const PhaseArgs &A1,
                  uint32_t *counts_spin, unsigned long long *total, hipStream_t s);
+// Clears the held-trial queue (a kernel, so that a captured call records no memset).
+void launch_held_clear(uint32_t *held, hipStream_t s);
 // Whether every field (2 nsub + kFieldPad elements of at most 8 bytes) spans
 // less than 4 GiB, so that kernels may address it with 32-bit byte offsets.
 bool field_off32(const Geom &G);

@@ -775,6 +775,18 @@ size_t flipbit_words(const Geom &G)
     return (size_t)grid1(G.nsub / 2u).x * (kBlock / 64u) * 2u;
 }
 
+// Clears the held-trial queue's header (count and blocks done): a call whose
+// launches stopped between a colour-1 launch and its k_held leaves them set.
+__global__ void k_held_clear(uint32_t *__restrict__ held)
+{
+    if (threadIdx.x < kHeldHead) held[threadIdx.x] = 0u;
+}
+
+void launch_held_clear(uint32_t *held, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_held_clear, dim3(1), dim3(64), 0, s, held);
+}
+
 template <int RNG, int BC>
 static void launch_held_t(const Geom &G, const Fields &F, const PhaseArgs &A0, const PhaseArgs &A1,
                           uint32_t *counts_spin, unsigned long long *total, hipStream_t s)
