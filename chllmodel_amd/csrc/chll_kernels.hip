@@ -269,15 +269,15 @@ __global__ __launch_bounds__(kBlock) void k_held(Geom G, Fields F, PhaseArgs A0,
         }
     }
     if (nflip) atomic_add_global(counts_spin + (blockIdx.x % kSlots) * kSlotStride, nflip);
-    // the last block applies every decision: each block's decisions are
-    // released by its fence before it counts itself done, and the last one
-    // acquires them with its own fence
+    // The last block applies every decision.  Each wave's fence waits for
+    // that wave's own stores, so every thread fences before the barrier:
+    // then all of the block's decisions are in device-coherent memory before
+    // thread 0 counts the block done, and the last block acquires them with
+    // its own fence.
     __shared__ uint32_t is_last;
+    __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        is_last = atomicAdd(held + 1, 1u) == gridDim.x - 1u;
-    }
+    if (threadIdx.x == 0) is_last = atomicAdd(held + 1, 1u) == gridDim.x - 1u;
     __syncthreads();
     if (!is_last) return;
     __threadfence();
