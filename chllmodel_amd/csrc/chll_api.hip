@@ -661,6 +661,10 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
             return (launch_check() || tend(L, ti)) ? -1 : 0;
         };
         if (!d0_done && half(0, 0, grp(kGroupAcc0))) return -1;
+        const bool meas = nout > 0 && (istep % nout) == 0 && row < max_rows;
+        const bool chain = fuse2 && !meas && t + 1 < nsw;
+        FA.istep = istep; FA.measure = meas ? 1 : 0; FA.row = row; FA.sweep_index = t;
+        FA.close_prev = prev_open ? 1 : 0;
         if (fuse1) {
             set_u(0, 1);
             if (cfuse) {
@@ -681,24 +685,26 @@ int chll_lattice_sweeps(chll_lattice *L, chll_params *p, int32_t nstart, int32_t
                     A0.u_acc = L->ustream + 4 * ns;
                     A1.u_acc = L->ustream + 5 * ns;
                 }
+                // in a chain, the finalize that closes sweep t's director
+                // accepts shares the launch
+                FA.close_this = 0;
                 if (tbegin(L, KIND_HELD, &ti)) return -1;
-                launch_held(rngk, L->bc, G, L->F, A0, A1, grp(flip_group(istep, 1)), L->held_total, st);
+                launch_held(rngk, L->bc, G, L->F, A0, A1, grp(flip_group(istep, 1)), L->held_total,
+                            chain ? &FA : nullptr, st);
                 if (launch_check() || tend(L, ti)) return -1;
             }
         } else if (half(0, 1, grp(kGroupAcc1))) {
             return -1;
         }
-        const bool meas = nout > 0 && (istep % nout) == 0 && row < max_rows;
-        const bool chain = fuse2 && !meas && t + 1 < nsw;
-        FA.istep = istep; FA.measure = meas ? 1 : 0; FA.row = row; FA.sweep_index = t;
-        FA.close_prev = prev_open ? 1 : 0;
         if (chain) {
             // close sweep t's director accepts (cone step of sweep t + 1), then
             // C0 of sweep t with D0 of sweep t + 1
             FA.close_this = 0;
-            if (tbegin(L, KIND_FINAL, &ti)) return -1;
-            launch_finalize(FA, st);
-            if (launch_check() || tend(L, ti)) return -1;
+            if (!cfuse) {
+                if (tbegin(L, KIND_FINAL, &ti)) return -1;
+                launch_finalize(FA, st);
+                if (launch_check() || tend(L, ti)) return -1;
+            }
             PhaseArgs A2 = A;
             A2.istep = (uint32_t)(istep + 1);
             A2.istep_spin = (uint32_t)istep;

@@ -122,8 +122,10 @@ size_t flipbit_words(const Geom &G);
 // The held chirality trials of colour 1 (A1: sweep t's colour-1 chirality
 // arguments; A0: its colour-0 ones): decide each from colour 0's outcomes,
 // count the flips into counts_spin, then apply them and clear the queue.
+// With fin, the same launch also does that finalize kernel's work (it reads
+// no counter the held trials write).
 void launch_held(int rng, int bc, const Geom &G, const Fields &F, const PhaseArgs &A0, const PhaseArgs &A1,
-                 uint32_t *counts_spin, unsigned long long *total, hipStream_t s);
+                 uint32_t *counts_spin, unsigned long long *total, const FinArgs *fin, hipStream_t s);
 // Clears the held-trial queue (a kernel, so that a captured call records no memset).
 void launch_held_clear(uint32_t *held, hipStream_t s);
 // Whether every field (2 nsub + kFieldPad elements of at most 8 bytes) spans
