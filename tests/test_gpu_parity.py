@@ -600,6 +600,38 @@ def test_compat_fused_equals_reference_order(dims, bc, rng, scale, kbt_end):
         assert on[4] > 100
 
 
+@pytest.mark.parametrize("scale", [2.0 ** -21, 2.0 ** -10], ids=["s21", "s10"])
+def test_compat_fused_full_size(scale):
+    """The L=256 workload (K=1, g=0.5, kT=0.7, Philox, nout=4) in reference
+    arithmetic: 12 fused sweeps with held trials over two calls equal the
+    reference's order of four half-sweeps bit for bit (lattice, cone step,
+    counts, rows).  At the default scale about 90 trials a sweep are held; at
+    2^-10 about 120,000, so the held-trial kernel runs dozens of passes over
+    its queue."""
+    L = 256
+    st = isotropic_state(L, L, L, 20261203)
+    kw = dict(g=0.5, KK=1.0, kbt=0.7, arith="compat", rng="philox", seed=20261203, adapt=True)
+    res = []
+    for fuse in (True, False):
+        print(f"compat full size L={L} scale={scale} fuse={fuse}", flush=True)
+        with Lattice(L, L, L) as lat:
+            lat.set_fusion(fuse)
+            lat.set_screen(scale)
+            lat.upload_state(st)
+            lat.set_step(0.3)
+            o1 = lat.sweeps(1, 7, 4, want_counts=True, **kw)
+            o2 = lat.sweeps(8, 12, 4, want_counts=True, **kw)
+            res.append((lat.download(), lat.get_step(), np.concatenate([o1["counts"], o2["counts"]]),
+                        np.concatenate([o1["rows"], o2["rows"]]), lat.held_trials()))
+    on, off = res
+    for f in ("nx", "ny", "nz", "s"):
+        assert np.array_equal(on[0][f], off[0][f]), f
+    assert on[1] == off[1]
+    assert np.array_equal(on[2], off[2]) and np.array_equal(on[3], off[3])
+    print(f"held trials: {on[4]} over 12 sweeps", flush=True)
+    assert off[4] == 0 and on[4] > (12 * 500 if scale > 2.0 ** -21 else 12 * 20)
+
+
 def test_compat_screen_scale_validation():
     with Lattice(8, 8, 4) as lat:
         for bad in (2.0 ** -22, -1.0, 2.0, float("nan"), float("inf")):
