@@ -42,6 +42,10 @@ struct RepDesc {
     uint32_t nmeas;            // its measurement partials (blocks of the measure launch)
     double g, KK;
     uint32_t *counts;          // kGroups x kSlots x kSlotStride (groups as the single lattice)
+    // reference arithmetic fused through held trials (chll_site.h), else null:
+    // colour 1's flip bitmap of the current sweep and its held-trial queue
+    unsigned long long *flipbits;
+    uint32_t *held;
 };
 
 // Ensemble-wide device state.
@@ -172,6 +176,85 @@ __global__ __launch_bounds__(BS) void k_pair_ens(EnsDev E, uint32_t istep_off, i
         block_sum2<BS>(wa, wf, D.counts + kGroupAcc1 * gs, D.counts + flip_group(A.istep, 1) * gs, lb);
     else
         block_sum2<BS>(wa, wf, D.counts + kGroupAcc0 * gs, D.counts + flip_group(A.istep_spin, 0) * gs, lb);
+}
+
+// Reference-arithmetic kinds of k_pair_ens, fused through held trials
+// (chll_site.h): 0 director trials of colour 0 (a block's first launch), 2
+// colour-1 director trials each followed by the site's screened chirality
+// trial (held trials to the replica's queue, flips to its bitmap), 3 colour-0
+// chirality trials of sweep t each followed by the site's director trial of
+// sweep t + 1, 4 colour-0 chirality trials alone after colour 1's (a block's
+// last sweep).  Needs every replica to satisfy pairs_ok.
+template <int KIND, int BC, int BS>
+__global__ __launch_bounds__(BS) void k_pair_ens_h(EnsDev E, uint32_t istep_off, int rev, uint32_t base)
+{
+    const uint2 ent = E.blk_rep2[rev ? base - blockIdx.x : base + blockIdx.x];
+    const uint32_t r = ent.x >> 24, lb = ent.x & 0xFFFFFFu;
+    const RepDesc &D = E.R[r];
+    const int s = E.rep_slot[r];
+    PhaseArgs A;
+    A.g = D.g; A.KK = D.KK; A.kbt = E.kbt[s]; A.inv_kbt = E.inv_kbt[s]; A.inv_kbt_spin = A.inv_kbt; A.d = E.d + s;
+    A.screen = kScreenScale;
+    A.kbt_spin = A.kbt; A.flipbits = D.flipbits; A.held = D.held;
+    A.u_phi = nullptr; A.u_acc = nullptr; A.u_spin = nullptr;
+    A.seed = E.seed; A.run_index = E.run_index; A.replica = D.replica; A.istep = *E.istep + istep_off;
+    A.istep_spin = A.istep;
+    const uint32_t m0 = 2u * (lb * BS + threadIdx.x);
+    const size_t gs = (size_t)kSlots * kSlotStride;
+    bool a0 = false, a1 = false, f0 = false, f1 = false, h0 = false, h1 = false;
+    if constexpr (KIND == 3) A.istep = A.istep_spin + 1;
+    const double d = KIND == 4 ? 0.0 : *A.d;
+    if (threadIdx.x < ent.y) {
+        SiteIn S0, S1;
+        const uint32_t own = load_pair<BC, true>(D.G, D.F, KIND == 2 ? 1 : 0, m0, S0, S1);
+        SiteOut O0, O1;
+        if constexpr (KIND == 0) {
+            a0 = dir_apply<ARITH_COMPAT, RNG_PHILOX, true, true>(D.F, 0, A, d, m0, own, S0, &O0);
+            a1 = dir_apply<ARITH_COMPAT, RNG_PHILOX, true, true>(D.F, 0, A, d, m0 + 1u, own + 1u, S1, &O1);
+        } else if constexpr (KIND == 2) {
+            a0 = dir_spin_held<RNG_PHILOX>(A, d, m0, S0, f0, h0, &O0);
+            a1 = dir_spin_held<RNG_PHILOX>(A, d, m0 + 1u, S1, f1, h1, &O1);
+        } else if constexpr (KIND == 3) {
+            a0 = spin_dir_held<BC>(D.G, A, d, m0, S0, f0, &O0);
+            a1 = spin_dir_held<BC>(D.G, A, d, m0 + 1u, S1, f1, &O1);
+        } else {
+            f0 = spin_c0_apply<RNG_PHILOX, BC>(D.G, A, m0, S0, &O0);
+            f1 = spin_c0_apply<RNG_PHILOX, BC>(D.G, A, m0 + 1u, S1, &O1);
+        }
+        store_pair<KIND != 4, KIND != 0, true>(D.F, own, O0, O1);
+    }
+    if constexpr (KIND == 2) held_epilogue(A, D.G.nsub, true, lb * (BS / 64u) + (threadIdx.x >> 6), m0, f0, f1, h0, h1);
+    const uint32_t wa = (uint32_t)(__popcll(__ballot(a0)) + __popcll(__ballot(a1)));
+    const uint32_t wf = (uint32_t)(__popcll(__ballot(f0)) + __popcll(__ballot(f1)));
+    if constexpr (KIND == 0)
+        block_sum2<BS>(wa, 0u, D.counts + kGroupAcc0 * gs, D.counts + kGroupAcc0 * gs, lb);
+    else if constexpr (KIND == 4)
+        block_sum2<BS>(0u, wf, D.counts + flip_group(A.istep, 0) * gs, D.counts + flip_group(A.istep, 0) * gs, lb);
+    else if constexpr (KIND == 2)
+        block_sum2<BS>(wa, wf, D.counts + kGroupAcc1 * gs, D.counts + flip_group(A.istep, 1) * gs, lb);
+    else
+        block_sum2<BS>(wa, wf, D.counts + kGroupAcc0 * gs, D.counts + flip_group(A.istep_spin, 0) * gs, lb);
+}
+
+// Held chirality trials (held_group) of the replicas order[first ..
+// first + gridDim.x / hb - 1], hb blocks per replica.
+constexpr uint32_t kHeldEnsBlocks = 2;
+template <int BC>
+__global__ __launch_bounds__(kHeldBlockThreads) void k_held_ens(EnsDev E, uint32_t first, uint32_t hb,
+                                                              uint32_t istep_off)
+{
+    const uint32_t r = E.order[first + blockIdx.x / hb], blk = blockIdx.x % hb;
+    const RepDesc &D = E.R[r];
+    const int s = E.rep_slot[r];
+    PhaseArgs A;
+    A.g = D.g; A.KK = D.KK; A.kbt = E.kbt[s]; A.inv_kbt = E.inv_kbt[s]; A.inv_kbt_spin = A.inv_kbt; A.d = E.d + s;
+    A.screen = kScreenScale;
+    A.kbt_spin = A.kbt; A.flipbits = D.flipbits; A.held = D.held;
+    A.u_phi = nullptr; A.u_acc = nullptr; A.u_spin = nullptr;
+    A.seed = E.seed; A.run_index = E.run_index; A.replica = D.replica; A.istep = *E.istep + istep_off;
+    A.istep_spin = A.istep;
+    const size_t gs = (size_t)kSlots * kSlotStride;
+    held_group<RNG_PHILOX, BC>(D.G, D.F, A, A, D.held, D.counts + flip_group(A.istep, 1) * gs, nullptr, blk, hb);
 }
 
 template <int BC, int BS>
@@ -635,7 +718,9 @@ int launch_block_t(chll_ensemble *X, int K, hipStream_t st)
         }
     }
     const dim3 b(BS), bf(kEnsFin);
-    const bool fuse = ARITH == ARITH_FP64 && X->fuse != 0;
+    // reference arithmetic fuses only through held trials, on site pairs
+    const bool hfuse = ARITH == ARITH_COMPAT && X->fuse == 1 && X->pairs;
+    const bool fuse = (ARITH == ARITH_FP64 && X->fuse != 0) || hfuse;
     const bool pairs = fuse && X->fuse == 1 && X->pairs;
     // observables: four sites per thread wherever every replica allows site
     // pairs, whatever the fusion setting, so rows do not depend on it
@@ -663,17 +748,28 @@ int launch_block_t(chll_ensemble *X, int K, hipStream_t st)
                 continue;
             }
             if (t == 0) {
-                if (pairs) hipLaunchKernelGGL((k_pair_ens<0, BC, BS>), g2, b, 0, cs, E, off, 1, r2);
+                if (hfuse) hipLaunchKernelGGL((k_pair_ens_h<0, BC, BS>), g2, b, 0, cs, E, off, 1, r2);
+                else if (pairs) hipLaunchKernelGGL((k_pair_ens<0, BC, BS>), g2, b, 0, cs, E, off, 1, r2);
                 else hipLaunchKernelGGL((k_phase_ens<0, ARITH, BC, BS>), g, b, 0, cs, E, 0, off, 1, r1);
             }
-            if (pairs) hipLaunchKernelGGL((k_pair_ens<2, BC, BS>), g2, b, 0, cs, E, off, 0, c.t2);
-            else hipLaunchKernelGGL((k_phase_ens<2, ARITH, BC, BS>), g, b, 0, cs, E, 1, off, 0, c.t1);
+            if (hfuse) {
+                // colour 1's director + chirality trials, then its held trials
+                hipLaunchKernelGGL((k_pair_ens_h<2, BC, BS>), g2, b, 0, cs, E, off, 0, c.t2);
+                hipLaunchKernelGGL((k_held_ens<BC>), dim3(c.nf * kHeldEnsBlocks), dim3(kHeldBlockThreads), 0, cs, E,
+                                   c.f, kHeldEnsBlocks, off);
+            } else if (pairs) {
+                hipLaunchKernelGGL((k_pair_ens<2, BC, BS>), g2, b, 0, cs, E, off, 0, c.t2);
+            } else {
+                hipLaunchKernelGGL((k_phase_ens<2, ARITH, BC, BS>), g, b, 0, cs, E, 1, off, 0, c.t1);
+            }
             if (!last) {
                 hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, cs, E, off, 0, t > 0 ? 1 : 0, 0, c.f);
-                if (pairs) hipLaunchKernelGGL((k_pair_ens<3, BC, BS>), g2, b, 0, cs, E, off, 1, r2);
+                if (hfuse) hipLaunchKernelGGL((k_pair_ens_h<3, BC, BS>), g2, b, 0, cs, E, off, 1, r2);
+                else if (pairs) hipLaunchKernelGGL((k_pair_ens<3, BC, BS>), g2, b, 0, cs, E, off, 1, r2);
                 else hipLaunchKernelGGL((k_phase_ens<3, ARITH, BC, BS>), g, b, 0, cs, E, 0, off, 1, r1);
             } else {
-                if (pairs) hipLaunchKernelGGL((k_pair_ens<1, BC, BS>), g2, b, 0, cs, E, off, 1, r2);
+                if (hfuse) hipLaunchKernelGGL((k_pair_ens_h<4, BC, BS>), g2, b, 0, cs, E, off, 1, r2);
+                else if (pairs) hipLaunchKernelGGL((k_pair_ens<1, BC, BS>), g2, b, 0, cs, E, off, 1, r2);
                 else hipLaunchKernelGGL((k_phase_ens<1, ARITH, BC, BS>), g, b, 0, cs, E, 0, off, 1, r1);
                 measure(c, cs);
                 hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, cs, E, off, 1, t > 0 ? 1 : 0, 1, c.f);
@@ -788,9 +884,17 @@ int chll_ensemble_create(const chll_replica_desc *reps, int32_t nrep, int32_t bc
         D.g = reps[r].g;
         D.KK = reps[r].KK;
     }
-    // two sites per thread in the fused FP64 launches when every replica allows it
-    X->pairs = arith == ARITH_FP64;
+    // two sites per thread in the fused launches when every replica allows it
+    X->pairs = true;
     for (int r = 0; r < nrep; ++r) X->pairs = X->pairs && pairs_ok(X->rep[r].G);
+    // reference arithmetic fuses through held trials: a flip bitmap (two words
+    // per wave of every block size) and a held-trial queue per replica
+    for (int r = 0; r < nrep && !rc && arith == ARITH_COMPAT && X->pairs; ++r) {
+        RepDesc &D = X->rep[r];
+        const size_t words = 16 * (((size_t)D.G.nsub / 2 + 511) / 512);
+        rc |= ealloc(&D.flipbits, words) | ealloc(&D.held, kHeldHead + (size_t)D.G.nsub);
+        if (!rc && hipMemset(D.held, 0, kHeldHead * sizeof(uint32_t)) != hipSuccess) rc = efail("hipMemset failed");
+    }
     // block order: largest replicas first
     X->order.resize(nrep);
     for (int r = 0; r < nrep; ++r) X->order[r] = r;
@@ -850,6 +954,8 @@ int chll_ensemble_destroy(chll_ensemble *X)
     for (RepDesc &D : X->rep) {
         (void)hipFree(D.F.nx); (void)hipFree(D.F.ny); (void)hipFree(D.F.nz); (void)hipFree(D.F.s);
         (void)hipFree(D.counts);
+        if (D.flipbits) (void)hipFree(D.flipbits);
+        if (D.held) (void)hipFree(D.held);
     }
     EnsDev &E = X->E;
     void *ptrs[] = {X->R, X->blk_rep, X->blk_rep2, X->blk_rep3, X->order_dev, E.kbt, E.inv_kbt, E.d, E.rep_slot,

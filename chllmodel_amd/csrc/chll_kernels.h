@@ -40,6 +40,7 @@ struct PhaseArgs {
 // the flag of a decided flip.
 constexpr int kHeldLanes = 8;
 constexpr uint32_t kHeldBlocks = 64;
+constexpr uint32_t kHeldBlockThreads = 256;
 constexpr uint32_t kHeldHead = 2;
 constexpr uint32_t kHeldFlip = 0x80000000u;
 

@@ -170,32 +170,8 @@ __global__ __launch_bounds__(kBlock) void k_pair(Geom G, Fields F, int colour, P
         }
         store_pair<OP != 1 && OP != 4, OP != 0>(F, own, O0, O1);
     }
-    if constexpr (OP == 2 && ARITH == ARITH_COMPAT) {
-        const uint64_t b0 = __ballot(f0), b1 = __ballot(f1), q0 = __ballot(h0), q1 = __ballot(h1);
-        const uint32_t lane = threadIdx.x & 63u;
-        if (blk != kIdleBlock) {
-            // wave w of the launch's logical blocks holds sites 128 w .. 128 w + 127
-            const uint32_t w = blk * (kBlock / 64u) + (threadIdx.x >> 6);
-            if (lane == 0u) {
-                typedef unsigned long long u2v __attribute__((ext_vector_type(2)));
-                typedef __attribute__((address_space(1))) u2v gu2v;
-                *(gu2v *)(A.flipbits + 2u * w) =
-                    u2v{spread32((uint32_t)b0) | (spread32((uint32_t)b1) << 1),
-                        spread32((uint32_t)(b0 >> 32)) | (spread32((uint32_t)(b1 >> 32)) << 1)};
-            }
-            if (q0 | q1) {
-                uint32_t base = 0;
-                if (lane == 0u) base = atomicAdd(A.held, (uint32_t)(__popcll(q0) + __popcll(q1)));
-                base = __shfl(base, 0);
-                // the queue holds nsub entries, one per colour-1 site and sweep
-                const uint64_t below = (1ull << lane) - 1ull;
-                const uint32_t e0 = base + (uint32_t)__popcll(q0 & below);
-                const uint32_t e1 = base + (uint32_t)(__popcll(q0) + __popcll(q1 & below));
-                if (h0 && e0 < G.nsub) A.held[kHeldHead + e0] = m0;
-                if (h1 && e1 < G.nsub) A.held[kHeldHead + e1] = m0 + 1u;
-            }
-        }
-    }
+    if constexpr (OP == 2 && ARITH == ARITH_COMPAT)
+        held_epilogue(A, G.nsub, blk != kIdleBlock, blk * (kBlock / 64u) + (threadIdx.x >> 6), m0, f0, f1, h0, h1);
     const uint32_t wa = (uint32_t)(__popcll(__ballot(a0)) + __popcll(__ballot(a1)));
     const uint32_t wf = (uint32_t)(__popcll(__ballot(f0)) + __popcll(__ballot(f1)));
     block_sum2<kBlock>(wa, wf, c0, (OP < 2 || OP == 4) ? c0 : c1, blockIdx.x);
@@ -426,20 +402,11 @@ __device__ inline void finalize_block(const FinArgs &A)
 
 __global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs A) { finalize_block(A); }
 
-// Held chirality trials of colour 1 (chll_site.h), kHeldLanes lanes per
-// trial.  Lane b < 6 computes the chirality outcome of the trial site's
-// colour-0 neighbour b: the trial colour 0's next launch runs, on the same
-// inputs and draw (A0, spin_c0_compat).  Lane 6 loads the held site's own
-// inputs meanwhile and then decides its trial on the colour-0 spins those
-// outcomes leave, as the reference's order has them (A1).  The decisions
-// read only the state the colour-1 launch left; the last block to finish
-// applies the flips (spin and flip bit) and clears the queue.  The loop over
-// groups of trials is uniform, so every lane of a group takes part in the
-// shuffles.
-// With fin, one more block (the last) runs the finalize kernel's work
-// (finalize_block) for a fused sweep's chain: it reads the director accepts of
-// the sweep and the flips of the sweep before, none of which this kernel
-// writes, so the two share a launch.
+// Held chirality trials (held_group) of the single lattice.  With fin, one
+// more block (the last) runs the finalize kernel's work (finalize_block) for a
+// fused sweep's chain: it reads the director accepts of the sweep and the
+// flips of the sweep before, none of which the held trials write, so the two
+// share a launch.
 template <int RNG, int BC>
 __global__ __launch_bounds__(kBlock) void k_held(Geom G, Fields F, PhaseArgs A0, PhaseArgs A1,
                                              uint32_t *__restrict__ held, uint32_t *__restrict__ counts_spin,
@@ -450,89 +417,7 @@ __global__ __launch_bounds__(kBlock) void k_held(Geom G, Fields F, PhaseArgs A0,
         finalize_block(FA);
         return;
     }
-    // the first entry of this group is loaded with the count (in bounds
-    // whatever the count; used only below it)
-    const uint32_t e_first = blockIdx.x * (kBlock / kHeldLanes) + threadIdx.x / kHeldLanes;
-    const uint32_t m_first = held[kHeldHead + min(e_first, G.nsub - 1u)];
-    const uint32_t n = min(held[0], G.nsub);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && total) atomicAdd(total, (unsigned long long)n);
-    const uint32_t lane = threadIdx.x % kHeldLanes, per = nheld_blocks * (kBlock / kHeldLanes);
-    uint32_t nflip = 0;
-    for (uint32_t e0 = 0; e0 < n; e0 += per) {
-        const uint32_t e = e0 + e_first;
-        const bool live = e < n;
-        const uint32_t m1 = live ? (e0 == 0 ? m_first : held[kHeldHead + e]) : 0u;
-        int after_c0 = 0;
-        SiteIn S;
-        uint32_t mask = 0u;
-        if (live) {
-            if (lane < 6u) {
-                uint32_t nb[6];
-                mask = neighbours<BC>(G, m1, decode(G, m1, 1), 0u, nb);  // packed colour-0 indices
-                if ((mask >> lane) & 1u) {
-                    const uint32_t i = nb[lane];
-                    SiteIn Si;
-                    load_site<BC, true>(G, F, 0, i, Si);
-                    const double u =
-                        RNG == RNG_PHILOX
-                            ? site_u_spin(site_block(A0.seed, A0.run_index, A0.replica, i, spin_sweep(0, A0.istep), 0))
-                            : A0.u_acc[i];
-                    int8_t fl[6] = {0, 0, 0, 0, 0, 0};
-                    const bool f =
-                        spin_c0_compat<BC>(G, A0, A0.kbt, A0.inv_kbt, i, Si.N, Si.nx, Si.ny, Si.nz, Si.sg, u, fl);
-                    after_c0 = f ? -Si.sg : Si.sg;
-                }
-            } else if (lane == 6u) {
-                load_site<BC, true>(G, F, 1, m1, S);
-                mask = S.N.mask;
-            }
-        }
-        int sb[6];
-#pragma unroll
-        for (int b = 0; b < 6; ++b) sb[b] = __shfl(after_c0, b, kHeldLanes);
-        if (live && lane == 6u) {
-#pragma unroll
-            for (int b = 0; b < 6; ++b)
-                if ((mask >> b) & 1u) S.N.s[b] = sb[b];
-            const double u =
-                RNG == RNG_PHILOX
-                    ? site_u_spin(site_block(A1.seed, A1.run_index, A1.replica, m1, spin_sweep(1, A1.istep), 1))
-                    : A1.u_acc[m1];
-            const int sc = spin_screen_compat(A1, A1.kbt, A1.inv_kbt, S.N, S.nx, S.ny, S.nz, S.sg, u);
-            const bool f = sc >= 0 ? sc != 0 : spin_exact_compat(A1, A1.kbt, A1.inv_kbt, S.N, S.nx, S.ny, S.nz, S.sg, u);
-            if (f) {
-                held[kHeldHead + e] = m1 | kHeldFlip;
-                ++nflip;
-            }
-        }
-    }
-    if (nflip) atomic_add_global(counts_spin + (blockIdx.x % kSlots) * kSlotStride, nflip);
-    // The last block applies every decision.  Each wave's fence waits for
-    // that wave's own stores, so every thread fences before the barrier:
-    // then all of the block's decisions are in device-coherent memory before
-    // thread 0 counts the block done, and the last block acquires them with
-    // its own fence.
-    __shared__ uint32_t is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = atomicAdd(held + 1, 1u) == nheld_blocks - 1u;
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
-        const uint32_t v = __hip_atomic_load(held + kHeldHead + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (v & kHeldFlip) {
-            const uint32_t m = v & ~kHeldFlip;
-            int8_t *p = F.s + G.nsub + m;
-            *p = (int8_t)(-*p);
-            atomicOr(A1.flipbits + (m >> 6), 1ull << (m & 63u));
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        held[0] = 0u;
-        held[1] = 0u;
-    }
+    held_group<RNG, BC>(G, F, A0, A1, held, counts_spin, total, blockIdx.x, nheld_blocks);
 }
 
 

@@ -620,6 +620,137 @@ __device__ inline unsigned long long spread32(uint32_t v)
     return x;
 }
 
+// After a fused colour-1 launch's trials in reference arithmetic (k_pair,
+// k_pair_ens): each wave whose lanes hold consecutive site pairs m0 = 2 (64 w
+// + lane) writes the flip bits of its 128 sites (two 64-bit words, site m at
+// bit m % 64 of word m / 64), and appends the sites of its held trials to
+// A.held.  The ballots take every lane; live_wave: the wave belongs to a live
+// block.  The queue holds nsub entries, one per colour-1 site and sweep.
+__device__ inline void held_epilogue(const PhaseArgs &A, uint32_t nsub, bool live_wave, uint32_t w, uint32_t m0,
+                                     bool f0, bool f1, bool h0, bool h1)
+{
+    const uint64_t b0 = __ballot(f0), b1 = __ballot(f1), q0 = __ballot(h0), q1 = __ballot(h1);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (!live_wave) return;
+    if (lane == 0u) {
+        typedef unsigned long long u2v __attribute__((ext_vector_type(2)));
+        typedef __attribute__((address_space(1))) u2v gu2v;
+        *(gu2v *)(A.flipbits + 2u * w) = u2v{spread32((uint32_t)b0) | (spread32((uint32_t)b1) << 1),
+                                            spread32((uint32_t)(b0 >> 32)) | (spread32((uint32_t)(b1 >> 32)) << 1)};
+    }
+    if (q0 | q1) {
+        uint32_t base = 0;
+        if (lane == 0u) base = atomicAdd(A.held, (uint32_t)(__popcll(q0) + __popcll(q1)));
+        base = __shfl(base, 0);
+        const uint64_t below = (1ull << lane) - 1ull;
+        const uint32_t e0 = base + (uint32_t)__popcll(q0 & below);
+        const uint32_t e1 = base + (uint32_t)(__popcll(q0) + __popcll(q1 & below));
+        if (h0 && e0 < nsub) A.held[kHeldHead + e0] = m0;
+        if (h1 && e1 < nsub) A.held[kHeldHead + e1] = m0 + 1u;
+    }
+}
+
+// Held chirality trials of colour 1 of one lattice, block blk of the nblk
+// blocks that serve its queue, kHeldLanes lanes per trial.  Lane b < 6
+// computes the chirality outcome of the trial site's colour-0 neighbour b:
+// the trial colour 0's next launch runs, on the same inputs and draw (A0,
+// spin_c0_compat).  Lane 6 loads the held site's own inputs meanwhile and then
+// decides its trial on the colour-0 spins those outcomes leave, as the
+// reference's order has them (A1).  The decisions read only the state the
+// colour-1 launch left; the last of the nblk blocks to finish applies the
+// flips (spin and flip bit) and clears the queue.  The loop over groups of
+// trials is uniform, so every lane of a group takes part in the shuffles.
+template <int RNG, int BC>
+__device__ inline void held_group(const Geom &G, const Fields &F, const PhaseArgs &A0, const PhaseArgs &A1,
+                                  uint32_t *held, uint32_t *counts_spin, unsigned long long *total, uint32_t blk,
+                                  uint32_t nblk)
+{
+    constexpr uint32_t kGroupsPerBlock = kHeldBlockThreads / kHeldLanes;
+    // the first entry of this group is loaded with the count (in bounds
+    // whatever the count; used only below it)
+    const uint32_t e_first = blk * kGroupsPerBlock + threadIdx.x / kHeldLanes;
+    const uint32_t m_first = held[kHeldHead + min(e_first, G.nsub - 1u)];
+    const uint32_t n = min(held[0], G.nsub);
+    if (blk == 0 && threadIdx.x == 0 && total) atomicAdd(total, (unsigned long long)n);
+    const uint32_t lane = threadIdx.x % kHeldLanes, per = nblk * kGroupsPerBlock;
+    uint32_t nflip = 0;
+    for (uint32_t e0 = 0; e0 < n; e0 += per) {
+        const uint32_t e = e0 + e_first;
+        const bool live = e < n;
+        const uint32_t m1 = live ? (e0 == 0 ? m_first : held[kHeldHead + e]) : 0u;
+        int after_c0 = 0;
+        SiteIn S;
+        uint32_t mask = 0u;
+        if (live) {
+            if (lane < 6u) {
+                uint32_t nb[6];
+                mask = neighbours<BC>(G, m1, decode(G, m1, 1), 0u, nb);  // packed colour-0 indices
+                if ((mask >> lane) & 1u) {
+                    const uint32_t i = nb[lane];
+                    SiteIn Si;
+                    load_site<BC, true>(G, F, 0, i, Si);
+                    const double u =
+                        RNG == RNG_PHILOX
+                            ? site_u_spin(site_block(A0.seed, A0.run_index, A0.replica, i, spin_sweep(0, A0.istep), 0))
+                            : A0.u_acc[i];
+                    int8_t fl[6] = {0, 0, 0, 0, 0, 0};
+                    const bool f =
+                        spin_c0_compat<BC>(G, A0, A0.kbt, A0.inv_kbt, i, Si.N, Si.nx, Si.ny, Si.nz, Si.sg, u, fl);
+                    after_c0 = f ? -Si.sg : Si.sg;
+                }
+            } else if (lane == 6u) {
+                load_site<BC, true>(G, F, 1, m1, S);
+                mask = S.N.mask;
+            }
+        }
+        int sb[6];
+#pragma unroll
+        for (int b = 0; b < 6; ++b) sb[b] = __shfl(after_c0, b, kHeldLanes);
+        if (live && lane == 6u) {
+#pragma unroll
+            for (int b = 0; b < 6; ++b)
+                if ((mask >> b) & 1u) S.N.s[b] = sb[b];
+            const double u =
+                RNG == RNG_PHILOX
+                    ? site_u_spin(site_block(A1.seed, A1.run_index, A1.replica, m1, spin_sweep(1, A1.istep), 1))
+                    : A1.u_acc[m1];
+            const int sc = spin_screen_compat(A1, A1.kbt, A1.inv_kbt, S.N, S.nx, S.ny, S.nz, S.sg, u);
+            const bool f = sc >= 0 ? sc != 0 : spin_exact_compat(A1, A1.kbt, A1.inv_kbt, S.N, S.nx, S.ny, S.nz, S.sg, u);
+            if (f) {
+                held[kHeldHead + e] = m1 | kHeldFlip;
+                ++nflip;
+            }
+        }
+    }
+    if (nflip) atomic_add_global(counts_spin + (blk % kSlots) * kSlotStride, nflip);
+    // The last block applies every decision.  Each wave's fence waits for
+    // that wave's own stores, so every thread fences before the barrier:
+    // then all of the block's decisions are in device-coherent memory before
+    // thread 0 counts the block done, and the last block acquires them with
+    // its own fence.
+    __shared__ uint32_t is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = atomicAdd(held + 1, 1u) == nblk - 1u;
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    for (uint32_t e = threadIdx.x; e < n; e += kHeldBlockThreads) {
+        const uint32_t v = __hip_atomic_load(held + kHeldHead + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v & kHeldFlip) {
+            const uint32_t m = v & ~kHeldFlip;
+            int8_t *p = F.s + G.nsub + m;
+            *p = (int8_t)(-*p);
+            atomicOr(A1.flipbits + (m >> 6), 1ull << (m & 63u));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        held[0] = 0u;
+        held[1] = 0u;
+    }
+}
+
 // Single sites: load, then apply.
 template <int ARITH, int RNG, int BC, bool OFF32 = false>
 __device__ inline bool dir_site(const Geom &G, const Fields &F, int colour, const PhaseArgs &A, uint32_t m)

@@ -132,10 +132,11 @@ class Ensemble:
         self.block_size = int(bs)
 
     def set_fusion(self, on: bool, pairs: bool = True):
-        """FP64: fused director + chirality launches within each block of
-        sweeps (same results, fewer passes over the replicas); with `pairs`,
-        two sites per thread in those launches when every replica's ni is a
-        multiple of 4."""
+        """Fused director + chirality launches within each block of sweeps
+        (same results, fewer passes over the replicas); with `pairs`, two
+        sites per thread in those launches when every replica's ni is a
+        multiple of 4.  Reference arithmetic fuses only with `pairs`, where
+        every replica allows them, through held chirality trials."""
         _check(lib().chll_ensemble_set_fusion(self._h, (1 if pairs else 2) if on else 0))
 
     def set_chunk(self, nbytes: int):

@@ -239,10 +239,12 @@ int chll_ensemble_destroy(chll_ensemble *e);
 int chll_ensemble_set_stream(chll_ensemble *e, void *hip_stream);
 /* threads per block of the batched kernels: 64, 128, 256 or 512 */
 int chll_ensemble_set_block(chll_ensemble *e, int32_t block_size);
-/* FP64 ensembles: on = 1 (default) runs the fused launches of
- * chll_lattice_set_fusion inside each block of K sweeps, two sites per thread
- * when every replica has ni % 4 == 0 (the single lattice's site pairs); 2 the
- * same with one site per thread; 0 four half-sweeps.  Results are unchanged. */
+/* on = 1 (default) runs the fused launches of chll_lattice_set_fusion inside
+ * each block of K sweeps, two sites per thread when every replica has
+ * ni % 4 == 0 (the single lattice's site pairs); 2 the same with one site per
+ * thread; 0 four half-sweeps.  Reference arithmetic fuses only with 1 on site
+ * pairs, through held chirality trials, and otherwise runs four half-sweeps.
+ * Results are unchanged. */
 int chll_ensemble_set_fusion(chll_ensemble *e, int32_t on);
 /* Replicas are independent between swaps, so a block of K sweeps can run chunk
  * by chunk: all K sweeps of consecutive replicas (largest first) holding at

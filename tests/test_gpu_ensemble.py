@@ -35,11 +35,28 @@ def small_replicas():
     return reps
 
 
-@pytest.mark.parametrize("arith", ["compat", "fp64"])
+def pair_replicas():
+    # every replica with ni % 4 == 0: the batched kernels take two sites per
+    # thread, and reference arithmetic fuses through held trials
+    reps = []
+    for kT in (0.6, 0.7, 0.8):
+        reps.append({"ni": 4, "nj": 4, "nk": 4, "ladder": 0, "g": 0.5, "KK": 1.0, "kbt": kT, "d0": 0.3})
+    for kT in (0.9, 1.0, 1.1, 1.25):
+        reps.append({"ni": 8, "nj": 4, "nk": 3, "ladder": 1, "g": 1.0, "KK": 0.7, "kbt": kT, "d0": 0.2})
+    reps.append({"ni": 8, "nj": 6, "nk": 4, "ladder": 2, "g": 0.0, "KK": 1.5, "kbt": 0.8, "d0": 0.25})
+    return reps
+
+
+@pytest.mark.parametrize("arith,repset,fuse", [("compat", "small", 1), ("fp64", "small", 1), ("compat", "pairs", 1),
+                                               ("compat", "pairs", 0)])
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_ensemble_matches_oracle_with_swaps(arith, use_graph):
+def test_ensemble_matches_oracle_with_swaps(arith, repset, fuse, use_graph):
+    """Every block of K sweeps bit-exact against oracle runs that follow the
+    GPU's slot permutation, with the swap decisions recomputed.  The "pairs"
+    replicas all take site pairs, so reference arithmetic runs fused with
+    held trials (fuse 1) or as four half-sweeps (fuse 0)."""
     orc = Oracle()
-    reps = small_replicas()
+    reps = small_replicas() if repset == "small" else pair_replicas()
     seed, run_index, K, nblocks = 0xC0FFEE1234, 2, 3, 5
     n = len(reps)
     states = [isotropic_state(r["ni"], r["nj"], r["nk"], 500 + i) for i, r in enumerate(reps)]
@@ -48,8 +65,9 @@ def test_ensemble_matches_oracle_with_swaps(arith, use_graph):
     rep_slot = np.arange(n)
     lad_start = [0, 3, 7, 8]
     oarith = ARITH_COMPAT if arith == "compat" else ARITH_FP64
-    print(f"ensemble parity arith={arith} graph={use_graph}", flush=True)
+    print(f"ensemble parity arith={arith} replicas={repset} fuse={fuse} graph={use_graph}", flush=True)
     with Ensemble(reps, bc="open", arith=arith) as ens:
+        ens.set_fusion(bool(fuse))
         for r in range(n):
             ens.upload(r, states[r])
         for b in range(nblocks):
@@ -87,7 +105,10 @@ def test_ensemble_matches_oracle_with_swaps(arith, use_graph):
             rep_slot = np.argsort(slot_rep)
             assert np.array_equal(st["rep_slot"], rep_slot), (b, st["rep_slot"], rep_slot)
         assert st["istep"] == nblocks * K + 1
-        assert st["swap_acc"].sum() > 0 and st["swap_acc"].sum() < st["swap_att"].sum()
+        if repset == "small":
+            assert st["swap_acc"].sum() > 0 and st["swap_acc"].sum() < st["swap_att"].sum()
+        else:
+            assert st["swap_att"].sum() > 0
 
 
 def test_graph_replays_equal_eager():
@@ -200,7 +221,7 @@ def test_pair_measure_rows_match_oracle():
             assert abs(row[14] - e) <= np.spacing(np.float32(abs(e))), r
 
 
-@pytest.mark.parametrize("arith,fuse", [("fp64", True), ("fp64", False), ("compat", False)])
+@pytest.mark.parametrize("arith,fuse", [("fp64", True), ("fp64", False), ("compat", False), ("compat", True)])
 def test_chunks_equal_one_launch(arith, fuse):
     """A block of sweeps run chunk by chunk (a few replicas at a time, all K
     sweeps each), on one stream or with the chunks taking turns on two or
