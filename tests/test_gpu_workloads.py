@@ -338,11 +338,14 @@ def test_l128_anneal_20000_sweeps():
 
 
 @pytest.mark.timeout(900)
-def test_phase_scan_10000_sweeps():
+@pytest.mark.parametrize("arith", ["fp64", "compat"])
+def test_phase_scan_10000_sweeps(arith):
+    """The scan workload's full 10k sweeps; in reference arithmetic its blocks
+    run fused with held chirality trials (every replica takes site pairs)."""
     reps = phase_scan_replicas()
     n = len(reps)
-    print("phase scan, 10k sweeps: 1000 blocks of 10", flush=True)
-    with Ensemble(reps, bc="open", arith="fp64") as ens:
+    print(f"phase scan, 10k sweeps ({arith}): 1000 blocks of 10", flush=True)
+    with Ensemble(reps, bc="open", arith=arith) as ens:
         # the configuration the bench's autotuner picks on MI355X: chunks of
         # 128 MiB taking turns on two graph branches
         ens.set_chunk(128 << 20)
