@@ -236,27 +236,6 @@ __global__ __launch_bounds__(BS) void k_pair_ens_h(EnsDev E, uint32_t istep_off,
         block_sum2<BS>(wa, wf, D.counts + kGroupAcc0 * gs, D.counts + flip_group(A.istep_spin, 0) * gs, lb);
 }
 
-// Held chirality trials (held_group) of the replicas order[first ..
-// first + gridDim.x / hb - 1], hb blocks per replica.
-constexpr uint32_t kHeldEnsBlocks = 2;
-template <int BC>
-__global__ __launch_bounds__(kHeldBlockThreads) void k_held_ens(EnsDev E, uint32_t first, uint32_t hb,
-                                                              uint32_t istep_off)
-{
-    const uint32_t r = E.order[first + blockIdx.x / hb], blk = blockIdx.x % hb;
-    const RepDesc &D = E.R[r];
-    const int s = E.rep_slot[r];
-    PhaseArgs A;
-    A.g = D.g; A.KK = D.KK; A.kbt = E.kbt[s]; A.inv_kbt = E.inv_kbt[s]; A.inv_kbt_spin = A.inv_kbt; A.d = E.d + s;
-    A.screen = kScreenScale;
-    A.kbt_spin = A.kbt; A.flipbits = D.flipbits; A.held = D.held;
-    A.u_phi = nullptr; A.u_acc = nullptr; A.u_spin = nullptr;
-    A.seed = E.seed; A.run_index = E.run_index; A.replica = D.replica; A.istep = *E.istep + istep_off;
-    A.istep_spin = A.istep;
-    const size_t gs = (size_t)kSlots * kSlotStride;
-    held_group<RNG_PHILOX, BC>(D.G, D.F, A, A, D.held, D.counts + flip_group(A.istep, 1) * gs, nullptr, blk, hb);
-}
-
 template <int BC, int BS>
 __global__ __launch_bounds__(BS) void k_measure_ens(EnsDev E, uint32_t base)
 {
@@ -325,10 +304,12 @@ constexpr int kEnsFin = 256;
 // The director accepts of sweep istep are always closed (cone-step update);
 // close_prev zeroes the flips of sweep istep - 1, close_this counts those of
 // sweep istep (for the row) and zeroes them.
-__global__ __launch_bounds__(kEnsFin) void k_finalize_ens(EnsDev E, uint32_t istep_off, int measure, int close_prev,
-                                                          int close_this, uint32_t fbase)
+// The per-replica epilogue of replica order[fbase + b], one block of kEnsFin
+// threads (k_finalize_ens, and blocks of k_held_ens).
+__device__ inline void finalize_ens_block(const EnsDev &E, uint32_t istep_off, int measure, int close_prev,
+                                          int close_this, uint32_t fbase, uint32_t b)
 {
-    const int r = (int)E.order[fbase + blockIdx.x];
+    const int r = (int)E.order[fbase + b];
     const RepDesc &D = E.R[r];
     const uint32_t t = threadIdx.x;
     const uint32_t istep = *E.istep + istep_off;
@@ -438,6 +419,42 @@ __global__ __launch_bounds__(kEnsFin) void k_finalize_ens(EnsDev E, uint32_t ist
         }
     }
 }
+
+__global__ __launch_bounds__(kEnsFin) void k_finalize_ens(EnsDev E, uint32_t istep_off, int measure, int close_prev,
+                                                          int close_this, uint32_t fbase)
+{
+    finalize_ens_block(E, istep_off, measure, close_prev, close_this, fbase, blockIdx.x);
+}
+
+// Held chirality trials (held_group) of the nf replicas order[first ..
+// first + nf - 1], hb blocks per replica.  With fin, nf more blocks run those
+// replicas' epilogue that closes the sweep's director accepts
+// (finalize_ens_block, close_prev as given): it reads the accepts and the
+// previous sweep's flips, none of which the held trials write.
+constexpr uint32_t kHeldEnsBlocks = 2;
+static_assert(kHeldBlockThreads == kEnsFin, "the epilogue blocks of k_held_ens take the held blocks' size");
+template <int BC>
+__global__ __launch_bounds__(kHeldBlockThreads) void k_held_ens(EnsDev E, uint32_t first, uint32_t nf, uint32_t hb,
+                                                              uint32_t istep_off, int fin, int close_prev)
+{
+    if (blockIdx.x >= nf * hb) {
+        if (fin) finalize_ens_block(E, istep_off, 0, close_prev, 0, first, blockIdx.x - nf * hb);
+        return;
+    }
+    const uint32_t r = E.order[first + blockIdx.x / hb], blk = blockIdx.x % hb;
+    const RepDesc &D = E.R[r];
+    const int s = E.rep_slot[r];
+    PhaseArgs A;
+    A.g = D.g; A.KK = D.KK; A.kbt = E.kbt[s]; A.inv_kbt = E.inv_kbt[s]; A.inv_kbt_spin = A.inv_kbt; A.d = E.d + s;
+    A.screen = kScreenScale;
+    A.kbt_spin = A.kbt; A.flipbits = D.flipbits; A.held = D.held;
+    A.u_phi = nullptr; A.u_acc = nullptr; A.u_spin = nullptr;
+    A.seed = E.seed; A.run_index = E.run_index; A.replica = D.replica; A.istep = *E.istep + istep_off;
+    A.istep_spin = A.istep;
+    const size_t gs = (size_t)kSlots * kSlotStride;
+    held_group<RNG_PHILOX, BC>(D.G, D.F, A, A, D.held, D.counts + flip_group(A.istep, 1) * gs, nullptr, blk, hb);
+}
+
 
 // Parallel-tempering swaps, one thread per ladder, then advance the sweep and
 // row counters.  Round q = istep / K alternates even and odd slot pairs.
@@ -755,15 +772,17 @@ int launch_block_t(chll_ensemble *X, int K, hipStream_t st)
             if (hfuse) {
                 // colour 1's director + chirality trials, then its held trials
                 hipLaunchKernelGGL((k_pair_ens_h<2, BC, BS>), g2, b, 0, cs, E, off, 0, c.t2);
-                hipLaunchKernelGGL((k_held_ens<BC>), dim3(c.nf * kHeldEnsBlocks), dim3(kHeldBlockThreads), 0, cs, E,
-                                   c.f, kHeldEnsBlocks, off);
+                // with, in a chain, the epilogue closing the director accepts
+                hipLaunchKernelGGL((k_held_ens<BC>), dim3(c.nf * (kHeldEnsBlocks + (last ? 0u : 1u))),
+                                   dim3(kHeldBlockThreads), 0, cs, E, c.f, c.nf, kHeldEnsBlocks, off, last ? 0 : 1,
+                                   t > 0 ? 1 : 0);
             } else if (pairs) {
                 hipLaunchKernelGGL((k_pair_ens<2, BC, BS>), g2, b, 0, cs, E, off, 0, c.t2);
             } else {
                 hipLaunchKernelGGL((k_phase_ens<2, ARITH, BC, BS>), g, b, 0, cs, E, 1, off, 0, c.t1);
             }
             if (!last) {
-                hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, cs, E, off, 0, t > 0 ? 1 : 0, 0, c.f);
+                if (!hfuse) hipLaunchKernelGGL(k_finalize_ens, gf, bf, 0, cs, E, off, 0, t > 0 ? 1 : 0, 0, c.f);
                 if (hfuse) hipLaunchKernelGGL((k_pair_ens_h<3, BC, BS>), g2, b, 0, cs, E, off, 1, r2);
                 else if (pairs) hipLaunchKernelGGL((k_pair_ens<3, BC, BS>), g2, b, 0, cs, E, off, 1, r2);
                 else hipLaunchKernelGGL((k_phase_ens<3, ARITH, BC, BS>), g, b, 0, cs, E, 0, off, 1, r1);
