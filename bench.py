@@ -343,6 +343,8 @@ def main():
                            "kernels": {k: round(v["avg_us"], 2) for k, v in v_k.items()}}
         if other == "compat":
             variants[other]["held_trials_per_sweep"] = round(held_rate.get(other, 0.0), 1)
+        # the reference algorithm's accounting (125 B per site-update) against 8 TB/s
+        variants[other]["sweep_level_frac"] = v_sups * 125.0 / (HBM_PEAK_GBS * 1e9)
         # the same workload with periodic boundaries, and without measurement sweeps
         other_bc = "periodic" if args.bc == "open" else "open"
         for name, kw in ((f"{args.arith}_{other_bc}", {"bc": other_bc}), (f"{args.arith}_no_measure", {"nout": 0})):
