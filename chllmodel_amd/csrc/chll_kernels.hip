@@ -129,8 +129,15 @@ __global__ __launch_bounds__(kBlock) void k_spin_dir(Geom G, Fields F, PhaseArgs
 // flip bits of its 128 sites (two 64-bit words, site m at bit m % 64 of word
 // m / 64) and appends its held trials to A.held.  Needs an even half
 // (ni % 4 == 0) and fields below 4 GiB.
+// Waves per SIMD the compiler must fit: the reference-arithmetic colour-0
+// fused launch (OP 3) fits 3 with a few spilled registers and runs 9 % faster
+// than at the 2 its natural 176-184 VGPRs give; the colour-1 one (OP 2) would
+// spill over 200 bytes per lane.
+template <int OP, int ARITH>
+constexpr int kPairMinWaves = (OP == 3 && ARITH == ARITH_COMPAT) ? 3 : 1;
+
 template <int OP, int ARITH, int RNG, int BC>
-__global__ __launch_bounds__(kBlock) void k_pair(Geom G, Fields F, int colour, PhaseArgs A,
+__global__ __launch_bounds__(kBlock, (kPairMinWaves<OP, ARITH>)) void k_pair(Geom G, Fields F, int colour, PhaseArgs A,
                                              uint32_t *__restrict__ c0, uint32_t *__restrict__ c1,
                                              uint32_t band)
 {
